@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark of the DG RHS + RK4 hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): scalar wave
+(scalarwave1) on a cubed-sphere shell r in [2, 20], six patches of 8x8x8
+elements (3072 elements), N=7 (8 LGL points per direction, 1,572,864 nodes),
+FP64, upwind flux, outgoing boundaries (scalarwave1_k = 0 0 0), RK4 at
+dtfac 0.25.  Initial data: a Gaussian shell psi = exp(-(r-8)^2) (1 + sum_m
+a_m Y_2m) with a_m ~ U[-0.1, 0.1] from numpy's PCG64 seeded with 2026, the
+constraint-satisfying Phi_i = d_i psi and Pi = 0.
+
+One "step" is one full RK4 step (4 fused RHS+stage launches).  The unit of
+work is a node-update: one RHS evaluation at one node for all 5 fields.
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def shell_initial_data(x, seed=2026):
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(-0.1, 0.1, 5)
+    c1 = 0.5 * np.sqrt(15.0 / np.pi)
+    c0 = 0.25 * np.sqrt(5.0 / np.pi)
+    # S = x^T A x / r^2 = sum_m a_m Y_2m (real harmonics)
+    A = np.zeros((3, 3))
+    A[0, 1] = A[1, 0] = 0.5 * c1 * a[0]            # Y_2,-2 ~ xy
+    A[1, 2] = A[2, 1] = 0.5 * c1 * a[1]            # Y_2,-1 ~ yz
+    A += c0 * a[2] * np.diag([-1.0, -1.0, 2.0])    # Y_2,0 ~ 3z^2 - r^2
+    A[0, 2] = A[2, 0] = 0.5 * c1 * a[3]            # Y_2,1 ~ xz
+    A[0, 0] += 0.5 * c1 * a[4]                     # Y_2,2 ~ x^2 - y^2
+    A[1, 1] -= 0.5 * c1 * a[4]
+    r2 = (x ** 2).sum(0)
+    r = np.sqrt(r2)
+    Ax = A @ x
+    Q = (x * Ax).sum(0)
+    S = Q / r2
+    G = np.exp(-(r - 8.0) ** 2)
+    dG = -2.0 * (r - 8.0) * G
+    psi = G * (1.0 + S)
+    dS = (2.0 * Ax * r2 - 2.0 * x * Q) / r2 ** 2
+    dpsi = dG * x / r * (1.0 + S) + G * dS
+    u = np.zeros((5, x.shape[1]))
+    u[1:4] = dpsi
+    u[4] = psi
+    return u.ravel()
+
+
+def c2_params():
+    return {
+        "amr_mesh_type": "Shell", "amr_Shell_rin": 2.0, "amr_Shell_rout": 20.0,
+        "amr_luni": 3, "amr_n0": 8, "amr_n1": 8, "amr_n2": 8,
+        "scalarwave1_k": "0 0 0", "scalarwave1_numflux": "upwind",
+        "evolve_method": "RK4", "dtfac": 0.25, "dt": "auto",
+    }
+
+
+def load_traffic(name):
+    p = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
+def cpu_baseline(x_unused, budget_s=12.0):
+    """Oracle (OpenMP C restatement of the reference algorithm) on the same
+    mesh and operator, timed on the host cores for a bounded number of RK4
+    steps."""
+    from oracle.orc import OMesh, lib
+    om = OMesh("shell", 3, 8, rin=2.0, rout=20.0)
+    X, x = om.coords()
+    u = shell_initial_data(x)
+    dt = om.dt()
+    k = [0.0, 0.0, 0.0]
+    # one step first, then as many as fit the budget
+    t0 = time.perf_counter()
+    u, t = om.evolve(u, dt, 1, 0.0, k, "RK4")
+    one = time.perf_counter() - t0
+    nmore = max(0, min(8, int((budget_s - one) / max(one, 1e-3))))
+    t0 = time.perf_counter()
+    if nmore:
+        u, t = om.evolve(u, dt, nmore, t, k, "RK4")
+    el = time.perf_counter() - t0
+    steps = nmore if nmore else 1
+    secs = el if nmore else one
+    rate = steps * 4 * om.nnodes / secs
+    return {"value": rate, "unit": "node-updates/s", "cores": int(lib().orc_num_threads()),
+            "kind": "port",
+            "sample": "%d RK4 step(s) of the same 3072-element N=7 shell (%d node-updates) "
+                      "with the oracle's OpenMP C restatement" % (steps, steps * 4 * om.nnodes)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--path", default="plain")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU: the hot path is measured on one "
+              "GPU's share of the mesh; multi-GPU runs are not part of this build.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import nmesh_amd as nm
+    try:
+        import torch
+        have_torch = torch.cuda.is_available()
+    except Exception:
+        have_torch = False
+
+    m = nm.Mesh(c2_params(), device=0)
+    m.set_par("nmx_path", args.path)
+    nm.amr_setup_mesh(m)
+    x = m.coords()
+    m.upload(shell_initial_data(x))
+    nn = m.nnodes
+    if args.warmup:
+        nm.evolve_myln(m, args.warmup)
+    m.synchronize()
+    if have_torch:
+        torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nm.evolve_myln(m, args.steps)
+    m.synchronize()
+    if have_torch:
+        torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    dev_ms = m.last_evolve_ms()    # HIP events on the library's stream around the steps
+    launches = 4 * args.steps
+    avg_launch_s = dev_ms * 1e-3 / launches
+    node_updates = 4.0 * args.steps * nn
+    value = node_updates / wall
+    # algorithmic bytes per node-update, averaged over the 4 RK4 stages:
+    # stage 1 reads u and writes u_acc, w (15 doubles), stages 2-3 read w,
+    # u_acc, u_p and write u_acc, w (25), stage 4 reads w, u_acc and writes
+    # u_acc (15); plus 9 doubles of dXdx per node; face-neighbour re-reads count 0
+    bytes_per_nu = 8.0 * ((15 + 25 + 25 + 15) / 4.0 + 9.0)
+    achieved = bytes_per_nu * nn / avg_launch_s / 1e9
+    traffic = load_traffic("traffic_dg_stage_shell_n8.json")
+    out = {
+        "metric": "DG RHS node-updates/s (FP64)",
+        "value": value,
+        "unit": "node-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "scalarwave1 DG, cubed-sphere shell r in [2,20], 6x8^3 elements, "
+                               "N=7 (1,572,864 nodes), upwind, outgoing, RK4, fused RHS+stage",
+                   "elements": m.nelem, "nodes": nn, "dt": m.dt, "path": args.path},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "kernel": "dg_stage_kernel<8,true,512>",
+                     "avg_launch_us": avg_launch_s * 1e6,
+                     "bytes_per_node_update": bytes_per_nu},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(x)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

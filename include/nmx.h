@@ -1,0 +1,129 @@
+/* nmx.h -- C ABI of the MI355X-native DG/FV evolution library (libnmx.so).
+ *
+ * The library replaces nmesh's element-local evolution hot path: the DG
+ * right-hand side of the first-order scalar wave (scalarwave1), the face
+ * numerical flux and lift, the Runge-Kutta stage update, the FV WENO5 + LLF
+ * reconstruction, and a discrete adjoint of the RK step.  Everything below is
+ * plain C: opaque handles, pointers and sizes.  Every entry returns 0 on
+ * success and a negative NMX_E* code on failure; nmx_last_error() then holds a
+ * message (per thread).  One host thread drives one context; work is ordered
+ * on the context's HIP stream (nmx_set_stream).
+ *
+ * Reference call surface each group replaces (paths relative to the nmesh
+ * source tree):
+ *   - parameters by name: AddPar/Par/Get* (src/main/main/nmesh_main.h:89-107);
+ *     names are nmesh's own (amr_*, dt, dtfac, evolve_method, scalarwave1_*).
+ *   - mesh set-up: amr_setup_mesh (src/main/amr/mesh.c:181-319).
+ *   - per-element variable arrays: Vard(node, vi) (src/main/amr/nmesh_amr_loops.h:181).
+ *   - RHS: evolve_setrhs_mesh (src/main/evolve/evolve.h:15, evolve.c:182-232).
+ *   - steppers: evolve_myln / evolve_RK4_mesh / evolve_sspRK3_mesh /
+ *     evolve_Euler_mesh (src/main/evolve/evolve.c:25, RungeKutta.c:19-188).
+ *   - time step: update_mesh_myelms_elm_eid_dt (src/main/amr/storage.c:1370-1386).
+ */
+#ifndef NMX_H
+#define NMX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define NMX_OK 0
+#define NMX_EINVAL (-1)   /* bad argument or parameter value */
+#define NMX_ESTATE (-2)   /* call out of order (e.g. no mesh yet) */
+#define NMX_EHIP (-3)     /* HIP runtime error */
+#define NMX_ENOMEM (-4)   /* allocation failed */
+#define NMX_EUNSUP (-5)   /* configuration not supported */
+
+/* state registers */
+#define NMX_REG_U 0       /* evolved variables u (scalarwave1 vlu) */
+#define NMX_REG_R 1       /* RHS r = RHS(u) ("_r" vars, evosys.c:256) */
+#define NMX_REG_P 2       /* u at previous time ("_p" vars, evosys.c:257) */
+
+/* variables of scalarwave1's vlu, in order (scalarwave1.c:670-672) */
+#define NMX_NVARS 5       /* pi, cx, cy, cz, phi */
+
+typedef struct nmx_ctx nmx_ctx;
+
+const char *nmx_last_error(void);
+
+/* context: device is the HIP device ordinal. */
+int nmx_ctx_create(nmx_ctx **ctx, int device);
+int nmx_ctx_destroy(nmx_ctx *ctx);
+/* stream is a hipStream_t (NULL = the library's own stream) */
+int nmx_set_stream(nmx_ctx *ctx, void *stream);
+int nmx_synchronize(nmx_ctx *ctx);
+
+/* Parameters under nmesh's names (replaces AddPar/Par, nmesh_main.h:89-107):
+ * amr_mesh_type (BoxMesh|Shell), amr_n0/1/2, amr_luni, amr_BoxMesh_dout,
+ * amr_mesh_xc, amr_Shell_rin, amr_Shell_rout, amr_hrefine_sphere_radius,
+ * amr_hrefine_sphere_levels, dt (auto|number), dtfac, evolve_method
+ * (RK4|sspRK3|Euler), scalarwave1_k, scalarwave1_numflux (upwind|LLF),
+ * scalarwave1_profile (sin), plus library knobs prefixed nmx_ (nmx_path:
+ * plain|batched|perelement, nmx_graph: yes|no, nmx_geometry: stored|analytic). */
+int nmx_set_param(nmx_ctx *ctx, const char *name, const char *value);
+int nmx_get_param(nmx_ctx *ctx, const char *name, char *buf, int buflen);
+
+/* Build the mesh, geometry, neighbour tables and device mirror from the
+ * parameters (replaces amr_setup_mesh, mesh.c:181-319, and coordinates_init,
+ * coordinates.c:133-443).  Sets mesh time 0 and dt per the dt parameter. */
+int nmx_mesh_build(nmx_ctx *ctx);
+/* p-refine: set points per direction of every element (n[nelem]) and rebuild
+ * tables (replaces update_node_n_pt_typ in src/main/amr/storage.c). */
+int nmx_mesh_set_elm_n(nmx_ctx *ctx, const int *n, int nelem);
+
+int nmx_mesh_nelem(nmx_ctx *ctx, int *nelem);
+int nmx_mesh_nnodes(nmx_ctx *ctx, long *nnodes);
+/* info[22]: patch, level, n, node offset, nfnb[6], ajsurf mode[6] (0 none,
+ * 1 alias, 2 oriented copy, 3 interpolation), orientation code[6]
+ * (intrch*4 + rev1*2 + rev2 as in copy_ajsurf_from_nbsurf0, surface.c:1361) */
+int nmx_mesh_elm_info(nmx_ctx *ctx, int elm, long *info);
+int nmx_mesh_elm_bbox(nmx_ctx *ctx, int elm, double *bbox6);
+/* face neighbours of elm on face f (node->fnb[f], nmesh_amr.h:96-98); nb and
+ * nbface must hold 8 entries; returns the count in *nnb */
+int nmx_mesh_elm_fnb(nmx_ctx *ctx, int elm, int f, int *nb, int *nbface, int *nnb);
+/* per-node Cartesian coordinates x,y,z as x[d*nnodes + node] */
+int nmx_mesh_coords(nmx_ctx *ctx, double *x);
+/* dXdx[(3a+b)*nnodes+node], sqrtgdiag[a*nnodes+node] (coordinates.c:133-224,728-805) */
+int nmx_mesh_geometry(nmx_ctx *ctx, double *dXdx, double *sqrtgdiag);
+/* time step from the dt/dtfac parameters (storage.c:1305-1386) and time */
+int nmx_get_dt(nmx_ctx *ctx, double *dt);
+int nmx_set_dt(nmx_ctx *ctx, double dt);
+int nmx_get_time(nmx_ctx *ctx, double *t);
+int nmx_set_time(nmx_ctx *ctx, double t);
+
+/* State transfer in the flat layout u[var*nnodes + node], node = element
+ * offset + ijk, ijk = i + n*(j + n*k) (nmesh_amr_loops.h:279). */
+int nmx_state_upload(nmx_ctx *ctx, int reg, const double *u);
+int nmx_state_download(nmx_ctx *ctx, int reg, double *u);
+/* one variable of one element, as Vard(node, vi) would hold it */
+int nmx_state_upload_elm(nmx_ctx *ctx, int reg, int elm, int var, const double *d, int np);
+int nmx_state_download_elm(nmx_ctx *ctx, int reg, int elm, int var, double *d, int np);
+/* set u to scalarwave1's profile at time t (scalarwave1_set_profile, scalarwave1.c:574-642) */
+int nmx_state_set_profile(nmx_ctx *ctx, int reg, double t);
+
+/* r = RHS(u, t) on the device (evolve_setrhs_mesh, evolve.c:182-232) */
+int nmx_setrhs(nmx_ctx *ctx, int reg_out, int reg_in, double t);
+/* nsteps full time steps with evolve_method (evolve_myln, evolve.c:25-176;
+ * RungeKutta.c:19-188).  Advances NMX_REG_U and the mesh time. */
+int nmx_evolve(nmx_ctx *ctx, int nsteps);
+
+/* device pointer and length (doubles) of a register, for zero-copy use from
+ * frameworks; valid until the mesh is rebuilt */
+int nmx_state_device_ptr(nmx_ctx *ctx, int reg, void **dptr, long *len);
+
+/* Discrete adjoint of nsteps RK steps (no counterpart in the reference; the
+ * forward operator it transposes is evolve_RK4_mesh above).  Given lambda_T
+ * = dL/du(T) in reg NMX_REG_R ... see nmx_adjoint_* below. */
+int nmx_adjoint_rhs(nmx_ctx *ctx, const double *lam, double *out, double t);
+/* lam := (dU_{n}/dU_{n-1})^T lam for one RK step of the linear operator */
+int nmx_adjoint_step(nmx_ctx *ctx, double *lam_inout, double t);
+
+/* profiling helpers: time (ms) of the last nmx_evolve on the device and the
+ * number of kernel launches it issued */
+int nmx_last_evolve_ms(nmx_ctx *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

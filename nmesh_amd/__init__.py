@@ -1,0 +1,10 @@
+"""nmesh_amd: MI355X-native DG/FV evolution hot path for nmesh meshes.
+
+The compute path is libnmx.so (hand-written HIP kernels for gfx950 behind a
+C ABI, include/nmx.h); this package holds its ctypes binding and a host-side
+mirror of nmesh's evolution interface.
+"""
+from ._lib import LIB_PATH, NmxError, lib  # noqa: F401
+from .evolve import (Mesh, amr_setup_mesh, evolve_Euler_mesh, evolve_myln,  # noqa: F401
+                     evolve_RK4_mesh, evolve_setrhs_mesh, evolve_sspRK3_mesh,
+                     scalarwave1_set_profile)

@@ -1,0 +1,683 @@
+// api.cpp -- the C ABI of libnmx (include/nmx.h): context, parameters under
+// nmesh's names, device mirror of the mesh, state transfer and the RK
+// drivers that issue the fused stage kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/nmx.h"
+#include "kernels.h"
+#include "mesh.h"
+
+using namespace nmx;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess)                                                           \
+      return fail(NMX_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+struct DevBuf {
+  double *p = nullptr;
+  size_t n = 0;
+};
+
+struct nmx_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  std::map<std::string, std::string> par;
+  bool have_mesh = false;
+  Mesh mesh;
+  // derived
+  int np1 = 0;           // uniform points per direction (0: mixed)
+  bool curv = false;
+  long nstate = 0;       // doubles per register
+  double dt = 0, time = 0;
+  long double Te = 0, te = 0;   // time = Te + te (main.c:588-624)
+  // device mirror
+  double *dU = nullptr, *dA = nullptr, *dW0 = nullptr, *dW1 = nullptr, *dR = nullptr, *dP = nullptr;
+  double *dXdx = nullptr, *dXbdX = nullptr, *dxyz = nullptr, *dD = nullptr, *dT = nullptr;
+  int4 *dface = nullptr;
+  std::vector<long> eoff;   // node offset per element
+  // graph cache
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_steps = 0;
+  double graph_dt = 0;
+  int graph_method = -1;
+  double *hT = nullptr;     // pinned stage-time staging
+  int tcap = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0;
+};
+
+extern "C" const char *nmx_last_error(void) { return g_err.c_str(); }
+
+static void set_defaults(nmx_ctx *c) {
+  // defaults of the reference parameters (nmesh_amr.c:31-121, nmesh_main.c:29-35,
+  // nmesh_evolve.c:26, nmesh_scalarwave1.c:31-34)
+  c->par = {
+      {"amr_mesh_type", "BoxMesh"},
+      {"amr_n0", "5"},
+      {"amr_n1", "5"},
+      {"amr_n2", "5"},
+      {"amr_luni", "0"},
+      {"amr_BoxMesh_dout", "1"},
+      {"amr_mesh_xc", "0 0 0"},
+      {"amr_Shell_rin", "0.5"},
+      {"amr_Shell_rout", "1"},
+      {"amr_hrefine_sphere_radius", "10"},
+      {"amr_hrefine_sphere_levels", "0"},
+      {"dt", "auto"},
+      {"dtfac", "0.25"},
+      {"evolve_method", "RK4"},
+      {"scalarwave1_k", "0.8 -0.5 0.33166247903554"},
+      {"scalarwave1_numflux", "upwind"},
+      {"scalarwave1_profile", "sin"},
+      {"nmx_path", "plain"},
+      {"nmx_graph", "yes"},
+      {"nmx_graph_steps", "8"},
+  };
+}
+
+static std::string P(nmx_ctx *c, const char *k) { return c->par[k]; }
+static double Pd(nmx_ctx *c, const char *k) { return std::atof(c->par[k].c_str()); }
+static int Pi(nmx_ctx *c, const char *k) { return std::atoi(c->par[k].c_str()); }
+
+extern "C" int nmx_ctx_create(nmx_ctx **out, int device) {
+  if (!out) return fail(NMX_EINVAL, "ctx pointer is NULL");
+  if (device == -1) {   // host-only context: mesh and tables, no device mirror
+    nmx_ctx *c = new nmx_ctx();
+    c->device = -1;
+    set_defaults(c);
+    *out = c;
+    return NMX_OK;
+  }
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(NMX_EINVAL, "no such HIP device");
+  nmx_ctx *c = new nmx_ctx();
+  c->device = device;
+  set_defaults(c);
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+  c->stream = c->own_stream;
+  HIPCHK(hipEventCreate(&c->ev0));
+  HIPCHK(hipEventCreate(&c->ev1));
+  *out = c;
+  return NMX_OK;
+}
+
+static void free_mesh_dev(nmx_ctx *c) {
+  double **bufs[] = {&c->dU, &c->dA, &c->dW0, &c->dW1, &c->dR, &c->dP,
+                     &c->dXdx, &c->dXbdX, &c->dxyz, &c->dD, &c->dT};
+  for (auto b : bufs)
+    if (*b) { (void)hipFree(*b); *b = nullptr; }
+  if (c->dface) { (void)hipFree(c->dface); c->dface = nullptr; }
+  if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+  c->graph_steps = 0;
+}
+
+extern "C" int nmx_ctx_destroy(nmx_ctx *c) {
+  if (!c) return NMX_OK;
+  if (c->device < 0) { delete c; return NMX_OK; }
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_mesh_dev(c);
+  if (c->hT) (void)hipHostFree(c->hT);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+  return NMX_OK;
+}
+
+extern "C" int nmx_set_stream(nmx_ctx *c, void *s) {
+  if (!c) return fail(NMX_EINVAL, "ctx is NULL");
+  c->stream = s ? (hipStream_t)s : c->own_stream;
+  return NMX_OK;
+}
+
+extern "C" int nmx_synchronize(nmx_ctx *c) {
+  if (!c) return fail(NMX_EINVAL, "ctx is NULL");
+  if (c->device < 0) return NMX_OK;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return NMX_OK;
+}
+
+extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
+  if (!c || !name || !value) return fail(NMX_EINVAL, "NULL argument");
+  std::string k(name), v(value);
+  if (!c->par.count(k)) return fail(NMX_EINVAL, "unknown parameter " + k);
+  if (k == "evolve_method" && v != "RK4" && v != "sspRK3" && v != "Euler")
+    return fail(NMX_EINVAL, "evolve_method must be RK4, sspRK3 or Euler");
+  if (k == "scalarwave1_numflux" && v != "upwind" && v != "LLF")
+    return fail(NMX_EINVAL, "scalarwave1_numflux must be upwind or LLF");
+  if (k == "scalarwave1_profile" && v != "sin")
+    return fail(NMX_EUNSUP, "only scalarwave1_profile = sin is supported");
+  if (k == "nmx_path" && v != "plain" && v != "perelement" && v != "batched")
+    return fail(NMX_EINVAL, "nmx_path must be plain, perelement or batched");
+  c->par[k] = v;
+  if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+  c->graph_steps = 0;
+  if (k == "dt" && c->have_mesh) {
+    if (v == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"));
+    else c->dt = std::atof(v.c_str());
+  }
+  return NMX_OK;
+}
+
+extern "C" int nmx_get_param(nmx_ctx *c, const char *name, char *buf, int len) {
+  if (!c || !name || !buf || len <= 0) return fail(NMX_EINVAL, "NULL argument");
+  auto it = c->par.find(name);
+  if (it == c->par.end()) return fail(NMX_EINVAL, std::string("unknown parameter ") + name);
+  std::snprintf(buf, len, "%s", it->second.c_str());
+  return NMX_OK;
+}
+
+template <class T>
+static int dev_alloc(T **p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(NMX_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return NMX_OK;
+}
+
+// host -> device mirror of the mesh (element-block layout)
+static int upload_mesh(nmx_ctx *c) {
+  free_mesh_dev(c);
+  Mesh &M = c->mesh;
+  const long nn = M.nnodes;
+  const size_t ne = M.e.size();
+  c->np1 = M.e[0].n;
+  for (auto &E : M.e)
+    if (E.n != c->np1) c->np1 = 0;
+  c->curv = false;
+  for (auto &Pt : M.pat)
+    if (Pt.type == PAT_CUBEDSHELL) c->curv = true;
+  if (c->np1 == 0) return fail(NMX_EUNSUP, "mixed element orders need nmx_path = batched");
+  for (size_t i = 0; i < ne; ++i)
+    for (int f = 0; f < 6; ++f)
+      if (M.e[i].ajmode[f] == 3) return fail(NMX_EUNSUP, "non-conforming faces need nmx_path = batched");
+  c->nstate = 5L * nn;
+  c->eoff.resize(ne);
+  for (size_t i = 0; i < ne; ++i) c->eoff[i] = M.e[i].off;
+  int r;
+  if ((r = dev_alloc(&c->dU, c->nstate)) || (r = dev_alloc(&c->dA, c->nstate)) ||
+      (r = dev_alloc(&c->dW0, c->nstate)) || (r = dev_alloc(&c->dW1, c->nstate)) ||
+      (r = dev_alloc(&c->dR, c->nstate)) || (r = dev_alloc(&c->dP, c->nstate)))
+    return r;
+  HIPCHK(hipMemset(c->dU, 0, c->nstate * sizeof(double)));
+  HIPCHK(hipMemset(c->dR, 0, c->nstate * sizeof(double)));
+  HIPCHK(hipMemset(c->dP, 0, c->nstate * sizeof(double)));
+  // geometry in element blocks
+  std::vector<double> h9(c->curv ? 9 * nn : 1), h3(3 * ne), hx(3 * nn);
+  for (size_t i = 0; i < ne; ++i) {
+    const Elm &E = M.e[i];
+    long np = (long)E.n * E.n * E.n;
+    for (int d = 0; d < 3; ++d) h3[3 * i + d] = 2. / (E.bbox[2 * d + 1] - E.bbox[2 * d]);
+    for (long q = 0; q < np; ++q) {
+      long g = E.off + q;
+      if (c->curv)
+        for (int m = 0; m < 9; ++m) h9[9 * E.off + m * np + q] = M.dXdx[m * nn + g];
+      for (int d = 0; d < 3; ++d) hx[3 * E.off + d * np + q] = M.x[d * nn + g];
+    }
+  }
+  if ((r = dev_alloc(&c->dXdx, h9.size())) || (r = dev_alloc(&c->dXbdX, h3.size())) ||
+      (r = dev_alloc(&c->dxyz, hx.size())))
+    return r;
+  HIPCHK(hipMemcpy(c->dXdx, h9.data(), h9.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dXbdX, h3.data(), h3.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dxyz, hx.data(), hx.size() * 8, hipMemcpyHostToDevice));
+  // face gather descriptors {nb, local base, s1, s2}
+  std::vector<int4> hf(6 * ne);
+  for (size_t i = 0; i < ne; ++i)
+    for (int f = 0; f < 6; ++f) {
+      const Elm &E = M.e[i];
+      int4 d = {-1, 0, 0, 0};
+      if (E.ajmode[f] == 1 || E.ajmode[f] == 2) {
+        const FaceCopy &fc = M.fcopy[6 * i + f];
+        int nb = E.fnb[f][0];
+        d.x = nb;
+        d.y = (int)(fc.nbase - M.e[nb].off);
+        d.z = fc.s1;
+        d.w = fc.s2;
+        // every gathered node must lie inside the neighbour (the map is affine,
+        // so checking the four corners covers the face)
+        const long np3 = (long)c->np1 * c->np1 * c->np1;
+        for (int cb = 0; cb < 2; ++cb)
+          for (int ca = 0; ca < 2; ++ca) {
+            long q = d.y + (long)ca * (c->np1 - 1) * d.z + (long)cb * (c->np1 - 1) * d.w;
+            if (nb < 0 || nb >= (int)ne || q < 0 || q >= np3)
+              return fail(NMX_EINVAL, "face gather table out of range");
+          }
+      }
+      hf[6 * i + f] = d;
+    }
+  if ((r = dev_alloc(&c->dface, hf.size()))) return r;
+  HIPCHK(hipMemcpy(c->dface, hf.data(), hf.size() * sizeof(int4), hipMemcpyHostToDevice));
+  const Basis &b = lgl_basis(c->np1);
+  std::vector<double> hD(b.D, b.D + c->np1 * c->np1);
+  if ((r = dev_alloc(&c->dD, hD.size()))) return r;
+  HIPCHK(hipMemcpy(c->dD, hD.data(), hD.size() * 8, hipMemcpyHostToDevice));
+  c->tcap = 4 * 64;
+  if ((r = dev_alloc(&c->dT, c->tcap))) return r;
+  if (!c->hT) HIPCHK(hipHostMalloc((void **)&c->hT, c->tcap * sizeof(double), hipHostMallocDefault));
+  return NMX_OK;
+}
+
+extern "C" int nmx_mesh_build(nmx_ctx *c) {
+  if (!c) return fail(NMX_EINVAL, "ctx is NULL");
+  if (c->device >= 0) HIPCHK(hipSetDevice(c->device));
+  std::string type = P(c, "amr_mesh_type");
+  int n0 = Pi(c, "amr_n0"), n1 = Pi(c, "amr_n1"), n2 = Pi(c, "amr_n2");
+  if (n0 != n1 || n0 != n2) return fail(NMX_EUNSUP, "amr_n0, amr_n1, amr_n2 must be equal");
+  int luni = Pi(c, "amr_luni");
+  if (luni < 0 || luni > 8) return fail(NMX_EINVAL, "amr_luni out of range");
+  std::string err;
+  if (type.find("BoxMesh") != std::string::npos) {
+    double xc[3] = {0, 0, 0};
+    std::sscanf(P(c, "amr_mesh_xc").c_str(), "%lg %lg %lg", &xc[0], &xc[1], &xc[2]);
+    err = build_box(c->mesh, luni, n0, Pd(c, "amr_BoxMesh_dout"), xc);
+  } else if (type.find("Shell") != std::string::npos) {
+    err = build_shell(c->mesh, luni, n0, Pd(c, "amr_Shell_rin"), Pd(c, "amr_Shell_rout"),
+                      (double)Pi(c, "amr_hrefine_sphere_radius"), Pi(c, "amr_hrefine_sphere_levels"));
+  } else {
+    return fail(NMX_EUNSUP, "amr_mesh_type must be BoxMesh or Shell");
+  }
+  if (!err.empty()) return fail(NMX_EINVAL, err);
+  c->have_mesh = true;
+  c->time = 0; c->Te = 0; c->te = 0;
+  if (P(c, "dt") == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"));
+  else c->dt = Pd(c, "dt");
+  if (c->device < 0) return NMX_OK;                    // host-only context
+  if (P(c, "nmx_path") == "batched") return NMX_OK;   // device mirror built lazily
+  return upload_mesh(c);
+}
+
+extern "C" int nmx_mesh_set_elm_n(nmx_ctx *c, const int *n, int nelem) {
+  if (!c || !n) return fail(NMX_EINVAL, "NULL argument");
+  if (!c->have_mesh) return fail(NMX_ESTATE, "no mesh");
+  std::vector<int> v(n, n + nelem);
+  std::string err = set_elm_n(c->mesh, v);
+  if (!err.empty()) return fail(NMX_EINVAL, err);
+  if (P(c, "dt") == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"));
+  if (c->device < 0) return NMX_OK;
+  if (P(c, "nmx_path") == "batched") return NMX_OK;
+  return upload_mesh(c);
+}
+
+#define NEED_MESH(c)                                        \
+  do {                                                      \
+    if (!(c)) return fail(NMX_EINVAL, "ctx is NULL");      \
+    if (!(c)->have_mesh) return fail(NMX_ESTATE, "no mesh"); \
+  } while (0)
+
+#define NEED_DEV(c)                                                              \
+  do {                                                                           \
+    NEED_MESH(c);                                                                \
+    if ((c)->device < 0) return fail(NMX_ESTATE, "host-only context has no device"); \
+    if (!(c)->dU) return fail(NMX_ESTATE, "no device mirror");                   \
+  } while (0)
+
+extern "C" int nmx_mesh_nelem(nmx_ctx *c, int *ne) {
+  NEED_MESH(c);
+  *ne = (int)c->mesh.e.size();
+  return NMX_OK;
+}
+
+extern "C" int nmx_mesh_nnodes(nmx_ctx *c, long *nn) {
+  NEED_MESH(c);
+  *nn = c->mesh.nnodes;
+  return NMX_OK;
+}
+
+extern "C" int nmx_mesh_elm_info(nmx_ctx *c, int e, long *info) {
+  NEED_MESH(c);
+  if (e < 0 || e >= (int)c->mesh.e.size()) return fail(NMX_EINVAL, "element out of range");
+  const Elm &E = c->mesh.e[e];
+  info[0] = E.p; info[1] = E.l; info[2] = E.n; info[3] = E.off;
+  for (int f = 0; f < 6; ++f) {
+    info[4 + f] = E.nfnb[f];
+    info[10 + f] = E.ajmode[f];
+    info[16 + f] = E.orient[f];
+  }
+  return NMX_OK;
+}
+
+extern "C" int nmx_mesh_elm_bbox(nmx_ctx *c, int e, double *bb) {
+  NEED_MESH(c);
+  if (e < 0 || e >= (int)c->mesh.e.size()) return fail(NMX_EINVAL, "element out of range");
+  std::memcpy(bb, c->mesh.e[e].bbox, 6 * sizeof(double));
+  return NMX_OK;
+}
+
+extern "C" int nmx_mesh_elm_fnb(nmx_ctx *c, int e, int f, int *nb, int *nbf, int *nnb) {
+  NEED_MESH(c);
+  if (e < 0 || e >= (int)c->mesh.e.size() || f < 0 || f > 5) return fail(NMX_EINVAL, "out of range");
+  const Elm &E = c->mesh.e[e];
+  for (int i = 0; i < E.nfnb[f]; ++i) {
+    nb[i] = E.fnb[f][i];
+    if (nbf) nbf[i] = E.nbf[f][i];
+  }
+  *nnb = E.nfnb[f];
+  return NMX_OK;
+}
+
+extern "C" int nmx_mesh_coords(nmx_ctx *c, double *x) {
+  NEED_MESH(c);
+  std::memcpy(x, c->mesh.x.data(), c->mesh.x.size() * sizeof(double));
+  return NMX_OK;
+}
+
+extern "C" int nmx_mesh_geometry(nmx_ctx *c, double *dXdx, double *sg) {
+  NEED_MESH(c);
+  if (dXdx) std::memcpy(dXdx, c->mesh.dXdx.data(), c->mesh.dXdx.size() * sizeof(double));
+  if (sg) std::memcpy(sg, c->mesh.sqrtgdiag.data(), c->mesh.sqrtgdiag.size() * sizeof(double));
+  return NMX_OK;
+}
+
+extern "C" int nmx_get_dt(nmx_ctx *c, double *dt) { NEED_MESH(c); *dt = c->dt; return NMX_OK; }
+extern "C" int nmx_set_dt(nmx_ctx *c, double dt) { NEED_MESH(c); c->dt = dt; return NMX_OK; }
+extern "C" int nmx_get_time(nmx_ctx *c, double *t) { NEED_MESH(c); *t = c->time; return NMX_OK; }
+extern "C" int nmx_set_time(nmx_ctx *c, double t) {
+  NEED_MESH(c);
+  c->time = t; c->Te = t; c->te = 0;
+  return NMX_OK;
+}
+
+static double *reg_ptr(nmx_ctx *c, int reg) {
+  switch (reg) {
+    case NMX_REG_U: return c->dU;
+    case NMX_REG_R: return c->dR;
+    case NMX_REG_P: return c->dP;
+    default: return nullptr;
+  }
+}
+
+// flat [var][node] <-> element blocks [e][var][np]
+static void flat_to_blocks(const Mesh &M, const double *u, double *b) {
+  long nn = M.nnodes;
+  for (const Elm &E : M.e) {
+    long np = (long)E.n * E.n * E.n;
+    for (int v = 0; v < 5; ++v) std::memcpy(b + 5 * E.off + v * np, u + v * nn + E.off, np * 8);
+  }
+}
+static void blocks_to_flat(const Mesh &M, const double *b, double *u) {
+  long nn = M.nnodes;
+  for (const Elm &E : M.e) {
+    long np = (long)E.n * E.n * E.n;
+    for (int v = 0; v < 5; ++v) std::memcpy(u + v * nn + E.off, b + 5 * E.off + v * np, np * 8);
+  }
+}
+
+extern "C" int nmx_state_upload(nmx_ctx *c, int reg, const double *u) {
+  NEED_DEV(c);
+  double *d = reg_ptr(c, reg);
+  if (!d) return fail(NMX_EINVAL, "bad register");
+  std::vector<double> b(c->nstate);
+  flat_to_blocks(c->mesh, u, b.data());
+  HIPCHK(hipMemcpyAsync(d, b.data(), c->nstate * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return NMX_OK;
+}
+
+extern "C" int nmx_state_download(nmx_ctx *c, int reg, double *u) {
+  NEED_DEV(c);
+  double *d = reg_ptr(c, reg);
+  if (!d) return fail(NMX_EINVAL, "bad register");
+  std::vector<double> b(c->nstate);
+  HIPCHK(hipMemcpyAsync(b.data(), d, c->nstate * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  blocks_to_flat(c->mesh, b.data(), u);
+  return NMX_OK;
+}
+
+extern "C" int nmx_state_upload_elm(nmx_ctx *c, int reg, int e, int v, const double *d, int np) {
+  NEED_DEV(c);
+  double *p = reg_ptr(c, reg);
+  if (!p || e < 0 || e >= (int)c->mesh.e.size() || v < 0 || v > 4) return fail(NMX_EINVAL, "out of range");
+  const Elm &E = c->mesh.e[e];
+  long enp = (long)E.n * E.n * E.n;
+  if (np != enp) return fail(NMX_EINVAL, "np does not match the element");
+  HIPCHK(hipMemcpyAsync(p + 5 * E.off + v * enp, d, enp * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return NMX_OK;
+}
+
+extern "C" int nmx_state_download_elm(nmx_ctx *c, int reg, int e, int v, double *d, int np) {
+  NEED_DEV(c);
+  double *p = reg_ptr(c, reg);
+  if (!p || e < 0 || e >= (int)c->mesh.e.size() || v < 0 || v > 4) return fail(NMX_EINVAL, "out of range");
+  const Elm &E = c->mesh.e[e];
+  long enp = (long)E.n * E.n * E.n;
+  if (np != enp) return fail(NMX_EINVAL, "np does not match the element");
+  HIPCHK(hipMemcpyAsync(d, p + 5 * E.off + v * enp, enp * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return NMX_OK;
+}
+
+static void kvec(nmx_ctx *c, double k[3]) {
+  k[0] = k[1] = k[2] = 0;
+  std::sscanf(P(c, "scalarwave1_k").c_str(), "%lg %lg %lg", &k[0], &k[1], &k[2]);
+}
+
+extern "C" int nmx_state_set_profile(nmx_ctx *c, int reg, double t) {
+  NEED_MESH(c);
+  double k[3];
+  kvec(c, k);
+  double om = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+  long nn = c->mesh.nnodes;
+  std::vector<double> u(5 * nn);
+  for (long g = 0; g < nn; ++g) {
+    double arg = k[0] * c->mesh.x[g] + k[1] * c->mesh.x[nn + g] + k[2] * c->mesh.x[2 * nn + g] - om * t;
+    double s = std::sin(arg), co = std::cos(arg);
+    u[4 * nn + g] = s;
+    u[0 * nn + g] = -om * co;
+    u[1 * nn + g] = k[0] * co;
+    u[2 * nn + g] = k[1] * co;
+    u[3 * nn + g] = k[2] * co;
+  }
+  return nmx_state_upload(c, reg, u.data());
+}
+
+// ---------------------------------------------------------------------------
+// stage launches
+// ---------------------------------------------------------------------------
+
+static StageArgs base_args(nmx_ctx *c) {
+  StageArgs a;
+  a.dXdx = c->dXdx;
+  a.dXbdX = c->dXbdX;
+  a.xyz = c->dxyz;
+  a.face = c->dface;
+  a.D = c->dD;
+  a.tdev = c->dT;
+  const Basis &b = lgl_basis(c->np1);
+  a.wend_inv = 1.0 / b.w[0];
+  kvec(c, a.k);
+  a.om = std::sqrt(a.k[0] * a.k[0] + a.k[1] * a.k[1] + a.k[2] * a.k[2]);
+  a.profile = (a.k[0] != 0 || a.k[1] != 0 || a.k[2] != 0);
+  a.numflux = (P(c, "scalarwave1_numflux") == "LLF") ? 1 : 0;
+  a.e0 = 0;
+  a.e1 = (int)c->mesh.e.size();
+  return a;
+}
+
+static int launch(nmx_ctx *c, StageArgs a) {
+  bool perelm = (P(c, "nmx_path") == "perelement");
+  if (perelm) {
+    int ne = a.e1;
+    for (int e = 0; e < ne; ++e) {
+      a.e0 = e; a.e1 = e + 1;
+      hipError_t err = launch_dg_stage(c->np1, c->curv, a, 1, c->stream);
+      if (err != hipSuccess) return fail(NMX_EHIP, std::string("stage launch: ") + hipGetErrorString(err));
+    }
+    return NMX_OK;
+  }
+  hipError_t err = launch_dg_stage(c->np1, c->curv, a, a.e1 - a.e0, c->stream);
+  if (err != hipSuccess) return fail(NMX_EHIP, std::string("stage launch: ") + hipGetErrorString(err));
+  return NMX_OK;
+}
+
+extern "C" int nmx_setrhs(nmx_ctx *c, int reg_out, int reg_in, double t) {
+  NEED_DEV(c);
+  double *out = reg_ptr(c, reg_out), *in = reg_ptr(c, reg_in);
+  if (!out || !in || out == in) return fail(NMX_EINVAL, "bad registers");
+  c->hT[0] = t;
+  HIPCHK(hipMemcpyAsync(c->dT, c->hT, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  StageArgs a = base_args(c);
+  a.win = in;
+  a.R = out;
+  a.tidx = 0;
+  return launch(c, a);
+}
+
+static int method_id(nmx_ctx *c) {
+  std::string m = P(c, "evolve_method");
+  return m == "RK4" ? 0 : m == "sspRK3" ? 1 : 2;
+}
+
+// issue the stage launches of one step; stage times at dT[tbase + s]
+static int issue_step(nmx_ctx *c, int method, double dt, int tbase, bool swap_parity) {
+  double *U = swap_parity ? c->dA : c->dU;
+  double *A = swap_parity ? c->dU : c->dA;
+  StageArgs a = base_args(c);
+  int r;
+  if (method == 0) {   // evolve_RK4_mesh (RungeKutta.c:19-82)
+    a.win = U; a.U = U; a.Ain = nullptr; a.Aout = A; a.cA = dt / 6.; a.Wout = c->dW0; a.wmode = 1;
+    a.cW = dt / 2.; a.tidx = tbase + 0;
+    if ((r = launch(c, a))) return r;
+    a.win = c->dW0; a.Ain = A; a.cA = dt / 3.; a.Wout = c->dW1; a.cW = dt / 2.; a.tidx = tbase + 1;
+    if ((r = launch(c, a))) return r;
+    a.win = c->dW1; a.cA = dt / 3.; a.Wout = c->dW0; a.cW = dt; a.tidx = tbase + 2;
+    if ((r = launch(c, a))) return r;
+    a.win = c->dW0; a.cA = dt / 6.; a.Wout = nullptr; a.wmode = 0; a.tidx = tbase + 3;
+    if ((r = launch(c, a))) return r;
+  } else if (method == 1) {   // evolve_sspRK3_mesh (RungeKutta.c:130-188)
+    a.win = U; a.U = U; a.Aout = A; a.cA = dt / 6.; a.Wout = c->dW0; a.wmode = 1; a.cW = dt;
+    a.tidx = tbase + 0;
+    if ((r = launch(c, a))) return r;
+    a.win = c->dW0; a.Ain = A; a.cA = dt / 6.; a.Wout = c->dW1; a.wmode = 2; a.cW = dt; a.tidx = tbase + 1;
+    if ((r = launch(c, a))) return r;
+    a.win = c->dW1; a.cA = dt * 2. / 3.; a.Wout = nullptr; a.wmode = 0; a.tidx = tbase + 2;
+    if ((r = launch(c, a))) return r;
+  } else {   // evolve_Euler_mesh (RungeKutta.c:85-115)
+    a.win = U; a.U = U; a.Aout = A; a.cA = dt; a.wmode = 0; a.tidx = tbase;
+    if ((r = launch(c, a))) return r;
+  }
+  return NMX_OK;
+}
+
+static void stage_times(int method, double t, double dt, double *ts) {
+  if (method == 0) { ts[0] = t; ts[1] = t + 0.5 * dt; ts[2] = t + 0.5 * dt; ts[3] = t + dt; }
+  else if (method == 1) { ts[0] = t; ts[1] = t + dt; ts[2] = t + 0.5 * dt; ts[3] = 0; }
+  else { ts[0] = t; ts[1] = ts[2] = ts[3] = 0; }
+}
+
+extern "C" int nmx_evolve(nmx_ctx *c, int nsteps) {
+  NEED_DEV(c);
+  if (nsteps < 0) return fail(NMX_EINVAL, "nsteps < 0");
+  HIPCHK(hipSetDevice(c->device));
+  const int method = method_id(c);
+  const double dt = c->dt;
+  const bool use_graph = (P(c, "nmx_graph") == "yes") && P(c, "nmx_path") != "perelement";
+  int G = Pi(c, "nmx_graph_steps");
+  if (G < 2) G = 2;
+  if (G > 64) G = 64;
+  G &= ~1;   // even: the U/A ping-pong returns to its start
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  int done = 0;
+  bool parity = false;   // false: current state in dU
+  if (use_graph && nsteps >= G) {
+    if (!c->gexec || c->graph_steps != G || c->graph_dt != dt || c->graph_method != method) {
+      if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+      if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int r = NMX_OK;
+      for (int s = 0; s < G && !r; ++s) r = issue_step(c, method, dt, 4 * s, (s & 1) != 0);
+      hipGraph_t g = nullptr;
+      hipError_t ec = hipStreamEndCapture(c->stream, &g);
+      if (r) return r;
+      if (ec != hipSuccess) return fail(NMX_EHIP, std::string("capture: ") + hipGetErrorString(ec));
+      c->graph = g;
+      HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+      c->graph_steps = G; c->graph_dt = dt; c->graph_method = method;
+    }
+    while (nsteps - done >= G) {
+      // stage times of the G steps (time = Te + te, main.c:588-624)
+      HIPCHK(hipStreamSynchronize(c->stream));   // hT is reused
+      long double te = c->te;
+      for (int s = 0; s < G; ++s) {
+        double t = (double)(c->Te + te);
+        stage_times(method, t, dt, c->hT + 4 * s);
+        te += dt;
+      }
+      HIPCHK(hipMemcpyAsync(c->dT, c->hT, 4 * G * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+      c->te = te;
+      c->time = (double)(c->Te + c->te);
+      done += G;
+    }
+  }
+  for (; done < nsteps; ++done) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    stage_times(method, c->time, dt, c->hT);
+    HIPCHK(hipMemcpyAsync(c->dT, c->hT, 4 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int r = issue_step(c, method, dt, 0, parity);
+    if (r) return r;
+    parity = !parity;
+    c->te += dt;
+    c->time = (double)(c->Te + c->te);
+  }
+  // keep register pointers fixed (captured graphs refer to them)
+  if (parity) HIPCHK(hipMemcpyAsync(c->dU, c->dA, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  return NMX_OK;
+}
+
+extern "C" int nmx_last_evolve_ms(nmx_ctx *c, double *ms) {
+  if (!c) return fail(NMX_EINVAL, "ctx is NULL");
+  *ms = c->last_ms;
+  return NMX_OK;
+}
+
+extern "C" int nmx_state_device_ptr(nmx_ctx *c, int reg, void **p, long *len) {
+  NEED_DEV(c);
+  double *d = reg_ptr(c, reg);
+  if (!d) return fail(NMX_EINVAL, "bad register");
+  *p = d;
+  *len = c->nstate;
+  return NMX_OK;
+}
+
+extern "C" int nmx_adjoint_rhs(nmx_ctx *c, const double *, double *, double) {
+  (void)c;
+  return fail(NMX_EUNSUP, "adjoint not built yet");
+}
+extern "C" int nmx_adjoint_step(nmx_ctx *c, double *, double) {
+  (void)c;
+  return fail(NMX_EUNSUP, "adjoint not built yet");
+}

@@ -1,0 +1,94 @@
+// mesh.h -- host-side mesh, geometry and face tables of libnmx.
+//
+// The mesh is nmesh's forest of patches whose leaves (elements) are kept in
+// depth-first octant order, exactly as the reference's element list
+// (src/main/amr/storage.c:586-621): children replace their parent in place,
+// child index c has bit d set for the upper half in direction d
+// (src/main/amr/connections.c:1175-1206).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace nmx {
+
+constexpr int kMaxN = 16;     // points per direction supported by the host code
+constexpr int kMaxNb = 8;     // face neighbours per face
+
+enum PatchType { PAT_BOX = 0, PAT_CUBEDSHELL = 1 };
+
+struct Patch {
+  int type = PAT_BOX;
+  int dom = 0;                // cubed-sphere domain 0..5 (dir = dom/2)
+  double s0 = 0, s1 = 0;      // inner and outer radius (sigma) of a CubedShell
+  double xc[3] = {0, 0, 0};
+  double bbox[6] = {0, 0, 0, 0, 0, 0};
+  int bound[6] = {1, 1, 1, 1, 1, 1};
+};
+
+// One element.  path[l] is the child index taken at level l+1.
+struct Elm {
+  int p = 0, l = 0;
+  uint8_t path[24] = {0};
+  uint32_t ic[3] = {0, 0, 0};   // integer position at level l
+  double bbox[6] = {0, 0, 0, 0, 0, 0};
+  int n = 2;
+  long off = 0;                 // first node
+  int nfnb[6] = {0, 0, 0, 0, 0, 0};
+  int fnb[6][kMaxNb];
+  int nbf[6][kMaxNb];
+  int ajmode[6] = {0, 0, 0, 0, 0, 0};
+  int orient[6] = {0, 0, 0, 0, 0, 0};
+  double hmin = 0;
+};
+
+struct Basis {
+  int n = 0;
+  double x[kMaxN], w[kMaxN], wb[kMaxN];   // LGL nodes, quadrature, barycentric
+  double D[kMaxN * kMaxN];                // D[i*n + l] = l_l'(x_i)
+};
+const Basis &lgl_basis(int n);
+
+// conforming-face gather: neighbour node = nbase + a*s1 + b*s2 for my face
+// point (a,b); a runs along Dir1_norm(dir), b along Dir2_norm(dir)
+struct FaceCopy {
+  long nbase = -1;
+  int s1 = 0, s2 = 0;
+};
+
+// one neighbour's contribution to an interpolated face
+struct InterpSlot {
+  int nb = -1, nbface = 0, m = 0;   // neighbour element, its face, its n
+  long nbase = 0;                   // neighbour plane: node = nbase + q1*t1 + q2*t2
+  int t1 = 0, t2 = 0;
+  int swap = 0;                     // 1: my a-axis maps to the neighbour's q2 axis
+  int a0 = 0, a1 = 0, b0 = 0, b1 = 0;   // my covered point ranges [a0,a1]x[b0,b1]
+  std::vector<double> Ba, Bb;       // Ba[a*m+q]: l_q at my a-th point (covered rows)
+};
+
+struct Mesh {
+  std::vector<Patch> pat;
+  std::vector<Elm> e;
+  long nnodes = 0;
+  std::vector<double> x;            // x[d*nnodes + node]
+  std::vector<double> dXdx;         // dXdx[(3a+b)*nnodes + node]
+  std::vector<double> sqrtgdiag;    // sqrtgdiag[a*nnodes + node]
+  std::vector<FaceCopy> fcopy;      // 6 per element
+  std::vector<std::vector<InterpSlot>> fint;  // 6 per element
+  int maxlevel = 0;
+};
+
+// builders (return "" on success, else an error message)
+std::string build_box(Mesh &M, int luni, int n, double dout, const double xc[3]);
+std::string build_shell(Mesh &M, int luni, int n, double rin, double rout,
+                        double sph_radius, int sph_levels);
+std::string set_elm_n(Mesh &M, const std::vector<int> &n);
+
+double mesh_dt(const Mesh &M, double dtfac);
+
+// maps
+void patch_xyz(const Patch &P, const double X[3], double x[3]);
+void patch_XYZ(const Patch &P, const double x[3], double X[3]);
+void patch_dXdx(const Patch &P, const double X[3], double x[3], double M[3][3]);
+
+}  // namespace nmx

@@ -1,0 +1,209 @@
+"""Host-side mirror of nmesh's evolution interface over libnmx.
+
+Names follow the reference (paths relative to the nmesh source tree):
+
+- ``Mesh`` stands in for ``tMesh`` plus its parameter database
+  (src/main/amr/nmesh_amr.h:227-251, src/main/main/parameters.c); parameters
+  are set under nmesh's own names (``amr_mesh_type``, ``amr_luni``, ``dt``,
+  ``evolve_method``, ``scalarwave1_k`` ...).
+- ``amr_setup_mesh`` (src/main/amr/mesh.c:181-319)
+- ``evolve_setrhs_mesh`` (src/main/evolve/evolve.c:182-232)
+- ``evolve_RK4_mesh`` / ``evolve_sspRK3_mesh`` / ``evolve_Euler_mesh``
+  (src/main/evolve/RungeKutta.c:19-188) and ``evolve_myln`` (evolve.c:25-176)
+- ``scalarwave1_set_profile`` (src/tests/scalarwave1/scalarwave1.c:574-642)
+
+Errors raise ``NmxError`` where the reference would call ``errorexit``
+(src/main/main/nmesh_main.h:244-252).
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, lib
+
+NVARS = 5
+VARS = ("scalarwave1_pi", "scalarwave1_cx", "scalarwave1_cy", "scalarwave1_cz",
+        "scalarwave1_phi")
+REG_U, REG_R, REG_P = 0, 1, 2
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Mesh:
+    """A mesh with its device mirror; one context per Mesh."""
+
+    def __init__(self, pars=None, device=0):
+        L = lib()
+        self._ctx = C.c_void_p()
+        check(L.nmx_ctx_create(C.byref(self._ctx), int(device)))
+        for k, v in (pars or {}).items():
+            self.set_par(k, v)
+
+    def close(self):
+        if self._ctx:
+            lib().nmx_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # parameters (AddPar / Par / Gets)
+    def set_par(self, name, value):
+        check(lib().nmx_set_param(self._ctx, name.encode(), str(value).encode()))
+
+    def par(self, name):
+        buf = C.create_string_buffer(256)
+        check(lib().nmx_get_param(self._ctx, name.encode(), buf, 256))
+        return buf.value.decode()
+
+    @property
+    def ctx(self):
+        return self._ctx
+
+    @property
+    def nelem(self):
+        n = C.c_int()
+        check(lib().nmx_mesh_nelem(self._ctx, C.byref(n)))
+        return n.value
+
+    @property
+    def nnodes(self):
+        n = C.c_long()
+        check(lib().nmx_mesh_nnodes(self._ctx, C.byref(n)))
+        return n.value
+
+    @property
+    def dt(self):
+        d = C.c_double()
+        check(lib().nmx_get_dt(self._ctx, C.byref(d)))
+        return d.value
+
+    @dt.setter
+    def dt(self, v):
+        check(lib().nmx_set_dt(self._ctx, float(v)))
+
+    @property
+    def time(self):
+        d = C.c_double()
+        check(lib().nmx_get_time(self._ctx, C.byref(d)))
+        return d.value
+
+    @time.setter
+    def time(self, v):
+        check(lib().nmx_set_time(self._ctx, float(v)))
+
+    def elm_info(self, e):
+        a = np.zeros(22, dtype=np.int64)
+        check(lib().nmx_mesh_elm_info(self._ctx, int(e), _p(a)))
+        return a
+
+    def elm_bbox(self, e):
+        a = np.zeros(6)
+        check(lib().nmx_mesh_elm_bbox(self._ctx, int(e), _p(a)))
+        return a
+
+    def elm_fnb(self, e, f):
+        nb = np.zeros(8, dtype=np.int32)
+        nbf = np.zeros(8, dtype=np.int32)
+        n = C.c_int()
+        check(lib().nmx_mesh_elm_fnb(self._ctx, int(e), int(f), _p(nb), _p(nbf), C.byref(n)))
+        return nb[:n.value].copy(), nbf[:n.value].copy()
+
+    def coords(self):
+        x = np.zeros(3 * self.nnodes)
+        check(lib().nmx_mesh_coords(self._ctx, _p(x)))
+        return x.reshape(3, -1)
+
+    def geometry(self):
+        nn = self.nnodes
+        dXdx = np.zeros(9 * nn)
+        sg = np.zeros(3 * nn)
+        check(lib().nmx_mesh_geometry(self._ctx, _p(dXdx), _p(sg)))
+        return dXdx.reshape(9, -1), sg.reshape(3, -1)
+
+    def set_elm_n(self, n):
+        n = np.ascontiguousarray(n, dtype=np.int32)
+        check(lib().nmx_mesh_set_elm_n(self._ctx, _p(n), int(n.size)))
+
+    # state (Vard over all elements, flat [var][node])
+    def upload(self, u, reg=REG_U):
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        if u.size != NVARS * self.nnodes:
+            raise ValueError("state must hold 5*nnodes values")
+        check(lib().nmx_state_upload(self._ctx, int(reg), _p(u)))
+
+    def download(self, reg=REG_U):
+        u = np.zeros(NVARS * self.nnodes)
+        check(lib().nmx_state_download(self._ctx, int(reg), _p(u)))
+        return u
+
+    def Vard(self, e, vi, reg=REG_U):
+        info = self.elm_info(e)
+        np_ = int(info[2]) ** 3
+        d = np.zeros(np_)
+        check(lib().nmx_state_download_elm(self._ctx, int(reg), int(e), int(vi), _p(d), np_))
+        return d
+
+    def synchronize(self):
+        check(lib().nmx_synchronize(self._ctx))
+
+    def last_evolve_ms(self):
+        d = C.c_double()
+        check(lib().nmx_last_evolve_ms(self._ctx, C.byref(d)))
+        return d.value
+
+
+def amr_setup_mesh(mesh):
+    """Build patches, elements, tables, geometry and dt (mesh.c:181-319)."""
+    check(lib().nmx_mesh_build(mesh.ctx))
+    return 0
+
+
+def scalarwave1_set_profile(mesh, t=None, reg=REG_U):
+    """u = plane wave of scalarwave1_k at time t (scalarwave1.c:574-642)."""
+    check(lib().nmx_state_set_profile(mesh.ctx, int(reg), float(mesh.time if t is None else t)))
+    return 0
+
+
+def evolve_setrhs_mesh(mesh, t=None, reg_out=REG_R, reg_in=REG_U):
+    """r = RHS(u, t) on the device; returns r as a flat array (evolve.c:182-232)."""
+    check(lib().nmx_setrhs(mesh.ctx, int(reg_out), int(reg_in),
+                           float(mesh.time if t is None else t)))
+    return mesh.download(reg_out)
+
+
+def _step(mesh, method, nsteps):
+    old = mesh.par("evolve_method")
+    if old != method:
+        mesh.set_par("evolve_method", method)
+    try:
+        check(lib().nmx_evolve(mesh.ctx, int(nsteps)))
+    finally:
+        if old != method:
+            mesh.set_par("evolve_method", old)
+
+
+def evolve_RK4_mesh(mesh, nsteps=1):
+    """RK4 step(s) (RungeKutta.c:19-82)."""
+    _step(mesh, "RK4", nsteps)
+
+
+def evolve_sspRK3_mesh(mesh, nsteps=1):
+    """SSP-RK3 step(s) (RungeKutta.c:130-188)."""
+    _step(mesh, "sspRK3", nsteps)
+
+
+def evolve_Euler_mesh(mesh, nsteps=1):
+    """Euler step(s) (RungeKutta.c:85-115)."""
+    _step(mesh, "Euler", nsteps)
+
+
+def evolve_myln(mesh, nsteps=1):
+    """Step(s) with the method named by evolve_method (evolve.c:25-176)."""
+    check(lib().nmx_evolve(mesh.ctx, int(nsteps)))
+    return 0

@@ -1,0 +1,1357 @@
+/* nmx_oracle.c -- CPU restatement of nmesh's DG evolution hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle the HIP product path is
+ * checked against.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it, and only as the checker or the
+ * timed CPU baseline; the product library never links or calls it.
+ *
+ * It restates, in plain C99 + OpenMP, the algorithm of the reference files
+ * cited at each function (paths relative to the nmesh source tree).  The
+ * reference itself cannot be compiled in this image (every source includes
+ * nmesh.h, which includes the build-generated nmesh_automatic_include.h), so
+ * this restatement is pinned by closed forms, invariants and convergence
+ * tests under tests/ instead of by a reference build.
+ *
+ * Layout used at this file's C-ABI: a state is double u[var*nnodes + node]
+ * with node = elm_offset + ijk and ijk = i + n*(j + n*k) (nmesh_amr_loops.h:279).
+ * Variable order is scalarwave1's vlu: (pi, cx, cy, cz, phi)
+ * (src/tests/scalarwave1/scalarwave1.c:670-672).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_MAXN 24
+#define ORC_MAXNB 8
+#define DEQUALEPS 1e-10 /* src/main/main/nmesh_main.h:32 */
+
+/* ------------------------------------------------------------------ */
+/* basis: LGL points, weights, barycentric weights, transposed D       */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  int set;
+  double Xb[ORC_MAXN], Wq[ORC_MAXN], WL[ORC_MAXN], DT[ORC_MAXN * ORC_MAXN];
+  double Xu[ORC_MAXN], Wu[ORC_MAXN];        /* uniform points (FV) */
+} OBasis;
+
+static OBasis g_basis[ORC_MAXN + 1];
+
+/* P_l, P_l', Q = P_{l+1}-P_{l-1}, Q'  (src/main/basis/Legendre.c:61-98) */
+static void leg_P_dP_Q_dQ(int l, double x, double *P, double *dP, double *Q,
+                          double *dQ)
+{
+  double P0 = 1., P1 = x, dP0 = 0., dP1 = 1.;
+  double Pk = 0., dPk = 0.;
+  int k;
+  if (l == 0) { *P = 1.; *dP = 0.; return; }
+  if (l == 1) { *P = x; *dP = 1.; return; }
+  for (k = 2; k <= l + 1; k++) {
+    Pk = (P1 * x * (2 * k - 1) - P0 * (k - 1)) / k;
+    dPk = P1 * (2 * k - 1) + dP0;
+    if (k >= l + 1) break;
+    P0 = P1; P1 = Pk; dP0 = dP1; dP1 = dPk;
+  }
+  *P = P1; *dP = dP1; *Q = Pk - P0; *dQ = dPk - dP0;
+}
+
+/* LGL nodes and weights on [-1,1] (src/main/basis/Legendre.c:149-213) */
+void orc_lgl(int npoints, double *x, double *w)
+{
+  int N = npoints - 1, i, j;
+  double P, dP, Q, dQ, y, dy;
+  if (N == 0) { x[0] = 0.; if (w) w[0] = 2.; return; }
+  x[0] = -1.; x[N] = 1.;
+  if (w) { w[0] = 2.0 / (N * (N + 1.0)); w[N] = w[0]; }
+  if (N == 1) return;
+  for (j = 1; j <= N / 2; j++) {
+    if (j == N / 2.0) {
+      y = 0.;
+    } else {
+      y = -cos(M_PI * (j + 0.25) / N - 3 / (8 * N * M_PI * (j + 0.25)));
+      for (i = 0; i < 100; i++) {
+        leg_P_dP_Q_dQ(N, y, &P, &dP, &Q, &dQ);
+        dy = -Q / dQ;
+        y += dy;
+        if (fabs(dy) < fabs(1e-15 * y)) break;
+      }
+    }
+    x[N - j] = -y;
+    x[j] = y;
+    leg_P_dP_Q_dQ(N, y, &P, &dP, &Q, &dQ);
+    if (w) { w[j] = 2.0 / ((N * (N + 1)) * P * P); w[N - j] = w[j]; }
+  }
+}
+
+/* barycentric weights (src/main/basis/Lagrange.c:15-28) */
+static void lagrange_winterp(int n, const double *x, double *wi)
+{
+  for (int i = 0; i < n; i++) {
+    double denom = 1.;
+    for (int m = 0; m < n; m++) if (m != i) denom *= (x[i] - x[m]);
+    wi[i] = 1. / denom;
+  }
+}
+
+/* DT[j + i*n] = D_ij,  D_ii = -sum_{j!=i} D_ij (src/main/basis/Lagrange.c:126-147) */
+static void lagrange_DT(int n, const double *x, const double *wi, double *DT)
+{
+  for (int i = 0; i < n; i++) {
+    double Dii = 0;
+    for (int j = 0; j < n; j++)
+      if (i != j) {
+        double Dij = (wi[j] / wi[i]) / (x[i] - x[j]);
+        Dii -= Dij;
+        DT[j + i * n] = Dij;
+      }
+    DT[i * n + i] = Dii;
+  }
+}
+
+/* Lagrange basis l_k(x) (src/main/basis/Lagrange.c:151-158) */
+static double lagrange_of_x(int k, double x, int np, const double *xp,
+                            const double *wi)
+{
+  double prod = 1.;
+  for (int m = 0; m < np; m++) if (m != k) prod *= (x - xp[m]);
+  return prod * wi[k];
+}
+
+static OBasis *basis(int n)
+{
+  OBasis *b = &g_basis[n];
+  if (!b->set) {
+#pragma omp critical(orc_basis)
+    if (!b->set) {
+      orc_lgl(n, b->Xb, b->Wq);
+      lagrange_winterp(n, b->Xb, b->WL);
+      lagrange_DT(n, b->Xb, b->WL, b->DT);
+      /* uniform points incl. both faces, trapezoid weights
+         (src/main/basis/gridpoints.c:49-53) */
+      for (int i = 0; i < n; i++)
+        b->Xu[i] = (n > 1) ? -1. + 2. * i / (n - 1) : 0.;
+      for (int i = 0; i < n; i++)
+        b->Wu[i] = (n > 1) ? ((i == 0 || i == n - 1) ? 1. / (n - 1)
+                                                      : 2. / (n - 1))
+                           : 2.;
+      b->set = 1;
+    }
+  }
+  return b;
+}
+
+void orc_dt_matrix(int n, double *DT)
+{
+  OBasis *b = basis(n);
+  memcpy(DT, b->DT, sizeof(double) * n * n);
+}
+
+/* ------------------------------------------------------------------ */
+/* patches and coordinate maps                                         */
+/* ------------------------------------------------------------------ */
+
+enum { PAT_BOX = 0, PAT_CUBEDSHELL = 1 };
+
+typedef struct {
+  int type;
+  int dom;          /* cubed-sphere domain 0..5 */
+  double s0, s1;    /* CI->s[0], CI->s[1] */
+  double xc[3];
+  double bbox[6];
+  int bound[6];     /* 1: patch face is an outer boundary */
+} OPat;
+
+/* x(X) for a CubedShell with constant sigma
+   (src/main/coordinates/coordtrans_CubedSphere.c:38-121) */
+static void cubsph_xyz(const OPat *P, const double X[3], double x[3])
+{
+  double lam = X[0], A = X[1], B = X[2];
+  int dir = P->dom / 2;
+  double pm = 2 * (P->dom % 2) - 1;
+  double s = sqrt(1.0 + A * A + B * B);
+  double a0 = pm * P->s0 / s, a1 = pm * P->s1 / s;
+  double r = (a1 - a0) * lam + a0;
+  if (dir == 0) { x[0] = P->xc[0] + r; x[1] = P->xc[1] + A * r; x[2] = P->xc[2] + B * r; }
+  else if (dir == 1) { x[1] = P->xc[1] + r; x[0] = P->xc[0] + A * r; x[2] = P->xc[2] + B * r; }
+  else { x[2] = P->xc[2] + r; x[1] = P->xc[1] + A * r; x[0] = P->xc[0] + B * r; }
+}
+
+/* inverse map X(x) for a CubedShell with constant sigma
+   (src/main/coordinates/coordtrans_CubedSphere.c:125-256) */
+static void cubsph_lamAB(const OPat *P, const double x[3], double X[3])
+{
+  int dir = P->dom / 2;
+  double pm = 2 * (P->dom % 2) - 1;
+  double rx = x[0] - P->xc[0], ry = x[1] - P->xc[1], rz = x[2] - P->xc[2];
+  double r0, A, B;
+  if (dir == 0) { r0 = rx; A = ry / rx; B = rz / rx; }
+  else if (dir == 1) { r0 = ry; A = rx / ry; B = rz / ry; }
+  else { r0 = rz; A = ry / rz; B = rx / rz; }
+  double s = sqrt(1.0 + A * A + B * B);
+  double a0 = pm * P->s0 / s, a1 = pm * P->s1 / s;
+  X[0] = (r0 - a0) / (a1 - a0);
+  X[1] = A;
+  X[2] = B;
+}
+
+/* dX/dx for a CubedShell with constant sigma (dsigma = 0)
+   (src/main/coordinates/coordtrans_CubedSphere.c:259-509) */
+static void cubsph_dXdx(const OPat *P, const double X[3], double x[3],
+                        double M[3][3])
+{
+  double lam = X[0], A = X[1], B = X[2];
+  int dir = P->dom / 2;
+  double pm = 2 * (P->dom % 2) - 1;
+  double s = sqrt(1.0 + A * A + B * B);
+  double a0 = pm * P->s0 / s, a1 = pm * P->s1 / s;
+  double rx, ry, rz, rx2, ry2, rz2, rc2, mx, my, mz;
+  double da0x = a0, da0y = a0, da0z = a0, da1x = a1, da1y = a1, da1z = a1;
+  if (dir == 0) {
+    rx = (a1 - a0) * lam + a0;
+    x[0] = P->xc[0] + rx; x[1] = P->xc[1] + A * rx; x[2] = P->xc[2] + B * rx;
+    ry = x[1] - P->xc[1]; rz = x[2] - P->xc[2];
+    rx2 = rx * rx; ry2 = ry * ry; rz2 = rz * rz; rc2 = rx2 + ry2 + rz2;
+    mx = (1.0 / rx - rx / rc2); my = (-ry / rc2); mz = (-rz / rc2);
+    da0x *= (mx + 0.0); da0y *= (my + 0.0); da0z *= (mz + 0.0);
+    da1x *= (mx + 0.0); da1y *= (my + 0.0); da1z *= (mz + 0.0);
+    M[0][0] = (1.0 - da0x - (da1x - da0x) * lam) / (a1 - a0);
+    M[0][1] = (-da0y - (da1y - da0y) * lam) / (a1 - a0);
+    M[0][2] = (-da0z - (da1z - da0z) * lam) / (a1 - a0);
+    M[1][0] = -ry / rx2; M[1][1] = 1.0 / rx; M[1][2] = 0.0;
+    M[2][0] = -rz / rx2; M[2][1] = 0.0; M[2][2] = M[1][1];
+  } else if (dir == 1) {
+    ry = (a1 - a0) * lam + a0;
+    x[1] = P->xc[1] + ry; x[0] = P->xc[0] + A * ry; x[2] = P->xc[2] + B * ry;
+    rx = x[0] - P->xc[0]; rz = x[2] - P->xc[2];
+    rx2 = rx * rx; ry2 = ry * ry; rz2 = rz * rz; rc2 = rx2 + ry2 + rz2;
+    mx = (-rx / rc2); my = (1.0 / ry - ry / rc2); mz = (-rz / rc2);
+    da0x *= (mx + 0.0); da0y *= (my + 0.0); da0z *= (mz + 0.0);
+    da1x *= (mx + 0.0); da1y *= (my + 0.0); da1z *= (mz + 0.0);
+    M[0][0] = (-da0x - (da1x - da0x) * lam) / (a1 - a0);
+    M[0][1] = (1.0 - da0y - (da1y - da0y) * lam) / (a1 - a0);
+    M[0][2] = (-da0z - (da1z - da0z) * lam) / (a1 - a0);
+    M[1][0] = 1.0 / ry; M[1][1] = -rx / ry2; M[1][2] = 0.0;
+    M[2][0] = 0.0; M[2][1] = -rz / ry2; M[2][2] = M[1][0];
+  } else {
+    rz = (a1 - a0) * lam + a0;
+    x[2] = P->xc[2] + rz; x[1] = P->xc[1] + A * rz; x[0] = P->xc[0] + B * rz;
+    rx = x[0] - P->xc[0]; ry = x[1] - P->xc[1];
+    rx2 = rx * rx; ry2 = ry * ry; rz2 = rz * rz; rc2 = rx2 + ry2 + rz2;
+    mx = (-rx / rc2); my = (-ry / rc2); mz = (1.0 / rz - rz / rc2);
+    da0x *= (mx + 0.0); da0y *= (my + 0.0); da0z *= (mz + 0.0);
+    da1x *= (mx + 0.0); da1y *= (my + 0.0); da1z *= (mz + 0.0);
+    M[0][0] = (-da0x - (da1x - da0x) * lam) / (a1 - a0);
+    M[0][1] = (-da0y - (da1y - da0y) * lam) / (a1 - a0);
+    M[0][2] = (1.0 - da0z - (da1z - da0z) * lam) / (a1 - a0);
+    M[1][0] = 0.0; M[1][1] = 1.0 / rz; M[1][2] = -ry / rz2;
+    M[2][0] = M[1][1]; M[2][1] = 0.0; M[2][2] = -rx / rz2;
+  }
+}
+
+static void pat_xyz(const OPat *P, const double X[3], double x[3])
+{
+  if (P->type == PAT_CUBEDSHELL) cubsph_xyz(P, X, x);
+  else { x[0] = X[0]; x[1] = X[1]; x[2] = X[2]; }
+}
+
+static void pat_XYZ(const OPat *P, const double x[3], double X[3])
+{
+  if (P->type == PAT_CUBEDSHELL) cubsph_lamAB(P, x, X);
+  else { X[0] = x[0]; X[1] = x[1]; X[2] = x[2]; }
+}
+
+/* ------------------------------------------------------------------ */
+/* mesh: oct-tree leaves in Morton order                               */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  int p, l;
+  char loc[32];     /* child octant digits per level (connections.c:1219-1233) */
+  double bbox[6];
+  int n;            /* points per direction */
+  int use_fv;       /* 1: uniform points and fv_divf */
+  long off;         /* first node of this element */
+  int nfnb[6];
+  int fnb[6][ORC_MAXNB];
+  int nbf[6][ORC_MAXNB]; /* face of the neighbour that faces me */
+  int ajmode[6];    /* 0 none, 1 alias, 2 oriented copy, 3 interpolation */
+  int orient[6];    /* intrch*4 + rev1*2 + rev2 for ajmode 2 */
+  double hmin;
+} OElm;
+
+typedef struct {
+  int npat;
+  OPat pat[8];
+  int nelem;
+  OElm *e;
+  long nnodes;
+  double *X, *x, *dXdx, *sqrtgdiag, *det;
+  /* per-element face-plane offsets for the exchange buffers */
+  long *foff;       /* foff[6*e+f]: start of face plane (n*n values) */
+  long nface;       /* total face-plane values */
+} OMesh;
+
+static int elm_np(const OElm *E) { return E->n * E->n * E->n; }
+
+/* element bbox from its location (src/main/amr/connections.c:1175-1206) */
+static void elm_bbox_from_loc(const OPat *P, OElm *E)
+{
+  double LX[3];
+  for (int f = 0; f < 6; f++) E->bbox[f] = P->bbox[f];
+  for (int d = 0; d < 3; d++) LX[d] = E->bbox[2 * d + 1] - E->bbox[2 * d];
+  for (int ll = 1; ll <= E->l; ll++) {
+    int ijk = E->loc[ll - 1] - '0';
+    for (int d = 0; d < 3; d++) {
+      int bt = 1 << d;
+      LX[d] /= 2.;
+      if (ijk & bt) E->bbox[2 * d] = E->bbox[2 * d + 1] - LX[d];
+      else E->bbox[2 * d + 1] = E->bbox[2 * d] + LX[d];
+    }
+  }
+}
+
+/* replace every flagged element by its 8 children in place, so the element
+   list stays in depth-first (Morton) order
+   (src/main/amr/storage.c:586-621, src/main/amr/refine.c:174-221) */
+static void refine_flagged(OMesh *M, const int *flag)
+{
+  int cnt = 0;
+  for (int e = 0; e < M->nelem; e++) cnt += flag[e] ? 8 : 1;
+  OElm *ne = calloc(cnt, sizeof(OElm));
+  int k = 0;
+  for (int e = 0; e < M->nelem; e++) {
+    if (!flag[e]) { ne[k++] = M->e[e]; continue; }
+    for (int c = 0; c < 8; c++) {
+      OElm C;
+      memset(&C, 0, sizeof(C));
+      C.p = M->e[e].p;
+      C.l = M->e[e].l + 1;
+      memcpy(C.loc, M->e[e].loc, sizeof(C.loc));
+      C.loc[C.l - 1] = '0' + c;
+      C.loc[C.l] = 0;
+      C.n = M->e[e].n;
+      C.use_fv = M->e[e].use_fv;
+      elm_bbox_from_loc(&M->pat[C.p], &C);
+      ne[k++] = C;
+    }
+  }
+  free(M->e);
+  M->e = ne;
+  M->nelem = cnt;
+}
+
+static void mesh_add_roots(OMesh *M, int n)
+{
+  M->nelem = M->npat;
+  M->e = calloc(M->nelem, sizeof(OElm));
+  for (int p = 0; p < M->npat; p++) {
+    OElm *E = &M->e[p];
+    E->p = p; E->l = 0; E->loc[0] = 0; E->n = n;
+    elm_bbox_from_loc(&M->pat[p], E);
+  }
+}
+
+static void refine_uniform(OMesh *M, int luni)
+{
+  for (int l = 0; l < luni; l++) {
+    int *flag = calloc(M->nelem, sizeof(int));
+    for (int e = 0; e < M->nelem; e++) flag[e] = (M->e[e].l < luni);
+    refine_flagged(M, flag);
+    free(flag);
+  }
+}
+
+/* ---- geometry helpers ---- */
+
+static void elm_X_of_Xb(const OElm *E, const double Xb[3], double X[3])
+{ /* src/main/coordinates/get_coords.c:384-394 */
+  for (int d = 0; d < 3; d++) {
+    int f = 2 * d;
+    X[d] = 0.5 * ((E->bbox[f + 1] - E->bbox[f]) * Xb[d] + (E->bbox[f + 1] + E->bbox[f]));
+  }
+}
+
+static void elm_dXbdX(const OElm *E, double dXbdX[3])
+{ /* src/main/coordinates/get_coords.c:409-419 */
+  for (int d = 0; d < 3; d++) dXbdX[d] = 2. / (E->bbox[2 * d + 1] - E->bbox[2 * d]);
+}
+
+static const double *elm_Xb1(const OElm *E)
+{
+  OBasis *b = basis(E->n);
+  return E->use_fv ? b->Xu : b->Xb;
+}
+
+static void elm_point_X(const OElm *E, int i, int j, int k, double X[3])
+{
+  const double *xb = elm_Xb1(E);
+  double Xb[3] = {xb[i], xb[j], xb[k]};
+  elm_X_of_Xb(E, Xb, X);
+}
+
+/* does patch face f of patch p touch another patch? (bfaces) */
+static int elm_on_patface(const OMesh *M, const OElm *E, int f)
+{
+  return E->bbox[f] == M->pat[E->p].bbox[f];
+}
+
+/* ---- neighbour search ---- */
+
+static int rect_overlap(const double a[4], const double b[4], double tol)
+{
+  double lo0 = fmax(a[0], b[0]), hi0 = fmin(a[1], b[1]);
+  double lo1 = fmax(a[2], b[2]), hi1 = fmin(a[3], b[3]);
+  return (hi0 - lo0 > tol) && (hi1 - lo1 > tol);
+}
+
+/* face f of element E as an in-plane rectangle in patch coordinates */
+static void face_rect(const OElm *E, int f, double r[4])
+{
+  int dir = f / 2, d1 = (dir == 0), d2 = (dir == 2) ? 1 : 2;
+  r[0] = E->bbox[2 * d1]; r[1] = E->bbox[2 * d1 + 1];
+  r[2] = E->bbox[2 * d2]; r[3] = E->bbox[2 * d2 + 1];
+}
+
+/* physical point at in-plane fractions (s,t) of face f */
+static void face_point(const OMesh *M, const OElm *E, int f, double s,
+                       double t, double x[3])
+{
+  int dir = f / 2, d1 = (dir == 0), d2 = (dir == 2) ? 1 : 2;
+  double X[3];
+  X[dir] = E->bbox[f];
+  X[d1] = E->bbox[2 * d1] + s * (E->bbox[2 * d1 + 1] - E->bbox[2 * d1]);
+  X[d2] = E->bbox[2 * d2] + t * (E->bbox[2 * d2 + 1] - E->bbox[2 * d2]);
+  pat_xyz(&M->pat[E->p], X, x);
+}
+
+/* is physical point x on face f2 of element E2 (inside its rectangle)? */
+static int point_on_elmface(const OMesh *M, const OElm *E2, int f2,
+                            const double x[3], double tol)
+{
+  double X[3];
+  int dir = f2 / 2, d1 = (dir == 0), d2 = (dir == 2) ? 1 : 2;
+  pat_XYZ(&M->pat[E2->p], x, X);
+  double L = fmax(fabs(E2->bbox[f2]), 1.0);
+  if (fabs(X[dir] - E2->bbox[f2]) > tol * L) return 0;
+  if (X[d1] < E2->bbox[2 * d1] - tol || X[d1] > E2->bbox[2 * d1 + 1] + tol) return 0;
+  if (X[d2] < E2->bbox[2 * d2] - tol || X[d2] > E2->bbox[2 * d2 + 1] + tol) return 0;
+  return 1;
+}
+
+/* face neighbours: same patch -> touching opposite faces with overlapping
+   rectangles; other patch -> faces that share a 2D piece of surface.
+   Neighbour lists are kept in element-id order. */
+static void find_neighbours(OMesh *M)
+{
+  int ne = M->nelem;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int e = 0; e < ne; e++) {
+    OElm *E = &M->e[e];
+    for (int f = 0; f < 6; f++) {
+      E->nfnb[f] = 0;
+      double r[4];
+      face_rect(E, f, r);
+      int onpf = elm_on_patface(M, E, f);
+      for (int e2 = 0; e2 < ne; e2++) {
+        if (e2 == e) continue;
+        OElm *E2 = &M->e[e2];
+        int hit = -1;
+        if (E2->p == E->p && !onpf) {
+          int f2 = f ^ 1;
+          if (E2->bbox[f2] == E->bbox[f]) {
+            double r2[4];
+            face_rect(E2, f2, r2);
+            if (rect_overlap(r, r2, 1e-13)) hit = f2;
+          }
+        } else if (E2->p != E->p && onpf) {
+          for (int f2 = 0; f2 < 6 && hit < 0; f2++) {
+            if (!elm_on_patface(M, E2, f2)) continue;
+            double xc[3], xc2[3];
+            face_point(M, E, f, 0.5, 0.5, xc);
+            face_point(M, E2, f2, 0.5, 0.5, xc2);
+            if (point_on_elmface(M, E2, f2, xc, 1e-10) ||
+                point_on_elmface(M, E, f, xc2, 1e-10)) {
+              /* make sure it is not only an edge contact: an interior
+                 point of the smaller face must be on the other face */
+              double q[3];
+              int ok = 0;
+              face_point(M, E, f, 0.5, 0.5, q);
+              if (point_on_elmface(M, E2, f2, q, 1e-10)) ok = 1;
+              face_point(M, E2, f2, 0.5, 0.5, q);
+              if (point_on_elmface(M, E, f, q, 1e-10)) ok = 1;
+              if (ok) hit = f2;
+            }
+          }
+        }
+        if (hit >= 0) {
+          if (E->nfnb[f] >= ORC_MAXNB) { fprintf(stderr, "too many nb\n"); abort(); }
+          E->fnb[f][E->nfnb[f]] = e2;
+          E->nbf[f][E->nfnb[f]] = hit;
+          E->nfnb[f]++;
+        }
+      }
+    }
+  }
+}
+
+/* patch faces that touch no other patch are outer boundaries
+   (src/main/amr/bfaces.c:894-910) */
+static void mark_boundaries(OMesh *M)
+{
+  for (int p = 0; p < M->npat; p++)
+    for (int f = 0; f < 6; f++) M->pat[p].bound[f] = 1;
+  for (int e = 0; e < M->nelem; e++) {
+    OElm *E = &M->e[e];
+    for (int f = 0; f < 6; f++)
+      if (elm_on_patface(M, E, f) && E->nfnb[f] > 0) M->pat[E->p].bound[f] = 0;
+  }
+}
+
+/* nearest corner of the neighbour's face plane to physical point x0
+   (src/main/amr/surface.c:885-930, nearest_corner_of_xyz_inplaneN) */
+static double nearest_corner(const OMesh *M, const OElm *N, int nf,
+                             const double x0[3], int inb[3])
+{
+  int dir = nf / 2, pl = (N->n - 1) * (nf % 2);
+  double best = 1e300;
+  for (int c2 = 0; c2 < 2; c2++)
+    for (int c1 = 0; c1 < 2; c1++) {
+      int I[3];
+      int d1 = (dir == 0), d2 = (dir == 2) ? 1 : 2;
+      I[dir] = pl; I[d1] = c1 * (N->n - 1); I[d2] = c2 * (N->n - 1);
+      double X[3], x[3];
+      elm_point_X(N, I[0], I[1], I[2], X);
+      pat_xyz(&M->pat[N->p], X, x);
+      double d = sqrt((x[0] - x0[0]) * (x[0] - x0[0]) + (x[1] - x0[1]) * (x[1] - x0[1]) +
+                      (x[2] - x0[2]) * (x[2] - x0[2]));
+      if (d < best) { best = d; inb[0] = I[0]; inb[1] = I[1]; inb[2] = I[2]; }
+    }
+  return best;
+}
+
+/* decide how ajsurf is built for every face (src/main/amr/surface.c:799-1080) */
+static void set_ajmodes(OMesh *M)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int e = 0; e < M->nelem; e++) {
+    OElm *E = &M->e[e];
+    for (int f = 0; f < 6; f++) {
+      E->ajmode[f] = 0; E->orient[f] = 0;
+      if (E->nfnb[f] == 0) continue;
+      E->ajmode[f] = 3;
+      if (E->nfnb[f] != 1) continue;
+      OElm *N = &M->e[E->fnb[f][0]];
+      int nbf = E->nbf[f][0];
+      if (N->p == E->p) {
+        if (N->l == E->l && N->n == E->n && N->use_fv == E->use_fv) E->ajmode[f] = 1;
+        continue;
+      }
+      if (N->nfnb[nbf] != 1 || N->n != E->n || N->use_fv != E->use_fv) continue;
+      int dir = f / 2, pl = (E->n - 1) * (f % 2);
+      int d1 = (dir == 0), d2 = (dir == 2) ? 1 : 2;
+      int n1 = E->n, n2 = E->n;
+      int odir = nbf / 2, od1 = (odir == 0), od2 = (odir == 2) ? 1 : 2;
+      int on1 = N->n, on2 = N->n;
+      int inb[3][3];
+      double dist[3];
+      int pts[3][2] = {{0, 0}, {n1 - 1, 0}, {0, n2 - 1}};
+      for (int q = 0; q < 3; q++) {
+        int I[3];
+        I[dir] = pl; I[d1] = pts[q][0]; I[d2] = pts[q][1];
+        double X[3], x0[3];
+        elm_point_X(E, I[0], I[1], I[2], X);
+        pat_xyz(&M->pat[E->p], X, x0);
+        dist[q] = nearest_corner(M, N, nbf, x0, inb[q]);
+        double mx0 = sqrt(x0[0] * x0[0] + x0[1] * x0[1] + x0[2] * x0[2]);
+        if (mx0 > 0.) dist[q] /= mx0;
+      }
+      if (!(dist[0] <= DEQUALEPS && dist[1] <= DEQUALEPS && dist[2] <= DEQUALEPS)) continue;
+      int a00 = inb[0][od1], a02 = inb[0][od2];
+      int a10 = inb[1][od1], a12 = inb[1][od2];
+      int a20 = inb[2][od1], a22 = inb[2][od2];
+      int code = -1;
+      if (a00 == 0 && a02 == 0) {
+        if (a10 == on1 - 1 && a12 == 0 && a20 == 0 && a22 == on2 - 1) code = 0;
+        else if (a10 == 0 && a12 == on2 - 1 && a20 == on1 - 1 && a22 == 0) code = 4;
+      }
+      if (code < 0 && a00 == on1 - 1 && a02 == 0) {
+        if (a10 == 0 && a12 == 0 && a20 == on1 - 1 && a22 == on2 - 1) code = 2;
+        else if (a10 == on1 - 1 && a12 == on2 - 1 && a20 == 0 && a22 == 0) code = 6;
+      }
+      if (code < 0 && a00 == 0 && a02 == on2 - 1) {
+        if (a10 == on1 - 1 && a12 == on2 - 1 && a20 == 0 && a22 == 0) code = 1;
+        else if (a10 == 0 && a12 == 0 && a20 == on1 - 1 && a22 == on2 - 1) code = 5;
+      }
+      if (code < 0 && a00 == on1 - 1 && a02 == on2 - 1) {
+        if (a10 == 0 && a12 == on2 - 1 && a20 == on1 - 1 && a22 == 0) code = 3;
+        else if (a10 == on1 - 1 && a12 == 0 && a20 == 0 && a22 == on2 - 1) code = 7;
+      }
+      if (code >= 0) { E->ajmode[f] = 2; E->orient[f] = code; }
+    }
+  }
+}
+
+/* smallest axial spacing at the 8 corners (src/main/coordinates/utilities.c:1028-1124) */
+static double elm_find_hmin(const OMesh *M, const OElm *E)
+{
+  const OPat *P = &M->pat[E->p];
+  int n = E->n;
+  double X0[3] = {E->bbox[0], E->bbox[2], E->bbox[4]};
+  double X1[3] = {E->bbox[1], E->bbox[3], E->bbox[5]};
+  double x0[3], x1[3];
+  pat_xyz(P, X0, x0); pat_xyz(P, X1, x1);
+  double d2 = 0.;
+  for (int d = 0; d < 3; d++) d2 += (x1[d] - x0[d]) * (x1[d] - x0[d]);
+  double hmin = sqrt(d2) / sqrt(3.);
+  int cs[8][3] = {{0, 0, 0}, {n - 1, 0, 0}, {0, n - 1, 0}, {n - 1, n - 1, 0},
+                  {0, 0, n - 1}, {n - 1, 0, n - 1}, {0, n - 1, n - 1}, {n - 1, n - 1, n - 1}};
+  for (int c = 0; c < 8; c++) {
+    int i = cs[c][0], j = cs[c][1], k = cs[c][2];
+    double Xa[3], xa[3];
+    elm_point_X(E, i, j, k, Xa);
+    pat_xyz(P, Xa, xa);
+    for (int kk = -1; kk <= 1; kk++) {
+      int kn = k + kk;
+      if (kn < 0 || kn >= n) continue;
+      for (int jj = -1; jj <= 1; jj++) {
+        int jn = j + jj;
+        if (jn < 0 || jn >= n) continue;
+        for (int ii = -1; ii <= 1; ii++) {
+          int in = i + ii;
+          if (in < 0 || in >= n) continue;
+          if (in == i && jn == j && kn == k) continue;
+          if (in != i && jn != j) continue;
+          if (in != i && kn != k) continue;
+          if (jn != j && kn != k) continue;
+          double Xb_[3], xb_[3];
+          elm_point_X(E, in, jn, kn, Xb_);
+          pat_xyz(P, Xb_, xb_);
+          double dd = 0.;
+          for (int d = 0; d < 3; d++) dd += (xb_[d] - xa[d]) * (xb_[d] - xa[d]);
+          dd = sqrt(dd);
+          if (dd < hmin) hmin = dd;
+        }
+      }
+    }
+  }
+  return hmin;
+}
+
+/* per-node geometry (src/main/coordinates/coordinates.c:133-224,728-805) */
+static void mesh_geometry(OMesh *M)
+{
+  long off = 0, foff = 0;
+  for (int e = 0; e < M->nelem; e++) {
+    M->e[e].off = off;
+    off += elm_np(&M->e[e]);
+  }
+  M->nnodes = off;
+  free(M->foff);
+  M->foff = malloc(sizeof(long) * 6 * M->nelem);
+  for (int e = 0; e < M->nelem; e++)
+    for (int f = 0; f < 6; f++) { M->foff[6 * e + f] = foff; foff += M->e[e].n * M->e[e].n; }
+  M->nface = foff;
+  long nn = M->nnodes;
+  free(M->X); free(M->x); free(M->dXdx); free(M->sqrtgdiag); free(M->det);
+  M->X = malloc(sizeof(double) * 3 * nn);
+  M->x = malloc(sizeof(double) * 3 * nn);
+  M->dXdx = malloc(sizeof(double) * 9 * nn);
+  M->sqrtgdiag = malloc(sizeof(double) * 3 * nn);
+  M->det = malloc(sizeof(double) * nn);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int e = 0; e < M->nelem; e++) {
+    OElm *E = &M->e[e];
+    const OPat *P = &M->pat[E->p];
+    int n = E->n;
+    double dXbdX[3];
+    elm_dXbdX(E, dXbdX);
+    double detb = dXbdX[0] * dXbdX[1] * dXbdX[2];
+    for (int k = 0; k < n; k++)
+      for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+          long g = E->off + i + n * (j + n * k);
+          double X[3], x[3], D[3][3] = {{0}};
+          elm_point_X(E, i, j, k, X);
+          if (P->type == PAT_CUBEDSHELL) {
+            cubsph_dXdx(P, X, x, D);
+          } else {
+            for (int d = 0; d < 3; d++) { x[d] = X[d]; D[d][d] = 1.; }
+          }
+          for (int d = 0; d < 3; d++) { M->X[d * nn + g] = X[d]; M->x[d * nn + g] = x[d]; }
+          for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) M->dXdx[(3 * a + b) * nn + g] = D[a][b];
+          double det3 = D[0][0] * (D[1][1] * D[2][2] - D[1][2] * D[2][1]) -
+                        D[0][1] * (D[1][0] * D[2][2] - D[1][2] * D[2][0]) +
+                        D[0][2] * (D[1][0] * D[2][1] - D[1][1] * D[2][0]);
+          M->det[g] = (P->type == PAT_CUBEDSHELL) ? detb * det3 : detb;
+          for (int a = 0; a < 3; a++) {
+            double gam = 0.0;
+            for (int l = 0; l < 3; l++) gam += D[a][l] * D[a][l];
+            M->sqrtgdiag[a * nn + g] = dXbdX[a] * sqrt(gam);
+          }
+        }
+    E->hmin = elm_find_hmin(M, E);
+  }
+}
+
+static void mesh_finish(OMesh *M)
+{
+  find_neighbours(M);
+  mark_boundaries(M);
+  set_ajmodes(M);
+  mesh_geometry(M);
+}
+
+/* BoxMesh with one patch xc +- dout (src/main/amr/mesh.c:354-445,
+   src/main/coordinates/setup_Boxes.c:11-32) */
+OMesh *orc_mesh_box(int luni, int n, double dout, const double xc[3])
+{
+  OMesh *M = calloc(1, sizeof(OMesh));
+  M->npat = 1;
+  OPat *P = &M->pat[0];
+  P->type = PAT_BOX;
+  for (int d = 0; d < 3; d++) {
+    P->xc[d] = xc ? xc[d] : 0.;
+    P->bbox[2 * d] = P->xc[d] - dout;
+    P->bbox[2 * d + 1] = P->xc[d] + dout;
+  }
+  mesh_add_roots(M, n);
+  refine_uniform(M, luni);
+  mesh_finish(M);
+  return M;
+}
+
+/* CubedSphere shell of 6 CubedShell patches with constant sigma
+   (src/main/amr/mesh.c:625-669, src/main/coordinates/setup_CubedSpheres.c:428-443,653-684,935-944) */
+static OMesh *shell_base(int n, double rin, double rout)
+{
+  OMesh *M = calloc(1, sizeof(OMesh));
+  M->npat = 6;
+  for (int p = 0; p < 6; p++) {
+    OPat *P = &M->pat[p];
+    P->type = PAT_CUBEDSHELL;
+    P->dom = p;
+    P->s0 = rin; P->s1 = rout;
+    P->xc[0] = P->xc[1] = P->xc[2] = 0.;
+    /* set_AB_min_max_from_Din with all Din equal gives [-1,1]x[-1,1] */
+    double b[6] = {0., 1., -rin / rin, rin / rin, -rin / rin, rin / rin};
+    memcpy(P->bbox, b, sizeof(b));
+  }
+  mesh_add_roots(M, n);
+  return M;
+}
+
+OMesh *orc_mesh_shell(int luni, int n, double rin, double rout)
+{
+  OMesh *M = shell_base(n, rin, rout);
+  refine_uniform(M, luni);
+  mesh_finish(M);
+  return M;
+}
+
+/* shell refined uniformly to luni, then in nested spheres about the origin:
+   elements whose centre lies within radius*2^(levels-1-i) are refined in
+   pass i (src/main/amr/refine.c:1025-1100) */
+OMesh *orc_mesh_amr_shell(int luni, int n, double rin, double rout,
+                          double radius, int levels)
+{
+  OMesh *M = shell_base(n, rin, rout);
+  refine_uniform(M, luni);
+  double r = radius * pow(2., levels - 1);
+  for (int i = 0; i < levels; i++) {
+    int *flag = calloc(M->nelem, sizeof(int));
+    for (int e = 0; e < M->nelem; e++) {
+      OElm *E = &M->e[e];
+      double Xc[3], xcn[3];
+      for (int d = 0; d < 3; d++) Xc[d] = 0.5 * (E->bbox[2 * d + 1] + E->bbox[2 * d]);
+      pat_xyz(&M->pat[E->p], Xc, xcn);
+      double rr = sqrt(xcn[0] * xcn[0] + xcn[1] * xcn[1] + xcn[2] * xcn[2]);
+      flag[e] = (rr <= r);
+    }
+    refine_flagged(M, flag);
+    free(flag);
+    r *= 0.5;
+  }
+  mesh_finish(M);
+  return M;
+}
+
+/* change the number of points per direction of every element (p-refinement,
+   src/main/amr/storage.c update_node_n_pt_typ) and rebuild tables */
+void orc_mesh_set_n(OMesh *M, const int *n)
+{
+  for (int e = 0; e < M->nelem; e++) M->e[e].n = n[e];
+  set_ajmodes(M);
+  mesh_geometry(M);
+}
+
+void orc_mesh_free(OMesh *M)
+{
+  if (!M) return;
+  free(M->e); free(M->X); free(M->x); free(M->dXdx); free(M->sqrtgdiag);
+  free(M->det); free(M->foff); free(M);
+}
+
+/* ---- queries for the test harness ---- */
+int orc_mesh_nelem(const OMesh *M) { return M->nelem; }
+long orc_mesh_nnodes(const OMesh *M) { return M->nnodes; }
+int orc_mesh_npat(const OMesh *M) { return M->npat; }
+
+/* info[0..]: p, l, n, off(lo 31 bits), nfnb[6], ajmode[6], orient[6] */
+void orc_mesh_elm_info(const OMesh *M, int e, long *info)
+{
+  const OElm *E = &M->e[e];
+  info[0] = E->p; info[1] = E->l; info[2] = E->n; info[3] = E->off;
+  for (int f = 0; f < 6; f++) {
+    info[4 + f] = E->nfnb[f];
+    info[10 + f] = E->ajmode[f];
+    info[16 + f] = E->orient[f];
+  }
+}
+
+void orc_mesh_elm_bbox(const OMesh *M, int e, double *bb)
+{
+  memcpy(bb, M->e[e].bbox, 6 * sizeof(double));
+}
+
+int orc_mesh_elm_fnb(const OMesh *M, int e, int f, int *nb, int *nbf)
+{
+  const OElm *E = &M->e[e];
+  for (int i = 0; i < E->nfnb[f]; i++) { nb[i] = E->fnb[f][i]; if (nbf) nbf[i] = E->nbf[f][i]; }
+  return E->nfnb[f];
+}
+
+int orc_mesh_pat_bound(const OMesh *M, int p, int f) { return M->pat[p].bound[f]; }
+
+void orc_mesh_coords(const OMesh *M, double *X, double *x)
+{
+  if (X) memcpy(X, M->X, sizeof(double) * 3 * M->nnodes);
+  if (x) memcpy(x, M->x, sizeof(double) * 3 * M->nnodes);
+}
+
+void orc_mesh_geom(const OMesh *M, double *dXdx, double *sqrtgdiag, double *det)
+{
+  if (dXdx) memcpy(dXdx, M->dXdx, sizeof(double) * 9 * M->nnodes);
+  if (sqrtgdiag) memcpy(sqrtgdiag, M->sqrtgdiag, sizeof(double) * 3 * M->nnodes);
+  if (det) memcpy(det, M->det, sizeof(double) * M->nnodes);
+}
+
+/* dt = dtfac * min_elements hmin (src/main/amr/storage.c:1305-1366,
+   src/main/coordinates/utilities.c:1264-1304) */
+double orc_mesh_dt(const OMesh *M, double dtfac)
+{
+  double dt = 0.1 * 1.7976931348623157e308;
+  for (int e = 0; e < M->nelem; e++) {
+    double dtm = dtfac * M->e[e].hmin;
+    if (dtm < dt) dt = dtm;
+  }
+  return dt;
+}
+
+double orc_mesh_elm_hmin(const OMesh *M, int e) { return M->e[e].hmin; }
+
+/* ------------------------------------------------------------------ */
+/* scalarwave1                                                         */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  int numflux;     /* 0 upwind (scalarwave1_numflux1d_upwind), 1 LLF */
+  double k[3];     /* scalarwave1_k */
+} OSW;
+
+#define NV 5
+
+/* plane wave (src/tests/scalarwave1/scalarwave1.c:503-524) */
+static void sw_profile_pt(const double k[3], const double xyz[3], double t, double *u)
+{
+  double kx = k[0], ky = k[1], kz = k[2];
+  double om = sqrt(kx * kx + ky * ky + kz * kz);
+  double arg = kx * xyz[0] + ky * xyz[1] + kz * xyz[2] - om * t;
+  double s = sin(arg), c = cos(arg);
+  u[4] = s; u[0] = -om * c; u[1] = kx * c; u[2] = ky * c; u[3] = kz * c;
+}
+
+/* d/dt of the plane wave (src/tests/scalarwave1/scalarwave1.c:544-566) */
+static void sw_dt_profile_pt(const double k[3], const double xyz[3], double t, double *du)
+{
+  double u[5];
+  double om = sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+  sw_profile_pt(k, xyz, t, u);
+  du[0] = -om * om * u[4]; du[1] = om * k[0] * u[4]; du[2] = om * k[1] * u[4];
+  du[3] = om * k[2] * u[4]; du[4] = u[0];
+}
+
+void orc_set_profile(const OMesh *M, const double k[3], double t, double *u, double *dudt)
+{
+  long nn = M->nnodes;
+#pragma omp parallel for
+  for (long g = 0; g < nn; g++) {
+    double xyz[3] = {M->x[g], M->x[nn + g], M->x[2 * nn + g]}, v[5];
+    sw_profile_pt(k, xyz, t, v);
+    for (int l = 0; l < NV; l++) u[l * nn + g] = v[l];
+    if (dudt) {
+      sw_dt_profile_pt(k, xyz, t, v);
+      for (int l = 0; l < NV; l++) dudt[l * nn + g] = v[l];
+    }
+  }
+}
+
+/* f.n for u = (pi,cx,cy,cz,phi) (src/tests/scalarwave1/scalarwave1.c:39-48) */
+static void sw_flux1d(double *f, const double nrm[3], const double *u)
+{
+  f[0] = -(u[1] * nrm[0] + u[2] * nrm[1] + u[3] * nrm[2]);
+  f[1] = -u[0] * nrm[0];
+  f[2] = -u[0] * nrm[1];
+  f[3] = -u[0] * nrm[2];
+  f[4] = 0.;
+}
+
+/* eigen-decomposition upwind flux (src/tests/scalarwave1/scalarwave1.c:324-499) */
+static void sw_numflux_upwind(const double nrm[3], const double *lami, const double *ui,
+                              const double *ua, double *fnum)
+{
+  const int nf = 4;
+  double lp[4][4] = {{0}}, ln[4][4] = {{0}}, ip[4][4] = {{0}}, in_[4][4] = {{0}};
+  double sp[4][4] = {{0}}, sn[4][4] = {{0}}, fi[4] = {0}, fa[4] = {0};
+  double nx = nrm[0], ny = nrm[1], nz = nrm[2];
+  double R[4][4], iR[4][4];
+  for (int l = 0; l < nf; l++) {
+    if (lami[l] > 0.) lp[l][l] = lami[l];
+    else ln[l][l] = lami[l];
+  }
+  if (nz != 0) {
+    double R_[4][4] = {{1.0, 1.0, 0.0, 0.0}, {-nx, nx, 0.0, nz}, {-ny, ny, nz, 0.0}, {-nz, nz, -ny, -nx}};
+    double iR_[4][4] = {{1.0 / 2.0, -nx / 2.0, -ny / 2.0, -nz / 2.0},
+                        {1.0 / 2.0, nx / 2.0, ny / 2.0, nz / 2.0},
+                        {0.0, -(nx * ny) / nz, (nx * nx + nz * nz) / nz, -ny},
+                        {0.0, (ny * ny + nz * nz) / nz, -(nx * ny) / nz, -nx}};
+    memcpy(R, R_, sizeof(R)); memcpy(iR, iR_, sizeof(iR));
+  } else if (nx != 0) {
+    double R_[4][4] = {{1.0, 1.0, 0.0, 0.0}, {-nx, nx, -nz, -ny}, {-ny, ny, 0.0, nx}, {-nz, nz, nx, 0.0}};
+    double iR_[4][4] = {{1.0 / 2.0, -nx / 2.0, -ny / 2.0, -nz / 2.0},
+                        {1.0 / 2.0, nx / 2.0, ny / 2.0, nz / 2.0},
+                        {0.0, -nz, -(ny * nz) / nx, (nx * nx + ny * ny) / nx},
+                        {0.0, -ny, (nx * nx + nz * nz) / nx, -(ny * nz) / nx}};
+    memcpy(R, R_, sizeof(R)); memcpy(iR, iR_, sizeof(iR));
+  } else {
+    double R_[4][4] = {{1.0, 1.0, 0.0, 0.0}, {-nx, nx, 0.0, ny}, {-ny, ny, -nz, -nx}, {-nz, nz, ny, 0.0}};
+    double iR_[4][4] = {{1.0 / 2.0, -nx / 2.0, -ny / 2.0, -nz / 2.0},
+                        {1.0 / 2.0, nx / 2.0, ny / 2.0, nz / 2.0},
+                        {0.0, -(nx * nz) / ny, -nz, (nx * nx + ny * ny) / ny},
+                        {0.0, (ny * ny + nz * nz) / ny, -nx, -(nx * nz) / ny}};
+    memcpy(R, R_, sizeof(R)); memcpy(iR, iR_, sizeof(iR));
+  }
+  for (int l = 0; l < nf; l++)
+    for (int m = 0; m < nf; m++)
+      for (int s = 0; s < nf; s++) { ip[l][m] += lp[l][s] * iR[s][m]; in_[l][m] += ln[l][s] * iR[s][m]; }
+  for (int l = 0; l < nf; l++)
+    for (int m = 0; m < nf; m++)
+      for (int s = 0; s < nf; s++) { sp[l][m] += R[l][s] * ip[s][m]; sn[l][m] += R[l][s] * in_[s][m]; }
+  for (int l = 0; l < nf; l++)
+    for (int m = 0; m < nf; m++) { fi[l] += sp[l][m] * ui[m]; fa[l] += sn[l][m] * ua[m]; }
+  for (int l = 0; l < nf; l++) fnum[l] = fi[l] + fa[l];
+  fnum[4] = 0.;
+}
+
+/* LLF flux (src/main/dg/numflux1d.c:85-112) */
+static void numflux_LLF(int nf, const double *lami, const double *lama, const double *ui,
+                        const double *ua, const double *fi, const double *fa, double *fnum)
+{
+  double apl = 0., ami = 0.;
+  for (int i = 0; i < nf; i++) {
+    apl = fmax(apl, fmax(lama[i], lami[i]));
+    ami = fmin(ami, fmin(lama[i], lami[i]));
+  }
+  apl = fabs(apl); ami = fabs(ami);
+  double amax = fmax(ami, apl);
+  for (int i = 0; i < nf; i++) fnum[i] = 0.5 * (fa[i] + fi[i] - amax * (ua[i] - ui[i]));
+}
+
+/* out = D_dir u (src/main/amr/array.c:36-45,100-300 via src/main/basis/basis.c:61-95) */
+static void deriv1(int n, const double *DT, int dir, const double *u, double *du)
+{
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        double sum = 0.0;
+        for (int l = 0; l < n; l++) {
+          int src = (dir == 0) ? l + n * (j + n * k) : (dir == 1) ? i + n * (l + n * k)
+                                                                  : i + n * (j + n * l);
+          int ii = (dir == 0) ? i : (dir == 1) ? j : k;
+          sum += DT[l + n * ii] * u[src];
+        }
+        du[i + n * (j + n * k)] = sum;
+      }
+}
+
+/* Cartesian partials of one scalar: du[m] = sum_i dXdx[i][m] dXbdX[i] D_i u
+   (src/main/coordinates/derivs.c:102-181) */
+static void cart_partials(const OMesh *M, const OElm *E, const double *DT, const double *u,
+                          double *du[3])
+{
+  int n = E->n, np = elm_np(E);
+  double dXbdX[3];
+  elm_dXbdX(E, dXbdX);
+  for (int d = 0; d < 3; d++) deriv1(n, DT, d, u, du[d]);
+  for (int i = 0; i < np; i++)
+    for (int m = 0; m < 3; m++) du[m][i] *= dXbdX[m];
+  if (M->pat[E->p].type == PAT_CUBEDSHELL) {
+    long nn = M->nnodes;
+    for (int i = 0; i < np; i++) {
+      long g = E->off + i;
+      double dw[3];
+      for (int m = 0; m < 3; m++) {
+        dw[m] = 0.;
+        for (int a = 0; a < 3; a++) dw[m] += M->dXdx[(3 * a + m) * nn + g] * du[a][i];
+      }
+      for (int m = 0; m < 3; m++) du[m][i] = dw[m];
+    }
+  }
+}
+
+/* divU = d_x Ux + d_y Uy + d_z Uz (src/main/coordinates/derivs.c:756-815) */
+static void cart_divergence(const OMesh *M, const OElm *E, const double *DT, const double *Ux,
+                            const double *Uy, const double *Uz, double *divU, double *tmp)
+{
+  int np = elm_np(E);
+  double *t1 = tmp, *t2 = tmp + np, *t3 = tmp + 2 * np;
+  double *dU[3] = {divU, t2, t3};
+  cart_partials(M, E, DT, Ux, dU);
+  dU[0] = t1;
+  cart_partials(M, E, DT, Uy, dU);
+  for (int i = 0; i < np; i++) divU[i] += t2[i];
+  cart_partials(M, E, DT, Uz, dU);
+  for (int i = 0; i < np; i++) divU[i] += t3[i];
+}
+
+/* volume part (src/tests/scalarwave1/scalarwave1.c:62-134,242-284) */
+static void sw_vol_rhs(const OMesh *M, const OElm *E, const double *u, double *r,
+                       double *divf, double *work)
+{
+  int n = E->n, np = elm_np(E);
+  long nn = M->nnodes, o = E->off;
+  const double *DT = basis(n)->DT;
+  double *fpi[3], *fc[3][3];
+  for (int d = 0; d < 3; d++) fpi[d] = work + d * np;
+  for (int c = 0; c < 3; c++)
+    for (int d = 0; d < 3; d++) fc[c][d] = work + (3 + 3 * c + d) * np;
+  double *tmp = work + 12 * np;
+  for (int i = 0; i < np; i++) {
+    double uu[5] = {u[0 * nn + o + i], u[1 * nn + o + i], u[2 * nn + o + i], u[3 * nn + o + i],
+                    u[4 * nn + o + i]};
+    for (int d = 0; d < 3; d++) {
+      double no[3] = {d == 0, d == 1, d == 2}, f[5];
+      sw_flux1d(f, no, uu);
+      fpi[d][i] = f[0];
+      for (int c = 0; c < 3; c++) fc[c][d][i] = f[1 + c];
+    }
+  }
+  cart_divergence(M, E, DT, fpi[0], fpi[1], fpi[2], divf + 0 * np, tmp);
+  for (int c = 0; c < 3; c++)
+    cart_divergence(M, E, DT, fc[c][0], fc[c][1], fc[c][2], divf + (1 + c) * np, tmp);
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        int ijk = i + n * (j + n * k);
+        int onface = (i == 0 || i == n - 1 || j == 0 || j == n - 1 || k == 0 || k == n - 1);
+        double interior = !onface;
+        r[4 * nn + o + ijk] = u[0 * nn + o + ijk];
+        for (int l = 0; l < 4; l++) r[l * nn + o + ijk] = -divf[l * np + ijk] * interior;
+      }
+}
+
+/* fill face plane f of element E from state u (set_mysurf, surface.c:231-256) */
+static void set_mysurf(const OMesh *M, const OElm *E, int f, const double *u, int v,
+                       double *plane)
+{
+  int n = E->n, dir = f / 2, p = (n - 1) * (f % 2);
+  long nn = M->nnodes;
+  for (int b = 0; b < n; b++)
+    for (int a = 0; a < n; a++) {
+      int I[3];
+      int d1 = (dir == 0), d2 = (dir == 2) ? 1 : 2;
+      I[dir] = p; I[d1] = a; I[d2] = b;
+      /* Ind_n_norm layout: index a + n*b in both cases */
+      plane[a + n * b] = u[v * nn + E->off + I[0] + n * (I[1] + n * I[2])];
+    }
+}
+
+/* oriented copy (src/main/amr/surface.c:1361-1466); face planes use the
+   in-plane index a + n1*b with a along Dir1_norm(dir), b along Dir2_norm(dir) */
+static void copy_oriented(int n, int odir, int code, const double *nb, double *aj)
+{
+  int intrch = (code >> 2) & 1, rev1 = (code >> 1) & 1, rev2 = code & 1;
+  for (int I2 = 0; I2 < n; I2++)
+    for (int I1 = 0; I1 < n; I1++) {
+      int oi, oj, ok;
+      if (odir == 0) { oi = 0; oj = I1; ok = I2; }
+      else if (odir == 1) { oi = I1; oj = 0; ok = I2; }
+      else { oi = I1; oj = I2; ok = 0; }
+      int i2 = oi, j2 = oj, k2 = ok;
+      if (odir == 0) {
+        if (!intrch) { if (rev1) j2 = n - 1 - oj; if (rev2) k2 = n - 1 - ok; }
+        else { j2 = ok; k2 = oj; if (rev1) j2 = n - 1 - ok; if (rev2) k2 = n - 1 - oj; }
+      } else if (odir == 1) {
+        if (!intrch) { if (rev1) i2 = n - 1 - oi; if (rev2) k2 = n - 1 - ok; }
+        else { i2 = ok; k2 = oi; if (rev1) i2 = n - 1 - ok; if (rev2) k2 = n - 1 - oi; }
+      } else {
+        if (!intrch) { if (rev1) i2 = n - 1 - oi; if (rev2) j2 = n - 1 - oj; }
+        else { i2 = oj; j2 = oi; if (rev1) i2 = n - 1 - oj; if (rev2) j2 = n - 1 - oi; }
+      }
+      /* neighbour plane has n[odir]=1: Ind_n over (n0,n1) with the normal
+         dimension collapsed */
+      int ind2;
+      if (odir == 0) ind2 = j2 + n * k2;
+      else if (odir == 1) ind2 = i2 + n * k2;
+      else ind2 = i2 + n * j2;
+      aj[I1 + n * I2] = nb[ind2];
+    }
+}
+
+/* build ajsurf of face f for variable plane data (src/main/amr/surface.c:799-1357) */
+static void build_ajsurf(const OMesh *M, const OElm *E, int f, const double *planes, int v,
+                         double *aj)
+{
+  int n = E->n;
+  int mode = E->ajmode[f];
+  long nf = M->nface;
+  if (mode == 1) {
+    int e2 = E->fnb[f][0], f2 = E->nbf[f][0];
+    memcpy(aj, planes + v * nf + M->foff[6 * e2 + f2], sizeof(double) * n * n);
+    return;
+  }
+  if (mode == 2) {
+    int e2 = E->fnb[f][0], f2 = E->nbf[f][0];
+    copy_oriented(n, f2 / 2, E->orient[f], planes + v * nf + M->foff[6 * e2 + f2], aj);
+    return;
+  }
+  /* interpolation from every neighbour, averaged where several hold a point */
+  int dir = f / 2, pl = (n - 1) * (f % 2), d1 = (dir == 0), d2 = (dir == 2) ? 1 : 2;
+  const OPat *P = &M->pat[E->p];
+  for (int b = 0; b < n; b++)
+    for (int a = 0; a < n; a++) {
+      int I[3];
+      I[dir] = pl; I[d1] = a; I[d2] = b;
+      double X[3], x[3];
+      elm_point_X(E, I[0], I[1], I[2], X);
+      pat_xyz(P, X, x);
+      double sum = 0.;
+      int cnt = 0;
+      for (int ni = 0; ni < E->nfnb[f]; ni++) {
+        const OElm *N = &M->e[E->fnb[f][ni]];
+        int nbf = E->nbf[f][ni], nd = nbf / 2;
+        int nd1 = (nd == 0), nd2 = (nd == 2) ? 1 : 2;
+        double XN[3];
+        if (N->p == E->p) { XN[0] = X[0]; XN[1] = X[1]; XN[2] = X[2]; }
+        else pat_XYZ(&M->pat[N->p], x, XN);
+        /* inside neighbour (array_find_Xplane_in_node): within its face
+           rectangle, boundaries included */
+        double tol = 1e-10;
+        if (XN[nd1] < N->bbox[2 * nd1] - tol || XN[nd1] > N->bbox[2 * nd1 + 1] + tol) continue;
+        if (XN[nd2] < N->bbox[2 * nd2] - tol || XN[nd2] > N->bbox[2 * nd2 + 1] + tol) continue;
+        double Cb0 = (2. * XN[nd1] - (N->bbox[2 * nd1 + 1] + N->bbox[2 * nd1])) /
+                     (N->bbox[2 * nd1 + 1] - N->bbox[2 * nd1]);
+        double Cb1 = (2. * XN[nd2] - (N->bbox[2 * nd2 + 1] + N->bbox[2 * nd2])) /
+                     (N->bbox[2 * nd2 + 1] - N->bbox[2 * nd2]);
+        /* 2D tensor Lagrange interpolation (src/main/basis/interpolate.c:296-363) */
+        int nn_ = N->n;
+        OBasis *bn = basis(nn_);
+        double B1[ORC_MAXN], B2[ORC_MAXN];
+        for (int q = 0; q < nn_; q++) {
+          B1[q] = lagrange_of_x(q, Cb0, nn_, bn->Xb, bn->WL);
+          B2[q] = lagrange_of_x(q, Cb1, nn_, bn->Xb, bn->WL);
+        }
+        const double *np_ = planes + v * nf + M->foff[6 * E->fnb[f][ni] + nbf];
+        double s = 0.;
+        for (int q2 = 0; q2 < nn_; q2++)
+          for (int q1 = 0; q1 < nn_; q1++) s += np_[q1 + nn_ * q2] * B1[q1] * B2[q2];
+        sum += s;
+        cnt++;
+      }
+      if (cnt > 1) sum /= cnt;
+      aj[a + n * b] = sum;
+    }
+}
+
+/* surface part: numerical flux and lift on each face node, then r -= divf
+   on all face points (src/tests/scalarwave1/scalarwave1.c:139-209,287-321,
+   src/main/dg/dg.c:30-151, src/main/main/variables.c:1044-1071) */
+static void sw_surf_rhs(const OMesh *M, const OSW *sw, const OElm *E, double t, const double *u,
+                        const double *planes, double *r, double *divf)
+{
+  int n = E->n, np = elm_np(E);
+  long nn = M->nnodes, o = E->off;
+  OBasis *bb = basis(n);
+  double aj[NV][ORC_MAXN * ORC_MAXN];
+  for (int f = 0; f < 6; f++) {
+    int dir = f / 2, p = (n - 1) * (f % 2), d1 = (dir == 0), d2 = (dir == 2) ? 1 : 2;
+    double sig = 2 * (f % 2) - 1;
+    double oow = 1. / bb->Wq[p];
+    int have_aj = (E->nfnb[f] > 0);
+    int onbound = elm_on_patface(M, E, f) && M->pat[E->p].bound[f];
+    if (have_aj)
+      for (int v = 0; v < NV; v++) build_ajsurf(M, E, f, planes, v, aj[v]);
+    for (int b = 0; b < n; b++)
+      for (int a = 0; a < n; a++) {
+        int I[3];
+        I[dir] = p; I[d1] = a; I[d2] = b;
+        int ijk = I[0] + n * (I[1] + n * I[2]);
+        int JK = a + n * b;
+        long g = o + ijk;
+        double nrm[3];
+        for (int d = 0; d < 3; d++) nrm[d] = M->dXdx[(3 * dir + d) * nn + g];
+        double smag = sig * sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+        if (smag == 0.0) smag = sig;
+        for (int d = 0; d < 3; d++) nrm[d] /= smag;
+        double lam[5] = {1.0, -1.0, 0.0, 0.0, 0.0};
+        double ui[5], ua[5] = {0, 0, 0, 0, 0}, fi[5], fa[5], fnum[5];
+        for (int l = 0; l < NV; l++) ui[l] = u[l * nn + g];
+        if (have_aj) {
+          for (int l = 0; l < NV; l++) ua[l] = aj[l][JK];
+        } else if (onbound) {
+          double xyz[3] = {M->x[g], M->x[nn + g], M->x[2 * nn + g]};
+          sw_profile_pt(sw->k, xyz, t, ua);
+        }
+        sw_flux1d(fi, nrm, ui);
+        sw_flux1d(fa, nrm, ua);
+        if (sw->numflux == 1) numflux_LLF(NV, lam, lam, ui, ua, fi, fa, fnum);
+        else sw_numflux_upwind(nrm, lam, ui, ua, fnum);
+        double gd_ow = M->sqrtgdiag[dir * nn + g] * oow;
+        for (int l = 0; l < 4; l++) {
+          double F = (fnum[l] - fi[l] * 1.0) * 1.0;
+          divf[l * np + ijk] += F * (0.0 * 0.0 + 1.0 * gd_ow);
+        }
+      }
+  }
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++)
+        if (i == 0 || i == n - 1 || j == 0 || j == n - 1 || k == 0 || k == n - 1) {
+          int ijk = i + n * (j + n * k);
+          for (int l = 0; l < 4; l++) r[l * nn + o + ijk] += -1. * divf[l * np + ijk];
+        }
+}
+
+static OSW sw_from(int numflux, const double *k)
+{
+  OSW s;
+  s.numflux = numflux;
+  for (int d = 0; d < 3; d++) s.k[d] = k ? k[d] : 0.;
+  return s;
+}
+
+/* r = RHS(u, t) (src/main/evolve/evolve.c:182-232) */
+void orc_rhs(const OMesh *M, int numflux, const double *k, const double *u, double t, double *r)
+{
+  OSW sw = sw_from(numflux, k);
+  long nn = M->nnodes;
+  int maxnp = 0;
+  for (int e = 0; e < M->nelem; e++) if (elm_np(&M->e[e]) > maxnp) maxnp = elm_np(&M->e[e]);
+  double *divf = malloc(sizeof(double) * 4 * nn);
+  double *planes = malloc(sizeof(double) * NV * M->nface);
+  /* face planes of the state (MPIexchange_set_all_myln_localdata) */
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int e = 0; e < M->nelem; e++)
+    for (int f = 0; f < 6; f++)
+      for (int v = 0; v < NV; v++)
+        set_mysurf(M, &M->e[e], f, u, v, planes + v * M->nface + M->foff[6 * e + f]);
+#pragma omp parallel
+  {
+    double *work = malloc(sizeof(double) * 15 * maxnp);
+#pragma omp for schedule(dynamic, 8)
+    for (int e = 0; e < M->nelem; e++) {
+      const OElm *E = &M->e[e];
+      sw_vol_rhs(M, E, u, r, divf + 4 * E->off, work);
+    }
+#pragma omp for schedule(dynamic, 8)
+    for (int e = 0; e < M->nelem; e++) {
+      const OElm *E = &M->e[e];
+      sw_surf_rhs(M, &sw, E, t, u, planes, r, divf + 4 * E->off);
+    }
+    free(work);
+  }
+  free(planes);
+  free(divf);
+}
+
+/* ------------------------------------------------------------------ */
+/* Runge-Kutta steppers (src/main/evolve/RungeKutta.c:19-188,          */
+/* src/main/main/variables.c:831-1041)                                 */
+/* ------------------------------------------------------------------ */
+
+static void vladd(long N, double *r, double ca, const double *a, double cb, const double *b)
+{
+  if (ca == 0 && cb == 0) { for (long i = 0; i < N; i++) r[i] = 0; }
+  else if (ca == 0) { for (long i = 0; i < N; i++) r[i] = cb * b[i]; }
+  else if (cb == 0) { for (long i = 0; i < N; i++) r[i] = ca * a[i]; }
+  else { for (long i = 0; i < N; i++) r[i] = ca * a[i] + cb * b[i]; }
+}
+
+static void vladdto(long N, double *r, double ca, const double *a)
+{
+  if (ca == 0) return;
+  if (ca == 1) { for (long i = 0; i < N; i++) r[i] += a[i]; }
+  else if (ca == -1) { for (long i = 0; i < N; i++) r[i] -= a[i]; }
+  else { for (long i = 0; i < N; i++) r[i] += ca * a[i]; }
+}
+
+/* method: 0 RK4, 1 sspRK3, 2 Euler; u advanced in place; returns final t */
+double orc_evolve(const OMesh *M, int numflux, const double *k, int method, double dt,
+                  int nsteps, double t, double *u)
+{
+  long N = NV * M->nnodes;
+  double *up = malloc(sizeof(double) * N), *w = malloc(sizeof(double) * N);
+  double *r = malloc(sizeof(double) * N);
+  /* time = Te + te accumulated in long double (src/main/main/main.c:588-624) */
+  long double Te = t, te = 0.;
+  for (int s = 0; s < nsteps; s++) {
+    memcpy(up, u, sizeof(double) * N);
+    memcpy(w, u, sizeof(double) * N);
+    if (method == 0) {
+      orc_rhs(M, numflux, k, w, t, r);
+      memcpy(u, w, sizeof(double) * N);
+      vladdto(N, u, dt / 6., r);
+      vladd(N, w, 1., up, dt / 2., r);
+      orc_rhs(M, numflux, k, w, t + 0.5 * dt, r);
+      vladdto(N, u, dt / 3., r);
+      vladd(N, w, 1., up, dt / 2., r);
+      orc_rhs(M, numflux, k, w, t + 0.5 * dt, r);
+      vladdto(N, u, dt / 3., r);
+      vladd(N, w, 1., up, dt, r);
+      orc_rhs(M, numflux, k, w, t + dt, r);
+      vladdto(N, u, dt / 6., r);
+    } else if (method == 1) {
+      orc_rhs(M, numflux, k, w, t, r);
+      memcpy(u, w, sizeof(double) * N);
+      vladdto(N, u, dt / 6., r);
+      vladd(N, w, 1., up, dt, r);
+      orc_rhs(M, numflux, k, w, t + dt, r);
+      vladdto(N, u, dt / 6., r);
+      vladdto(N, w, dt, r);
+      vladd(N, w, 0.75, up, 0.25, w);
+      orc_rhs(M, numflux, k, w, t + 0.5 * dt, r);
+      vladdto(N, u, dt * 2. / 3., r);
+    } else {
+      orc_rhs(M, numflux, k, w, t, r);
+      memcpy(u, w, sizeof(double) * N);
+      vladdto(N, u, dt, r);
+    }
+    te += dt;
+    t = (double)(Te + te);
+  }
+  free(up); free(w); free(r);
+  return t;
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

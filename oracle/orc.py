@@ -1,0 +1,157 @@
+"""ctypes binding of the oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker / CPU baseline.  The product
+(nmesh_amd, libnmx.so) never imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_L = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        vp, i, l, d = C.c_void_p, C.c_int, C.c_long, C.c_double
+        sig = {
+            "orc_lgl": (None, [i, vp, vp]),
+            "orc_dt_matrix": (None, [i, vp]),
+            "orc_mesh_box": (vp, [i, i, d, vp]),
+            "orc_mesh_shell": (vp, [i, i, d, d]),
+            "orc_mesh_amr_shell": (vp, [i, i, d, d, d, i]),
+            "orc_mesh_set_n": (None, [vp, vp]),
+            "orc_mesh_free": (None, [vp]),
+            "orc_mesh_nelem": (i, [vp]),
+            "orc_mesh_nnodes": (l, [vp]),
+            "orc_mesh_npat": (i, [vp]),
+            "orc_mesh_elm_info": (None, [vp, i, vp]),
+            "orc_mesh_elm_bbox": (None, [vp, i, vp]),
+            "orc_mesh_elm_fnb": (i, [vp, i, i, vp, vp]),
+            "orc_mesh_pat_bound": (i, [vp, i, i]),
+            "orc_mesh_coords": (None, [vp, vp, vp]),
+            "orc_mesh_geom": (None, [vp, vp, vp, vp]),
+            "orc_mesh_dt": (d, [vp, d]),
+            "orc_mesh_elm_hmin": (d, [vp, i]),
+            "orc_set_profile": (None, [vp, vp, d, vp, vp]),
+            "orc_rhs": (None, [vp, i, vp, vp, d, vp]),
+            "orc_evolve": (d, [vp, i, vp, i, d, i, d, vp]),
+            "orc_num_threads": (i, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _L = L
+    return _L
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+NUMFLUX = {"upwind": 0, "LLF": 1}
+METHOD = {"RK4": 0, "sspRK3": 1, "Euler": 2}
+
+
+class OMesh:
+    """Oracle mesh built from nmesh parameter values."""
+
+    def __init__(self, kind, luni, n, dout=1.0, rin=2.0, rout=20.0, radius=10.0, levels=0):
+        L = lib()
+        if kind == "box":
+            self.h = L.orc_mesh_box(int(luni), int(n), float(dout), None)
+        elif kind == "shell" and levels == 0:
+            self.h = L.orc_mesh_shell(int(luni), int(n), float(rin), float(rout))
+        elif kind == "shell":
+            self.h = L.orc_mesh_amr_shell(int(luni), int(n), float(rin), float(rout),
+                                          float(radius), int(levels))
+        else:
+            raise ValueError(kind)
+
+    def __del__(self):
+        try:
+            lib().orc_mesh_free(self.h)
+        except Exception:
+            pass
+
+    @property
+    def nelem(self):
+        return lib().orc_mesh_nelem(self.h)
+
+    @property
+    def nnodes(self):
+        return lib().orc_mesh_nnodes(self.h)
+
+    def set_n(self, n):
+        n = np.ascontiguousarray(n, dtype=np.int32)
+        lib().orc_mesh_set_n(self.h, _p(n))
+
+    def elm_info(self, e):
+        a = np.zeros(22, dtype=np.int64)
+        lib().orc_mesh_elm_info(self.h, int(e), _p(a))
+        return a
+
+    def elm_bbox(self, e):
+        a = np.zeros(6)
+        lib().orc_mesh_elm_bbox(self.h, int(e), _p(a))
+        return a
+
+    def elm_fnb(self, e, f):
+        nb = np.zeros(8, dtype=np.int32)
+        nbf = np.zeros(8, dtype=np.int32)
+        k = lib().orc_mesh_elm_fnb(self.h, int(e), int(f), _p(nb), _p(nbf))
+        return nb[:k].copy(), nbf[:k].copy()
+
+    def pat_bound(self, p, f):
+        return lib().orc_mesh_pat_bound(self.h, int(p), int(f))
+
+    def coords(self):
+        X = np.zeros(3 * self.nnodes)
+        x = np.zeros(3 * self.nnodes)
+        lib().orc_mesh_coords(self.h, _p(X), _p(x))
+        return X.reshape(3, -1), x.reshape(3, -1)
+
+    def geometry(self):
+        nn = self.nnodes
+        g = np.zeros(9 * nn)
+        sg = np.zeros(3 * nn)
+        det = np.zeros(nn)
+        lib().orc_mesh_geom(self.h, _p(g), _p(sg), _p(det))
+        return g.reshape(9, -1), sg.reshape(3, -1), det
+
+    def dt(self, dtfac=0.25):
+        return lib().orc_mesh_dt(self.h, float(dtfac))
+
+    def profile(self, k, t):
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        u = np.zeros(5 * self.nnodes)
+        du = np.zeros(5 * self.nnodes)
+        lib().orc_set_profile(self.h, _p(k), float(t), _p(u), _p(du))
+        return u, du
+
+    def rhs(self, u, t, k, numflux="upwind"):
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        r = np.zeros_like(u)
+        lib().orc_rhs(self.h, NUMFLUX[numflux], _p(k), _p(u), float(t), _p(r))
+        return r
+
+    def evolve(self, u, dt, nsteps, t0, k, method="RK4", numflux="upwind"):
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        u = np.array(u, dtype=np.float64, copy=True)
+        t = lib().orc_evolve(self.h, NUMFLUX[numflux], _p(k), METHOD[method], float(dt),
+                             int(nsteps), float(t0), _p(u))
+        return u, t
