@@ -109,7 +109,25 @@ def main():
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--path", default="plain")
+    ap.add_argument("--graph", default="yes", help="capture RK steps in hipGraphs (yes|no)")
+    ap.add_argument("--geometry", default="analytic", help="dXdx recomputed (analytic) or streamed (stored)")
+    ap.add_argument("--xcd", default="auto", help="contiguous element ranges per XCD (yes|no|auto)")
+    ap.add_argument("--block", default="auto",
+                    help="nodes per thread: full (1), half (2), quarter (4) or auto (autotuned)")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="instead of the workload, run 10 device copies of MIB MiB with "
+                         "torch (counter calibration for rocprofv3 --pmc) and exit")
     args = ap.parse_args()
+    if args.copy_calib:
+        import torch
+        n = args.copy_calib * 1024 * 1024 // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_mib": args.copy_calib, "copies": 10}))
+        return
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the hot path is measured on one "
               "GPU's share of the mesh; multi-GPU runs are not part of this build.",
@@ -125,10 +143,18 @@ def main():
 
     m = nm.Mesh(c2_params(), device=0)
     m.set_par("nmx_path", args.path)
+    m.set_par("nmx_graph", args.graph)
+    m.set_par("nmx_geometry", args.geometry)
+    if args.xcd != "auto":
+        m.set_par("nmx_xcd", args.xcd)
+    if args.block != "auto":
+        m.set_par("nmx_block", args.block)
     nm.amr_setup_mesh(m)
     x = m.coords()
     m.upload(shell_initial_data(x))
     nn = m.nnodes
+    if args.block == "auto" and args.xcd == "auto":
+        nm.autotune(m)            # times tile size / XCD mapping on this mesh
     if args.warmup:
         nm.evolve_myln(m, args.warmup)
     m.synchronize()
@@ -148,8 +174,10 @@ def main():
     # algorithmic bytes per node-update, averaged over the 4 RK4 stages:
     # stage 1 reads u and writes u_acc, w (15 doubles), stages 2-3 read w,
     # u_acc, u_p and write u_acc, w (25), stage 4 reads w, u_acc and writes
-    # u_acc (15); plus 9 doubles of dXdx per node; face-neighbour re-reads count 0
-    bytes_per_nu = 8.0 * ((15 + 25 + 25 + 15) / 4.0 + 9.0)
+    # u_acc (15); plus 9 doubles of dXdx per node when the geometry is streamed
+    # (none when it is recomputed); face-neighbour re-reads count 0
+    g = 9.0 if args.geometry == "stored" else 0.0
+    bytes_per_nu = 8.0 * ((15 + 25 + 25 + 15) / 4.0 + g)
     achieved = bytes_per_nu * nn / avg_launch_s / 1e9
     traffic = load_traffic("traffic_dg_stage_shell_n8.json")
     out = {
@@ -171,7 +199,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "kernel": "dg_stage_kernel<8,true,512>",
+                     "kernel": "dg_stage_kernel<8,%d,%d>" % (2 if args.geometry == "analytic" else 1,
+                                                             {"full": 512, "half": 256, "quarter": 128}[m.par("nmx_block")]),
+                     "tile": m.par("nmx_block"), "xcd_map": m.par("nmx_xcd"),
                      "avg_launch_us": avg_launch_s * 1e6,
                      "bytes_per_node_update": bytes_per_nu},
     }

@@ -60,7 +60,9 @@ int nmx_synchronize(nmx_ctx *ctx);
  * amr_hrefine_sphere_levels, dt (auto|number), dtfac, evolve_method
  * (RK4|sspRK3|Euler), scalarwave1_k, scalarwave1_numflux (upwind|LLF),
  * scalarwave1_profile (sin), plus library knobs prefixed nmx_ (nmx_path:
- * plain|batched|perelement, nmx_graph: yes|no, nmx_geometry: stored|analytic). */
+ * plain|batched|perelement, nmx_graph: yes|no, nmx_graph_steps, nmx_geometry:
+ * stored|analytic, nmx_block: full|half|quarter, nmx_xcd: yes|no,
+ * nmx_autotune: yes|no). */
 int nmx_set_param(nmx_ctx *ctx, const char *name, const char *value);
 int nmx_get_param(nmx_ctx *ctx, const char *name, char *buf, int buflen);
 
@@ -101,6 +103,12 @@ int nmx_state_upload_elm(nmx_ctx *ctx, int reg, int elm, int var, const double *
 int nmx_state_download_elm(nmx_ctx *ctx, int reg, int elm, int var, double *d, int np);
 /* set u to scalarwave1's profile at time t (scalarwave1_set_profile, scalarwave1.c:574-642) */
 int nmx_state_set_profile(nmx_ctx *ctx, int reg, double t);
+
+/* Time the stage kernel variants (nodes per thread, XCD-aware element
+ * mapping) on the current mesh and keep the fastest; runs automatically
+ * before the first nmx_evolve when nmx_autotune = yes.  No counterpart in the
+ * reference (its element loop is an OpenMP for, nmesh_amr_loops.h:61-63). */
+int nmx_autotune(nmx_ctx *ctx);
 
 /* r = RHS(u, t) on the device (evolve_setrhs_mesh, evolve.c:182-232) */
 int nmx_setrhs(nmx_ctx *ctx, int reg_out, int reg_in, double t);

@@ -47,6 +47,7 @@ def _proto(lib):
         "nmx_state_download_elm": (i, [vp, i, i, i, vp, i]),
         "nmx_state_set_profile": (i, [vp, i, d]),
         "nmx_setrhs": (i, [vp, i, i, d]),
+        "nmx_autotune": (i, [vp]),
         "nmx_evolve": (i, [vp, i]),
         "nmx_state_device_ptr": (i, [vp, i, P(vp), P(l)]),
         "nmx_adjoint_rhs": (i, [vp, vp, vp, d]),

@@ -53,7 +53,10 @@ struct nmx_ctx {
   // device mirror
   double *dU = nullptr, *dA = nullptr, *dW0 = nullptr, *dW1 = nullptr, *dR = nullptr, *dP = nullptr;
   double *dXdx = nullptr, *dXbdX = nullptr, *dxyz = nullptr, *dD = nullptr, *dT = nullptr;
+  double *degeo = nullptr, *dxb = nullptr;
+  int *dedom = nullptr;
   int4 *dface = nullptr;
+  int geom = 0;          // 0 Cartesian, 1 stored dXdx, 2 recomputed CubedShell
   std::vector<long> eoff;   // node offset per element
   // graph cache
   hipGraph_t graph = nullptr;
@@ -65,6 +68,7 @@ struct nmx_ctx {
   int tcap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
+  bool tuned = false;
 };
 
 extern "C" const char *nmx_last_error(void) { return g_err.c_str(); }
@@ -93,6 +97,10 @@ static void set_defaults(nmx_ctx *c) {
       {"nmx_path", "plain"},
       {"nmx_graph", "yes"},
       {"nmx_graph_steps", "8"},
+      {"nmx_geometry", "analytic"},
+      {"nmx_xcd", "yes"},
+      {"nmx_block", "half"},
+      {"nmx_autotune", "yes"},
   };
 }
 
@@ -126,9 +134,10 @@ extern "C" int nmx_ctx_create(nmx_ctx **out, int device) {
 
 static void free_mesh_dev(nmx_ctx *c) {
   double **bufs[] = {&c->dU, &c->dA, &c->dW0, &c->dW1, &c->dR, &c->dP,
-                     &c->dXdx, &c->dXbdX, &c->dxyz, &c->dD, &c->dT};
+                     &c->dXdx, &c->dXbdX, &c->dxyz, &c->dD, &c->dT, &c->degeo, &c->dxb};
   for (auto b : bufs)
     if (*b) { (void)hipFree(*b); *b = nullptr; }
+  if (c->dedom) { (void)hipFree(c->dedom); c->dedom = nullptr; }
   if (c->dface) { (void)hipFree(c->dface); c->dface = nullptr; }
   if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
@@ -174,7 +183,14 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
     return fail(NMX_EUNSUP, "only scalarwave1_profile = sin is supported");
   if (k == "nmx_path" && v != "plain" && v != "perelement" && v != "batched")
     return fail(NMX_EINVAL, "nmx_path must be plain, perelement or batched");
+  if (k == "nmx_geometry" && v != "analytic" && v != "stored")
+    return fail(NMX_EINVAL, "nmx_geometry must be analytic or stored");
+  if (k == "nmx_geometry" && c->have_mesh)
+    return fail(NMX_ESTATE, "set nmx_geometry before amr_setup_mesh");
+  if (k == "nmx_block" && v != "full" && v != "half" && v != "quarter")
+    return fail(NMX_EINVAL, "nmx_block must be full, half or quarter");
   c->par[k] = v;
+  if (k == "nmx_block" || k == "nmx_xcd") c->tuned = true;   // explicit choice wins
   if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
   c->graph_steps = 0;
@@ -204,6 +220,7 @@ static int dev_alloc(T **p, size_t n) {
 // host -> device mirror of the mesh (element-block layout)
 static int upload_mesh(nmx_ctx *c) {
   free_mesh_dev(c);
+  c->tuned = false;
   Mesh &M = c->mesh;
   const long nn = M.nnodes;
   const size_t ne = M.e.size();
@@ -229,14 +246,25 @@ static int upload_mesh(nmx_ctx *c) {
   HIPCHK(hipMemset(c->dR, 0, c->nstate * sizeof(double)));
   HIPCHK(hipMemset(c->dP, 0, c->nstate * sizeof(double)));
   // geometry in element blocks
-  std::vector<double> h9(c->curv ? 9 * nn : 1), h3(3 * ne), hx(3 * nn);
+  c->geom = c->curv ? (P(c, "nmx_geometry") == "stored" ? 1 : 2) : 0;
+  if (c->geom == 2) {
+    for (auto &Pt : M.pat)
+      if (Pt.type == PAT_CUBEDSHELL && (Pt.xc[0] != 0 || Pt.xc[1] != 0 || Pt.xc[2] != 0))
+        c->geom = 1;   // recomputation assumes the shell centre at the origin
+  }
+  std::vector<double> h9(c->geom == 1 ? 9 * nn : 1), h3(3 * ne), hx(3 * nn), hg(8 * ne);
+  std::vector<int> hdom(ne);
   for (size_t i = 0; i < ne; ++i) {
     const Elm &E = M.e[i];
     long np = (long)E.n * E.n * E.n;
     for (int d = 0; d < 3; ++d) h3[3 * i + d] = 2. / (E.bbox[2 * d + 1] - E.bbox[2 * d]);
+    for (int q = 0; q < 6; ++q) hg[8 * i + q] = E.bbox[q];
+    hg[8 * i + 6] = M.pat[E.p].s0;
+    hg[8 * i + 7] = M.pat[E.p].s1;
+    hdom[i] = M.pat[E.p].dom;
     for (long q = 0; q < np; ++q) {
       long g = E.off + q;
-      if (c->curv)
+      if (c->geom == 1)
         for (int m = 0; m < 9; ++m) h9[9 * E.off + m * np + q] = M.dXdx[m * nn + g];
       for (int d = 0; d < 3; ++d) hx[3 * E.off + d * np + q] = M.x[d * nn + g];
     }
@@ -247,6 +275,9 @@ static int upload_mesh(nmx_ctx *c) {
   HIPCHK(hipMemcpy(c->dXdx, h9.data(), h9.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->dXbdX, h3.data(), h3.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->dxyz, hx.data(), hx.size() * 8, hipMemcpyHostToDevice));
+  if ((r = dev_alloc(&c->degeo, hg.size())) || (r = dev_alloc(&c->dedom, hdom.size()))) return r;
+  HIPCHK(hipMemcpy(c->degeo, hg.data(), hg.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dedom, hdom.data(), hdom.size() * sizeof(int), hipMemcpyHostToDevice));
   // face gather descriptors {nb, local base, s1, s2}
   std::vector<int4> hf(6 * ne);
   for (size_t i = 0; i < ne; ++i)
@@ -278,6 +309,8 @@ static int upload_mesh(nmx_ctx *c) {
   std::vector<double> hD(b.D, b.D + c->np1 * c->np1);
   if ((r = dev_alloc(&c->dD, hD.size()))) return r;
   HIPCHK(hipMemcpy(c->dD, hD.data(), hD.size() * 8, hipMemcpyHostToDevice));
+  if ((r = dev_alloc(&c->dxb, c->np1))) return r;
+  HIPCHK(hipMemcpy(c->dxb, b.x, c->np1 * 8, hipMemcpyHostToDevice));
   c->tcap = 4 * 64;
   if ((r = dev_alloc(&c->dT, c->tcap))) return r;
   if (!c->hT) HIPCHK(hipHostMalloc((void **)&c->hT, c->tcap * sizeof(double), hipHostMallocDefault));
@@ -511,6 +544,10 @@ static StageArgs base_args(nmx_ctx *c) {
   a.face = c->dface;
   a.D = c->dD;
   a.tdev = c->dT;
+  a.egeo = c->degeo;
+  a.edom = c->dedom;
+  a.xb = c->dxb;
+  a.xcd_map = (P(c, "nmx_xcd") == "yes") ? 1 : 0;
   const Basis &b = lgl_basis(c->np1);
   a.wend_inv = 1.0 / b.w[0];
   kvec(c, a.k);
@@ -524,16 +561,21 @@ static StageArgs base_args(nmx_ctx *c) {
 
 static int launch(nmx_ctx *c, StageArgs a) {
   bool perelm = (P(c, "nmx_path") == "perelement");
+  std::string blk = P(c, "nmx_block");
+  int nt = dg_stage_threads(c->np1, blk == "half" ? 2 : blk == "quarter" ? 4 : 1);
   if (perelm) {
     int ne = a.e1;
+    a.xcd_map = 0;
     for (int e = 0; e < ne; ++e) {
       a.e0 = e; a.e1 = e + 1;
-      hipError_t err = launch_dg_stage(c->np1, c->curv, a, 1, c->stream);
+      hipError_t err = launch_dg_stage(c->np1, c->geom, a, 1, nt, c->stream);
       if (err != hipSuccess) return fail(NMX_EHIP, std::string("stage launch: ") + hipGetErrorString(err));
     }
     return NMX_OK;
   }
-  hipError_t err = launch_dg_stage(c->np1, c->curv, a, a.e1 - a.e0, c->stream);
+  int nel = a.e1 - a.e0;
+  int grid = a.xcd_map ? 8 * ((nel + 7) / 8) : nel;
+  hipError_t err = launch_dg_stage(c->np1, c->geom, a, grid, nt, c->stream);
   if (err != hipSuccess) return fail(NMX_EHIP, std::string("stage launch: ") + hipGetErrorString(err));
   return NMX_OK;
 }
@@ -593,10 +635,63 @@ static void stage_times(int method, double t, double dt, double *ts) {
   else { ts[0] = t; ts[1] = ts[2] = ts[3] = 0; }
 }
 
+// Autotuner: time the stage kernel's tile size (nodes per thread) and the
+// XCD-aware element mapping on this mesh with a few RK steps, keep the
+// fastest.  The state and time are restored afterwards.
+extern "C" int nmx_autotune(nmx_ctx *c) {
+  NEED_DEV(c);
+  HIPCHK(hipSetDevice(c->device));
+  if (P(c, "nmx_path") == "perelement") { c->tuned = true; return NMX_OK; }
+  double *backup = nullptr;
+  int r;
+  if ((r = dev_alloc(&backup, c->nstate))) return r;
+  HIPCHK(hipMemcpyAsync(backup, c->dU, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream));
+  const int method = method_id(c);
+  stage_times(method, c->time, c->dt, c->hT);
+  HIPCHK(hipMemcpyAsync(c->dT, c->hT, 4 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const char *blocks[] = {"full", "half", "quarter"};
+  const char *xcds[] = {"yes", "no"};
+  std::string best_b = P(c, "nmx_block"), best_x = P(c, "nmx_xcd");
+  float best = 1e30f;
+  hipEvent_t t0, t1;
+  HIPCHK(hipEventCreate(&t0));
+  HIPCHK(hipEventCreate(&t1));
+  for (const char *b : blocks)
+    for (const char *x : xcds) {
+      c->par["nmx_block"] = b;
+      c->par["nmx_xcd"] = x;
+      if ((r = issue_step(c, method, c->dt, 0, false))) return r;   // warm-up
+      HIPCHK(hipEventRecord(t0, c->stream));
+      for (int s = 0; s < 4 && !r; ++s) r = issue_step(c, method, c->dt, 0, (s & 1) != 0);
+      if (r) return r;
+      HIPCHK(hipEventRecord(t1, c->stream));
+      HIPCHK(hipEventSynchronize(t1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+      if (ms < best) { best = ms; best_b = b; best_x = x; }
+    }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  c->par["nmx_block"] = best_b;
+  c->par["nmx_xcd"] = best_x;
+  HIPCHK(hipMemcpyAsync(c->dU, backup, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  (void)hipFree(backup);
+  if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+  c->graph_steps = 0;
+  c->tuned = true;
+  return NMX_OK;
+}
+
 extern "C" int nmx_evolve(nmx_ctx *c, int nsteps) {
   NEED_DEV(c);
   if (nsteps < 0) return fail(NMX_EINVAL, "nsteps < 0");
   HIPCHK(hipSetDevice(c->device));
+  if (!c->tuned && P(c, "nmx_autotune") == "yes" && nsteps > 0) {
+    int r = nmx_autotune(c);
+    if (r) return r;
+  }
   const int method = method_id(c);
   const double dt = c->dt;
   const bool use_graph = (P(c, "nmx_graph") == "yes") && P(c, "nmx_path") != "perelement";

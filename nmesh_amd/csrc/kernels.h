@@ -23,7 +23,11 @@ struct StageArgs {
   const int4 *face = nullptr;    // [e][6]: {nb elm (-1 none), local base, s1, s2}
   const double *D = nullptr;     // D[i*np + l]
   const double *tdev = nullptr;  // stage times on the device; t = tdev[tidx]
+  const double *egeo = nullptr;  // [e][8]: patch bbox of the element (6), s0, s1
+  const int *edom = nullptr;     // [e]: cubed-sphere domain of the element's patch
+  const double *xb = nullptr;    // LGL nodes on [-1,1]
   int tidx = 0;
+  int xcd_map = 0;               // 1: contiguous element ranges per XCD
   double cA = 0, cW = 0, wend_inv = 0;
   double k[3] = {0, 0, 0};
   double om = 0;
@@ -31,6 +35,8 @@ struct StageArgs {
   int wmode = 0, numflux = 0, profile = 0;
 };
 
-hipError_t launch_dg_stage(int np, bool curv, const StageArgs &a, int grid, hipStream_t s);
+// geom: 0 Cartesian, 1 stored dXdx, 2 CubedShell recomputed; nt from dg_stage_threads
+hipError_t launch_dg_stage(int np, int geom, const StageArgs &a, int grid, int nt, hipStream_t s);
+int dg_stage_threads(int np, int div);   // div: nodes per thread 1, 2 or 4
 
 }  // namespace nmx

@@ -203,6 +203,11 @@ def evolve_Euler_mesh(mesh, nsteps=1):
     _step(mesh, "Euler", nsteps)
 
 
+def autotune(mesh):
+    """Time the stage-kernel variants on this mesh and keep the fastest."""
+    check(lib().nmx_autotune(mesh.ctx))
+
+
 def evolve_myln(mesh, nsteps=1):
     """Step(s) with the method named by evolve_method (evolve.c:25-176)."""
     check(lib().nmx_evolve(mesh.ctx, int(nsteps)))

@@ -18,12 +18,14 @@ TOL = 1e-12
 pytestmark = pytest.mark.gpu
 
 
-def make(kind, luni, n, k=K_DEFAULT, numflux="upwind", path="plain", graph="yes", **kw):
+def make(kind, luni, n, k=K_DEFAULT, numflux="upwind", path="plain", graph="yes",
+         geometry="analytic", xcd="yes", block="full", **kw):
     pars = {
         "amr_mesh_type": "BoxMesh" if kind == "box" else "Shell",
         "amr_luni": luni, "amr_n0": n, "amr_n1": n, "amr_n2": n,
         "scalarwave1_k": "%.17g %.17g %.17g" % tuple(k),
         "scalarwave1_numflux": numflux, "nmx_path": path, "nmx_graph": graph,
+        "nmx_geometry": geometry, "nmx_xcd": xcd, "nmx_block": block,
     }
     if kind == "box":
         pars["amr_BoxMesh_dout"] = kw.get("dout", 8.0)
@@ -139,3 +141,20 @@ def test_config1_gaussian_100_steps():
         u, t = om.evolve(u, m.dt, 10, t, k, "RK4")
         nm.evolve_myln(m, 10)
         assert rel(m.download(), u) <= TOL, "block %d" % blk
+
+
+@pytest.mark.parametrize("geometry,xcd,block", [("stored", "yes", "full"), ("analytic", "no", "full"),
+                                                ("stored", "no", "half"), ("analytic", "yes", "half"),
+                                                ("analytic", "yes", "quarter")])
+def test_kernel_variants_match_oracle(geometry, xcd, block):
+    """Streamed and recomputed geometry, XCD mapping on/off and both tile
+    sizes give the same RHS and RK4 steps to the oracle tolerance."""
+    print("variant", geometry, xcd, block, flush=True)
+    m, om = make("shell", 1, 8, geometry=geometry, xcd=xcd, block=block)
+    u = noisy_state(om, K_DEFAULT, 0.2, seed=21)
+    m.upload(u)
+    assert rel(nm.evolve_setrhs_mesh(m, t=0.2), om.rhs(u, 0.2, K_DEFAULT)) <= TOL
+    m.time = 0.2
+    u_o, t = om.evolve(u, m.dt, 3, 0.2, K_DEFAULT, "RK4")
+    nm.evolve_myln(m, 3)
+    assert rel(m.download(), u_o) <= TOL
