@@ -7,12 +7,14 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <sstream>
 #include <string>
 #include <vector>
 
 #include "../../include/nmx.h"
+#include "amr.h"
 #include "kernels.h"
 #include "mesh.h"
 
@@ -57,6 +59,13 @@ struct nmx_ctx {
   int *dedom = nullptr;
   int4 *dface = nullptr;
   int geom = 0;          // 0 Cartesian, 1 stored dXdx, 2 recomputed CubedShell
+  // batched (order-bucketed) mirror
+  bool amr = false;
+  long *dsoff = nullptr;
+  int *dfaceB = nullptr, *derounds = nullptr, *dslot = nullptr;
+  double *dBtab = nullptr, *dDall = nullptr, *dxball = nullptr;
+  struct Bucket { int n; int count; int *dlist; long doff; long xoff; };
+  std::vector<Bucket> buckets;
   std::vector<long> eoff;   // node offset per element
   // graph cache
   hipGraph_t graph = nullptr;
@@ -138,6 +147,15 @@ static void free_mesh_dev(nmx_ctx *c) {
   for (auto b : bufs)
     if (*b) { (void)hipFree(*b); *b = nullptr; }
   if (c->dedom) { (void)hipFree(c->dedom); c->dedom = nullptr; }
+  {
+    void **ptrs[] = {(void **)&c->dsoff, (void **)&c->dfaceB, (void **)&c->derounds, (void **)&c->dslot,
+                     (void **)&c->dBtab, (void **)&c->dDall, (void **)&c->dxball};
+    for (auto q : ptrs)
+      if (*q) { (void)hipFree(*q); *q = nullptr; }
+    for (auto &b : c->buckets)
+      if (b.dlist) (void)hipFree(b.dlist);
+    c->buckets.clear();
+  }
   if (c->dface) { (void)hipFree(c->dface); c->dface = nullptr; }
   if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
@@ -217,6 +235,115 @@ static int dev_alloc(T **p, size_t n) {
   return NMX_OK;
 }
 
+template <class T>
+static int dev_upload(T **p, const std::vector<T> &h) {
+  int r = dev_alloc(p, h.size());
+  if (r) return r;
+  HIPCHK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return NMX_OK;
+}
+
+// Tables of the batched path: per-element state offsets, face descriptors,
+// interpolation slots with deduplicated 1D Lagrange rows, staging rounds,
+// order buckets.  Every gather a kernel will make is bounds-checked here.
+static int upload_amr_tables(nmx_ctx *c) {
+  Mesh &M = c->mesh;
+  const size_t ne = M.e.size();
+  std::vector<long> soff(ne);
+  for (size_t i = 0; i < ne; ++i) soff[i] = 5 * M.e[i].off;
+  std::vector<int> fd(48 * ne, 0), rounds(ne, 0), slots;
+  std::vector<double> btab;
+  std::map<std::vector<double>, int> bdedup;
+  auto brow = [&](const std::vector<double> &rows) {
+    auto it = bdedup.find(rows);
+    if (it != bdedup.end()) return it->second;
+    int o = (int)btab.size();
+    btab.insert(btab.end(), rows.begin(), rows.end());
+    bdedup[rows] = o;
+    return o;
+  };
+  for (size_t i = 0; i < ne; ++i) {
+    const Elm &E = M.e[i];
+    const int n = E.n;
+    int round = 0, used = 0;
+    for (int f = 0; f < 6; ++f) {
+      int *d = &fd[48 * i + 8 * f];
+      if (E.ajmode[f] == 0) { d[0] = 0; continue; }
+      if (E.ajmode[f] == 1 || E.ajmode[f] == 2) {
+        const FaceCopy &fc = M.fcopy[6 * i + f];
+        const int nb = E.fnb[f][0];
+        const Elm &N = M.e[nb];
+        if (N.n != n) return fail(NMX_EINVAL, "copy face between different orders");
+        d[0] = 1; d[1] = nb; d[2] = (int)(fc.nbase - N.off); d[3] = fc.s1; d[4] = fc.s2;
+        const long np3 = (long)n * n * n;
+        for (int cb = 0; cb < 2; ++cb)
+          for (int ca = 0; ca < 2; ++ca) {
+            long q = d[2] + (long)ca * (n - 1) * d[3] + (long)cb * (n - 1) * d[4];
+            if (q < 0 || q >= np3) return fail(NMX_EINVAL, "face gather table out of range");
+          }
+        continue;
+      }
+      const std::vector<InterpSlot> &sl = M.fint[6 * i + f];
+      if (sl.empty() || sl.size() > 4) return fail(NMX_EUNSUP, "face with no or more than 4 interpolation neighbours");
+      int need = 0;
+      for (const InterpSlot &q : sl) need += 4 * q.m * q.m;
+      if (need > kAmrStage) return fail(NMX_EUNSUP, "face interpolation exceeds the staging area");
+      if (used + need > kAmrStage) { ++round; used = 0; }
+      used += need;
+      d[0] = 2; d[5] = (int)sl.size(); d[6] = (int)(slots.size() / 12); d[7] = round;
+      for (const InterpSlot &q : sl) {
+        const Elm &N = M.e[q.nb];
+        const int m = q.m;
+        const long base = q.nbase - N.off;
+        const long m3 = (long)m * m * m;
+        for (int c2 = 0; c2 < 2; ++c2)
+          for (int c1 = 0; c1 < 2; ++c1) {
+            long idx = base + (long)c1 * (m - 1) * q.t1 + (long)c2 * (m - 1) * q.t2;
+            if (idx < 0 || idx >= m3) return fail(NMX_EINVAL, "interpolation plane out of range");
+          }
+        if (q.a0 < 0 || q.a1 >= n || q.b0 < 0 || q.b1 >= n || q.a0 > q.a1 || q.b0 > q.b1)
+          return fail(NMX_EINVAL, "interpolation coverage out of range");
+        if ((int)q.Ba.size() != n * m || (int)q.Bb.size() != n * m)
+          return fail(NMX_EINVAL, "interpolation rows have the wrong size");
+        int oa = brow(q.Ba), ob = brow(q.Bb);
+        int rec[12] = {q.nb, m, (int)base, q.t1, q.t2, q.swap, q.a0, q.a1, q.b0, q.b1, oa, ob};
+        slots.insert(slots.end(), rec, rec + 12);
+      }
+    }
+    rounds[i] = 0;
+    for (int f = 0; f < 6; ++f)
+      if (fd[48 * i + 8 * f] == 2) rounds[i] = std::max(rounds[i], fd[48 * i + 8 * f + 7] + 1);
+  }
+  if (slots.empty()) slots.assign(12, 0);
+  if (btab.empty()) btab.assign(1, 0.0);
+  int r;
+  if ((r = dev_upload(&c->dsoff, soff)) || (r = dev_upload(&c->dfaceB, fd)) ||
+      (r = dev_upload(&c->derounds, rounds)) || (r = dev_upload(&c->dslot, slots)) ||
+      (r = dev_upload(&c->dBtab, btab)))
+    return r;
+  // order buckets with their D matrices and nodes
+  std::vector<double> dall, xball;
+  for (int n = 2; n <= 10; ++n) {
+    std::vector<int> list;
+    for (size_t i = 0; i < ne; ++i)
+      if (M.e[i].n == n) list.push_back((int)i);
+    if (list.empty()) continue;
+    nmx_ctx::Bucket b;
+    b.n = n;
+    b.count = (int)list.size();
+    b.dlist = nullptr;
+    if ((r = dev_upload(&b.dlist, list))) return r;
+    const Basis &bs = lgl_basis(n);
+    b.doff = (long)dall.size();
+    dall.insert(dall.end(), bs.D, bs.D + n * n);
+    b.xoff = (long)xball.size();
+    xball.insert(xball.end(), bs.x, bs.x + n);
+    c->buckets.push_back(b);
+  }
+  if ((r = dev_upload(&c->dDall, dall)) || (r = dev_upload(&c->dxball, xball))) return r;
+  return NMX_OK;
+}
+
 // host -> device mirror of the mesh (element-block layout)
 static int upload_mesh(nmx_ctx *c) {
   free_mesh_dev(c);
@@ -230,10 +357,12 @@ static int upload_mesh(nmx_ctx *c) {
   c->curv = false;
   for (auto &Pt : M.pat)
     if (Pt.type == PAT_CUBEDSHELL) c->curv = true;
-  if (c->np1 == 0) return fail(NMX_EUNSUP, "mixed element orders need nmx_path = batched");
+  c->amr = (c->np1 == 0) || P(c, "nmx_path") == "batched";
   for (size_t i = 0; i < ne; ++i)
     for (int f = 0; f < 6; ++f)
-      if (M.e[i].ajmode[f] == 3) return fail(NMX_EUNSUP, "non-conforming faces need nmx_path = batched");
+      if (M.e[i].ajmode[f] == 3) c->amr = true;
+  for (auto &E : M.e)
+    if (E.n > 10) return fail(NMX_EUNSUP, "the device kernels take at most 10 points per direction");
   c->nstate = 5L * nn;
   c->eoff.resize(ne);
   for (size_t i = 0; i < ne; ++i) c->eoff[i] = M.e[i].off;
@@ -278,6 +407,10 @@ static int upload_mesh(nmx_ctx *c) {
   if ((r = dev_alloc(&c->degeo, hg.size())) || (r = dev_alloc(&c->dedom, hdom.size()))) return r;
   HIPCHK(hipMemcpy(c->degeo, hg.data(), hg.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->dedom, hdom.data(), hdom.size() * sizeof(int), hipMemcpyHostToDevice));
+  c->tcap = 4 * 64;
+  if ((r = dev_alloc(&c->dT, c->tcap))) return r;
+  if (!c->hT) HIPCHK(hipHostMalloc((void **)&c->hT, c->tcap * sizeof(double), hipHostMallocDefault));
+  if (c->amr) return upload_amr_tables(c);
   // face gather descriptors {nb, local base, s1, s2}
   std::vector<int4> hf(6 * ne);
   for (size_t i = 0; i < ne; ++i)
@@ -311,9 +444,6 @@ static int upload_mesh(nmx_ctx *c) {
   HIPCHK(hipMemcpy(c->dD, hD.data(), hD.size() * 8, hipMemcpyHostToDevice));
   if ((r = dev_alloc(&c->dxb, c->np1))) return r;
   HIPCHK(hipMemcpy(c->dxb, b.x, c->np1 * 8, hipMemcpyHostToDevice));
-  c->tcap = 4 * 64;
-  if ((r = dev_alloc(&c->dT, c->tcap))) return r;
-  if (!c->hT) HIPCHK(hipHostMalloc((void **)&c->hT, c->tcap * sizeof(double), hipHostMallocDefault));
   return NMX_OK;
 }
 
@@ -342,7 +472,6 @@ extern "C" int nmx_mesh_build(nmx_ctx *c) {
   if (P(c, "dt") == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"));
   else c->dt = Pd(c, "dt");
   if (c->device < 0) return NMX_OK;                    // host-only context
-  if (P(c, "nmx_path") == "batched") return NMX_OK;   // device mirror built lazily
   return upload_mesh(c);
 }
 
@@ -354,7 +483,6 @@ extern "C" int nmx_mesh_set_elm_n(nmx_ctx *c, const int *n, int nelem) {
   if (!err.empty()) return fail(NMX_EINVAL, err);
   if (P(c, "dt") == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"));
   if (c->device < 0) return NMX_OK;
-  if (P(c, "nmx_path") == "batched") return NMX_OK;
   return upload_mesh(c);
 }
 
@@ -548,8 +676,7 @@ static StageArgs base_args(nmx_ctx *c) {
   a.edom = c->dedom;
   a.xb = c->dxb;
   a.xcd_map = (P(c, "nmx_xcd") == "yes") ? 1 : 0;
-  const Basis &b = lgl_basis(c->np1);
-  a.wend_inv = 1.0 / b.w[0];
+  a.wend_inv = c->np1 ? 1.0 / lgl_basis(c->np1).w[0] : 0.0;
   kvec(c, a.k);
   a.om = std::sqrt(a.k[0] * a.k[0] + a.k[1] * a.k[1] + a.k[2] * a.k[2]);
   a.profile = (a.k[0] != 0 || a.k[1] != 0 || a.k[2] != 0);
@@ -559,7 +686,42 @@ static StageArgs base_args(nmx_ctx *c) {
   return a;
 }
 
+static int launch_amr(nmx_ctx *c, const StageArgs &sa) {
+  const bool perelm = (P(c, "nmx_path") == "perelement");
+  AmrArgs a;
+  a.win = sa.win; a.U = sa.U; a.Ain = sa.Ain; a.Aout = sa.Aout; a.Wout = sa.Wout; a.R = sa.R;
+  a.soff = c->dsoff; a.face = c->dfaceB; a.erounds = c->derounds; a.slot = c->dslot;
+  a.Btab = c->dBtab; a.dXbdX = c->dXbdX; a.egeo = c->degeo; a.edom = c->dedom;
+  a.dXdx = c->dXdx; a.xyz = c->dxyz; a.tdev = sa.tdev; a.tidx = sa.tidx;
+  a.cA = sa.cA; a.cW = sa.cW;
+  for (int d = 0; d < 3; ++d) a.k[d] = sa.k[d];
+  a.om = sa.om; a.wmode = sa.wmode; a.numflux = sa.numflux; a.profile = sa.profile;
+  for (const auto &b : c->buckets) {
+    a.D = c->dDall + b.doff;
+    a.xb = c->dxball + b.xoff;
+    a.wend_inv = 1.0 / lgl_basis(b.n).w[0];
+    if (perelm) {
+      a.xcd_map = 0;
+      a.nlist = 1;
+      for (int q = 0; q < b.count; ++q) {
+        a.elist = b.dlist + q;
+        hipError_t err = launch_dg_amr(b.n, c->geom, a, 1, c->stream);
+        if (err != hipSuccess) return fail(NMX_EHIP, std::string("stage launch: ") + hipGetErrorString(err));
+      }
+      continue;
+    }
+    a.elist = b.dlist;
+    a.nlist = b.count;
+    a.xcd_map = sa.xcd_map;
+    int grid = a.xcd_map ? 8 * ((b.count + 7) / 8) : b.count;
+    hipError_t err = launch_dg_amr(b.n, c->geom, a, grid, c->stream);
+    if (err != hipSuccess) return fail(NMX_EHIP, std::string("stage launch: ") + hipGetErrorString(err));
+  }
+  return NMX_OK;
+}
+
 static int launch(nmx_ctx *c, StageArgs a) {
+  if (c->amr) return launch_amr(c, a);
   bool perelm = (P(c, "nmx_path") == "perelement");
   std::string blk = P(c, "nmx_block");
   int nt = dg_stage_threads(c->np1, blk == "half" ? 2 : blk == "quarter" ? 4 : 1);

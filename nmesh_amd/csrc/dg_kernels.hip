@@ -23,76 +23,16 @@
 // +v*np, node ijk = i + n*(j + n*k) inside.
 #include <hip/hip_runtime.h>
 
+#include "dg_common.h"
 #include "kernels.h"
 
 namespace nmx {
-
-// upwind: F = fnum - f_inner for the 4 flux-carrying fields.  With A = f.n
-// Jacobian [[0,-n^T],[-n,0]], A+/- = (A +- |A|)/2 gives
-// F_pi = (dpi + n.dc)/2 and F_c = n F_pi with d = inner - adjacent.
-// LLF (amax = 1): F_pi = (dpi + n.dc)/2, F_c = (dc + n dpi)/2.
-__device__ __forceinline__ void face_flux(int numflux, const double n[3], double dpi,
-                                          const double dc[3], double F[4]) {
-  double ndc = n[0] * dc[0] + n[1] * dc[1] + n[2] * dc[2];
-  double Fp = 0.5 * (dpi + ndc);
-  F[0] = Fp;
-  if (numflux == 0) {
-    F[1] = n[0] * Fp; F[2] = n[1] * Fp; F[3] = n[2] * Fp;
-  } else {
-    F[1] = 0.5 * (dc[0] + n[0] * dpi);
-    F[2] = 0.5 * (dc[1] + n[1] * dpi);
-    F[3] = 0.5 * (dc[2] + n[2] * dpi);
-  }
-}
-
-// plane wave of scalarwave1_set_profile_pt (scalarwave1.c:503-524)
-__device__ __forceinline__ void profile(const StageArgs &a, double x, double y, double z,
-                                        double t, double u[4]) {
-  double arg = a.k[0] * x + a.k[1] * y + a.k[2] * z - a.om * t;
-  double s, c;
-  sincos(arg, &s, &c);
-  (void)s;
-  u[0] = -a.om * c; u[1] = a.k[0] * c; u[2] = a.k[1] * c; u[3] = a.k[2] * c;
-}
-
-// dX/dx of a CubedShell (constant radii s0, s1, centre 0) at patch point X
-// (lam, A, B); the same expressions as the host geometry.  Written with
-// compile-time component indices only (runtime-indexed private arrays would
-// live in scratch memory).
-__device__ __forceinline__ void shell_dXdx(int dom, double s0, double s1, const double X[3],
-                                           double M[3][3]) {
-  const double lam = X[0], A = X[1], B = X[2];
-  const double pm = (dom & 1) ? 1.0 : -1.0;
-  const double s = sqrt(1.0 + A * A + B * B);
-  const double a0 = pm * s0 / s, a1 = pm * s1 / s;
-  const double r = (a1 - a0) * lam + a0;
-  const int dir = dom >> 1;
-  const int ia = (dir == 1) ? 0 : 1;      // axis scaled by A
-  const double ra = A * r, rb = B * r;
-  double rv[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) rv[d] = (d == dir) ? r : ((d == ia) ? ra : rb);
-  const double rc2 = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
-  const double ooda = 1.0 / (a1 - a0);
-  const double oor = 1.0 / r, rr2 = r * r;
-  const double mA = -ra / rr2, mB = -rb / rr2;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const bool isr = (d == dir), isa = (d == ia);
-    const double md = isr ? (oor - r / rc2) : (-rv[d] / rc2);
-    const double da0 = a0 * md, da1 = a1 * md;
-    M[0][d] = ((isr ? 1.0 : 0.0) - da0 - (da1 - da0) * lam) * ooda;
-    M[1][d] = isr ? mA : (isa ? oor : 0.0);
-    M[2][d] = isr ? mB : ((!isa) ? oor : 0.0);
-  }
-}
 
 template <int NP, int GEOM, int NT>
 __global__ __launch_bounds__(NT, NMX_DG_WAVES) void dg_stage_kernel(StageArgs a) {
   constexpr int NP2 = NP * NP;
   constexpr int NP3 = NP * NP * NP;
   constexpr int NC = (NP3 + NT - 1) / NT;     // nodes per thread
-  constexpr bool CURV = (GEOM != 0);
   __shared__ double s_u[4][NP3];        // own pi, cx, cy, cz
   __shared__ double s_f[4][6][NP2];     // adjacent values (ajsurf) on the 6 faces
   __shared__ double s_D[NP2];
@@ -151,7 +91,7 @@ __global__ __launch_bounds__(NT, NMX_DG_WAVES) void dg_stage_kernel(StageArgs a)
         const int mynode = ii + NP * (jj + NP * kk);
         const double *xp = a.xyz + 3L * NP3 * e + mynode;
         double ua[4];
-        profile(a, xp[0], xp[NP3], xp[2 * NP3], a.tdev[a.tidx], ua);
+        plane_wave(a.k, a.om, xp[0], xp[NP3], xp[2 * NP3], a.tdev[a.tidx], ua);
 #pragma unroll
         for (int v = 0; v < 4; ++v) s_f[v][f][q] = ua[v];
       } else {
@@ -171,132 +111,23 @@ __global__ __launch_bounds__(NT, NMX_DG_WAVES) void dg_stage_kernel(StageArgs a)
     __syncthreads();
 
     // ---- phase 2: compute from LDS and registers, node by node ----
+    NodeOut o;
+    o.win = a.win; o.U = a.U; o.Ain = a.Ain; o.Aout = a.Aout; o.Wout = a.Wout; o.R = a.R;
+    o.cA = a.cA; o.cW = a.cW; o.wend_inv = a.wend_inv; o.wmode = a.wmode; o.numflux = a.numflux;
+    ElmGeo g;
+    g.dXbdX[0] = dXbdX[0]; g.dXbdX[1] = dXbdX[1]; g.dXbdX[2] = dXbdX[2];
+    g.dXdx = a.dXdx + 9L * NP3 * e;
+    if (GEOM == 2) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) g.bb[q] = a.egeo[8 * e + q];
+      g.s0 = a.egeo[8 * e + 6];
+      g.s1 = a.egeo[8 * e + 7];
+      g.dom = a.edom[e];
+    }
 #pragma unroll 1
     for (int c = 0; c < NC; ++c) {
       const int node = tid + c * NT;
-      if (node >= NP3) continue;
-      const int k = node / NP2;
-      const int j = (node / NP) % NP;
-      const int i = node % NP;
-      const long gi = ebase + node;
-
-      double M[3][3];
-      if (GEOM == 1) {
-        const double *g = a.dXdx + 9L * NP3 * e + node;
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-#pragma unroll
-          for (int q = 0; q < 3; ++q) M[m][q] = g[(3 * m + q) * NP3];
-      } else if (GEOM == 2) {
-        const int I[3] = {i, j, k};
-        double X[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const double lo = a.egeo[8 * e + 2 * d], hi = a.egeo[8 * e + 2 * d + 1];
-          X[d] = 0.5 * ((hi - lo) * s_xb[I[d]] + (hi + lo));
-        }
-        shell_dXdx(a.edom[e], a.egeo[8 * e + 6], a.egeo[8 * e + 7], X, M);
-      }
-
-      // tensor-product derivative along one direction m at a time, folded at
-      // once into the Cartesian partials: d_q f = sum_m dXdx[m][q] dXbdX[m] D_m f
-      // (r_pi = d_q c_q, r_cq = d_q pi)
-      double rpi = 0.0, rc[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        const int im = (m == 0) ? i : (m == 1) ? j : k;
-        const int stride = (m == 0) ? 1 : (m == 1) ? NP : NP2;
-        const int base = node - im * stride;
-        double dp = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
-#pragma unroll
-        for (int l = 0; l < NP; ++l) {
-          const double Dl = s_D[im * NP + l];
-          const int src = base + l * stride;
-          dp += Dl * s_u[0][src];
-          if (CURV) {
-            d0 += Dl * s_u[1][src];
-            d1 += Dl * s_u[2][src];
-            d2 += Dl * s_u[3][src];
-          } else {
-            d0 += Dl * s_u[1 + m][src];
-          }
-        }
-        if (CURV) {
-          const double sp = dXbdX[m] * dp;
-          rpi += M[m][0] * (dXbdX[m] * d0) + M[m][1] * (dXbdX[m] * d1) + M[m][2] * (dXbdX[m] * d2);
-          rc[0] += M[m][0] * sp;
-          rc[1] += M[m][1] * sp;
-          rc[2] += M[m][2] * sp;
-        } else {
-          rpi += dXbdX[m] * d0;
-          rc[m] = dXbdX[m] * dp;
-        }
-      }
-
-      // numerical flux and lift on the faces this node lies on
-      const int I[3] = {i, j, k};
-#pragma unroll
-      for (int dir = 0; dir < 3; ++dir) {
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-          if (I[dir] != side * (NP - 1)) continue;
-          const int f = 2 * dir + side;
-          const double sig = side ? 1.0 : -1.0;
-          double nrm[3], scale;
-          if (CURV) {
-            double s2 = M[dir][0] * M[dir][0] + M[dir][1] * M[dir][1] + M[dir][2] * M[dir][2];
-            double s = sqrt(s2);
-            double smag = sig * s;
-            nrm[0] = M[dir][0] / smag; nrm[1] = M[dir][1] / smag; nrm[2] = M[dir][2] / smag;
-            scale = dXbdX[dir] * s * a.wend_inv;
-          } else {
-            nrm[0] = nrm[1] = nrm[2] = 0.0;
-            nrm[dir] = sig;
-            scale = dXbdX[dir] * a.wend_inv;
-          }
-          const int ia = (dir == 0) ? j : i;          // index along Dir1_norm(dir)
-          const int ib = (dir == 2) ? j : k;          // index along Dir2_norm(dir)
-          const int q = ia + NP * ib;
-          const double dcv[3] = {s_u[1][node] - s_f[1][f][q], s_u[2][node] - s_f[2][f][q],
-                                 s_u[3][node] - s_f[3][f][q]};
-          double F[4];
-          face_flux(a.numflux, nrm, s_u[0][node] - s_f[0][f][q], dcv, F);
-          rpi -= F[0] * scale;
-          rc[0] -= F[1] * scale;
-          rc[1] -= F[2] * scale;
-          rc[2] -= F[3] * scale;
-        }
-      }
-
-      const double r[5] = {rpi, rc[0], rc[1], rc[2], s_u[0][node]};
-      double Ubuf[5], Abuf[5];
-      if (a.wmode) {
-#pragma unroll
-        for (int v = 0; v < 5; ++v) Ubuf[v] = a.U[gi + v * NP3];
-      }
-      if (a.Aout) {
-        const double *bas = a.Ain ? a.Ain : a.win;
-#pragma unroll
-        for (int v = 0; v < 5; ++v) Abuf[v] = bas[gi + v * NP3];
-      }
-      if (a.R) {
-#pragma unroll
-        for (int v = 0; v < 5; ++v) a.R[gi + v * NP3] = r[v];
-      }
-      if (a.Aout) {
-#pragma unroll
-        for (int v = 0; v < 5; ++v) a.Aout[gi + v * NP3] = Abuf[v] + a.cA * r[v];
-      }
-      if (a.wmode == 1) {
-#pragma unroll
-        for (int v = 0; v < 5; ++v) a.Wout[gi + v * NP3] = Ubuf[v] + a.cW * r[v];
-      } else if (a.wmode == 2) {
-        // sspRK3: win's own values; phi is not staged in LDS
-        const double wv[5] = {s_u[0][node], s_u[1][node], s_u[2][node], s_u[3][node],
-                              a.win[gi + 4 * NP3]};
-#pragma unroll
-        for (int v = 0; v < 5; ++v) a.Wout[gi + v * NP3] = 0.75 * Ubuf[v] + 0.25 * (wv[v] + a.cW * r[v]);
-      }
+      if (node < NP3) node_update<NP, GEOM>(o, g, s_u, s_f, s_D, s_xb, ebase, node);
     }
   }
 }
