@@ -4,16 +4,27 @@
 
 namespace nmx {
 
-constexpr int kAmrStage = 1600;   // doubles of LDS for staged neighbour planes per round
+// face interpolation job, kJobInts ints per (face, neighbour slot), the
+// slots of one face consecutive:
+//   [0] n (my points per direction), [1] m (neighbour's), [2] offset of the
+//   face's values in the face buffer ([v][n*n]), [3] slots of this face,
+//   [4] base, [5] t1, [6] t2 (neighbour plane node = base + q1*t1 + q2*t2
+//   inside the neighbour), [7] swap, [8..11] a0,a1,b0,b1 covered point
+//   ranges, [12] offset of the rows for my a, [13] for my b, in Btab (rows of
+//   m doubles), [14..15] the neighbour's first state double (a long)
+constexpr int kJobInts = 16;
+// LDS of the face interpolation kernel per face, in doubles: first-pass
+// results (4 * covered rows * m per slot) and the rows of each pass (covered
+// rows or columns * m per slot); covered rows and columns of a slot <= kMaxFaceN.
+// Single-neighbour faces (T1, R1) and faces with up to 4 neighbours (T4, R4).
+constexpr int kFaceT1 = 400, kFaceR1 = 100;
+constexpr int kFaceT4 = 800, kFaceR4 = 400;
+constexpr int kMaxFaceN = 10;
 
 // face descriptor, 8 ints per element face:
 //   [0] kind: 0 outer boundary, 1 conforming copy, 2 interpolation
 //   [1] nb, [2] base node in nb, [3] s1, [4] s2   (kind 1: nb node = base + a*s1 + b*s2)
-//   [5] nslot, [6] first slot, [7] staging round  (kind 2)
-// slot, 12 ints:
-//   [0] nb, [1] m (nb points per direction), [2] base, [3] t1, [4] t2 (nb plane node =
-//   base + q1*t1 + q2*t2), [5] swap, [6..9] a0,a1,b0,b1 covered point ranges,
-//   [10] offset of the rows for my a, [11] for my b, in Btab (rows of m doubles)
+//   kind 2: [2] offset of the face's values in the face buffer ([v][n*n])
 struct AmrArgs {
   const double *win = nullptr, *U = nullptr, *Ain = nullptr;
   double *Aout = nullptr, *Wout = nullptr, *R = nullptr;
@@ -21,9 +32,7 @@ struct AmrArgs {
   int nlist = 0;
   const long *soff = nullptr;     // [e] first state double of element e (5 * node offset)
   const int *face = nullptr;      // [e][6][8]
-  const int *erounds = nullptr;   // [e] staging rounds
-  const int *slot = nullptr;      // [s][12]
-  const double *Btab = nullptr;
+  const double *aj = nullptr;     // interpolated face values
   const double *dXbdX = nullptr;  // [e][3]
   const double *egeo = nullptr;   // [e][8]
   const int *edom = nullptr;      // [e]
@@ -39,6 +48,17 @@ struct AmrArgs {
   int wmode = 0, numflux = 0, profile = 0, xcd_map = 0;
 };
 
-hipError_t launch_dg_amr(int np, int geom, const AmrArgs &a, int grid, hipStream_t s);
+struct FaceArgs {
+  const double *win = nullptr;
+  const int *jobs = nullptr;      // [job][kJobInts]
+  const int *first = nullptr;     // [face] first job, or null: face i is job i
+  int nfaces = 0;
+  const double *Btab = nullptr;
+  double *aj = nullptr;
+};
+
+// div: nodes per thread (1, 2 or 4)
+hipError_t launch_dg_amr(int np, int geom, const AmrArgs &a, int grid, int div, hipStream_t s);
+hipError_t launch_face_interp(const FaceArgs &a, bool multi, hipStream_t s);
 
 }  // namespace nmx

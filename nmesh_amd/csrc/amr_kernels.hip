@@ -1,18 +1,16 @@
 // amr_kernels.hip -- batched DG RHS + RK stage for meshes with mixed
 // refinement levels and mixed element orders.
 //
-// Elements are grouped into order buckets; one launch per bucket runs the
-// kernel instantiated for that order over the bucket's element list (all
-// buckets of a stage are captured in one hipGraph per RK step).  Per element:
-//   phase 1: own fields and conforming neighbour faces (aliased or oriented
-//            copies, surface.c:825-1072) go to LDS; for non-conforming or
-//            mixed-order faces the neighbours' face planes are staged in LDS;
-//   phase 1b: those faces get ajsurf by 2D tensor Lagrange interpolation of
-//            each neighbour's plane at this element's face points, averaged
-//            where several neighbours hold a point (surface.c:1081-1337,
-//            interpolate.c:296-363); the 1D Lagrange rows come from a small
-//            deduplicated table;
-//   phase 2: the per-node RHS and stage update shared with the plain kernel.
+// Each RK stage is two steps:
+//   1. face_interp_kernel: one workgroup per non-conforming or mixed-order
+//      face computes ajsurf (the neighbours' values at this face's points)
+//      by 2D tensor Lagrange interpolation of each neighbour's face plane,
+//      averaged where several neighbours hold a point (surface.c:1081-1337,
+//      interpolate.c:296-363), into a compact face buffer;
+//   2. dg_amr_kernel, one launch per order bucket (buckets run concurrently):
+//      own fields to LDS, conforming faces copied from the neighbour
+//      (surface.c:825-1072), interpolated faces read from the face buffer,
+//      then the per-node RHS and stage update shared with the plain kernel.
 #include <hip/hip_runtime.h>
 
 #include "amr.h"
@@ -20,6 +18,137 @@
 
 namespace nmx {
 
+// ---------------------------------------------------------------------------
+// face interpolation, one wave per face:
+//   pass 1: T[v][x1][q2] = sum_q1 P[v](q1, q2) R1[x1][q1]; a lane owns one
+//           (slot, v, q2) line of the neighbour plane, requests all of it
+//           from L2 at once and keeps the x1 sums in registers;
+//   pass 2: V[v][a][b] = sum_q2 T[v][x1][q2] R2[x2][q2], (x1, x2) = (a, b),
+//           or (b, a) when the neighbour's first in-plane axis runs along my
+//           b; averaged over the neighbours that hold the point.
+// TC, RC: LDS capacity (doubles) for T and for the rows of one pass.
+// ---------------------------------------------------------------------------
+constexpr int kFaceThreads = 64;
+
+struct JobView {
+  int n, m, ajoff, nslot, base, t1, t2, sw, a0, a1, b0, b1, offA, offB;
+  long soff;
+  __device__ JobView(const int *j) {
+    n = j[0]; m = j[1]; ajoff = j[2]; nslot = j[3]; base = j[4]; t1 = j[5]; t2 = j[6];
+    sw = j[7]; a0 = j[8]; a1 = j[9]; b0 = j[10]; b1 = j[11]; offA = j[12]; offB = j[13];
+    soff = *reinterpret_cast<const long *>(j + 14);
+  }
+  __device__ int x1lo() const { return sw ? b0 : a0; }
+  __device__ int nx1() const { return (sw ? b1 - b0 : a1 - a0) + 1; }
+  __device__ int x2lo() const { return sw ? a0 : b0; }
+  __device__ int nx2() const { return (sw ? a1 - a0 : b1 - b0) + 1; }
+};
+
+template <int TC, int RC>
+__global__ __launch_bounds__(kFaceThreads) void face_interp_kernel(FaceArgs a) {
+  __shared__ double s_T[TC];      // [slot][v][x1][m]
+  __shared__ double s_R1[RC];     // [slot][x1][m]
+  __shared__ double s_R2[RC];     // [slot][x2][m]
+  const int tid = threadIdx.x;
+  // contiguous face ranges per XCD (faces are listed in element order, so
+  // neighbour planes gathered by nearby faces share that XCD's L2)
+  const int xcd = blockIdx.x & 7, g8 = gridDim.x >> 3;
+  const int chunk = (a.nfaces + 7) >> 3;
+  const int f_hi = min((xcd + 1) * chunk, a.nfaces);
+  for (int fi = xcd * chunk + (blockIdx.x >> 3); fi < f_hi; fi += g8) {
+    const int *job0 = a.jobs + kJobInts * (a.first ? a.first[fi] : fi);
+    const int n = job0[0], ns = job0[3], ajoff = job0[2];
+    const int n2 = n * n;
+    __syncthreads();
+    // rows of both passes to LDS
+    int ro1 = 0, ro2 = 0, total = 0;
+    for (int si = 0; si < ns; ++si) {
+      const JobView J(job0 + kJobInts * si);
+      const double *R1 = a.Btab + (J.sw ? J.offB : J.offA) + J.x1lo() * J.m;
+      const double *R2 = a.Btab + (J.sw ? J.offA : J.offB) + J.x2lo() * J.m;
+      for (int idx = tid; idx < J.nx1() * J.m; idx += kFaceThreads) s_R1[ro1 + idx] = R1[idx];
+      for (int idx = tid; idx < J.nx2() * J.m; idx += kFaceThreads) s_R2[ro2 + idx] = R2[idx];
+      ro1 += J.nx1() * J.m;
+      ro2 += J.nx2() * J.m;
+      total += 4 * J.m;
+    }
+    // pass 1 over the (slot, v, q2) lines of all slots
+    for (int idx0 = 0; idx0 < total; idx0 += kFaceThreads) {
+      const int idx = idx0 + tid;
+      const bool act = idx < total;
+      int si = 0, rem = act ? idx : 0, to = 0, r1 = 0;
+      for (;;) {
+        const JobView J(job0 + kJobInts * si);
+        if (rem < 4 * J.m || si == ns - 1) break;
+        rem -= 4 * J.m;
+        to += 4 * J.nx1() * J.m;
+        r1 += J.nx1() * J.m;
+        ++si;
+      }
+      const JobView J(job0 + kJobInts * si);
+      const int m = J.m, nx1 = J.nx1();
+      const int v = rem / m, q2 = rem % m;
+      const double *P = a.win + J.soff + J.base + (long)v * m * m * m + q2 * J.t2;
+      double p[kMaxFaceN];
+#pragma unroll
+      for (int q1 = 0; q1 < kMaxFaceN; ++q1) p[q1] = (act && q1 < m) ? P[q1 * J.t1] : 0.0;
+      __syncthreads();   // rows in LDS
+      if (act) {
+        double *T = s_T + to + v * nx1 * m + q2;
+        const double *R1 = s_R1 + r1;
+#pragma unroll
+        for (int x = 0; x < kMaxFaceN; ++x) {
+          if (x < nx1) {
+            double t = 0.0;
+#pragma unroll
+            for (int q1 = 0; q1 < kMaxFaceN; ++q1)
+              if (q1 < m) t += p[q1] * R1[x * m + q1];
+            T[x * m] = t;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // pass 2 and averaging
+    for (int idx = tid; idx < 4 * n2; idx += kFaceThreads) {
+      const int v = idx / n2, q = idx % n2;
+      const int fa = q % n, fb = q / n;
+      double sum = 0.0;
+      int cnt = 0, to = 0, ro = 0;
+      for (int si = 0; si < ns; ++si) {
+        const JobView J(job0 + kJobInts * si);
+        const int m = J.m, nx1 = J.nx1();
+        if (fa >= J.a0 && fa <= J.a1 && fb >= J.b0 && fb <= J.b1) {
+          const int x1 = (J.sw ? fb : fa) - J.x1lo(), x2 = (J.sw ? fa : fb) - J.x2lo();
+          const double *T = s_T + to + (v * nx1 + x1) * m;
+          const double *row = s_R2 + ro + x2 * m;
+          double sv = 0.0;
+          for (int q2 = 0; q2 < m; ++q2) sv += T[q2] * row[q2];
+          sum += sv;
+          ++cnt;
+        }
+        to += 4 * nx1 * m;
+        ro += J.nx2() * m;
+      }
+      if (cnt > 1) sum /= cnt;
+      a.aj[ajoff + idx] = sum;
+    }
+  }
+}
+
+hipError_t launch_face_interp(const FaceArgs &a, bool multi, hipStream_t s) {
+  if (a.nfaces == 0) return hipSuccess;
+  const int grid = 8 * ((a.nfaces + 7) / 8);   // a multiple of 8 (XCD mapping)
+  if (multi)
+    hipLaunchKernelGGL((face_interp_kernel<kFaceT4, kFaceR4>), dim3(grid), dim3(kFaceThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL((face_interp_kernel<kFaceT1, kFaceR1>), dim3(grid), dim3(kFaceThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// stage kernel per order bucket
+// ---------------------------------------------------------------------------
 template <int NP, int GEOM, int NT>
 __global__ __launch_bounds__(NT, NMX_DG_WAVES) void dg_amr_kernel(AmrArgs a) {
   constexpr int NP2 = NP * NP;
@@ -27,7 +156,6 @@ __global__ __launch_bounds__(NT, NMX_DG_WAVES) void dg_amr_kernel(AmrArgs a) {
   constexpr int NC = (NP3 + NT - 1) / NT;
   __shared__ double s_u[4][NP3];
   __shared__ double s_f[4][6][NP2];
-  __shared__ double s_stage[kAmrStage];
   __shared__ double s_D[NP2];
   __shared__ double s_xb[NP];
 
@@ -53,10 +181,9 @@ __global__ __launch_bounds__(NT, NMX_DG_WAVES) void dg_amr_kernel(AmrArgs a) {
     const int e = a.elist[it];
     const long gbase = a.soff[e];
     const int *fdesc = a.face + 48 * e;
-    const int nrounds = a.erounds[e];
     __syncthreads();
 
-    // ---- phase 1 ----
+    // ---- phase 1: own fields and all six adjacent face planes ----
     double uown[NC][4];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -66,56 +193,34 @@ __global__ __launch_bounds__(NT, NMX_DG_WAVES) void dg_amr_kernel(AmrArgs a) {
         for (int v = 0; v < 4; ++v) uown[c][v] = a.win[gbase + node + v * NP3];
       }
     }
-    for (int f = 0; f < 6; ++f) {
-      const int kind = fdesc[8 * f];
+    for (int idx = tid; idx < 6 * NP2; idx += NT) {
+      const int f = idx / NP2, q = idx % NP2;
+      const int *d = fdesc + 8 * f;
+      const int kind = d[0];
       if (kind == 1) {          // conforming copy from a same-order neighbour
-        const int nb = fdesc[8 * f + 1], base = fdesc[8 * f + 2];
-        const int s1 = fdesc[8 * f + 3], s2 = fdesc[8 * f + 4];
-        const double *nw = a.win + a.soff[nb] + base;
-        for (int q = tid; q < NP2; q += NT) {
+        const double *p = a.win + a.soff[d[1]] + d[2] + (q % NP) * d[3] + (q / NP) * d[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s_f[v][f][q] = p[v * NP3];
+      } else if (kind == 2) {   // interpolated this stage by face_interp_kernel
+        const double *p = a.aj + d[2] + q;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s_f[v][f][q] = p[v * NP2];
+      } else {                  // outer boundary: scalarwave1's exterior profile
+        double ua[4] = {0.0, 0.0, 0.0, 0.0};
+        if (a.profile) {
           const int fa = q % NP, fb = q / NP;
-          const double *p = nw + fa * s1 + fb * s2;
-#pragma unroll
-          for (int v = 0; v < 4; ++v) s_f[v][f][q] = p[v * NP3];
+          const int dir = f >> 1, pl = (f & 1) * (NP - 1);
+          const int ii = (dir == 0) ? pl : fa;
+          const int jj = (dir == 1) ? pl : ((dir == 0) ? fa : fb);
+          const int kk = (dir == 2) ? pl : fb;
+          const int mynode = ii + NP * (jj + NP * kk);
+          const double *xp = a.xyz + 3 * (gbase / 5) + mynode;
+          plane_wave(a.k, a.om, xp[0], xp[NP3], xp[2 * NP3], a.tdev[a.tidx], ua);
         }
-      } else if (kind == 0) {   // outer boundary: scalarwave1's exterior profile
-        for (int q = tid; q < NP2; q += NT) {
-          double ua[4] = {0.0, 0.0, 0.0, 0.0};
-          if (a.profile) {
-            const int fa = q % NP, fb = q / NP;
-            const int dir = f >> 1, pl = (f & 1) * (NP - 1);
-            const int ii = (dir == 0) ? pl : fa;
-            const int jj = (dir == 1) ? pl : ((dir == 0) ? fa : fb);
-            const int kk = (dir == 2) ? pl : fb;
-            const int mynode = ii + NP * (jj + NP * kk);
-            const double *xp = a.xyz + 3 * (gbase / 5) + mynode;
-            plane_wave(a.k, a.om, xp[0], xp[NP3], xp[2 * NP3], a.tdev[a.tidx], ua);
-          }
 #pragma unroll
-          for (int v = 0; v < 4; ++v) s_f[v][f][q] = ua[v];
-        }
+        for (int v = 0; v < 4; ++v) s_f[v][f][q] = ua[v];
       }
     }
-    // stage the neighbour planes of round 0
-    auto stage_round = [&](int r) {
-      int off = 0;
-      for (int f = 0; f < 6; ++f) {
-        if (fdesc[8 * f] != 2 || fdesc[8 * f + 7] != r) continue;
-        const int ns = fdesc[8 * f + 5], s0 = fdesc[8 * f + 6];
-        for (int si = 0; si < ns; ++si) {
-          const int *sl = a.slot + 12 * (s0 + si);
-          const int nb = sl[0], m = sl[1], base = sl[2], t1 = sl[3], t2 = sl[4];
-          const int m2 = m * m, m3 = m2 * m;
-          const double *nw = a.win + a.soff[nb] + base;
-          for (int idx = tid; idx < 4 * m2; idx += NT) {
-            const int v = idx / m2, q = idx % m2;
-            s_stage[off + idx] = nw[v * m3 + (q % m) * t1 + (q / m) * t2];
-          }
-          off += 4 * m2;
-        }
-      }
-    };
-    if (nrounds > 0) stage_round(0);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const int node = tid + c * NT;
@@ -125,49 +230,6 @@ __global__ __launch_bounds__(NT, NMX_DG_WAVES) void dg_amr_kernel(AmrArgs a) {
       }
     }
     __syncthreads();
-
-    // ---- phase 1b: interpolated faces, round by round ----
-    for (int r = 0; r < nrounds; ++r) {
-      if (r > 0) {
-        __syncthreads();
-        stage_round(r);
-        __syncthreads();
-      }
-      int off = 0;
-      for (int f = 0; f < 6; ++f) {
-        if (fdesc[8 * f] != 2 || fdesc[8 * f + 7] != r) continue;
-        const int ns = fdesc[8 * f + 5], s0 = fdesc[8 * f + 6];
-        for (int idx = tid; idx < 4 * NP2; idx += NT) {
-          const int v = idx / NP2, q = idx % NP2;
-          const int fa = q % NP, fb = q / NP;
-          double sum = 0.0;
-          int cnt = 0, soff = off;
-          for (int si = 0; si < ns; ++si) {
-            const int *sl = a.slot + 12 * (s0 + si);
-            const int m = sl[1], m2 = m * m;
-            if (fa >= sl[6] && fa <= sl[7] && fb >= sl[8] && fb <= sl[9]) {
-              const double *P = s_stage + soff + v * m2;
-              const double *ra = a.Btab + sl[10] + fa * m;
-              const double *rb = a.Btab + sl[11] + fb * m;
-              const double *B1 = sl[5] ? rb : ra;
-              const double *B2 = sl[5] ? ra : rb;
-              double sv = 0.0;
-              for (int q2 = 0; q2 < m; ++q2) {
-                const double b2 = B2[q2];
-                for (int q1 = 0; q1 < m; ++q1) sv += P[q1 + m * q2] * B1[q1] * b2;
-              }
-              sum += sv;
-              ++cnt;
-            }
-            soff += 4 * m2;
-          }
-          if (cnt > 1) sum /= cnt;
-          s_f[v][f][q] = sum;
-        }
-        for (int si = 0; si < ns; ++si) off += 4 * a.slot[12 * (s0 + si) + 1] * a.slot[12 * (s0 + si) + 1];
-      }
-    }
-    if (nrounds > 0) __syncthreads();
 
     // ---- phase 2 ----
     NodeOut o;
@@ -191,20 +253,31 @@ __global__ __launch_bounds__(NT, NMX_DG_WAVES) void dg_amr_kernel(AmrArgs a) {
   }
 }
 
-template <int NP, int GEOM>
-static hipError_t launch_amr_np(const AmrArgs &a, int grid, hipStream_t s) {
-  constexpr int NP3 = NP * NP * NP;
-  constexpr int NT = NP3 <= 64 ? 64 : (NP3 <= 128 ? 128 : 256);
+template <int NP, int GEOM, int NT>
+static hipError_t launch_amr_t(const AmrArgs &a, int grid, hipStream_t s) {
   hipLaunchKernelGGL((dg_amr_kernel<NP, GEOM, NT>), dim3(grid), dim3(NT), 0, s, a);
   return hipGetLastError();
 }
 
-hipError_t launch_dg_amr(int np, int geom, const AmrArgs &a, int grid, hipStream_t s) {
-#define NMX_CASE(N)                                              \
-  case N:                                                        \
-    if (geom == 0) return launch_amr_np<N, 0>(a, grid, s);       \
-    if (geom == 1) return launch_amr_np<N, 1>(a, grid, s);       \
-    return launch_amr_np<N, 2>(a, grid, s);
+// nodes per thread: div = 1, 2 or 4 (threads rounded to whole waves)
+static constexpr int amr_threads(int np3, int div) {
+  return (((np3 + div - 1) / div + 63) / 64 * 64) > 1024 ? 1024 : (((np3 + div - 1) / div + 63) / 64 * 64);
+}
+
+template <int NP, int GEOM>
+static hipError_t launch_amr_np(const AmrArgs &a, int grid, int div, hipStream_t s) {
+  constexpr int NP3 = NP * NP * NP;
+  if (div == 4) return launch_amr_t<NP, GEOM, amr_threads(NP3, 4)>(a, grid, s);
+  if (div == 2) return launch_amr_t<NP, GEOM, amr_threads(NP3, 2)>(a, grid, s);
+  return launch_amr_t<NP, GEOM, amr_threads(NP3, 1)>(a, grid, s);
+}
+
+hipError_t launch_dg_amr(int np, int geom, const AmrArgs &a, int grid, int div, hipStream_t s) {
+#define NMX_CASE(N)                                                   \
+  case N:                                                             \
+    if (geom == 0) return launch_amr_np<N, 0>(a, grid, div, s);       \
+    if (geom == 1) return launch_amr_np<N, 1>(a, grid, div, s);       \
+    return launch_amr_np<N, 2>(a, grid, div, s);
   switch (np) {
     NMX_CASE(2)
     NMX_CASE(3)

@@ -62,8 +62,9 @@ struct nmx_ctx {
   // batched (order-bucketed) mirror
   bool amr = false;
   long *dsoff = nullptr;
-  int *dfaceB = nullptr, *derounds = nullptr, *dslot = nullptr;
-  double *dBtab = nullptr, *dDall = nullptr, *dxball = nullptr;
+  int *dfaceB = nullptr, *djobs1 = nullptr, *djobs4 = nullptr, *dfirst4 = nullptr;
+  double *dBtab = nullptr, *dDall = nullptr, *dxball = nullptr, *daj = nullptr;
+  int nfint1 = 0, nfint4 = 0;   // interpolated faces with one / several neighbours
   struct Bucket { int n; int count; int *dlist; long doff; long xoff; };
   std::vector<Bucket> buckets;
   std::vector<long> eoff;   // node offset per element
@@ -76,6 +77,10 @@ struct nmx_ctx {
   double *hT = nullptr;     // pinned stage-time staging
   int tcap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // helper streams for concurrent order buckets (fork/join inside graphs)
+  static constexpr int kHelpers = 9;
+  hipStream_t helper[kHelpers] = {};
+  hipEvent_t evfork = nullptr, evjoin[kHelpers] = {};
   double last_ms = 0;
   bool tuned = false;
 };
@@ -110,6 +115,7 @@ static void set_defaults(nmx_ctx *c) {
       {"nmx_xcd", "yes"},
       {"nmx_block", "half"},
       {"nmx_autotune", "yes"},
+      {"nmx_concurrent", "yes"},
   };
 }
 
@@ -137,6 +143,11 @@ extern "C" int nmx_ctx_create(nmx_ctx **out, int device) {
   c->stream = c->own_stream;
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
+  HIPCHK(hipEventCreateWithFlags(&c->evfork, hipEventDisableTiming));
+  for (int i = 0; i < nmx_ctx::kHelpers; ++i) {
+    HIPCHK(hipStreamCreateWithFlags(&c->helper[i], hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->evjoin[i], hipEventDisableTiming));
+  }
   *out = c;
   return NMX_OK;
 }
@@ -148,8 +159,9 @@ static void free_mesh_dev(nmx_ctx *c) {
     if (*b) { (void)hipFree(*b); *b = nullptr; }
   if (c->dedom) { (void)hipFree(c->dedom); c->dedom = nullptr; }
   {
-    void **ptrs[] = {(void **)&c->dsoff, (void **)&c->dfaceB, (void **)&c->derounds, (void **)&c->dslot,
-                     (void **)&c->dBtab, (void **)&c->dDall, (void **)&c->dxball};
+    void **ptrs[] = {(void **)&c->dsoff, (void **)&c->dfaceB, (void **)&c->djobs1, (void **)&c->djobs4,
+                     (void **)&c->dfirst4, (void **)&c->dBtab, (void **)&c->dDall, (void **)&c->dxball,
+                     (void **)&c->daj};
     for (auto q : ptrs)
       if (*q) { (void)hipFree(*q); *q = nullptr; }
     for (auto &b : c->buckets)
@@ -171,6 +183,11 @@ extern "C" int nmx_ctx_destroy(nmx_ctx *c) {
   if (c->hT) (void)hipHostFree(c->hT);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->evfork) (void)hipEventDestroy(c->evfork);
+  for (int i = 0; i < nmx_ctx::kHelpers; ++i) {
+    if (c->evjoin[i]) (void)hipEventDestroy(c->evjoin[i]);
+    if (c->helper[i]) (void)hipStreamDestroy(c->helper[i]);
+  }
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return NMX_OK;
@@ -244,16 +261,18 @@ static int dev_upload(T **p, const std::vector<T> &h) {
 }
 
 // Tables of the batched path: per-element state offsets, face descriptors,
-// interpolation slots with deduplicated 1D Lagrange rows, staging rounds,
-// order buckets.  Every gather a kernel will make is bounds-checked here.
+// face interpolation jobs (single-neighbour faces and faces with several
+// neighbours in separate lists) with deduplicated 1D Lagrange rows, order
+// buckets.  Every gather a kernel will make is bounds-checked here.
 static int upload_amr_tables(nmx_ctx *c) {
   Mesh &M = c->mesh;
   const size_t ne = M.e.size();
   std::vector<long> soff(ne);
   for (size_t i = 0; i < ne; ++i) soff[i] = 5 * M.e[i].off;
-  std::vector<int> fd(48 * ne, 0), rounds(ne, 0), slots;
+  std::vector<int> fd(48 * ne, 0), jobs1, jobs4, first4;
   std::vector<double> btab;
   std::map<std::vector<double>, int> bdedup;
+  long ajlen = 0;
   auto brow = [&](const std::vector<double> &rows) {
     auto it = bdedup.find(rows);
     if (it != bdedup.end()) return it->second;
@@ -265,7 +284,6 @@ static int upload_amr_tables(nmx_ctx *c) {
   for (size_t i = 0; i < ne; ++i) {
     const Elm &E = M.e[i];
     const int n = E.n;
-    int round = 0, used = 0;
     for (int f = 0; f < 6; ++f) {
       int *d = &fd[48 * i + 8 * f];
       if (E.ajmode[f] == 0) { d[0] = 0; continue; }
@@ -284,13 +302,27 @@ static int upload_amr_tables(nmx_ctx *c) {
         continue;
       }
       const std::vector<InterpSlot> &sl = M.fint[6 * i + f];
-      if (sl.empty() || sl.size() > 4) return fail(NMX_EUNSUP, "face with no or more than 4 interpolation neighbours");
-      int need = 0;
-      for (const InterpSlot &q : sl) need += 4 * q.m * q.m;
-      if (need > kAmrStage) return fail(NMX_EUNSUP, "face interpolation exceeds the staging area");
-      if (used + need > kAmrStage) { ++round; used = 0; }
-      used += need;
-      d[0] = 2; d[5] = (int)sl.size(); d[6] = (int)(slots.size() / 12); d[7] = round;
+      const int ns = (int)sl.size();
+      if (ns == 0 || ns > 4) return fail(NMX_EUNSUP, "face with no or more than 4 interpolation neighbours");
+      // LDS of the face kernel: first-pass results (4 * covered rows * m
+      // per slot) and the rows of both passes (covered rows or columns * m)
+      int needT = 0, needR1 = 0, needR2 = 0;
+      for (const InterpSlot &q : sl) {
+        const int nx1 = q.swap ? (q.b1 - q.b0 + 1) : (q.a1 - q.a0 + 1);
+        const int nx2 = q.swap ? (q.a1 - q.a0 + 1) : (q.b1 - q.b0 + 1);
+        if (nx1 > kMaxFaceN || nx2 > kMaxFaceN || q.m > kMaxFaceN)
+          return fail(NMX_EUNSUP, "face order above 10");
+        needT += 4 * nx1 * q.m;
+        needR1 += nx1 * q.m;
+        needR2 += nx2 * q.m;
+      }
+      const int capT = ns == 1 ? kFaceT1 : kFaceT4, capR = ns == 1 ? kFaceR1 : kFaceR4;
+      if (needT > capT || needR1 > capR || needR2 > capR)
+        return fail(NMX_EUNSUP, "face interpolation exceeds the staging area");
+      if (ajlen + 4L * n * n > (long)INT32_MAX) return fail(NMX_EUNSUP, "face buffer too large");
+      d[0] = 2; d[2] = (int)ajlen;
+      std::vector<int> &jobs = ns == 1 ? jobs1 : jobs4;
+      if (ns > 1) first4.push_back((int)(jobs4.size() / kJobInts));
       for (const InterpSlot &q : sl) {
         const Elm &N = M.e[q.nb];
         const int m = q.m;
@@ -306,20 +338,27 @@ static int upload_amr_tables(nmx_ctx *c) {
         if ((int)q.Ba.size() != n * m || (int)q.Bb.size() != n * m)
           return fail(NMX_EINVAL, "interpolation rows have the wrong size");
         int oa = brow(q.Ba), ob = brow(q.Bb);
-        int rec[12] = {q.nb, m, (int)base, q.t1, q.t2, q.swap, q.a0, q.a1, q.b0, q.b1, oa, ob};
-        slots.insert(slots.end(), rec, rec + 12);
+        int rec[kJobInts] = {n, m, (int)ajlen, ns, (int)base, q.t1, q.t2, q.swap,
+                             q.a0, q.a1, q.b0, q.b1, oa, ob, 0, 0};
+        const long so = soff[q.nb];
+        std::memcpy(rec + 14, &so, sizeof(long));
+        jobs.insert(jobs.end(), rec, rec + kJobInts);
       }
+      ajlen += 4L * n * n;
     }
-    rounds[i] = 0;
-    for (int f = 0; f < 6; ++f)
-      if (fd[48 * i + 8 * f] == 2) rounds[i] = std::max(rounds[i], fd[48 * i + 8 * f + 7] + 1);
   }
-  if (slots.empty()) slots.assign(12, 0);
+  c->nfint1 = (int)(jobs1.size() / kJobInts);
+  c->nfint4 = (int)first4.size();
+  if (jobs1.empty()) jobs1.assign(kJobInts, 0);
+  if (jobs4.empty()) jobs4.assign(kJobInts, 0);
+  if (first4.empty()) first4.assign(1, 0);
   if (btab.empty()) btab.assign(1, 0.0);
+  if (ajlen == 0) ajlen = 1;
   int r;
   if ((r = dev_upload(&c->dsoff, soff)) || (r = dev_upload(&c->dfaceB, fd)) ||
-      (r = dev_upload(&c->derounds, rounds)) || (r = dev_upload(&c->dslot, slots)) ||
-      (r = dev_upload(&c->dBtab, btab)))
+      (r = dev_upload(&c->djobs1, jobs1)) || (r = dev_upload(&c->djobs4, jobs4)) ||
+      (r = dev_upload(&c->dfirst4, first4)) || (r = dev_upload(&c->dBtab, btab)) ||
+      (r = dev_alloc(&c->daj, ajlen)))
     return r;
   // order buckets with their D matrices and nodes
   std::vector<double> dall, xball;
@@ -690,13 +729,36 @@ static int launch_amr(nmx_ctx *c, const StageArgs &sa) {
   const bool perelm = (P(c, "nmx_path") == "perelement");
   AmrArgs a;
   a.win = sa.win; a.U = sa.U; a.Ain = sa.Ain; a.Aout = sa.Aout; a.Wout = sa.Wout; a.R = sa.R;
-  a.soff = c->dsoff; a.face = c->dfaceB; a.erounds = c->derounds; a.slot = c->dslot;
-  a.Btab = c->dBtab; a.dXbdX = c->dXbdX; a.egeo = c->degeo; a.edom = c->dedom;
+  a.soff = c->dsoff; a.face = c->dfaceB; a.aj = c->daj; a.dXbdX = c->dXbdX; a.egeo = c->degeo; a.edom = c->dedom;
   a.dXdx = c->dXdx; a.xyz = c->dxyz; a.tdev = sa.tdev; a.tidx = sa.tidx;
   a.cA = sa.cA; a.cW = sa.cW;
   for (int d = 0; d < 3; ++d) a.k[d] = sa.k[d];
   a.om = sa.om; a.wmode = sa.wmode; a.numflux = sa.numflux; a.profile = sa.profile;
-  for (const auto &b : c->buckets) {
+  // interpolated faces of this stage first, all orders in one launch per list
+  {
+    FaceArgs fa;
+    fa.win = sa.win; fa.Btab = c->dBtab; fa.aj = c->daj;
+    fa.jobs = c->djobs1; fa.first = nullptr; fa.nfaces = c->nfint1;
+    hipError_t err = launch_face_interp(fa, false, c->stream);
+    if (err == hipSuccess) {
+      fa.jobs = c->djobs4; fa.first = c->dfirst4; fa.nfaces = c->nfint4;
+      err = launch_face_interp(fa, true, c->stream);
+    }
+    if (err != hipSuccess) return fail(NMX_EHIP, std::string("face launch: ") + hipGetErrorString(err));
+  }
+  std::string blk = P(c, "nmx_block");
+  const int div = blk == "half" ? 2 : blk == "quarter" ? 4 : 1;
+  // buckets run concurrently on helper streams forked from the context's
+  // stream (parallel branches when captured into a graph)
+  const bool fork = !perelm && c->buckets.size() > 1 && P(c, "nmx_concurrent") == "yes";
+  if (fork) HIPCHK(hipEventRecord(c->evfork, c->stream));
+  for (size_t bi = 0; bi < c->buckets.size(); ++bi) {
+    const auto &b = c->buckets[bi];
+    hipStream_t st = c->stream;
+    if (fork) {
+      st = c->helper[bi % nmx_ctx::kHelpers];
+      HIPCHK(hipStreamWaitEvent(st, c->evfork, 0));
+    }
     a.D = c->dDall + b.doff;
     a.xb = c->dxball + b.xoff;
     a.wend_inv = 1.0 / lgl_basis(b.n).w[0];
@@ -705,7 +767,7 @@ static int launch_amr(nmx_ctx *c, const StageArgs &sa) {
       a.nlist = 1;
       for (int q = 0; q < b.count; ++q) {
         a.elist = b.dlist + q;
-        hipError_t err = launch_dg_amr(b.n, c->geom, a, 1, c->stream);
+        hipError_t err = launch_dg_amr(b.n, c->geom, a, 1, div, c->stream);
         if (err != hipSuccess) return fail(NMX_EHIP, std::string("stage launch: ") + hipGetErrorString(err));
       }
       continue;
@@ -714,9 +776,13 @@ static int launch_amr(nmx_ctx *c, const StageArgs &sa) {
     a.nlist = b.count;
     a.xcd_map = sa.xcd_map;
     int grid = a.xcd_map ? 8 * ((b.count + 7) / 8) : b.count;
-    hipError_t err = launch_dg_amr(b.n, c->geom, a, grid, c->stream);
+    hipError_t err = launch_dg_amr(b.n, c->geom, a, grid, div, st);
     if (err != hipSuccess) return fail(NMX_EHIP, std::string("stage launch: ") + hipGetErrorString(err));
+    if (fork) HIPCHK(hipEventRecord(c->evjoin[bi % nmx_ctx::kHelpers], st));
   }
+  if (fork)
+    for (size_t bi = 0; bi < c->buckets.size() && bi < (size_t)nmx_ctx::kHelpers; ++bi)
+      HIPCHK(hipStreamWaitEvent(c->stream, c->evjoin[bi], 0));
   return NMX_OK;
 }
 
