@@ -39,6 +39,7 @@ extern "C" {
 #define NMX_REG_U 0       /* evolved variables u (scalarwave1 vlu) */
 #define NMX_REG_R 1       /* RHS r = RHS(u) ("_r" vars, evosys.c:256) */
 #define NMX_REG_P 2       /* u at previous time ("_p" vars, evosys.c:257) */
+#define NMX_REG_ADJ 3     /* adjoint variable lam = dJ/du (no counterpart in the reference) */
 
 /* variables of scalarwave1's vlu, in order (scalarwave1.c:670-672) */
 #define NMX_NVARS 5       /* pi, cx, cy, cz, phi */
@@ -120,12 +121,34 @@ int nmx_evolve(nmx_ctx *ctx, int nsteps);
  * frameworks; valid until the mesh is rebuilt */
 int nmx_state_device_ptr(nmx_ctx *ctx, int reg, void **dptr, long *len);
 
-/* Discrete adjoint of nsteps RK steps (no counterpart in the reference; the
- * forward operator it transposes is evolve_RK4_mesh above).  Given lambda_T
- * = dL/du(T) in reg NMX_REG_R ... see nmx_adjoint_* below. */
-int nmx_adjoint_rhs(nmx_ctx *ctx, const double *lam, double *out, double t);
-/* lam := (dU_{n}/dU_{n-1})^T lam for one RK step of the linear operator */
-int nmx_adjoint_step(nmx_ctx *ctx, double *lam_inout, double t);
+/* Discrete adjoint (no counterpart in the reference; it transposes the
+ * forward operators above, evolve_setrhs_mesh and evolve_RK4_mesh /
+ * evolve_sspRK3_mesh / evolve_Euler_mesh, RungeKutta.c:19-188).  The
+ * scalarwave1 RHS is linear in u, so its Jacobian L does not depend on the
+ * state or the time; the boundary profile enters only as a u-independent
+ * source.  Conforming meshes of one element order only (NMX_EUNSUP otherwise).
+ *
+ * reg_out = L^T reg_in */
+int nmx_adjoint_setrhs(nmx_ctx *ctx, int reg_out, int reg_in);
+/* NMX_REG_ADJ <- (du_{n+1}/du_n)^T NMX_REG_ADJ, nsteps times, for
+ * evolve_method and the current dt (the forward stages in reverse).  The
+ * mesh time is not changed. */
+int nmx_adjoint_evolve(nmx_ctx *ctx, int nsteps);
+/* time (ms) of the last nmx_adjoint_evolve on the device */
+int nmx_last_adjoint_ms(nmx_ctx *ctx, double *ms);
+
+/* Checkpoints for reverse passes: a pool of nslots device copies of a
+ * register; store/load copy reg <-> slot on the stream, and keep the mesh
+ * time with the slot (restored by loading into NMX_REG_U). */
+int nmx_checkpoint_alloc(nmx_ctx *ctx, int nslots);
+int nmx_checkpoint_store(nmx_ctx *ctx, int slot, int reg);
+int nmx_checkpoint_load(nmx_ctx *ctx, int slot, int reg);
+
+/* State transfer between a register and DEVICE memory in the flat layout
+ * u[var*nnodes + node], ordered on the context's stream with no host
+ * synchronisation (for frameworks that hold the state on the GPU). */
+int nmx_state_copy_in(nmx_ctx *ctx, int reg, const double *dflat);
+int nmx_state_copy_out(nmx_ctx *ctx, int reg, double *dflat);
 
 /* profiling helpers: time (ms) of the last nmx_evolve on the device and the
  * number of kernel launches it issued */

@@ -50,8 +50,14 @@ def _proto(lib):
         "nmx_autotune": (i, [vp]),
         "nmx_evolve": (i, [vp, i]),
         "nmx_state_device_ptr": (i, [vp, i, P(vp), P(l)]),
-        "nmx_adjoint_rhs": (i, [vp, vp, vp, d]),
-        "nmx_adjoint_step": (i, [vp, vp, d]),
+        "nmx_adjoint_setrhs": (i, [vp, i, i]),
+        "nmx_adjoint_evolve": (i, [vp, i]),
+        "nmx_last_adjoint_ms": (i, [vp, P(d)]),
+        "nmx_checkpoint_alloc": (i, [vp, i]),
+        "nmx_checkpoint_store": (i, [vp, i, i]),
+        "nmx_checkpoint_load": (i, [vp, i, i]),
+        "nmx_state_copy_in": (i, [vp, i, vp]),
+        "nmx_state_copy_out": (i, [vp, i, vp]),
         "nmx_last_evolve_ms": (i, [vp, P(d)]),
     }
     for name, (res, args) in sig.items():

@@ -83,6 +83,22 @@ struct nmx_ctx {
   hipEvent_t evfork = nullptr, evjoin[kHelpers] = {};
   double last_ms = 0;
   bool tuned = false;
+  // adjoint: lam register, neighbour face index per element face (plain
+  // path), its own graph cache, and the checkpoint pool
+  double *dL = nullptr;
+  int *dnbf = nullptr;
+  long *deoff = nullptr;
+  int *den = nullptr;
+  hipGraph_t agraph = nullptr;
+  hipGraphExec_t agexec = nullptr;
+  int agraph_steps = 0;
+  double agraph_dt = 0;
+  int agraph_method = -1;
+  double *dCk = nullptr;
+  int nck = 0;
+  std::vector<long double> ckTe, ckte;
+  std::vector<char> ckset;
+  double last_adj_ms = 0;
 };
 
 extern "C" const char *nmx_last_error(void) { return g_err.c_str(); }
@@ -172,6 +188,17 @@ static void free_mesh_dev(nmx_ctx *c) {
   if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
   c->graph_steps = 0;
+  {
+    void **ptrs[] = {(void **)&c->dL, (void **)&c->dnbf, (void **)&c->deoff, (void **)&c->den,
+                     (void **)&c->dCk};
+    for (auto q : ptrs)
+      if (*q) { (void)hipFree(*q); *q = nullptr; }
+  }
+  if (c->agexec) { (void)hipGraphExecDestroy(c->agexec); c->agexec = nullptr; }
+  if (c->agraph) { (void)hipGraphDestroy(c->agraph); c->agraph = nullptr; }
+  c->agraph_steps = 0;
+  c->nck = 0;
+  c->ckTe.clear(); c->ckte.clear(); c->ckset.clear();
 }
 
 extern "C" int nmx_ctx_destroy(nmx_ctx *c) {
@@ -229,6 +256,9 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
   if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
   c->graph_steps = 0;
+  if (c->agexec) { (void)hipGraphExecDestroy(c->agexec); c->agexec = nullptr; }
+  if (c->agraph) { (void)hipGraphDestroy(c->agraph); c->agraph = nullptr; }
+  c->agraph_steps = 0;
   if (k == "dt" && c->have_mesh) {
     if (v == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"));
     else c->dt = std::atof(v.c_str());
@@ -410,6 +440,14 @@ static int upload_mesh(nmx_ctx *c) {
       (r = dev_alloc(&c->dW0, c->nstate)) || (r = dev_alloc(&c->dW1, c->nstate)) ||
       (r = dev_alloc(&c->dR, c->nstate)) || (r = dev_alloc(&c->dP, c->nstate)))
     return r;
+  if ((r = dev_alloc(&c->dL, c->nstate))) return r;
+  HIPCHK(hipMemset(c->dL, 0, c->nstate * sizeof(double)));
+  {
+    std::vector<long> eo(ne);
+    std::vector<int> en(ne);
+    for (size_t i = 0; i < ne; ++i) { eo[i] = M.e[i].off; en[i] = M.e[i].n; }
+    if ((r = dev_upload(&c->deoff, eo)) || (r = dev_upload(&c->den, en))) return r;
+  }
   HIPCHK(hipMemset(c->dU, 0, c->nstate * sizeof(double)));
   HIPCHK(hipMemset(c->dR, 0, c->nstate * sizeof(double)));
   HIPCHK(hipMemset(c->dP, 0, c->nstate * sizeof(double)));
@@ -477,6 +515,16 @@ static int upload_mesh(nmx_ctx *c) {
     }
   if ((r = dev_alloc(&c->dface, hf.size()))) return r;
   HIPCHK(hipMemcpy(c->dface, hf.data(), hf.size() * sizeof(int4), hipMemcpyHostToDevice));
+  // the neighbour's face on each conforming face (adjoint gathers)
+  std::vector<int> hnbf(6 * ne, 0);
+  for (size_t i = 0; i < ne; ++i)
+    for (int f = 0; f < 6; ++f)
+      if (hf[6 * i + f].x >= 0) {
+        int nf = M.e[i].nbf[f][0];
+        if (nf < 0 || nf > 5) return fail(NMX_EINVAL, "neighbour face index out of range");
+        hnbf[6 * i + f] = nf;
+      }
+  if ((r = dev_upload(&c->dnbf, hnbf))) return r;
   const Basis &b = lgl_basis(c->np1);
   std::vector<double> hD(b.D, b.D + c->np1 * c->np1);
   if ((r = dev_alloc(&c->dD, hD.size()))) return r;
@@ -609,6 +657,7 @@ static double *reg_ptr(nmx_ctx *c, int reg) {
     case NMX_REG_U: return c->dU;
     case NMX_REG_R: return c->dR;
     case NMX_REG_P: return c->dP;
+    case NMX_REG_ADJ: return c->dL;
     default: return nullptr;
   }
 }
@@ -996,11 +1045,195 @@ extern "C" int nmx_state_device_ptr(nmx_ctx *c, int reg, void **p, long *len) {
   return NMX_OK;
 }
 
-extern "C" int nmx_adjoint_rhs(nmx_ctx *c, const double *, double *, double) {
-  (void)c;
-  return fail(NMX_EUNSUP, "adjoint not built yet");
+// ---------------------------------------------------------------------------
+// discrete adjoint
+// ---------------------------------------------------------------------------
+
+static AdjArgs adj_base_args(nmx_ctx *c) {
+  AdjArgs a;
+  a.dXdx = c->dXdx; a.dXbdX = c->dXbdX; a.egeo = c->degeo; a.xb = c->dxb; a.D = c->dD;
+  a.edom = c->dedom; a.face = c->dface; a.nbface = c->dnbf;
+  a.wend_inv = c->np1 ? 1.0 / lgl_basis(c->np1).w[0] : 0.0;
+  a.numflux = (P(c, "scalarwave1_numflux") == "LLF") ? 1 : 0;
+  a.xcd_map = (P(c, "nmx_xcd") == "yes") ? 1 : 0;
+  a.e0 = 0;
+  a.e1 = (int)c->mesh.e.size();
+  return a;
 }
-extern "C" int nmx_adjoint_step(nmx_ctx *c, double *, double) {
-  (void)c;
-  return fail(NMX_EUNSUP, "adjoint not built yet");
+
+#define NEED_ADJ(c)                                                                      \
+  do {                                                                                   \
+    NEED_DEV(c);                                                                         \
+    if ((c)->amr || !(c)->dnbf)                                                          \
+      return fail(NMX_EUNSUP, "the adjoint needs a conforming mesh of one element order"); \
+  } while (0)
+
+static int launch_adj(nmx_ctx *c, const AdjArgs &a) {
+  int nel = a.e1 - a.e0;
+  int grid = a.xcd_map ? 8 * ((nel + 7) / 8) : nel;
+  hipError_t err = launch_dg_adjoint(c->np1, c->geom, a, grid, c->stream);
+  if (err != hipSuccess) return fail(NMX_EHIP, std::string("adjoint launch: ") + hipGetErrorString(err));
+  return NMX_OK;
+}
+
+extern "C" int nmx_adjoint_setrhs(nmx_ctx *c, int reg_out, int reg_in) {
+  NEED_ADJ(c);
+  double *out = reg_ptr(c, reg_out), *in = reg_ptr(c, reg_in);
+  if (!out || !in || out == in) return fail(NMX_EINVAL, "bad registers");
+  AdjArgs a = adj_base_args(c);
+  a.lam = in;
+  a.R = out;
+  return launch_adj(c, a);
+}
+
+// One adjoint RK step lam <- (du_{n+1}/du_n)^T lam: the forward stages of
+// issue_step taken in reverse.  With y_s = L^T x_s:
+//   RK4:    x4 = lam; acc = lam + dt/6 y4; x3 = dt/3 lam + dt^2/6 y4;
+//           acc += y3; x2 = dt/3 lam + dt/2 y3; acc += y2; x1 = dt/6 lam + dt/2 y2;
+//           acc += y1
+//   sspRK3: x3 = lam; acc = lam + 2dt/3 y3; x2 = dt/6 lam + dt^2/6 y3;
+//           acc += y2; x1 = x2 + dt y2; acc += y1
+//   Euler:  acc = lam + dt y1
+static int issue_adj_step(nmx_ctx *c, int method, double dt, bool swap_parity) {
+  double *L = swap_parity ? c->dA : c->dL;
+  double *Acc = swap_parity ? c->dL : c->dA;
+  AdjArgs a = adj_base_args(c);
+  int r;
+  if (method == 0) {
+    a.lam = L; a.Abase = L; a.Aout = Acc; a.cA = dt / 6.;
+    a.Wbase = L; a.Wout = c->dW0; a.cU = dt / 3.; a.cW = dt * dt / 6.;
+    if ((r = launch_adj(c, a))) return r;
+    a.lam = c->dW0; a.Abase = Acc; a.cA = 1.0; a.Wbase = L; a.Wout = c->dW1; a.cU = dt / 3.; a.cW = dt / 2.;
+    if ((r = launch_adj(c, a))) return r;
+    a.lam = c->dW1; a.Wout = c->dW0; a.cU = dt / 6.; a.cW = dt / 2.;
+    if ((r = launch_adj(c, a))) return r;
+    a.lam = c->dW0; a.Wout = nullptr; a.Wbase = nullptr;
+    if ((r = launch_adj(c, a))) return r;
+  } else if (method == 1) {
+    a.lam = L; a.Abase = L; a.Aout = Acc; a.cA = 2. * dt / 3.;
+    a.Wbase = L; a.Wout = c->dW0; a.cU = dt / 6.; a.cW = dt * dt / 6.;
+    if ((r = launch_adj(c, a))) return r;
+    a.lam = c->dW0; a.Abase = Acc; a.cA = 1.0; a.Wbase = c->dW0; a.Wout = c->dW1; a.cU = 1.0; a.cW = dt;
+    if ((r = launch_adj(c, a))) return r;
+    a.lam = c->dW1; a.Wout = nullptr; a.Wbase = nullptr;
+    if ((r = launch_adj(c, a))) return r;
+  } else {
+    a.lam = L; a.Abase = L; a.Aout = Acc; a.cA = dt;
+    if ((r = launch_adj(c, a))) return r;
+  }
+  return NMX_OK;
+}
+
+extern "C" int nmx_adjoint_evolve(nmx_ctx *c, int nsteps) {
+  NEED_ADJ(c);
+  if (nsteps < 0) return fail(NMX_EINVAL, "nsteps < 0");
+  HIPCHK(hipSetDevice(c->device));
+  const int method = method_id(c);
+  const double dt = c->dt;
+  const bool use_graph = (P(c, "nmx_graph") == "yes");
+  int G = Pi(c, "nmx_graph_steps");
+  if (G < 2) G = 2;
+  if (G > 64) G = 64;
+  G &= ~1;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  int done = 0;
+  bool parity = false;
+  if (use_graph && nsteps >= G) {
+    if (!c->agexec || c->agraph_steps != G || c->agraph_dt != dt || c->agraph_method != method) {
+      if (c->agexec) { (void)hipGraphExecDestroy(c->agexec); c->agexec = nullptr; }
+      if (c->agraph) { (void)hipGraphDestroy(c->agraph); c->agraph = nullptr; }
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int r = NMX_OK;
+      for (int s = 0; s < G && !r; ++s) r = issue_adj_step(c, method, dt, (s & 1) != 0);
+      hipGraph_t g = nullptr;
+      hipError_t ec = hipStreamEndCapture(c->stream, &g);
+      if (r) return r;
+      if (ec != hipSuccess) return fail(NMX_EHIP, std::string("capture: ") + hipGetErrorString(ec));
+      c->agraph = g;
+      HIPCHK(hipGraphInstantiate(&c->agexec, c->agraph, nullptr, nullptr, 0));
+      c->agraph_steps = G; c->agraph_dt = dt; c->agraph_method = method;
+    }
+    for (; nsteps - done >= G; done += G) HIPCHK(hipGraphLaunch(c->agexec, c->stream));
+  }
+  for (; done < nsteps; ++done) {
+    int r = issue_adj_step(c, method, dt, parity);
+    if (r) return r;
+    parity = !parity;
+  }
+  if (parity) HIPCHK(hipMemcpyAsync(c->dL, c->dA, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_adj_ms = ms;
+  return NMX_OK;
+}
+
+extern "C" int nmx_last_adjoint_ms(nmx_ctx *c, double *ms) {
+  if (!c || !ms) return fail(NMX_EINVAL, "NULL argument");
+  *ms = c->last_adj_ms;
+  return NMX_OK;
+}
+
+// checkpoint pool: nslots device copies of a register plus the mesh time
+extern "C" int nmx_checkpoint_alloc(nmx_ctx *c, int nslots) {
+  NEED_DEV(c);
+  if (nslots < 0) return fail(NMX_EINVAL, "nslots < 0");
+  if (c->dCk) { HIPCHK(hipStreamSynchronize(c->stream)); (void)hipFree(c->dCk); c->dCk = nullptr; }
+  c->nck = 0;
+  if (nslots > 0) {
+    int r = dev_alloc(&c->dCk, (size_t)nslots * c->nstate);
+    if (r) return r;
+  }
+  c->nck = nslots;
+  c->ckTe.assign(nslots, 0);
+  c->ckte.assign(nslots, 0);
+  c->ckset.assign(nslots, 0);
+  return NMX_OK;
+}
+
+extern "C" int nmx_checkpoint_store(nmx_ctx *c, int slot, int reg) {
+  NEED_DEV(c);
+  double *d = reg_ptr(c, reg);
+  if (!d) return fail(NMX_EINVAL, "bad register");
+  if (slot < 0 || slot >= c->nck) return fail(NMX_EINVAL, "checkpoint slot out of range");
+  HIPCHK(hipMemcpyAsync(c->dCk + (size_t)slot * c->nstate, d, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream));
+  c->ckTe[slot] = c->Te;
+  c->ckte[slot] = c->te;
+  c->ckset[slot] = 1;
+  return NMX_OK;
+}
+
+extern "C" int nmx_checkpoint_load(nmx_ctx *c, int slot, int reg) {
+  NEED_DEV(c);
+  double *d = reg_ptr(c, reg);
+  if (!d) return fail(NMX_EINVAL, "bad register");
+  if (slot < 0 || slot >= c->nck || !c->ckset[slot]) return fail(NMX_EINVAL, "checkpoint slot empty or out of range");
+  HIPCHK(hipMemcpyAsync(d, c->dCk + (size_t)slot * c->nstate, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream));
+  if (reg == NMX_REG_U) {
+    c->Te = c->ckTe[slot];
+    c->te = c->ckte[slot];
+    c->time = (double)(c->Te + c->te);
+  }
+  return NMX_OK;
+}
+
+// state transfer from / to device memory in the flat layout, ordered on the
+// context's stream (no host synchronisation)
+extern "C" int nmx_state_copy_in(nmx_ctx *c, int reg, const double *dflat) {
+  NEED_DEV(c);
+  double *d = reg_ptr(c, reg);
+  if (!d || !dflat) return fail(NMX_EINVAL, "bad register or pointer");
+  hipError_t err = launch_layout(dflat, d, c->deoff, c->den, (int)c->mesh.e.size(), c->mesh.nnodes, 1, c->stream);
+  if (err != hipSuccess) return fail(NMX_EHIP, std::string("layout launch: ") + hipGetErrorString(err));
+  return NMX_OK;
+}
+
+extern "C" int nmx_state_copy_out(nmx_ctx *c, int reg, double *dflat) {
+  NEED_DEV(c);
+  double *d = reg_ptr(c, reg);
+  if (!d || !dflat) return fail(NMX_EINVAL, "bad register or pointer");
+  hipError_t err = launch_layout(d, dflat, c->deoff, c->den, (int)c->mesh.e.size(), c->mesh.nnodes, 0, c->stream);
+  if (err != hipSuccess) return fail(NMX_EHIP, std::string("layout launch: ") + hipGetErrorString(err));
+  return NMX_OK;
 }

@@ -35,8 +35,33 @@ struct StageArgs {
   int wmode = 0, numflux = 0, profile = 0;
 };
 
+// Arguments of one fused adjoint RHS + adjoint RK stage launch: y = L^T lam
+// (L the Jacobian of the scalarwave1 DG RHS with respect to u), then
+//   R    = y
+//   Aout = Abase + cA * y
+//   Wout = cU * Wbase + cW * y
+// (each optional).  Wout must not alias lam: neighbours read lam.
+struct AdjArgs {
+  const double *lam = nullptr;
+  const double *Abase = nullptr, *Wbase = nullptr;
+  double *Aout = nullptr, *Wout = nullptr, *R = nullptr;
+  double cA = 0, cU = 0, cW = 0;
+  const double *dXdx = nullptr, *dXbdX = nullptr, *egeo = nullptr, *xb = nullptr, *D = nullptr;
+  const int *edom = nullptr;
+  const int4 *face = nullptr;    // as StageArgs::face
+  const int *nbface = nullptr;   // [e][6]: the neighbour's face touching face f
+  double wend_inv = 0;
+  int numflux = 0, xcd_map = 0, e0 = 0, e1 = 0;
+};
+
 // geom: 0 Cartesian, 1 stored dXdx, 2 CubedShell recomputed; nt from dg_stage_threads
 hipError_t launch_dg_stage(int np, int geom, const StageArgs &a, int grid, int nt, hipStream_t s);
 int dg_stage_threads(int np, int div);   // div: nodes per thread 1, 2 or 4
+hipError_t launch_dg_adjoint(int np, int geom, const AdjArgs &a, int grid, hipStream_t s);
+
+// flat [v][node] <-> element blocks [e][v][np_e] on the device
+// (to_blocks: 1 flat -> blocks, 0 blocks -> flat)
+hipError_t launch_layout(const double *src, double *dst, const long *eoff, const int *en, int nelem,
+                         long nnodes, int to_blocks, hipStream_t s);
 
 }  // namespace nmx

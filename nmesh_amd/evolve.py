@@ -24,7 +24,7 @@ from ._lib import check, lib
 NVARS = 5
 VARS = ("scalarwave1_pi", "scalarwave1_cx", "scalarwave1_cy", "scalarwave1_cz",
         "scalarwave1_phi")
-REG_U, REG_R, REG_P = 0, 1, 2
+REG_U, REG_R, REG_P, REG_ADJ = 0, 1, 2, 3
 
 
 def _p(a):
@@ -148,6 +148,34 @@ class Mesh:
         d = np.zeros(np_)
         check(lib().nmx_state_download_elm(self._ctx, int(reg), int(e), int(vi), _p(d), np_))
         return d
+
+    # device-resident state (flat [var][node] at a device pointer), ordered
+    # on the context's stream
+    def set_stream(self, stream_handle):
+        """Run the library's work on an existing hipStream_t (an integer
+        handle such as torch.cuda.current_stream().cuda_stream)."""
+        check(lib().nmx_set_stream(self._ctx, C.c_void_p(int(stream_handle))))
+
+    def copy_in(self, dptr, reg=REG_U):
+        check(lib().nmx_state_copy_in(self._ctx, int(reg), C.c_void_p(int(dptr))))
+
+    def copy_out(self, dptr, reg=REG_U):
+        check(lib().nmx_state_copy_out(self._ctx, int(reg), C.c_void_p(int(dptr))))
+
+    # checkpoints (device pool; see nmx_checkpoint_*)
+    def checkpoint_alloc(self, nslots):
+        check(lib().nmx_checkpoint_alloc(self._ctx, int(nslots)))
+
+    def checkpoint_store(self, slot, reg=REG_U):
+        check(lib().nmx_checkpoint_store(self._ctx, int(slot), int(reg)))
+
+    def checkpoint_load(self, slot, reg=REG_U):
+        check(lib().nmx_checkpoint_load(self._ctx, int(slot), int(reg)))
+
+    def last_adjoint_ms(self):
+        d = C.c_double()
+        check(lib().nmx_last_adjoint_ms(self._ctx, C.byref(d)))
+        return d.value
 
     def synchronize(self):
         check(lib().nmx_synchronize(self._ctx))
