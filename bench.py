@@ -155,6 +155,90 @@ def batched_leg(nm, steps, plain_steps):
     return out
 
 
+def c1_params():
+    return {"amr_mesh_type": "BoxMesh", "amr_BoxMesh_dout": 8.0, "amr_mesh_xc": "0 0 0",
+            "amr_luni": 2, "amr_n0": 5, "amr_n1": 5, "amr_n2": 5,
+            "scalarwave1_k": "0 0 0", "scalarwave1_numflux": "upwind",
+            "evolve_method": "RK4", "dtfac": 0.25, "dt": "auto"}
+
+
+def adjoint_leg(nm, steps=256, every=16, reps=3):
+    """Discrete adjoint cost against the forward step.  C5: the config-1 box
+    (64 elements, N=4, 8,000 nodes), 256 RK4 steps, psi = exp(-r^2) with
+    Phi_i = d_i psi; device time of nmx_evolve(256) and nmx_adjoint_evolve(256)
+    (HIP events), and wall time of the torch.autograd forward (checkpoints
+    every 16 steps) and its backward.  Also the adjoint/forward device-time
+    ratio per step on the 3072-element N=7 shell."""
+    import torch
+    from nmesh_amd import adjoint as adj
+    out = {}
+    m = nm.Mesh(c1_params(), device=0)
+    nm.amr_setup_mesh(m)
+    x = m.coords()
+    r2 = (x ** 2).sum(0)
+    psi = np.exp(-r2)
+    u0 = np.zeros((5, m.nnodes))
+    u0[4] = psi
+    u0[1:4] = -2 * x * psi
+    u0 = u0.ravel()
+    rng = np.random.default_rng(2026)
+    lam0 = torch.from_numpy(rng.standard_normal(u0.shape)).cuda()
+    u0_t = torch.from_numpy(u0).cuda()
+    m.upload(u0)
+    nm.evolve_myln(m, every)                    # autotune, graph capture
+    adj.adjoint_evolve(m, lam0, every)
+    fwd, bwd = [], []
+    for _ in range(reps):
+        m.upload(u0)
+        nm.evolve_myln(m, steps)
+        fwd.append(m.last_evolve_ms())
+        adj.adjoint_evolve(m, lam0, steps)
+        bwd.append(m.last_adjoint_ms())
+    out["forward_ms"] = min(fwd)
+    out["adjoint_ms"] = min(bwd)
+    out["adjoint_vs_forward"] = min(bwd) / min(fwd)
+    bnd = adj.outer_boundary_nodes(m)
+    target = torch.zeros(bnd.numel(), dtype=torch.float64, device="cuda")
+    tf, tb = [], []
+    for _ in range(reps):
+        u = u0_t.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        traj = adj.evolve(m, u, steps, every)
+        loss = ((traj[-1][4 * m.nnodes:][bnd] - target) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        (g,) = torch.autograd.grad(loss, u)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        tf.append(t1 - t0)
+        tb.append(t2 - t1)
+    out["autograd_forward_ms"] = 1e3 * min(tf)
+    out["autograd_backward_ms"] = 1e3 * min(tb)
+    out["autograd_backward_vs_forward"] = min(tb) / min(tf)
+    del m
+    # the same ratio on the 3072-element N=7 shell (HBM-bound kernels)
+    m = nm.Mesh(c2_params(), device=0)
+    nm.amr_setup_mesh(m)
+    xs = m.coords()
+    us = shell_initial_data(xs)
+    m.upload(us)
+    lam = torch.from_numpy(rng.standard_normal(us.shape)).cuda()
+    nm.evolve_myln(m, 8)
+    adj.adjoint_evolve(m, lam, 8)
+    nm.evolve_myln(m, 32)
+    f2 = m.last_evolve_ms()
+    adj.adjoint_evolve(m, lam, 32)
+    b2 = m.last_adjoint_ms()
+    out["shell_forward_ms_per_step"] = f2 / 32
+    out["shell_adjoint_ms_per_step"] = b2 / 32
+    out["shell_adjoint_vs_forward"] = b2 / f2
+    out["workload"] = ("C5: config-1 box (64 elements, N=4, 8000 nodes), upwind, zero exterior, "
+                       "%d RK4 steps, checkpoints every %d steps; plus 32 RK4 steps of the "
+                       "3072-element N=7 shell" % (steps, every))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +248,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the AMR batched-vs-per-element leg")
     ap.add_argument("--amr-steps", type=int, default=40, help="RK4 steps of the batched AMR leg")
     ap.add_argument("--only-amr", action="store_true", help="run only the batched AMR leg (profiling)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-cost leg")
+    ap.add_argument("--only-adjoint", action="store_true", help="run only the adjoint-cost leg")
     ap.add_argument("--path", default="plain")
     ap.add_argument("--graph", default="yes", help="capture RK steps in hipGraphs (yes|no)")
     ap.add_argument("--geometry", default="analytic", help="dXdx recomputed (analytic) or streamed (stored)")
@@ -200,6 +286,9 @@ def main():
     if args.only_amr:
         out = batched_leg(nm, args.amr_steps, 0) if args.amr_steps else {}
         print(json.dumps(out), flush=True)
+        return
+    if args.only_adjoint:
+        print(json.dumps(adjoint_leg(nm)), flush=True)
         return
     m = nm.Mesh(c2_params(), device=0)
     m.set_par("nmx_path", args.path)
@@ -274,6 +363,8 @@ def main():
     del m
     if not args.no_batched:
         out["batched_amr"] = batched_leg(nm, args.amr_steps, 1)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(nm)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x)
     print(json.dumps(out), flush=True)
