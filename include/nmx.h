@@ -150,6 +150,50 @@ int nmx_checkpoint_load(nmx_ctx *ctx, int slot, int reg);
 int nmx_state_copy_in(nmx_ctx *ctx, int reg, const double *dflat);
 int nmx_state_copy_out(nmx_ctx *ctx, int reg, double *dflat);
 
+/* ------------------------------------------------------------------------
+ * FV WENO5 + LLF on a periodic grid of 16^3-cell blocks, special-relativistic
+ * Euler with an ideal gas (conserved D, S_i, tau; primitive rho, v_i, p).
+ * Replaces the reconstruction and flux of fv_divf (src/main/dg/fv.c:115-624)
+ * with rec1d_p/m_WENO5 (rec1d.c:1190-1352) and numflux1d_LLF
+ * (numflux1d.c:85-112), stepped by evolve_sspRK3_mesh / evolve_RK4_mesh /
+ * evolve_Euler_mesh (RungeKutta.c:19-188).  The SR-Euler physics has no
+ * counterpart in the reference (its SR projects are absent).  Parameters:
+ * fv_n (cells per edge, a multiple of 16), fv_block (16), fv_gamma,
+ * fv_c2p_tol (Newton tolerance, relative), dt (auto = fv_dtfac * dx, dx = 1),
+ * fv_dtfac, evolve_method, fv_rec (WENO5), fv_flux (LLF), nmx_graph,
+ * nmx_graph_steps, nmx_xcd.  Host arrays are flat: u[var*ncell + cell],
+ * cell = i + N (j + N k).
+ * ------------------------------------------------------------------------ */
+#define NMX_FV_CONS 0     /* conserved state (evolved) */
+#define NMX_FV_PRIM 1     /* primitives of the last c2p */
+#define NMX_FV_RHS 2      /* RHS of nmx_fv_setrhs */
+
+typedef struct nmx_fv nmx_fv;
+
+int nmx_fv_create(nmx_fv **fv, int device);
+int nmx_fv_destroy(nmx_fv *fv);
+int nmx_fv_set_stream(nmx_fv *fv, void *stream);
+int nmx_fv_synchronize(nmx_fv *fv);
+int nmx_fv_set_param(nmx_fv *fv, const char *name, const char *value);
+int nmx_fv_get_param(nmx_fv *fv, const char *name, char *buf, int buflen);
+/* allocate the grid from the parameters; time 0, dt per the dt parameter */
+int nmx_fv_setup(nmx_fv *fv);
+int nmx_fv_ncells(nmx_fv *fv, long *ncell);
+int nmx_fv_get_dt(nmx_fv *fv, double *dt);
+int nmx_fv_set_dt(nmx_fv *fv, double dt);
+int nmx_fv_get_time(nmx_fv *fv, double *t);
+int nmx_fv_set_time(nmx_fv *fv, double t);
+/* primitives -> the conserved state (and the c2p guess) */
+int nmx_fv_upload_prim(nmx_fv *fv, const double *prim);
+int nmx_fv_upload(nmx_fv *fv, int which, const double *u);
+int nmx_fv_download(nmx_fv *fv, int which, double *u);
+/* NMX_FV_RHS = RHS(conserved state), with c2p first */
+int nmx_fv_setrhs(nmx_fv *fv);
+int nmx_fv_evolve(nmx_fv *fv, int nsteps);
+int nmx_fv_last_evolve_ms(nmx_fv *fv, double *ms);
+/* number of blocks whose c2p Newton solve failed since the last call */
+int nmx_fv_status(nmx_fv *fv, long *nbad);
+
 /* profiling helpers: time (ms) of the last nmx_evolve on the device and the
  * number of kernel launches it issued */
 int nmx_last_evolve_ms(nmx_ctx *ctx, double *ms);

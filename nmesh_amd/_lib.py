@@ -58,6 +58,25 @@ def _proto(lib):
         "nmx_checkpoint_load": (i, [vp, i, i]),
         "nmx_state_copy_in": (i, [vp, i, vp]),
         "nmx_state_copy_out": (i, [vp, i, vp]),
+        "nmx_fv_create": (i, [P(vp), i]),
+        "nmx_fv_destroy": (i, [vp]),
+        "nmx_fv_set_stream": (i, [vp, vp]),
+        "nmx_fv_synchronize": (i, [vp]),
+        "nmx_fv_set_param": (i, [vp, C.c_char_p, C.c_char_p]),
+        "nmx_fv_get_param": (i, [vp, C.c_char_p, C.c_char_p, i]),
+        "nmx_fv_setup": (i, [vp]),
+        "nmx_fv_ncells": (i, [vp, P(l)]),
+        "nmx_fv_get_dt": (i, [vp, P(d)]),
+        "nmx_fv_set_dt": (i, [vp, d]),
+        "nmx_fv_get_time": (i, [vp, P(d)]),
+        "nmx_fv_set_time": (i, [vp, d]),
+        "nmx_fv_upload_prim": (i, [vp, vp]),
+        "nmx_fv_upload": (i, [vp, i, vp]),
+        "nmx_fv_download": (i, [vp, i, vp]),
+        "nmx_fv_setrhs": (i, [vp]),
+        "nmx_fv_evolve": (i, [vp, i]),
+        "nmx_fv_last_evolve_ms": (i, [vp, P(d)]),
+        "nmx_fv_status": (i, [vp, P(l)]),
         "nmx_last_evolve_ms": (i, [vp, P(d)]),
     }
     for name, (res, args) in sig.items():

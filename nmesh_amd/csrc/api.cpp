@@ -15,6 +15,7 @@
 
 #include "../../include/nmx.h"
 #include "amr.h"
+#include "errors.h"
 #include "kernels.h"
 #include "mesh.h"
 
@@ -22,10 +23,12 @@ using namespace nmx;
 
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string &msg) {
+int nmx::set_error(int code, const std::string &msg) {
   g_err = msg;
   return code;
 }
+
+static int fail(int code, const std::string &msg) { return set_error(code, msg); }
 
 #define HIPCHK(x)                                                                   \
   do {                                                                              \

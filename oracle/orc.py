@@ -49,6 +49,17 @@ def lib():
             "orc_rhs": (None, [vp, i, vp, vp, d, vp]),
             "orc_evolve": (d, [vp, i, vp, i, d, i, d, vp]),
             "orc_num_threads": (i, []),
+            "orc_rec1d_p_WENO5": (d, [vp, i]),
+            "orc_rec1d_m_WENO5": (d, [vp, i]),
+            "orc_sr_p2c": (None, [d, vp, vp]),
+            "orc_sr_c2p": (i, [d, vp, vp, d]),
+            "orc_sr_flux": (None, [d, i, vp, vp, vp, vp]),
+            "orc_sr_llf": (None, [d, i, vp, vp, vp]),
+            "orc_fv_c2p": (i, [i, d, vp, vp, d]),
+            "orc_fv_p2c": (None, [i, d, vp, vp]),
+            "orc_fv_rhs_prim": (None, [i, d, vp, vp]),
+            "orc_fv_rhs": (i, [i, d, vp, vp, vp, d]),
+            "orc_fv_evolve": (i, [i, d, d, i, vp, vp, d]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -155,3 +166,117 @@ class OMesh:
         t = lib().orc_evolve(self.h, NUMFLUX[numflux], _p(k), METHOD[method], float(dt),
                              int(nsteps), float(t0), _p(u))
         return u, t
+
+
+# ---------------------------------------------------------------------------
+# FV WENO5 + LLF, SR-Euler (fv_oracle.c)
+# ---------------------------------------------------------------------------
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def rec_p(u, im):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    return lib().orc_rec1d_p_WENO5(_ptr(u), int(im))
+
+
+def rec_m(u, im):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    return lib().orc_rec1d_m_WENO5(_ptr(u), int(im))
+
+
+def sr_p2c(gam, P):
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    U = np.zeros(5)
+    lib().orc_sr_p2c(float(gam), _ptr(P), _ptr(U))
+    return U
+
+
+def sr_c2p(gam, U, guess_p, tol=1e-12):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    P = np.zeros(5)
+    P[4] = guess_p
+    it = lib().orc_sr_c2p(float(gam), _ptr(U), _ptr(P), float(tol))
+    return P, it
+
+
+def sr_flux(gam, d, P):
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    F, U, lam = np.zeros(5), np.zeros(5), np.zeros(2)
+    lib().orc_sr_flux(float(gam), int(d), _ptr(P), _ptr(F), _ptr(U), _ptr(lam))
+    return F, U, lam
+
+
+def sr_llf(gam, d, PL, PR):
+    PL = np.ascontiguousarray(PL, dtype=np.float64)
+    PR = np.ascontiguousarray(PR, dtype=np.float64)
+    F = np.zeros(5)
+    lib().orc_sr_llf(float(gam), int(d), _ptr(PL), _ptr(PR), _ptr(F))
+    return F
+
+
+class OFV:
+    """Periodic N^3 grid of the FV oracle; states are flat [var][cell],
+    cell = i + N (j + N k)."""
+
+    def __init__(self, N, gam=2.0, tol=1e-12):
+        self.N, self.gam, self.tol = int(N), float(gam), float(tol)
+        self.ncell = self.N ** 3
+
+    def p2c(self, P):
+        P = np.ascontiguousarray(P, dtype=np.float64).ravel()
+        u = np.zeros_like(P)
+        lib().orc_fv_p2c(self.N, self.gam, _ptr(P), _ptr(u))
+        return u
+
+    def c2p(self, u, P_guess):
+        u = np.ascontiguousarray(u, dtype=np.float64).ravel()
+        P = np.ascontiguousarray(P_guess, dtype=np.float64).ravel().copy()
+        bad = lib().orc_fv_c2p(self.N, self.gam, _ptr(u), _ptr(P), self.tol)
+        return P, bad
+
+    def rhs_prim(self, P):
+        P = np.ascontiguousarray(P, dtype=np.float64).ravel()
+        r = np.zeros_like(P)
+        lib().orc_fv_rhs_prim(self.N, self.gam, _ptr(P), _ptr(r))
+        return r
+
+    def rhs(self, u, P_guess):
+        u = np.ascontiguousarray(u, dtype=np.float64).ravel()
+        P = np.ascontiguousarray(P_guess, dtype=np.float64).ravel().copy()
+        r = np.zeros_like(u)
+        bad = lib().orc_fv_rhs(self.N, self.gam, _ptr(u), _ptr(P), _ptr(r), self.tol)
+        return r, P, bad
+
+    def evolve(self, u, P_guess, dt, nsteps):
+        u = np.ascontiguousarray(u, dtype=np.float64).ravel().copy()
+        P = np.ascontiguousarray(P_guess, dtype=np.float64).ravel().copy()
+        bad = lib().orc_fv_evolve(self.N, self.gam, float(dt), int(nsteps), _ptr(u), _ptr(P), self.tol)
+        return u, P, bad
+
+
+def fv_blob_initial_data(N, seed=2026, nblobs=32):
+    """Primitives of the FV benchmark: rho = 1 + sum of nblobs Gaussian blobs
+    (amplitude U[0.1, 1], width U[2, 8] cells, centres U[0, N)^3, periodic
+    minimum-image distance), v_i = U[-0.3, 0.3] per cell, p = 1; numpy PCG64
+    with the given seed.  Returns P flat [var][cell]."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(0.1, 1.0, nblobs)
+    w = rng.uniform(2.0, 8.0, nblobs)
+    xc = rng.uniform(0.0, N, (nblobs, 3))
+    x = np.arange(N) + 0.5
+    rho = np.ones((N, N, N))          # indexed [k, j, i]
+    for b in range(nblobs):
+        g = []
+        for d in range(3):
+            dd = x - xc[b, d]
+            dd = dd - N * np.round(dd / N)
+            g.append(np.exp(-dd ** 2 / (2 * w[b] ** 2)))
+        rho += A[b] * g[2][:, None, None] * g[1][None, :, None] * g[0][None, None, :]
+    v = rng.uniform(-0.3, 0.3, (3, N, N, N))
+    P = np.empty((5, N, N, N))
+    P[0] = rho
+    P[1:4] = v
+    P[4] = 1.0
+    return P.reshape(5, -1)
