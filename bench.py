@@ -155,6 +155,60 @@ def batched_leg(nm, steps, plain_steps):
     return out
 
 
+# FV (config 3): algorithmic figures per cell-update (one RHS evaluation at
+# one cell, all 5 fields; sspRK3 does 3 per cell per step).  Bytes: the c2p
+# pass reads the conserved state and the pressure guess and writes the
+# primitives (11 doubles); the stage pass reads the primitives once and the
+# stage registers (win, u_p, u_acc: 5, 15 and 10 doubles in stages 1, 2, 3)
+# and writes u_acc, w (10, 10, 5): (11 + 5 + (15 + 25 + 15) / 3) * 8 = 275 B.
+# FLOPs (hand count of the kernel's formulas, a division or a square root
+# counted as one): 10 WENO5 reconstructions of 68 FLOPs, 2 SR fluxes with
+# eigenvalues of 60, LLF and accumulation 32, physicality checks 10: 852
+# per face, 3 faces per cell; c2p about 115 (5 Newton iterations of 20 and
+# the primitives) and the stage combination 15: about 2690 per cell-update.
+FV_BYTES_PER_CU = 275.0
+FV_FLOPS_PER_CU = 2690.0
+
+
+def fv_leg(steps=50, warmup=8, N=256, cpu=True):
+    """Config 3: FV WENO5 + LLF, special-relativistic Euler (Gamma = 2) on a
+    periodic 256^3 grid of 16^3-cell blocks, 32 seeded Gaussian density
+    blobs, per-cell velocities U[-0.3, 0.3], p = 1; sspRK3 at dt = 0.25."""
+    from nmesh_amd.fv import FvGrid, blob_initial_data
+    P = blob_initial_data(N)
+    g = FvGrid({"fv_n": N, "evolve_method": "sspRK3"}, device=0)
+    g.setup()
+    g.upload_prim(P)
+    g.evolve(warmup)                 # includes the graph capture
+    t0 = time.perf_counter()
+    g.evolve(steps)
+    wall = time.perf_counter() - t0
+    dev_s = g.last_evolve_ms() * 1e-3
+    cu = 3.0 * steps * N ** 3
+    out = {"workload": "SR-Euler FV WENO5 + LLF, %d^3 periodic cells in 16^3 blocks, sspRK3, %d steps" % (N, steps),
+           "value": cu / wall, "unit": "cell-updates/s", "ms_per_step": 1e3 * wall / steps,
+           "device_ms_per_step": 1e3 * dev_s / steps, "c2p_failed_blocks": g.c2p_failures(),
+           "bytes_per_cell_update": FV_BYTES_PER_CU, "flops_per_cell_update": FV_FLOPS_PER_CU,
+           "achieved_GBps": FV_BYTES_PER_CU * cu / dev_s / 1e9,
+           "achieved_TFLOPs": FV_FLOPS_PER_CU * cu / dev_s / 1e12}
+    out["hbm_frac"] = out["achieved_GBps"] / HBM_PEAK_GBS
+    out["fp64_frac"] = out["achieved_TFLOPs"] / FP64_PEAK_TFLOPS
+    del g
+    if cpu:
+        from oracle.orc import OFV, lib
+        n = 64
+        Pc = blob_initial_data(n)
+        ofv = OFV(n)
+        u = ofv.p2c(Pc)
+        t0 = time.perf_counter()
+        ofv.evolve(u, Pc, 0.25, 2)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 3 * 2 * n ** 3 / el, "unit": "cell-updates/s",
+                               "cores": int(lib().orc_num_threads()), "kind": "port",
+                               "sample": "2 sspRK3 steps of the same scheme on a 64^3 grid (oracle, OpenMP C)"}
+    return out
+
+
 def c1_params():
     return {"amr_mesh_type": "BoxMesh", "amr_BoxMesh_dout": 8.0, "amr_mesh_xc": "0 0 0",
             "amr_luni": 2, "amr_n0": 5, "amr_n1": 5, "amr_n2": 5,
@@ -249,6 +303,9 @@ def main():
     ap.add_argument("--amr-steps", type=int, default=40, help="RK4 steps of the batched AMR leg")
     ap.add_argument("--only-amr", action="store_true", help="run only the batched AMR leg (profiling)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-cost leg")
+    ap.add_argument("--no-fv", action="store_true", help="skip the FV (SR-Euler WENO5) leg")
+    ap.add_argument("--only-fv", action="store_true", help="run only the FV leg")
+    ap.add_argument("--fv-steps", type=int, default=50, help="sspRK3 steps of the FV leg")
     ap.add_argument("--only-adjoint", action="store_true", help="run only the adjoint-cost leg")
     ap.add_argument("--path", default="plain")
     ap.add_argument("--graph", default="yes", help="capture RK steps in hipGraphs (yes|no)")
@@ -289,6 +346,9 @@ def main():
         return
     if args.only_adjoint:
         print(json.dumps(adjoint_leg(nm)), flush=True)
+        return
+    if args.only_fv:
+        print(json.dumps(fv_leg(args.fv_steps, cpu=not args.no_cpu)), flush=True)
         return
     m = nm.Mesh(c2_params(), device=0)
     m.set_par("nmx_path", args.path)
@@ -365,6 +425,8 @@ def main():
         out["batched_amr"] = batched_leg(nm, args.amr_steps, 1)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(nm)
+    if not args.no_fv:
+        out["fv_sr_euler"] = fv_leg(args.fv_steps, cpu=not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x)
     print(json.dumps(out), flush=True)

@@ -111,3 +111,29 @@ class FvGrid:
         n = C.c_long()
         check(lib().nmx_fv_status(self._h, C.byref(n)))
         return n.value
+
+
+def blob_initial_data(N, seed=2026, nblobs=32):
+    """Primitives of the FV benchmark workload: rho = 1 + sum of nblobs
+    Gaussian blobs (amplitude U[0.1, 1], width U[2, 8] cells, centres
+    U[0, N)^3, periodic minimum-image distance), v_i = U[-0.3, 0.3] per cell,
+    p = 1; numpy PCG64 with the given seed.  Returns P flat [var][cell]."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(0.1, 1.0, nblobs)
+    w = rng.uniform(2.0, 8.0, nblobs)
+    xc = rng.uniform(0.0, N, (nblobs, 3))
+    x = np.arange(N) + 0.5
+    rho = np.ones((N, N, N))          # indexed [k, j, i]
+    for b in range(nblobs):
+        g = []
+        for d in range(3):
+            dd = x - xc[b, d]
+            dd = dd - N * np.round(dd / N)
+            g.append(np.exp(-dd ** 2 / (2 * w[b] ** 2)))
+        rho += A[b] * g[2][:, None, None] * g[1][None, :, None] * g[0][None, None, :]
+    v = rng.uniform(-0.3, 0.3, (3, N, N, N))
+    P = np.empty((5, N, N, N))
+    P[0] = rho
+    P[1:4] = v
+    P[4] = 1.0
+    return P.reshape(5, -1)

@@ -12,6 +12,7 @@ index or direction breaks the RHS convergence to the exact flux difference.
 import numpy as np
 import pytest
 
+from nmesh_amd.fv import blob_initial_data
 from oracle import orc
 
 GAM = 2.0
@@ -180,12 +181,12 @@ def test_rhs_converges_at_fifth_order_to_the_exact_flux_difference(d):
 
 def test_rhs_conserves_every_component():
     N = 16
-    P = orc.fv_blob_initial_data(N, seed=7, nblobs=4)
+    P = blob_initial_data(N, seed=7, nblobs=4)
     r = orc.OFV(N).rhs_prim(P).reshape(5, -1)
     assert np.abs(r.sum(1)).max() < 1e-11 * np.abs(r).max() * N ** 3
 
 
 def test_blob_initial_data_has_the_stated_ranges():
-    P = orc.fv_blob_initial_data(32, seed=2026)
+    P = blob_initial_data(32, seed=2026)
     assert P[0].min() >= 1.0 and P[0].max() < 1.0 + 32
     assert np.abs(P[1:4]).max() <= 0.3 and np.all(P[4] == 1.0)

@@ -9,6 +9,7 @@ each RK step to ~1e-15.  The bar is 1e-12 relative to the field maximum.
 import numpy as np
 import pytest
 
+from nmesh_amd.fv import blob_initial_data
 from oracle import orc
 
 pytestmark = pytest.mark.gpu
@@ -30,7 +31,7 @@ def rel(a, b):
 @pytest.mark.parametrize("N,xcd", [(32, "yes"), (48, "yes"), (48, "no")])
 def test_rhs_and_primitives_match_oracle(N, xcd):
     print("fv rhs", N, xcd, flush=True)
-    P = orc.fv_blob_initial_data(N, seed=11, nblobs=6)
+    P = blob_initial_data(N, seed=11, nblobs=6)
     g = make(N, xcd=xcd)
     g.upload_prim(P)
     ofv = orc.OFV(N)
@@ -47,7 +48,7 @@ def test_rhs_and_primitives_match_oracle(N, xcd):
 def test_steps_match_oracle_every_step(method, nsteps):
     print("fv steps", method, nsteps, flush=True)
     N = 32
-    P = orc.fv_blob_initial_data(N, seed=12, nblobs=6)
+    P = blob_initial_data(N, seed=12, nblobs=6)
     g = make(N, method=method, gsteps=2)
     g.upload_prim(P)
     ofv = orc.OFV(N)
@@ -103,7 +104,7 @@ def test_periodic_shift_commutes_with_the_step():
     """Shifting the data by a whole block shifts the result bitwise: periodic
     wrap, block addressing and tile mapping are consistent."""
     N = 48
-    P = orc.fv_blob_initial_data(N, seed=13, nblobs=4).reshape(5, N, N, N)
+    P = blob_initial_data(N, seed=13, nblobs=4).reshape(5, N, N, N)
     Ps = np.roll(P, shift=(16, -16, 16), axis=(1, 2, 3))
     g1, g2 = make(N), make(N)
     g1.upload_prim(P.reshape(5, -1))
