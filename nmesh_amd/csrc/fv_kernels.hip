@@ -13,6 +13,10 @@
 //                    (numflux1d.c:85-112) and accumulates the flux
 //                    difference of each cell in LDS; the epilogue applies the
 //                    RK stage combination (RungeKutta.c:19-188).
+// The WENO5 formulas are the reference's; two rearrangements save work: the
+// nonlinear weights are normalised with one division instead of four, and
+// the right-side smoothness indicators of a face are kept for the left side
+// of the next face, where the same stencils appear in the same roles.
 // The SR-Euler flux, eigenvalues and c2p are textbook (Marti & Mueller,
 // Living Rev. Relativ. 6 (2003) 7); the reference has no SR-Euler system.
 #include <hip/hip_runtime.h>
@@ -28,40 +32,55 @@ constexpr int kFvT3 = kFvT * kFvT * kFvT;
 constexpr double kWeno5Eps = 1e-40;
 constexpr int kC2pMaxIt = 100;
 
-// midpoint from the left over (a, b, c, d, e) = u[im-2 .. im+2]
-__device__ __forceinline__ double weno5_p(double a, double b, double c, double d, double e) {
-  const double s01 = a - 2. * b + c, s02 = a - 4. * b + 3. * c;
-  const double s11 = b - 2. * c + d, s12 = b - d;
-  const double s21 = c - 2. * d + e, s22 = 3. * c - 4. * d + e;
-  const double beta0 = (13. / 12.) * s01 * s01 + 0.25 * s02 * s02;
-  const double beta1 = (13. / 12.) * s11 * s11 + 0.25 * s12 * s12;
-  const double beta2 = (13. / 12.) * s21 * s21 + 0.25 * s22 * s22;
-  const double e0 = beta0 + kWeno5Eps, e1 = beta1 + kWeno5Eps, e2 = beta2 + kWeno5Eps;
-  const double w0 = 1. / (e0 * e0), w1 = 6. / (e1 * e1), w2 = 3. / (e2 * e2);
-  const double inv = 1. / (w0 + w1 + w2);
+// WENO5 smoothness indicators of a 3-cell stencil (x, y, z) by the position
+// of the midpoint it reconstructs: right end (role 0), centre (role 1), left
+// end (role 2); rec1d_p_WENO5 uses roles 0, 1, 2 on cells im-2.., im-1..,
+// im..; rec1d_m_WENO5 uses roles 0, 1, 2 on cells im-1.., im.., im+1..
+// (rec1d.c:1195-1300)
+__device__ __forceinline__ double beta_r0(double x, double y, double z) {
+  const double s1 = x - 2. * y + z, s2 = x - 4. * y + 3. * z;
+  return (13. / 12.) * s1 * s1 + 0.25 * s2 * s2;
+}
+__device__ __forceinline__ double beta_r1(double x, double y, double z) {
+  const double s1 = x - 2. * y + z, s2 = x - z;
+  return (13. / 12.) * s1 * s1 + 0.25 * s2 * s2;
+}
+__device__ __forceinline__ double beta_r2(double x, double y, double z) {
+  const double s1 = x - 2. * y + z, s2 = 3. * x - 4. * y + z;
+  return (13. / 12.) * s1 * s1 + 0.25 * s2 * s2;
+}
+
+// Nonlinear combination with one division: the weights g_k / (beta_k + eps)^2
+// normalised equal  g_k prod_{j != k} (beta_j + eps)^2  normalised (for the
+// data here every product stays between 1e-240 and 1e+60).
+__device__ __forceinline__ double weno5_combine(double g0, double g1, double g2, double b0, double b1,
+                                                double b2, double q0, double q1, double q2) {
+  const double e0 = b0 + kWeno5Eps, e1 = b1 + kWeno5Eps, e2 = b2 + kWeno5Eps;
+  const double E0 = e0 * e0, E1 = e1 * e1, E2 = e2 * e2;
+  const double A0 = g0 * (E1 * E2), A1 = g1 * (E0 * E2), A2 = g2 * (E0 * E1);
+  return (A0 * q0 + A1 * q1 + A2 * q2) / (A0 + A1 + A2);
+}
+
+// midpoint from the left over (a, b, c, d, e) = u[im-2 .. im+2], ideal
+// weights (1, 6, 3), stencils (2,-7,11)/6, (-1,5,2)/6, (2,5,-1)/6
+__device__ __forceinline__ double weno5_p(double a, double b, double c, double d, double e, double b0,
+                                          double b1, double b2) {
   const double o6 = 1. / 6.;
   const double q0 = (2. * o6) * a + (-7. * o6) * b + (11. * o6) * c;
   const double q1 = (-o6) * b + (5. * o6) * c + (2. * o6) * d;
   const double q2 = (2. * o6) * c + (5. * o6) * d + (-o6) * e;
-  return (w0 * inv) * q0 + (w1 * inv) * q1 + (w2 * inv) * q2;
+  return weno5_combine(1., 6., 3., b0, b1, b2, q0, q1, q2);
 }
 
-// midpoint from the right over (a, b, c, d, e) = u[im-1 .. im+3]
-__device__ __forceinline__ double weno5_m(double a, double b, double c, double d, double e) {
-  const double s21 = c - 2. * d + e, s22 = 3. * c - 4. * d + e;
-  const double s11 = b - 2. * c + d, s12 = b - d;
-  const double s01 = a - 2. * b + c, s02 = a - 4. * b + 3. * c;
-  const double beta2 = (13. / 12.) * s21 * s21 + 0.25 * s22 * s22;
-  const double beta1 = (13. / 12.) * s11 * s11 + 0.25 * s12 * s12;
-  const double beta0 = (13. / 12.) * s01 * s01 + 0.25 * s02 * s02;
-  const double e2 = beta2 + kWeno5Eps, e1 = beta1 + kWeno5Eps, e0 = beta0 + kWeno5Eps;
-  const double w2 = 1. / (e2 * e2), w1 = 6. / (e1 * e1), w0 = 3. / (e0 * e0);
-  const double inv = 1. / (w0 + w1 + w2);
+// midpoint from the right over (a, b, c, d, e) = u[im-1 .. im+3], ideal
+// weights (3, 6, 1) on the stencils starting at im-1, im, im+1
+__device__ __forceinline__ double weno5_m(double a, double b, double c, double d, double e, double b0,
+                                          double b1, double b2) {
   const double o6 = 1. / 6.;
   const double q2 = (11. * o6) * c + (-7. * o6) * d + (2. * o6) * e;
   const double q1 = (2. * o6) * b + (5. * o6) * c + (-o6) * d;
   const double q0 = (-o6) * a + (5. * o6) * b + (2. * o6) * c;
-  return (w0 * inv) * q0 + (w1 * inv) * q1 + (w2 * inv) * q2;
+  return weno5_combine(3., 6., 1., b0, b1, b2, q0, q1, q2);
 }
 
 __device__ __forceinline__ void sr_p2c(double gam, const double P[5], double U[5]) {
@@ -77,26 +96,34 @@ __device__ __forceinline__ void sr_p2c(double gam, const double P[5], double U[5
   U[4] = rhW2 - p - U[0];
 }
 
-// flux along DIR, conserved state, and max |acoustic eigenvalue|
+// flux along DIR, conserved state, and max |acoustic eigenvalue|;
+// g1 = gam / (gam - 1)
 template <int DIR>
-__device__ __forceinline__ double sr_flux(double gam, const double P[5], double F[5], double U[5]) {
-  sr_p2c(gam, P, U);
-  const double rho = P[0], p = P[4];
+__device__ __forceinline__ double sr_flux(double gam, double g1, const double P[5], double F[5], double U[5]) {
+  const double rho = P[0], vx = P[1], vy = P[2], vz = P[3], p = P[4];
+  const double v2 = vx * vx + vy * vy + vz * vz;
+  const double W = 1. / sqrt(1. - v2);
+  const double ph = p / rho;
+  const double h = 1. + g1 * ph;
+  const double rhW2 = rho * h * W * W;
+  U[0] = rho * W;
+  U[1] = rhW2 * vx;
+  U[2] = rhW2 * vy;
+  U[3] = rhW2 * vz;
+  U[4] = rhW2 - p - U[0];
   const double vd = P[1 + DIR];
-  const double v2 = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
   F[0] = U[0] * vd;
   F[1] = U[1] * vd;
   F[2] = U[2] * vd;
   F[3] = U[3] * vd;
   F[1 + DIR] += p;
   F[4] = (U[4] + p) * vd;
-  const double h = 1. + gam / (gam - 1.) * p / rho;
-  const double cs2 = gam * p / (rho * h);
+  const double cs2 = gam * ph / h;
   const double cs = sqrt(cs2);
-  const double den = 1. - v2 * cs2;
+  const double oden = 1. / (1. - v2 * cs2);
   const double root = cs * sqrt((1. - v2) * (1. - v2 * cs2 - vd * vd * (1. - cs2)));
-  const double lm = (vd * (1. - cs2) - root) / den;
-  const double lp = (vd * (1. - cs2) + root) / den;
+  const double lm = (vd * (1. - cs2) - root) * oden;
+  const double lp = (vd * (1. - cs2) + root) * oden;
   return fmax(fabs(lm), fabs(lp));
 }
 
@@ -244,6 +271,16 @@ __device__ __forceinline__ void fv_sweep(const FvStageArgs &a, int N, int x0, in
     }
     load_cell<DIR>(a, N, x0, y0, z0, ta, tb, 3, nx);
   }
+  // right-side smoothness indicators of the previous face = left-side ones
+  // of this face (same stencils, same roles)
+  double bm[5][3];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    bm[v][0] = beta_r0(q[v][0], q[v][1], q[v][2]);
+    bm[v][1] = beta_r1(q[v][1], q[v][2], q[v][3]);
+    bm[v][2] = beta_r2(q[v][2], q[v][3], q[v][4]);
+  }
+  const double g1 = gam / (gam - 1.);
   double Fprev[5] = {0, 0, 0, 0, 0};
 #pragma unroll 1
   for (int f = 0; f <= kFvT; ++f) {   // face between tile cells f-1 and f
@@ -259,8 +296,12 @@ __device__ __forceinline__ void fv_sweep(const FvStageArgs &a, int N, int x0, in
     double PL[5], PR[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-      PL[v] = weno5_p(q[v][0], q[v][1], q[v][2], q[v][3], q[v][4]);
-      PR[v] = weno5_m(q[v][1], q[v][2], q[v][3], q[v][4], q[v][5]);
+      const double bp0 = bm[v][0], bp1 = bm[v][1], bp2 = bm[v][2];
+      bm[v][0] = beta_r0(q[v][1], q[v][2], q[v][3]);
+      bm[v][1] = beta_r1(q[v][2], q[v][3], q[v][4]);
+      bm[v][2] = beta_r2(q[v][3], q[v][4], q[v][5]);
+      PL[v] = weno5_p(q[v][0], q[v][1], q[v][2], q[v][3], q[v][4], bp0, bp1, bp2);
+      PR[v] = weno5_m(q[v][1], q[v][2], q[v][3], q[v][4], q[v][5], bm[v][0], bm[v][1], bm[v][2]);
     }
     if (!physical(PL)) {
 #pragma unroll
@@ -271,8 +312,8 @@ __device__ __forceinline__ void fv_sweep(const FvStageArgs &a, int N, int x0, in
       for (int v = 0; v < 5; ++v) PR[v] = q[v][3];
     }
     double FL[5], FR[5], UL[5], UR[5];
-    const double aL = sr_flux<DIR>(gam, PL, FL, UL);
-    const double aR = sr_flux<DIR>(gam, PR, FR, UR);
+    const double aL = sr_flux<DIR>(gam, g1, PL, FL, UL);
+    const double aR = sr_flux<DIR>(gam, g1, PR, FR, UR);
     const double am = fmax(aL, aR);
     double F[5];
 #pragma unroll
