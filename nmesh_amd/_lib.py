@@ -101,6 +101,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 "libnmx.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        # A process must hold one HIP runtime.  torch, when installed, ships
+        # its own libamdhip64; loading it first lets libnmx bind to the same
+        # runtime (the other order leaves torch without devices).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         _proto_names = list(_proto(L).keys())
         _lib = L
