@@ -68,7 +68,8 @@ struct nmx_ctx {
   int *dfaceB = nullptr, *djobs1 = nullptr, *djobs4 = nullptr, *dfirst4 = nullptr;
   double *dBtab = nullptr, *dDall = nullptr, *dxball = nullptr, *daj = nullptr;
   int nfint1 = 0, nfint4 = 0;   // interpolated faces with one / several neighbours
-  struct Bucket { int n; int count; int *dlist; long doff; long xoff; };
+  // div: nodes per thread of this bucket's tile (0: the nmx_block parameter)
+  struct Bucket { int n; int count; int *dlist; long doff; long xoff; int div; };
   std::vector<Bucket> buckets;
   std::vector<long> eoff;   // node offset per element
   // graph cache
@@ -84,6 +85,9 @@ struct nmx_ctx {
   static constexpr int kHelpers = 9;
   hipStream_t helper[kHelpers] = {};
   hipEvent_t evfork = nullptr, evjoin[kHelpers] = {};
+  // the faces with several neighbours run beside the single-neighbour ones
+  hipStream_t fhelper = nullptr;
+  hipEvent_t evface0 = nullptr, evface1 = nullptr;
   double last_ms = 0;
   bool tuned = false;
   // adjoint: lam register, neighbour face index per element face (plain
@@ -163,6 +167,9 @@ extern "C" int nmx_ctx_create(nmx_ctx **out, int device) {
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
   HIPCHK(hipEventCreateWithFlags(&c->evfork, hipEventDisableTiming));
+  HIPCHK(hipStreamCreateWithFlags(&c->fhelper, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&c->evface0, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&c->evface1, hipEventDisableTiming));
   for (int i = 0; i < nmx_ctx::kHelpers; ++i) {
     HIPCHK(hipStreamCreateWithFlags(&c->helper[i], hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&c->evjoin[i], hipEventDisableTiming));
@@ -214,6 +221,9 @@ extern "C" int nmx_ctx_destroy(nmx_ctx *c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->evfork) (void)hipEventDestroy(c->evfork);
+  if (c->evface0) (void)hipEventDestroy(c->evface0);
+  if (c->evface1) (void)hipEventDestroy(c->evface1);
+  if (c->fhelper) (void)hipStreamDestroy(c->fhelper);
   for (int i = 0; i < nmx_ctx::kHelpers; ++i) {
     if (c->evjoin[i]) (void)hipEventDestroy(c->evjoin[i]);
     if (c->helper[i]) (void)hipStreamDestroy(c->helper[i]);
@@ -256,6 +266,8 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
     return fail(NMX_EINVAL, "nmx_block must be full, half or quarter");
   c->par[k] = v;
   if (k == "nmx_block" || k == "nmx_xcd") c->tuned = true;   // explicit choice wins
+  if (k == "nmx_block")
+    for (auto &b : c->buckets) b.div = 0;
   if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
   c->graph_steps = 0;
@@ -404,6 +416,7 @@ static int upload_amr_tables(nmx_ctx *c) {
     b.n = n;
     b.count = (int)list.size();
     b.dlist = nullptr;
+    b.div = 0;
     if ((r = dev_upload(&b.dlist, list))) return r;
     const Basis &bs = lgl_basis(n);
     b.doff = (long)dall.size();
@@ -777,8 +790,7 @@ static StageArgs base_args(nmx_ctx *c) {
   return a;
 }
 
-static int launch_amr(nmx_ctx *c, const StageArgs &sa) {
-  const bool perelm = (P(c, "nmx_path") == "perelement");
+static AmrArgs amr_args(nmx_ctx *c, const StageArgs &sa) {
   AmrArgs a;
   a.win = sa.win; a.U = sa.U; a.Ain = sa.Ain; a.Aout = sa.Aout; a.Wout = sa.Wout; a.R = sa.R;
   a.soff = c->dsoff; a.face = c->dfaceB; a.aj = c->daj; a.dXbdX = c->dXbdX; a.egeo = c->degeo; a.edom = c->dedom;
@@ -786,20 +798,52 @@ static int launch_amr(nmx_ctx *c, const StageArgs &sa) {
   a.cA = sa.cA; a.cW = sa.cW;
   for (int d = 0; d < 3; ++d) a.k[d] = sa.k[d];
   a.om = sa.om; a.wmode = sa.wmode; a.numflux = sa.numflux; a.profile = sa.profile;
+  return a;
+}
+
+// one order bucket's stage kernel with tile div (1, 2 or 4 nodes per thread)
+static int launch_bucket(nmx_ctx *c, AmrArgs a, const nmx_ctx::Bucket &b, int div, int xcd_map,
+                         hipStream_t st) {
+  a.D = c->dDall + b.doff;
+  a.xb = c->dxball + b.xoff;
+  a.wend_inv = 1.0 / lgl_basis(b.n).w[0];
+  a.elist = b.dlist;
+  a.nlist = b.count;
+  a.xcd_map = xcd_map;
+  int grid = a.xcd_map ? 8 * ((b.count + 7) / 8) : b.count;
+  hipError_t err = launch_dg_amr(b.n, c->geom, a, grid, div, st);
+  if (err != hipSuccess) return fail(NMX_EHIP, std::string("stage launch: ") + hipGetErrorString(err));
+  return NMX_OK;
+}
+
+static int launch_amr(nmx_ctx *c, const StageArgs &sa) {
+  const bool perelm = (P(c, "nmx_path") == "perelement");
+  AmrArgs a = amr_args(c, sa);
   // interpolated faces of this stage first, all orders in one launch per list
   {
     FaceArgs fa;
     fa.win = sa.win; fa.Btab = c->dBtab; fa.aj = c->daj;
+    const bool side = c->nfint4 > 0 && !perelm;
+    hipError_t err = hipSuccess;
+    if (side) {
+      HIPCHK(hipEventRecord(c->evface0, c->stream));
+      HIPCHK(hipStreamWaitEvent(c->fhelper, c->evface0, 0));
+      FaceArgs f4 = fa;
+      f4.jobs = c->djobs4; f4.first = c->dfirst4; f4.nfaces = c->nfint4;
+      err = launch_face_interp(f4, true, c->fhelper);
+      if (err == hipSuccess) HIPCHK(hipEventRecord(c->evface1, c->fhelper));
+    }
     fa.jobs = c->djobs1; fa.first = nullptr; fa.nfaces = c->nfint1;
-    hipError_t err = launch_face_interp(fa, false, c->stream);
-    if (err == hipSuccess) {
+    if (err == hipSuccess) err = launch_face_interp(fa, false, c->stream);
+    if (err == hipSuccess && !side) {
       fa.jobs = c->djobs4; fa.first = c->dfirst4; fa.nfaces = c->nfint4;
       err = launch_face_interp(fa, true, c->stream);
     }
     if (err != hipSuccess) return fail(NMX_EHIP, std::string("face launch: ") + hipGetErrorString(err));
+    if (side) HIPCHK(hipStreamWaitEvent(c->stream, c->evface1, 0));
   }
   std::string blk = P(c, "nmx_block");
-  const int div = blk == "half" ? 2 : blk == "quarter" ? 4 : 1;
+  const int div0 = blk == "half" ? 2 : blk == "quarter" ? 4 : 1;
   // buckets run concurrently on helper streams forked from the context's
   // stream (parallel branches when captured into a graph)
   const bool fork = !perelm && c->buckets.size() > 1 && P(c, "nmx_concurrent") == "yes";
@@ -814,6 +858,7 @@ static int launch_amr(nmx_ctx *c, const StageArgs &sa) {
     a.D = c->dDall + b.doff;
     a.xb = c->dxball + b.xoff;
     a.wend_inv = 1.0 / lgl_basis(b.n).w[0];
+    const int div = b.div ? b.div : div0;
     if (perelm) {
       a.xcd_map = 0;
       a.nlist = 1;
@@ -824,12 +869,8 @@ static int launch_amr(nmx_ctx *c, const StageArgs &sa) {
       }
       continue;
     }
-    a.elist = b.dlist;
-    a.nlist = b.count;
-    a.xcd_map = sa.xcd_map;
-    int grid = a.xcd_map ? 8 * ((b.count + 7) / 8) : b.count;
-    hipError_t err = launch_dg_amr(b.n, c->geom, a, grid, div, st);
-    if (err != hipSuccess) return fail(NMX_EHIP, std::string("stage launch: ") + hipGetErrorString(err));
+    int r = launch_bucket(c, a, b, div, sa.xcd_map, st);
+    if (r) return r;
     if (fork) HIPCHK(hipEventRecord(c->evjoin[bi % nmx_ctx::kHelpers], st));
   }
   if (fork)
@@ -950,10 +991,38 @@ extern "C" int nmx_autotune(nmx_ctx *c) {
       HIPCHK(hipEventElapsedTime(&ms, t0, t1));
       if (ms < best) { best = ms; best_b = b; best_x = x; }
     }
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
   c->par["nmx_block"] = best_b;
   c->par["nmx_xcd"] = best_x;
+  // batched path: each order bucket gets its own tile, timed alone on an
+  // RK4 second-stage launch (reads w, u_p, u_acc; writes u_acc, w)
+  if (c->amr) {
+    StageArgs sa = base_args(c);
+    sa.win = c->dW0; sa.U = c->dU; sa.Ain = c->dA; sa.Aout = c->dA; sa.Wout = c->dW1;
+    sa.wmode = 1; sa.cA = c->dt / 3.; sa.cW = c->dt / 2.; sa.tidx = 0;
+    HIPCHK(hipMemcpyAsync(c->dW0, c->dU, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dA, c->dU, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream));
+    if ((r = launch_amr(c, sa))) return r;   // fills the face buffer
+    const int xcd = best_x == "yes" ? 1 : 0;
+    AmrArgs a = amr_args(c, sa);
+    for (auto &b : c->buckets) {
+      int best_div = 0;
+      float bms = 1e30f;
+      for (int div : {1, 2, 4}) {
+        if ((r = launch_bucket(c, a, b, div, xcd, c->stream))) return r;   // warm-up
+        HIPCHK(hipEventRecord(t0, c->stream));
+        for (int s = 0; s < 5 && !r; ++s) r = launch_bucket(c, a, b, div, xcd, c->stream);
+        if (r) return r;
+        HIPCHK(hipEventRecord(t1, c->stream));
+        HIPCHK(hipEventSynchronize(t1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+        if (ms < bms) { bms = ms; best_div = div; }
+      }
+      b.div = best_div;
+    }
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
   HIPCHK(hipMemcpyAsync(c->dU, backup, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   (void)hipFree(backup);
