@@ -94,6 +94,9 @@ struct nmx_ctx {
   // path), its own graph cache, and the checkpoint pool
   double *dL = nullptr;
   int *dnbf = nullptr;
+  // FV elements (fv_divf): every element FV on uniform points
+  bool fvelm = false;
+  double *dwc = nullptr;    // distances between midpoints around each point
   long *deoff = nullptr;
   int *den = nullptr;
   hipGraph_t agraph = nullptr;
@@ -125,6 +128,13 @@ static void set_defaults(nmx_ctx *c) {
       {"amr_Shell_rout", "1"},
       {"amr_hrefine_sphere_radius", "10"},
       {"amr_hrefine_sphere_levels", "0"},
+      {"amr_uniform_p", ""},
+      {"amr_fv_p", ""},
+      {"fv_rec", "WENOm3_2"},
+      {"fv_flux", "fnum"},
+      {"fv_divf_extrap", "dnfn_extrap1"},
+      {"fv_divf_use_only_right_flux", "yes"},
+      {"fv_WENOm3_opt_weightratio", "2"},
       {"dt", "auto"},
       {"dtfac", "0.25"},
       {"evolve_method", "RK4"},
@@ -200,7 +210,7 @@ static void free_mesh_dev(nmx_ctx *c) {
   c->graph_steps = 0;
   {
     void **ptrs[] = {(void **)&c->dL, (void **)&c->dnbf, (void **)&c->deoff, (void **)&c->den,
-                     (void **)&c->dCk};
+                     (void **)&c->dCk, (void **)&c->dwc};
     for (auto q : ptrs)
       if (*q) { (void)hipFree(*q); *q = nullptr; }
   }
@@ -262,6 +272,11 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
     return fail(NMX_EINVAL, "nmx_geometry must be analytic or stored");
   if (k == "nmx_geometry" && c->have_mesh)
     return fail(NMX_ESTATE, "set nmx_geometry before amr_setup_mesh");
+  if (k == "fv_rec" && v != "WENOm3_2" && v != "WENOm5_2")
+    return fail(NMX_EUNSUP, "fv_rec: WENOm3_2 and WENOm5_2 are built");
+  if ((k == "fv_flux" && v != "fnum") || (k == "fv_divf_extrap" && v != "dnfn_extrap1") ||
+      (k == "fv_divf_use_only_right_flux" && v != "yes") || (k == "fv_WENOm3_opt_weightratio" && v != "2"))
+    return fail(NMX_EUNSUP, k + ": only nmesh's default is built");
   if (k == "nmx_block" && v != "full" && v != "half" && v != "quarter")
     return fail(NMX_EINVAL, "nmx_block must be full, half or quarter");
   c->par[k] = v;
@@ -503,6 +518,21 @@ static int upload_mesh(nmx_ctx *c) {
   c->tcap = 4 * 64;
   if ((r = dev_alloc(&c->dT, c->tcap))) return r;
   if (!c->hT) HIPCHK(hipHostMalloc((void **)&c->hT, c->tcap * sizeof(double), hipHostMallocDefault));
+  c->fvelm = M.e[0].fv != 0;
+  if (c->fvelm) {
+    if (c->amr || c->np1 < 3) return fail(NMX_EUNSUP, "FV elements need a conforming mesh of one order n >= 3");
+    if (P(c, "nmx_path") != "plain") return fail(NMX_EUNSUP, "FV elements run on the plain path");
+    // midpoints between the uniform points and the distances around each
+    // point, face to first midpoint at the ends (get_coords.c:1749-1860)
+    const Basis &ub = uniform_basis(c->np1);
+    const int n = c->np1;
+    std::vector<double> xm(n), wc(n);
+    for (int i = 0; i < n - 1; ++i) xm[i] = 0.5 * (ub.x[i] + ub.x[i + 1]);
+    wc[0] = std::fabs(xm[0] - (-1.));
+    wc[n - 1] = std::fabs(xm[n - 2] - 1.);
+    for (int i = 1; i < n - 1; ++i) wc[i] = xm[i] - xm[i - 1];
+    if ((r = dev_upload(&c->dwc, wc))) return r;
+  }
   if (c->amr) return upload_amr_tables(c);
   // face gather descriptors {nb, local base, s1, s2}
   std::vector<int4> hf(6 * ne);
@@ -570,6 +600,26 @@ extern "C" int nmx_mesh_build(nmx_ctx *c) {
     return fail(NMX_EUNSUP, "amr_mesh_type must be BoxMesh or Shell");
   }
   if (!err.empty()) return fail(NMX_EINVAL, err);
+  {
+    // FV elements: amr_fv_p patches on uniform points (amr_uniform_p)
+    std::vector<int> up, fp;
+    std::istringstream su(P(c, "amr_uniform_p")), sf(P(c, "amr_fv_p"));
+    for (int q; su >> q;) up.push_back(q);
+    for (int q; sf >> q;) fp.push_back(q);
+    std::sort(up.begin(), up.end());
+    std::sort(fp.begin(), fp.end());
+    if (up != fp)
+      return fail(NMX_EUNSUP, "amr_uniform_p and amr_fv_p must name the same patches (DG on uniform "
+                              "points and FV on LGL points are not built)");
+    if (!fp.empty()) {
+      err = set_fv_patches(c->mesh, fp);
+      if (!err.empty()) return fail(NMX_EINVAL, err);
+      for (auto &E : c->mesh.e)
+        if (!E.fv) return fail(NMX_EUNSUP, "meshes mixing DG and FV elements are not built");
+      for (auto &Pt : c->mesh.pat)
+        if (Pt.type != PAT_BOX) return fail(NMX_EUNSUP, "FV elements are built on Cartesian (BoxMesh) patches");
+    }
+  }
   c->have_mesh = true;
   c->time = 0; c->Te = 0; c->te = 0;
   if (P(c, "dt") == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"));
@@ -881,6 +931,14 @@ static int launch_amr(nmx_ctx *c, const StageArgs &sa) {
 
 static int launch(nmx_ctx *c, StageArgs a) {
   if (c->amr) return launch_amr(c, a);
+  if (c->fvelm) {
+    int nel = a.e1 - a.e0;
+    int grid = a.xcd_map ? 8 * ((nel + 7) / 8) : nel;
+    int rec = P(c, "fv_rec") == "WENOm5_2" ? 5 : 3;
+    hipError_t err = launch_fvelm_stage(c->np1, rec, a, c->dwc, grid, c->stream);
+    if (err != hipSuccess) return fail(NMX_EHIP, std::string("fv stage launch: ") + hipGetErrorString(err));
+    return NMX_OK;
+  }
   bool perelm = (P(c, "nmx_path") == "perelement");
   std::string blk = P(c, "nmx_block");
   int nt = dg_stage_threads(c->np1, blk == "half" ? 2 : blk == "quarter" ? 4 : 1);
@@ -962,7 +1020,7 @@ static void stage_times(int method, double t, double dt, double *ts) {
 extern "C" int nmx_autotune(nmx_ctx *c) {
   NEED_DEV(c);
   HIPCHK(hipSetDevice(c->device));
-  if (P(c, "nmx_path") == "perelement") { c->tuned = true; return NMX_OK; }
+  if (P(c, "nmx_path") == "perelement" || c->fvelm) { c->tuned = true; return NMX_OK; }
   double *backup = nullptr;
   int r;
   if ((r = dev_alloc(&backup, c->nstate))) return r;
@@ -1136,8 +1194,8 @@ static AdjArgs adj_base_args(nmx_ctx *c) {
 #define NEED_ADJ(c)                                                                      \
   do {                                                                                   \
     NEED_DEV(c);                                                                         \
-    if ((c)->amr || !(c)->dnbf)                                                          \
-      return fail(NMX_EUNSUP, "the adjoint needs a conforming mesh of one element order"); \
+    if ((c)->amr || !(c)->dnbf || (c)->fvelm)                                            \
+      return fail(NMX_EUNSUP, "the adjoint needs a conforming DG mesh of one element order"); \
   } while (0)
 
 static int launch_adj(nmx_ctx *c, const AdjArgs &a) {

@@ -1,0 +1,302 @@
+// fvelm_kernels.hip -- scalarwave1 RHS + RK stage on FV elements (uniform
+// points including both faces) of a Cartesian mesh, gfx950.
+//
+// nmesh's fv_divf (src/main/dg/fv.c:115-624) with scalarwave1's callbacks
+// (src/tests/scalarwave1/scalarwave1.c:211-237,755-790) and its default
+// parameters (src/main/dg/nmesh_dg.c:26-66, dg.c:285-400): fv_flux = fnum,
+// fv_divf_extrap = dnfn_extrap1 (so the face fluxes are set here and
+// dg_add_surface_fluxes adds nothing), fv_divf_use_only_right_flux = yes,
+// WENOm3 optimal weights (1, 2).  Per element (one workgroup):
+//   phase 1: own pi, c to LDS; the adjacent face values (ajsurf) to LDS;
+//   phase 2: one task per (direction, line): reconstruct every midpoint
+//            from both sides (WENOm5_2 or WENOm3_2, rec1d.c:115-232,
+//            875-1370), take upwind / LLF fnum with the midpoint normal,
+//            the face fluxes against ajsurf at both ends,
+//            df_i = Jg_R fnumR_i - Jg_L fnumR_{i-1} with Jg = sqrtgdiag /
+//            (ooJ dXb_i), dnfn_extrap1 at both ends (rec1d.c:542-626) and
+//            divf_d = df ooJ, one LDS slab per direction;
+//   epilogue: r_{pi,c} = -(divf_x + divf_y + divf_z), r_phi = pi, RK stage.
+// Cartesian elements only: their midpoint metric equals the node metric.
+#include <hip/hip_runtime.h>
+
+#include "dg_common.h"
+#include "kernels.h"
+
+namespace nmx {
+
+constexpr double kWeno3Eps = 1e-6;    // WENO3_epsilon (rec1d.c:35), u_scale = 1
+constexpr double kWeno5EpsE = 1e-40;  // WENO5_epsilon (rec1d.c:1193)
+
+// WENOm3 (rec1d_p_WENO3 with the uniform WENOm3 weights)
+__device__ __forceinline__ double wm3_p(double um1, double u0, double up1) {
+  const double d1 = u0 - um1, d2 = up1 - u0;
+  const double b1 = d1 * d1 + kWeno3Eps, b2 = d2 * d2 + kWeno3Eps;
+  const double o1 = 1. / (b1 * b1), o2 = 2. / (b2 * b2);
+  const double s = o1 + o2;
+  return (o1 / s) * (-0.5 * um1 + 1.5 * u0) + (o2 / s) * (0.5 * u0 + 0.5 * up1);
+}
+__device__ __forceinline__ double wm3_m(double u0, double up1, double up2) {
+  const double d1 = up1 - u0, d2 = up2 - up1;
+  const double b1 = d1 * d1 + kWeno3Eps, b2 = d2 * d2 + kWeno3Eps;
+  const double o1 = 2. / (b1 * b1), o2 = 1. / (b2 * b2);
+  const double s = o1 + o2;
+  return (o1 / s) * (0.5 * u0 + 0.5 * up1) + (o2 / s) * (1.5 * up1 - 0.5 * up2);
+}
+
+// WENO5 (rec1d_p/m_WENO5 with uniform weights), the reference's arithmetic
+__device__ __forceinline__ double w5_p(double a, double b, double c, double d, double e) {
+  const double s01 = a - 2. * b + c, s02 = a - 4. * b + 3. * c;
+  const double s11 = b - 2. * c + d, s12 = b - d;
+  const double s21 = c - 2. * d + e, s22 = 3. * c - 4. * d + e;
+  const double e0 = (13. / 12.) * s01 * s01 + 0.25 * s02 * s02 + kWeno5EpsE;
+  const double e1 = (13. / 12.) * s11 * s11 + 0.25 * s12 * s12 + kWeno5EpsE;
+  const double e2 = (13. / 12.) * s21 * s21 + 0.25 * s22 * s22 + kWeno5EpsE;
+  const double w0 = 1. / (e0 * e0), w1 = 6. / (e1 * e1), w2 = 3. / (e2 * e2);
+  const double inv = 1. / (w0 + w1 + w2);
+  const double o6 = 1. / 6.;
+  const double q0 = (2. * o6) * a + (-7. * o6) * b + (11. * o6) * c;
+  const double q1 = (-o6) * b + (5. * o6) * c + (2. * o6) * d;
+  const double q2 = (2. * o6) * c + (5. * o6) * d + (-o6) * e;
+  return (w0 * inv) * q0 + (w1 * inv) * q1 + (w2 * inv) * q2;
+}
+__device__ __forceinline__ double w5_m(double a, double b, double c, double d, double e) {
+  const double s21 = c - 2. * d + e, s22 = 3. * c - 4. * d + e;
+  const double s11 = b - 2. * c + d, s12 = b - d;
+  const double s01 = a - 2. * b + c, s02 = a - 4. * b + 3. * c;
+  const double e2 = (13. / 12.) * s21 * s21 + 0.25 * s22 * s22 + kWeno5EpsE;
+  const double e1 = (13. / 12.) * s11 * s11 + 0.25 * s12 * s12 + kWeno5EpsE;
+  const double e0 = (13. / 12.) * s01 * s01 + 0.25 * s02 * s02 + kWeno5EpsE;
+  const double w2 = 1. / (e2 * e2), w1 = 6. / (e1 * e1), w0 = 3. / (e0 * e0);
+  const double inv = 1. / (w0 + w1 + w2);
+  const double o6 = 1. / 6.;
+  const double q2 = (11. * o6) * c + (-7. * o6) * d + (2. * o6) * e;
+  const double q1 = (2. * o6) * b + (5. * o6) * c + (-o6) * d;
+  const double q0 = (-o6) * a + (5. * o6) * b + (2. * o6) * c;
+  return (w0 * inv) * q0 + (w1 * inv) * q1 + (w2 * inv) * q2;
+}
+
+// both sides of midpoint im of a line q[0..NP-1] (WENOm5_2 when REC == 5,
+// WENOm3_2 when REC == 3)
+template <int NP, int REC>
+__device__ __forceinline__ void rec_mid(const double (&q)[NP], int im, double &up, double &um) {
+  if (REC == 5 && im >= 2 && im <= NP - 3) up = w5_p(q[im - 2], q[im - 1], q[im], q[im + 1], q[im + 2]);
+  else if (im >= 1) up = wm3_p(q[im - 1], q[im], q[im + 1]);
+  else up = 0.5 * q[0] + 0.5 * q[1];
+  if (REC == 5 && im >= 1 && im <= NP - 4) um = w5_m(q[im - 1], q[im], q[im + 1], q[im + 2], q[im + 3]);
+  else if (im <= NP - 3) um = wm3_m(q[im], q[im + 1], q[im + 2]);
+  else um = 0.5 * q[NP - 2] + 0.5 * q[NP - 1];
+}
+
+// fnum = f(ui).n + F(ui - ua) for n = sig e_dir (scalarwave1 fluxes,
+// scalarwave1.c:39-48; F as in face_flux)
+template <int DIR>
+__device__ __forceinline__ void fnum_dir(int numflux, double sig, const double ui[4], const double ua[4],
+                                         double fn[4]) {
+  double nrm[3] = {0.0, 0.0, 0.0};
+  nrm[DIR] = sig;
+  const double dc[3] = {ui[1] - ua[1], ui[2] - ua[2], ui[3] - ua[3]};
+  double F[4];
+  face_flux(numflux, nrm, ui[0] - ua[0], dc, F);
+  fn[0] = -(ui[1 + DIR] * sig) + F[0];
+  fn[1] = (DIR == 0 ? -ui[0] * sig : 0.0) + F[1];
+  fn[2] = (DIR == 1 ? -ui[0] * sig : 0.0) + F[2];
+  fn[3] = (DIR == 2 ? -ui[0] * sig : 0.0) + F[3];
+}
+
+template <int NP, int REC, int DIR>
+__device__ __forceinline__ void fv_line(const StageArgs &a, const double *wc, int numflux, double g_dir,
+                                        double ooJ, int la, int lb, const double (*s_u)[NP * NP * NP],
+                                        const double (*s_f)[6][NP * NP], double (*s_df)[NP * NP * NP]) {
+  constexpr int NP2 = NP * NP;
+  const int stride = DIR == 0 ? 1 : DIR == 1 ? NP : NP2;
+  // line base: la, lb run along Dir1_norm, Dir2_norm of DIR
+  const int base = DIR == 0 ? (la * NP + lb * NP2) : DIR == 1 ? (la + lb * NP2) : (la + lb * NP);
+  const int JK = la + NP * lb;
+  double q[4][NP];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int i = 0; i < NP; ++i) q[v][i] = s_u[v][base + i * stride];
+  double fR[4], fL[4], df[4][NP];
+#pragma unroll
+  for (int i0 = 0; i0 < NP; ++i0) {
+    const double JR = g_dir / (ooJ * wc[i0]), JL = JR;
+    if (i0 > 0) {
+#pragma unroll
+      for (int l = 0; l < 4; ++l) fL[l] = -fR[l];
+    } else {
+      double ui[4], ua[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) { ui[v] = q[v][0]; ua[v] = s_f[v][2 * DIR][JK]; }
+      fnum_dir<DIR>(numflux, -1.0, ui, ua, fL);
+    }
+    double ui[4], ua[4];
+    if (i0 < NP - 1) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double up, um;
+        rec_mid<NP, REC>(q[v], i0, up, um);
+        ui[v] = up;
+        ua[v] = um;
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) { ui[v] = q[v][NP - 1]; ua[v] = s_f[v][2 * DIR + 1][JK]; }
+    }
+    fnum_dir<DIR>(numflux, 1.0, ui, ua, fR);
+#pragma unroll
+    for (int l = 0; l < 4; ++l) df[l][i0] = JR * fR[l] + JL * fL[l];
+  }
+  if (NP > 2) {
+    const double w0 = 1. / 0.75, w1 = 1. - w0;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      df[l][0] = w0 * df[l][0] + w1 * df[l][1];
+      df[l][NP - 1] = w0 * df[l][NP - 1] + w1 * df[l][NP - 2];
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < 4; ++l)
+#pragma unroll
+    for (int i = 0; i < NP; ++i) s_df[l][base + i * stride] = df[l][i] * ooJ;
+}
+
+template <int NP, int REC, int NT>
+__global__ __launch_bounds__(NT) void fvelm_stage_kernel(StageArgs a, const double *wc) {
+  constexpr int NP2 = NP * NP;
+  constexpr int NP3 = NP * NP * NP;
+  constexpr int NC = (NP3 + NT - 1) / NT;
+  __shared__ double s_u[4][NP3];
+  __shared__ double s_f[4][6][NP2];
+  __shared__ double s_df[3][4][NP3];
+  __shared__ double s_wc[NP];
+  const int tid = threadIdx.x;
+  for (int q = tid; q < NP; q += NT) s_wc[q] = wc[q];
+
+  int e_hi, e_step, e_first;
+  if (a.xcd_map) {
+    const int xcd = blockIdx.x & 7, q = blockIdx.x >> 3, g8 = gridDim.x >> 3;
+    const int chunk = (a.e1 - a.e0 + 7) >> 3;
+    const int e_lo = a.e0 + xcd * chunk;
+    e_hi = min(e_lo + chunk, a.e1);
+    e_first = e_lo + q;
+    e_step = g8;
+  } else {
+    e_hi = a.e1;
+    e_first = a.e0 + blockIdx.x;
+    e_step = gridDim.x;
+  }
+
+  for (int e = e_first; e < e_hi; e += e_step) {
+    const long ebase = 5L * NP3 * e;
+    __syncthreads();
+    // ---- phase 1 (as in dg_stage_kernel) ----
+    for (int idx = tid; idx < 4 * NP3; idx += NT) {
+      const int v = idx / NP3, node = idx % NP3;
+      s_u[v][node] = a.win[ebase + node + v * NP3];
+    }
+    for (int idx = tid; idx < 6 * NP2; idx += NT) {
+      const int f = idx / NP2, q = idx % NP2;
+      const int fa = q % NP, fb = q / NP;
+      const int4 fd = a.face[6 * e + f];
+      if (fd.x >= 0) {
+        const double *nw = a.win + 5L * NP3 * fd.x + fd.y + fa * fd.z + fb * fd.w;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s_f[v][f][q] = nw[v * NP3];
+      } else if (a.profile) {
+        const int dir = f >> 1, pl = (f & 1) * (NP - 1);
+        const int ii = (dir == 0) ? pl : fa;
+        const int jj = (dir == 1) ? pl : ((dir == 0) ? fa : fb);
+        const int kk = (dir == 2) ? pl : fb;
+        const int mynode = ii + NP * (jj + NP * kk);
+        const double *xp = a.xyz + 3L * NP3 * e + mynode;
+        double ua[4];
+        plane_wave(a.k, a.om, xp[0], xp[NP3], xp[2 * NP3], a.tdev[a.tidx], ua);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s_f[v][f][q] = ua[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s_f[v][f][q] = 0.0;
+      }
+    }
+    const double g0 = a.dXbdX[3 * e], g1 = a.dXbdX[3 * e + 1], g2 = a.dXbdX[3 * e + 2];
+    const double ooJ = g0 * g1 * g2;
+    __syncthreads();
+    // ---- phase 2: lines of all three directions ----
+    for (int task = tid; task < 3 * NP2; task += NT) {
+      const int dir = task / NP2, l = task % NP2;
+      const int la = l % NP, lb = l / NP;
+      if (dir == 0) fv_line<NP, REC, 0>(a, s_wc, a.numflux, g0, ooJ, la, lb, s_u, s_f, s_df[0]);
+      else if (dir == 1) fv_line<NP, REC, 1>(a, s_wc, a.numflux, g1, ooJ, la, lb, s_u, s_f, s_df[1]);
+      else fv_line<NP, REC, 2>(a, s_wc, a.numflux, g2, ooJ, la, lb, s_u, s_f, s_df[2]);
+    }
+    __syncthreads();
+    // ---- epilogue ----
+#pragma unroll 1
+    for (int c = 0; c < NC; ++c) {
+      const int node = tid + c * NT;
+      if (node >= NP3) continue;
+      const long gi = ebase + node;
+      double r[5];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) r[v] = -((s_df[0][v][node] + s_df[1][v][node]) + s_df[2][v][node]);
+      r[4] = s_u[0][node];
+      double Ubuf[5], Abuf[5];
+      if (a.wmode) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Ubuf[v] = a.U[gi + v * NP3];
+      }
+      if (a.Aout) {
+        const double *bas = a.Ain ? a.Ain : a.win;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Abuf[v] = bas[gi + v * NP3];
+      }
+      if (a.R) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) a.R[gi + v * NP3] = r[v];
+      }
+      if (a.Aout) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) a.Aout[gi + v * NP3] = Abuf[v] + a.cA * r[v];
+      }
+      if (a.wmode == 1) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) a.Wout[gi + v * NP3] = Ubuf[v] + a.cW * r[v];
+      } else if (a.wmode == 2) {
+        const double wv[5] = {s_u[0][node], s_u[1][node], s_u[2][node], s_u[3][node], a.win[gi + 4 * NP3]};
+#pragma unroll
+        for (int v = 0; v < 5; ++v) a.Wout[gi + v * NP3] = 0.75 * Ubuf[v] + 0.25 * (wv[v] + a.cW * r[v]);
+      }
+    }
+  }
+}
+
+template <int NP, int REC>
+static hipError_t launch_fvelm_t(const StageArgs &a, const double *wc, int grid, hipStream_t s) {
+  constexpr int NP3 = NP * NP * NP;
+  constexpr int NT0 = ((NP3 + 1) / 2 + 63) / 64 * 64;
+  constexpr int NT = NT0 < 3 * NP * NP ? ((3 * NP * NP + 63) / 64 * 64) : NT0;
+  hipLaunchKernelGGL((fvelm_stage_kernel<NP, REC, NT>), dim3(grid), dim3(NT), 0, s, a, wc);
+  return hipGetLastError();
+}
+
+hipError_t launch_fvelm_stage(int np, int rec, const StageArgs &a, const double *wc, int grid, hipStream_t s) {
+#define NMX_CASE(N)                                                   \
+  case N:                                                             \
+    return rec == 5 ? launch_fvelm_t<N, 5>(a, wc, grid, s) : launch_fvelm_t<N, 3>(a, wc, grid, s);
+  switch (np) {
+    NMX_CASE(3)
+    NMX_CASE(4)
+    NMX_CASE(5)
+    NMX_CASE(6)
+    NMX_CASE(7)
+    NMX_CASE(8)
+    NMX_CASE(9)
+    NMX_CASE(10)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef NMX_CASE
+}
+
+}  // namespace nmx

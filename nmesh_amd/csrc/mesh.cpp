@@ -82,6 +82,34 @@ static void make_basis(int n, Basis &b) {
   }
 }
 
+// uniform points with both faces: ends at -+1, -1 + j h on the left half,
+// mirrored, 0 exactly at the centre (uniform_x, Legendre.c:626-663)
+static void make_uniform(int n, Basis &b) {
+  b.n = n;
+  int N = n - 1;
+  if (N == 0) { b.x[0] = 0; return; }
+  b.x[0] = -1; b.x[N] = 1;
+  if (N == 1) return;
+  double h = 2. / N;
+  for (int j = 1; j <= N / 2; ++j) {
+    double y = (j == N / 2.0) ? 0. : -1. + j * h;
+    b.x[N - j] = -y;
+    b.x[j] = y;
+  }
+}
+
+const Basis &uniform_basis(int n) {
+  static Basis cache[kMaxN + 1];
+  static bool done[kMaxN + 1] = {false};
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!done[n]) {
+    make_uniform(n, cache[n]);
+    done[n] = true;
+  }
+  return cache[n];
+}
+
 const Basis &lgl_basis(int n) {
   static Basis cache[kMaxN + 1];
   static bool done[kMaxN + 1] = {false};
@@ -363,7 +391,7 @@ static void find_neighbours(Mesh &M) {
 // ------------------------------------------------------------------------
 
 static void node_X(const Elm &E, int i, int j, int k, double X[3]) {
-  const Basis &b = lgl_basis(E.n);
+  const Basis &b = E.fv ? uniform_basis(E.n) : lgl_basis(E.n);
   int I[3] = {i, j, k};
   for (int d = 0; d < 3; ++d)
     X[d] = 0.5 * ((E.bbox[2 * d + 1] - E.bbox[2 * d]) * b.x[I[d]] + (E.bbox[2 * d + 1] + E.bbox[2 * d]));
@@ -490,7 +518,9 @@ static std::string face_tables(Mesh &M) {
         const Elm &N = M.e[E.fnb[f][0]];
         int nf = E.nbf[f][0];
         bool copy = false;
-        if (N.p == E.p) {
+        if (N.fv != E.fv) {
+          copy = false;
+        } else if (N.p == E.p) {
           copy = (N.l == E.l && N.n == E.n);
         } else if (N.nfnb[nf] == 1 && N.n == E.n) {
           copy = true;   // if the corners coincide (checked below)
@@ -703,6 +733,16 @@ std::string set_elm_n(Mesh &M, const std::vector<int> &n) {
   for (size_t i = 0; i < n.size(); ++i) {
     if (n[i] < 2 || n[i] > kMaxN) return "n out of range";
     M.e[i].n = n[i];
+  }
+  geometry(M);
+  return face_tables(M);
+}
+
+std::string set_fv_patches(Mesh &M, const std::vector<int> &patches) {
+  for (auto &E : M.e) {
+    E.fv = 0;
+    for (int p : patches)
+      if (E.p == p) E.fv = 1;
   }
   geometry(M);
   return face_tables(M);

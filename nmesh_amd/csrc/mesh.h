@@ -33,6 +33,7 @@ struct Elm {
   uint32_t ic[3] = {0, 0, 0};   // integer position at level l
   double bbox[6] = {0, 0, 0, 0, 0, 0};
   int n = 2;
+  int fv = 0;                   // 1: FV element on uniform points (fv_divf)
   long off = 0;                 // first node
   int nfnb[6] = {0, 0, 0, 0, 0, 0};
   int fnb[6][kMaxNb];
@@ -48,6 +49,8 @@ struct Basis {
   double D[kMaxN * kMaxN];                // D[i*n + l] = l_l'(x_i)
 };
 const Basis &lgl_basis(int n);
+// uniform points incl. both faces (x only)
+const Basis &uniform_basis(int n);
 
 // conforming-face gather: neighbour node = nbase + a*s1 + b*s2 for my face
 // point (a,b); a runs along Dir1_norm(dir), b along Dir2_norm(dir)
@@ -83,6 +86,9 @@ std::string build_box(Mesh &M, int luni, int n, double dout, const double xc[3])
 std::string build_shell(Mesh &M, int luni, int n, double rin, double rout,
                         double sph_radius, int sph_levels);
 std::string set_elm_n(Mesh &M, const std::vector<int> &n);
+// put every element of the listed patches in FV mode on uniform points
+// (amr_uniform_p, amr_fv_p; src/main/amr/mesh.c:40-80) and rebuild
+std::string set_fv_patches(Mesh &M, const std::vector<int> &patches);
 
 double mesh_dt(const Mesh &M, double dtfac);
 

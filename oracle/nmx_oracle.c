@@ -130,10 +130,25 @@ static OBasis *basis(int n)
       orc_lgl(n, b->Xb, b->Wq);
       lagrange_winterp(n, b->Xb, b->WL);
       lagrange_DT(n, b->Xb, b->WL, b->DT);
-      /* uniform points incl. both faces, trapezoid weights
-         (src/main/basis/gridpoints.c:49-53) */
-      for (int i = 0; i < n; i++)
-        b->Xu[i] = (n > 1) ? -1. + 2. * i / (n - 1) : 0.;
+      /* uniform points incl. both faces (uniform_x, Legendre.c:626-663:
+         ends at -+1, -1 + j h on the left half, mirrored, 0 exactly at the
+         centre), trapezoid weights (gridpoints.c:49-53) */
+      {
+        int N = n - 1;
+        if (N == 0) b->Xu[0] = 0.;
+        else {
+          b->Xu[0] = -1.;
+          b->Xu[N] = 1.;
+          if (N > 1) {
+            double h = 2. / N;
+            for (int j = 1; j <= N / 2; j++) {
+              double y = (j == N / 2.0) ? 0. : -1. + j * h;
+              b->Xu[N - j] = -y;
+              b->Xu[j] = y;
+            }
+          }
+        }
+      }
       for (int i = 0; i < n; i++)
         b->Wu[i] = (n > 1) ? ((i == 0 || i == n - 1) ? 1. / (n - 1)
                                                       : 2. / (n - 1))
@@ -789,6 +804,15 @@ void orc_mesh_set_n(OMesh *M, const int *n)
   mesh_geometry(M);
 }
 
+/* elements of every patch in use_fv mode on uniform points (amr_uniform_p
+   and amr_fv_p naming all patches: mesh.c:40-80) */
+void orc_mesh_set_fv(OMesh *M, int use_fv)
+{
+  for (int e = 0; e < M->nelem; e++) M->e[e].use_fv = use_fv;
+  set_ajmodes(M);
+  mesh_geometry(M);
+}
+
 void orc_mesh_free(OMesh *M)
 {
   if (!M) return;
@@ -1244,6 +1268,202 @@ static OSW sw_from(int numflux, const double *k)
   return s;
 }
 
+/* ------------------------------------------------------------------ */
+/* FV elements: fv_divf for scalarwave1 (src/main/dg/fv.c:115-624,     */
+/* src/tests/scalarwave1/scalarwave1.c:211-237,755-790) with nmesh's   */
+/* defaults fv_flux = fnum, fv_divf_extrap = dnfn_extrap1,             */
+/* fv_divf_use_only_right_flux = yes, fv_WENOm3_opt_weightratio = 2,   */
+/* fv_rec_WENOm_s1/s2 and fv_divf_extrap_s1/s2/opt off                 */
+/* (src/main/dg/nmesh_dg.c:26-66, dg.c:285-400); Cartesian elements    */
+/* (the midpoint metric equals the node metric)                        */
+/* ------------------------------------------------------------------ */
+
+double orc_rec1d_p_WENO5(const double *u, int im);
+double orc_rec1d_m_WENO5(const double *u, int im);
+
+#define ORC_WENO3_EPS 1e-6 /* WENO3_epsilon, rec1d.c:35, u_scale = 1 */
+
+/* rec1d_p_WENO3 / rec1d_m_WENO3 with the WENOm3 uniform weights
+   (rec1d.c:115-232,875-905): optimal weights (1, 2) */
+double orc_rec1d_p_WENOm3(const double *u, int im)
+{
+  double u_im1 = u[im - 1], u_i = u[im], u_ip1 = u[im + 1];
+  double d1 = u_i - u_im1, d2 = u_ip1 - u_i;
+  double b1 = d1 * d1 + ORC_WENO3_EPS, b2 = d2 * d2 + ORC_WENO3_EPS;
+  double o1 = 1. / (b1 * b1), o2 = 2. / (b2 * b2);
+  double s = o1 + o2;
+  double w1 = o1 / s, w2 = o2 / s;
+  double v1 = -0.5 * u_im1 + 1.5 * u_i;
+  double v2 = 0.5 * u_i + 0.5 * u_ip1;
+  return w1 * v1 + w2 * v2;
+}
+
+double orc_rec1d_m_WENOm3(const double *u, int im)
+{
+  double u_i = u[im], u_ip1 = u[im + 1], u_ip2 = u[im + 2];
+  double d1 = u_ip1 - u_i, d2 = u_ip2 - u_ip1;
+  double b1 = d1 * d1 + ORC_WENO3_EPS, b2 = d2 * d2 + ORC_WENO3_EPS;
+  double o1 = 2. / (b1 * b1), o2 = 1. / (b2 * b2);
+  double s = o1 + o2;
+  double w1 = o1 / s, w2 = o2 / s;
+  double v1 = 0.5 * u_i + 0.5 * u_ip1;
+  double v2 = 1.5 * u_ip1 - 0.5 * u_ip2;
+  return w1 * v1 + w2 * v2;
+}
+
+/* WENOm3_2 at midpoint im = 0..n-2 (rec1d.c:909-1030): WENOm3 inside,
+   the average of the two points next to a face */
+static double rec_p_WENOm3_2(int n, const double *u, int im)
+{
+  if (im >= 1 && im <= n - 2) return orc_rec1d_p_WENOm3(u, im);
+  return 0.5 * u[0] + 0.5 * u[1];   /* im == 0 */
+}
+static double rec_m_WENOm3_2(int n, const double *u, int im)
+{
+  if (im >= 0 && im <= n - 3) return orc_rec1d_m_WENOm3(u, im);
+  return 0.5 * u[n - 2] + 0.5 * u[n - 1];   /* im == n-2 */
+}
+/* WENOm5_2 (rec1d.c:1354-1370): WENO5 inside, WENOm3_2 near the faces */
+static double rec_p_WENOm5_2(int n, const double *u, int im)
+{
+  if (im >= 2 && im <= n - 3) return orc_rec1d_p_WENO5(u, im);
+  return rec_p_WENOm3_2(n, u, im);
+}
+static double rec_m_WENOm5_2(int n, const double *u, int im)
+{
+  if (im >= 1 && im <= n - 4) return orc_rec1d_m_WENO5(u, im);
+  return rec_m_WENOm3_2(n, u, im);
+}
+
+/* the FV reconstruction used by fv_elm_rhs: 3 = WENOm3_2, 5 = WENOm5_2 */
+static int g_fv_rec = 5;
+void orc_set_fv_rec(int mode) { g_fv_rec = mode; }
+
+static double fv_rec_p(int n, const double *u, int im)
+{
+  return g_fv_rec == 5 ? rec_p_WENOm5_2(n, u, im) : rec_p_WENOm3_2(n, u, im);
+}
+static double fv_rec_m(int n, const double *u, int im)
+{
+  return g_fv_rec == 5 ? rec_m_WENOm5_2(n, u, im) : rec_m_WENOm3_2(n, u, im);
+}
+
+/* scalarwave1 numflux for normal nrm between ui and ua */
+static void sw_fnum(const OSW *sw, const double nrm[3], const double *ui, const double *ua, double *fnum)
+{
+  double lam[5] = {1.0, -1.0, 0.0, 0.0, 0.0}, fi[5], fa[5];
+  sw_flux1d(fi, nrm, ui);
+  sw_flux1d(fa, nrm, ua);
+  if (sw->numflux == 1) numflux_LLF(NV, lam, lam, ui, ua, fi, fa, fnum);
+  else sw_numflux_upwind(nrm, lam, ui, ua, fnum);
+}
+
+/* dnfn_extrap1 on both ends of one line (rec1d_LR_uface_to_uin_1_Carray
+   with forward = 0 and no s1/s2 limiting, rec1d.c:542-626): the end value
+   sits a quarter spacing inside the face and the next one a full spacing,
+   so the face value is df_end / 0.75 + (1 - 1/0.75) df_next */
+void orc_dnfn_extrap1(int n, double *df)
+{
+  if (n <= 2) return;
+  double w0 = 1. / 0.75, w1 = 1. - w0;
+  df[0] = w0 * df[0] + w1 * df[1];
+  df[n - 1] = w0 * df[n - 1] + w1 * df[n - 2];
+}
+
+/* r of one FV element: divf from fv_divf, r_{pi,c} = -divf, r_phi = pi */
+static void fv_elm_rhs(const OMesh *M, const OSW *sw, const OElm *E, double t, const double *u,
+                       const double *planes, double *r)
+{
+  const int n = E->n, np = elm_np(E);
+  const long nn = M->nnodes, o = E->off;
+  const double *Xb = basis(n)->Xu;
+  double (*divf)[ORC_MAXN * ORC_MAXN * ORC_MAXN] = calloc(4, sizeof(*divf));
+  double (*aj)[NV][ORC_MAXN * ORC_MAXN] = malloc(6 * sizeof(*aj));
+  int have[6];
+  /* adjacent face values: neighbour planes, or the exterior profile */
+  for (int f = 0; f < 6; f++) {
+    have[f] = (E->nfnb[f] > 0);
+    if (have[f]) {
+      for (int v = 0; v < NV; v++) build_ajsurf(M, E, f, planes, v, aj[f][v]);
+    } else {
+      int onbound = elm_on_patface(M, E, f) && M->pat[E->p].bound[f];
+      int dir = f / 2, p = (n - 1) * (f % 2), d1 = (dir == 0), d2 = (dir == 2) ? 1 : 2;
+      for (int b = 0; b < n; b++)
+        for (int a = 0; a < n; a++) {
+          int I[3];
+          I[dir] = p; I[d1] = a; I[d2] = b;
+          long g = o + I[0] + n * (I[1] + n * I[2]);
+          double ua[5] = {0, 0, 0, 0, 0};
+          if (onbound) {
+            double xyz[3] = {M->x[g], M->x[nn + g], M->x[2 * nn + g]};
+            sw_profile_pt(sw->k, xyz, t, ua);
+          }
+          for (int v = 0; v < NV; v++) aj[f][v][a + n * b] = ua[v];
+        }
+    }
+  }
+  /* midpoints and the distances between them (get_coords.c:1749-1860) */
+  double Xbm[ORC_MAXN] = {0}, dXb[ORC_MAXN] = {0};
+  for (int i = 0; i < n - 1; i++) Xbm[i] = 0.5 * (Xb[i] + Xb[i + 1]);
+  dXb[0] = fabs(Xbm[0] - (-1.));
+  dXb[n - 1] = fabs(Xbm[n - 2] - 1.);
+  for (int i = 1; i < n - 1; i++) dXb[i] = Xbm[i] - Xbm[i - 1];
+  for (int dir = 0; dir < 3; dir++) {
+    const int d1 = (dir == 0), d2 = (dir == 2) ? 1 : 2;
+    const double nR[3] = {dir == 0, dir == 1, dir == 2};
+    const double nL[3] = {-(double)(dir == 0), -(double)(dir == 1), -(double)(dir == 2)};
+    for (int b = 0; b < n; b++)
+      for (int a = 0; a < n; a++) {
+        double qc[NV][ORC_MAXN], df[4][ORC_MAXN];
+        long gl[ORC_MAXN];
+        for (int i0 = 0; i0 < n; i0++) {
+          int I[3];
+          I[dir] = i0; I[d1] = a; I[d2] = b;
+          gl[i0] = o + I[0] + n * (I[1] + n * I[2]);
+          for (int v = 0; v < NV; v++) qc[v][i0] = u[v * nn + gl[i0]];
+        }
+        double fnumR[5] = {0, 0, 0, 0, 0}, fnumL[5];
+        const int JK = a + n * b;
+        for (int i0 = 0; i0 < n; i0++) {
+          const long g = gl[i0];
+          const double sgd = M->sqrtgdiag[dir * nn + g], ooJ = M->det[g];
+          const double wc = dXb[i0];
+          /* midpoint metric = node metric on Cartesian elements */
+          double JR = sgd / (ooJ * wc), JL = sgd / (ooJ * wc);
+          if (i0 > 0) {
+            for (int l = 0; l < NV; l++) fnumL[l] = -fnumR[l];
+          } else {
+            double ui[5], ua[5];
+            for (int v = 0; v < NV; v++) { ui[v] = qc[v][0]; ua[v] = aj[2 * dir][v][JK]; }
+            sw_fnum(sw, nL, ui, ua, fnumL);
+          }
+          if (i0 < n - 1) {
+            double ui[5], ua[5];
+            for (int v = 0; v < NV; v++) {
+              ui[v] = fv_rec_p(n, qc[v], i0);
+              ua[v] = fv_rec_m(n, qc[v], i0);
+            }
+            sw_fnum(sw, nR, ui, ua, fnumR);
+          } else {
+            double ui[5], ua[5];
+            for (int v = 0; v < NV; v++) { ui[v] = qc[v][n - 1]; ua[v] = aj[2 * dir + 1][v][JK]; }
+            sw_fnum(sw, nR, ui, ua, fnumR);
+          }
+          for (int l = 0; l < 4; l++) df[l][i0] = JR * fnumR[l] + JL * fnumL[l];
+        }
+        for (int l = 0; l < 4; l++) orc_dnfn_extrap1(n, df[l]);
+        for (int i0 = 0; i0 < n; i0++)
+          for (int l = 0; l < 4; l++) divf[l][gl[i0] - o] += df[l][i0] * M->det[gl[i0]];
+      }
+  }
+  for (int q = 0; q < np; q++) {
+    r[4 * nn + o + q] = u[0 * nn + o + q];
+    for (int l = 0; l < 4; l++) r[l * nn + o + q] = -divf[l][q];
+  }
+  free(divf);
+  free(aj);
+}
+
 /* r = RHS(u, t) (src/main/evolve/evolve.c:182-232) */
 void orc_rhs(const OMesh *M, int numflux, const double *k, const double *u, double t, double *r)
 {
@@ -1265,12 +1485,13 @@ void orc_rhs(const OMesh *M, int numflux, const double *k, const double *u, doub
 #pragma omp for schedule(dynamic, 8)
     for (int e = 0; e < M->nelem; e++) {
       const OElm *E = &M->e[e];
-      sw_vol_rhs(M, E, u, r, divf + 4 * E->off, work);
+      if (E->use_fv) fv_elm_rhs(M, &sw, E, t, u, planes, r);
+      else sw_vol_rhs(M, E, u, r, divf + 4 * E->off, work);
     }
 #pragma omp for schedule(dynamic, 8)
     for (int e = 0; e < M->nelem; e++) {
       const OElm *E = &M->e[e];
-      sw_surf_rhs(M, &sw, E, t, u, planes, r, divf + 4 * E->off);
+      if (!E->use_fv) sw_surf_rhs(M, &sw, E, t, u, planes, r, divf + 4 * E->off);
     }
     free(work);
   }

@@ -49,6 +49,11 @@ def lib():
             "orc_rhs": (None, [vp, i, vp, vp, d, vp]),
             "orc_evolve": (d, [vp, i, vp, i, d, i, d, vp]),
             "orc_num_threads": (i, []),
+            "orc_mesh_set_fv": (None, [vp, i]),
+            "orc_set_fv_rec": (None, [i]),
+            "orc_dnfn_extrap1": (None, [i, vp]),
+            "orc_rec1d_p_WENOm3": (d, [vp, i]),
+            "orc_rec1d_m_WENOm3": (d, [vp, i]),
             "orc_rec1d_p_WENO5": (d, [vp, i]),
             "orc_rec1d_m_WENO5": (d, [vp, i]),
             "orc_sr_p2c": (None, [d, vp, vp]),
@@ -80,7 +85,7 @@ METHOD = {"RK4": 0, "sspRK3": 1, "Euler": 2}
 class OMesh:
     """Oracle mesh built from nmesh parameter values."""
 
-    def __init__(self, kind, luni, n, dout=1.0, rin=2.0, rout=20.0, radius=10.0, levels=0):
+    def __init__(self, kind, luni, n, dout=1.0, rin=2.0, rout=20.0, radius=10.0, levels=0, fv=False):
         L = lib()
         if kind == "box":
             self.h = L.orc_mesh_box(int(luni), int(n), float(dout), None)
@@ -91,6 +96,8 @@ class OMesh:
                                           float(radius), int(levels))
         else:
             raise ValueError(kind)
+        if fv:
+            L.orc_mesh_set_fv(self.h, 1)
 
     def __del__(self):
         try:
@@ -166,6 +173,11 @@ class OMesh:
         t = lib().orc_evolve(self.h, NUMFLUX[numflux], _p(k), METHOD[method], float(dt),
                              int(nsteps), float(t0), _p(u))
         return u, t
+
+
+def set_fv_rec(mode):
+    """reconstruction of the oracle's FV elements: "WENOm5_2" or "WENOm3_2" """
+    lib().orc_set_fv_rec({"WENOm5_2": 5, "WENOm3_2": 3}[mode])
 
 
 # ---------------------------------------------------------------------------
