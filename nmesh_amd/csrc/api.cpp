@@ -272,8 +272,8 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
     return fail(NMX_EINVAL, "nmx_geometry must be analytic or stored");
   if (k == "nmx_geometry" && c->have_mesh)
     return fail(NMX_ESTATE, "set nmx_geometry before amr_setup_mesh");
-  if (k == "fv_rec" && v != "WENOm3_2" && v != "WENOm5_2")
-    return fail(NMX_EUNSUP, "fv_rec: WENOm3_2 and WENOm5_2 are built");
+  if (k == "fv_rec" && v != "WENOm3_2" && v != "WENOm5_2" && v != "WENOmZ_2")
+    return fail(NMX_EUNSUP, "fv_rec: WENOm3_2, WENOm5_2 and WENOmZ_2 are built");
   if ((k == "fv_flux" && v != "fnum") || (k == "fv_divf_extrap" && v != "dnfn_extrap1") ||
       (k == "fv_divf_use_only_right_flux" && v != "yes") || (k == "fv_WENOm3_opt_weightratio" && v != "2"))
     return fail(NMX_EUNSUP, k + ": only nmesh's default is built");
@@ -934,7 +934,8 @@ static int launch(nmx_ctx *c, StageArgs a) {
   if (c->fvelm) {
     int nel = a.e1 - a.e0;
     int grid = a.xcd_map ? 8 * ((nel + 7) / 8) : nel;
-    int rec = P(c, "fv_rec") == "WENOm5_2" ? 5 : 3;
+    const std::string &fr = P(c, "fv_rec");
+    int rec = fr == "WENOm5_2" ? 5 : fr == "WENOmZ_2" ? 7 : 3;
     hipError_t err = launch_fvelm_stage(c->np1, rec, a, c->dwc, grid, c->stream);
     if (err != hipSuccess) return fail(NMX_EHIP, std::string("fv stage launch: ") + hipGetErrorString(err));
     return NMX_OK;

@@ -9,8 +9,9 @@
 // WENOm3 optimal weights (1, 2).  Per element (one workgroup):
 //   phase 1: own pi, c to LDS; the adjacent face values (ajsurf) to LDS;
 //   phase 2: one task per (direction, line): reconstruct every midpoint
-//            from both sides (WENOm5_2 or WENOm3_2, rec1d.c:115-232,
-//            875-1370), take upwind / LLF fnum with the midpoint normal,
+//            from both sides (WENOm5_2, WENOmZ_2 or WENOm3_2, rec1d.c:
+//            115-232,875-1370,1627-1798), take upwind / LLF fnum with the
+//            midpoint normal,
 //            the face fluxes against ajsurf at both ends,
 //            df_i = Jg_R fnumR_i - Jg_L fnumR_{i-1} with Jg = sqrtgdiag /
 //            (ooJ dXb_i), dnfn_extrap1 at both ends (rec1d.c:542-626) and
@@ -75,14 +76,52 @@ __device__ __forceinline__ double w5_m(double a, double b, double c, double d, d
   return (w0 * inv) * q0 + (w1 * inv) * q1 + (w2 * inv) * q2;
 }
 
-// both sides of midpoint im of a line q[0..NP-1] (WENOm5_2 when REC == 5,
-// WENOm3_2 when REC == 3)
+// WENO-Z (rec1d_p/m_WENOZ with uniform weights, rec1d.c:1627-1781)
+__device__ __forceinline__ double wz_p(double a, double b, double c, double d, double e) {
+  const double s01 = a - 2. * b + c, s02 = a - 4. * b + 3. * c;
+  const double s11 = b - 2. * c + d, s12 = b - d;
+  const double s21 = c - 2. * d + e, s22 = 3. * c - 4. * d + e;
+  const double beta0 = (13. / 12.) * s01 * s01 + 0.25 * s02 * s02;
+  const double beta1 = (13. / 12.) * s11 * s11 + 0.25 * s12 * s12;
+  const double beta2 = (13. / 12.) * s21 * s21 + 0.25 * s22 * s22;
+  const double tau5 = fabs(beta0 - beta2);
+  const double w0 = 1. * (1. + tau5 / (beta0 + kWeno5EpsE)), w1 = 6. * (1. + tau5 / (beta1 + kWeno5EpsE)),
+               w2 = 3. * (1. + tau5 / (beta2 + kWeno5EpsE));
+  const double inv = 1. / (w0 + w1 + w2);
+  const double o6 = 1. / 6.;
+  const double q0 = (2. * o6) * a + (-7. * o6) * b + (11. * o6) * c;
+  const double q1 = (-o6) * b + (5. * o6) * c + (2. * o6) * d;
+  const double q2 = (2. * o6) * c + (5. * o6) * d + (-o6) * e;
+  return (w0 * inv) * q0 + (w1 * inv) * q1 + (w2 * inv) * q2;
+}
+__device__ __forceinline__ double wz_m(double a, double b, double c, double d, double e) {
+  const double s21 = c - 2. * d + e, s22 = 3. * c - 4. * d + e;
+  const double s11 = b - 2. * c + d, s12 = b - d;
+  const double s01 = a - 2. * b + c, s02 = a - 4. * b + 3. * c;
+  const double beta2 = (13. / 12.) * s21 * s21 + 0.25 * s22 * s22;
+  const double beta1 = (13. / 12.) * s11 * s11 + 0.25 * s12 * s12;
+  const double beta0 = (13. / 12.) * s01 * s01 + 0.25 * s02 * s02;
+  const double tau5 = fabs(beta0 - beta2);
+  const double w2 = 1. * (1. + tau5 / (beta2 + kWeno5EpsE)), w1 = 6. * (1. + tau5 / (beta1 + kWeno5EpsE)),
+               w0 = 3. * (1. + tau5 / (beta0 + kWeno5EpsE));
+  const double inv = 1. / (w0 + w1 + w2);
+  const double o6 = 1. / 6.;
+  const double q2 = (11. * o6) * c + (-7. * o6) * d + (2. * o6) * e;
+  const double q1 = (2. * o6) * b + (5. * o6) * c + (-o6) * d;
+  const double q0 = (-o6) * a + (5. * o6) * b + (2. * o6) * c;
+  return (w0 * inv) * q0 + (w1 * inv) * q1 + (w2 * inv) * q2;
+}
+
+// both sides of midpoint im of a line q[0..NP-1]: WENOm5_2 (REC 5),
+// WENOmZ_2 (REC 7) or WENOm3_2 (REC 3)
 template <int NP, int REC>
 __device__ __forceinline__ void rec_mid(const double (&q)[NP], int im, double &up, double &um) {
   if (REC == 5 && im >= 2 && im <= NP - 3) up = w5_p(q[im - 2], q[im - 1], q[im], q[im + 1], q[im + 2]);
+  else if (REC == 7 && im >= 2 && im <= NP - 3) up = wz_p(q[im - 2], q[im - 1], q[im], q[im + 1], q[im + 2]);
   else if (im >= 1) up = wm3_p(q[im - 1], q[im], q[im + 1]);
   else up = 0.5 * q[0] + 0.5 * q[1];
   if (REC == 5 && im >= 1 && im <= NP - 4) um = w5_m(q[im - 1], q[im], q[im + 1], q[im + 2], q[im + 3]);
+  else if (REC == 7 && im >= 1 && im <= NP - 4) um = wz_m(q[im - 1], q[im], q[im + 1], q[im + 2], q[im + 3]);
   else if (im <= NP - 3) um = wm3_m(q[im], q[im + 1], q[im + 2]);
   else um = 0.5 * q[NP - 2] + 0.5 * q[NP - 1];
 }
@@ -281,9 +320,11 @@ static hipError_t launch_fvelm_t(const StageArgs &a, const double *wc, int grid,
 }
 
 hipError_t launch_fvelm_stage(int np, int rec, const StageArgs &a, const double *wc, int grid, hipStream_t s) {
-#define NMX_CASE(N)                                                   \
-  case N:                                                             \
-    return rec == 5 ? launch_fvelm_t<N, 5>(a, wc, grid, s) : launch_fvelm_t<N, 3>(a, wc, grid, s);
+#define NMX_CASE(N)                                                                 \
+  case N:                                                                           \
+    return rec == 5   ? launch_fvelm_t<N, 5>(a, wc, grid, s)                        \
+           : rec == 7 ? launch_fvelm_t<N, 7>(a, wc, grid, s)                        \
+                      : launch_fvelm_t<N, 3>(a, wc, grid, s);
   switch (np) {
     NMX_CASE(3)
     NMX_CASE(4)

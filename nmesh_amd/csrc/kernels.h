@@ -58,7 +58,8 @@ struct AdjArgs {
 hipError_t launch_dg_stage(int np, int geom, const StageArgs &a, int grid, int nt, hipStream_t s);
 int dg_stage_threads(int np, int div);   // div: nodes per thread 1, 2 or 4
 hipError_t launch_dg_adjoint(int np, int geom, const AdjArgs &a, int grid, hipStream_t s);
-// scalarwave1 on FV elements (fv_divf); rec 5 = WENOm5_2, 3 = WENOm3_2;
+// scalarwave1 on FV elements (fv_divf); rec 5 = WENOm5_2, 7 = WENOmZ_2,
+// 3 = WENOm3_2;
 // wc[np]: distances between the midpoints around each uniform point
 hipError_t launch_fvelm_stage(int np, int rec, const StageArgs &a, const double *wc, int grid, hipStream_t s);
 

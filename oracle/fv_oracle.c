@@ -85,6 +85,48 @@ double orc_rec1d_m_WENO5(const double *u, int im)
   return (w0 * inv) * q0 + (w1 * inv) * q1 + (w2 * inv) * q2;
 }
 
+/* WENO-Z (rec1d_p/m_WENOZ with uniform weights, rec1d.c:1627-1781):
+   tau5 = |beta0 - beta2|, unnormalised weights g_k (1 + tau5 / (beta_k + eps)) */
+double orc_rec1d_p_WENOZ(const double *u, int im)
+{
+  const double a = u[im - 2], b = u[im - 1], c = u[im], d = u[im + 1], e = u[im + 2];
+  const double s01 = a - 2. * b + c, s02 = a - 4. * b + 3. * c;
+  const double s11 = b - 2. * c + d, s12 = b - d;
+  const double s21 = c - 2. * d + e, s22 = 3. * c - 4. * d + e;
+  const double beta0 = (13. / 12.) * s01 * s01 + 0.25 * s02 * s02;
+  const double beta1 = (13. / 12.) * s11 * s11 + 0.25 * s12 * s12;
+  const double beta2 = (13. / 12.) * s21 * s21 + 0.25 * s22 * s22;
+  const double e0 = beta0 + FV_WENO5_EPS, e1 = beta1 + FV_WENO5_EPS, e2 = beta2 + FV_WENO5_EPS;
+  const double tau5 = fabs(beta0 - beta2);
+  const double w0 = 1. * (1. + tau5 / e0), w1 = 6. * (1. + tau5 / e1), w2 = 3. * (1. + tau5 / e2);
+  const double inv = 1. / (w0 + w1 + w2);
+  const double o6 = 1. / 6.;
+  const double q0 = (2. * o6) * a + (-7. * o6) * b + (11. * o6) * c;
+  const double q1 = (-o6) * b + (5. * o6) * c + (2. * o6) * d;
+  const double q2 = (2. * o6) * c + (5. * o6) * d + (-o6) * e;
+  return (w0 * inv) * q0 + (w1 * inv) * q1 + (w2 * inv) * q2;
+}
+
+double orc_rec1d_m_WENOZ(const double *u, int im)
+{
+  const double a = u[im - 1], b = u[im], c = u[im + 1], d = u[im + 2], e = u[im + 3];
+  const double s21 = c - 2. * d + e, s22 = 3. * c - 4. * d + e;
+  const double s11 = b - 2. * c + d, s12 = b - d;
+  const double s01 = a - 2. * b + c, s02 = a - 4. * b + 3. * c;
+  const double beta2 = (13. / 12.) * s21 * s21 + 0.25 * s22 * s22;
+  const double beta1 = (13. / 12.) * s11 * s11 + 0.25 * s12 * s12;
+  const double beta0 = (13. / 12.) * s01 * s01 + 0.25 * s02 * s02;
+  const double e2 = beta2 + FV_WENO5_EPS, e1 = beta1 + FV_WENO5_EPS, e0 = beta0 + FV_WENO5_EPS;
+  const double tau5 = fabs(beta0 - beta2);
+  const double w2 = 1. * (1. + tau5 / e2), w1 = 6. * (1. + tau5 / e1), w0 = 3. * (1. + tau5 / e0);
+  const double inv = 1. / (w0 + w1 + w2);
+  const double o6 = 1. / 6.;
+  const double q2 = (11. * o6) * c + (-7. * o6) * d + (2. * o6) * e;
+  const double q1 = (2. * o6) * b + (5. * o6) * c + (-o6) * d;
+  const double q0 = (-o6) * a + (5. * o6) * b + (2. * o6) * c;
+  return (w0 * inv) * q0 + (w1 * inv) * q1 + (w2 * inv) * q2;
+}
+
 /* ------------------------------------------------------------------ */
 /* SR-Euler, ideal gas p = (gam - 1) rho eps                           */
 /* ------------------------------------------------------------------ */

@@ -1280,6 +1280,8 @@ static OSW sw_from(int numflux, const double *k)
 
 double orc_rec1d_p_WENO5(const double *u, int im);
 double orc_rec1d_m_WENO5(const double *u, int im);
+double orc_rec1d_p_WENOZ(const double *u, int im);
+double orc_rec1d_m_WENOZ(const double *u, int im);
 
 #define ORC_WENO3_EPS 1e-6 /* WENO3_epsilon, rec1d.c:35, u_scale = 1 */
 
@@ -1335,17 +1337,32 @@ static double rec_m_WENOm5_2(int n, const double *u, int im)
   return rec_m_WENOm3_2(n, u, im);
 }
 
-/* the FV reconstruction used by fv_elm_rhs: 3 = WENOm3_2, 5 = WENOm5_2 */
+/* WENOmZ_2 (rec1d.c:1782-1798): WENO-Z inside, WENOm3_2 near the faces */
+static double rec_p_WENOmZ_2(int n, const double *u, int im)
+{
+  if (im >= 2 && im <= n - 3) return orc_rec1d_p_WENOZ(u, im);
+  return rec_p_WENOm3_2(n, u, im);
+}
+static double rec_m_WENOmZ_2(int n, const double *u, int im)
+{
+  if (im >= 1 && im <= n - 4) return orc_rec1d_m_WENOZ(u, im);
+  return rec_m_WENOm3_2(n, u, im);
+}
+
+/* the FV reconstruction used by fv_elm_rhs: 3 = WENOm3_2, 5 = WENOm5_2,
+   7 = WENOmZ_2 */
 static int g_fv_rec = 5;
 void orc_set_fv_rec(int mode) { g_fv_rec = mode; }
 
 static double fv_rec_p(int n, const double *u, int im)
 {
-  return g_fv_rec == 5 ? rec_p_WENOm5_2(n, u, im) : rec_p_WENOm3_2(n, u, im);
+  return g_fv_rec == 5 ? rec_p_WENOm5_2(n, u, im)
+                       : g_fv_rec == 7 ? rec_p_WENOmZ_2(n, u, im) : rec_p_WENOm3_2(n, u, im);
 }
 static double fv_rec_m(int n, const double *u, int im)
 {
-  return g_fv_rec == 5 ? rec_m_WENOm5_2(n, u, im) : rec_m_WENOm3_2(n, u, im);
+  return g_fv_rec == 5 ? rec_m_WENOm5_2(n, u, im)
+                       : g_fv_rec == 7 ? rec_m_WENOmZ_2(n, u, im) : rec_m_WENOm3_2(n, u, im);
 }
 
 /* scalarwave1 numflux for normal nrm between ui and ua */

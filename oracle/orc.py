@@ -56,6 +56,8 @@ def lib():
             "orc_rec1d_m_WENOm3": (d, [vp, i]),
             "orc_rec1d_p_WENO5": (d, [vp, i]),
             "orc_rec1d_m_WENO5": (d, [vp, i]),
+            "orc_rec1d_p_WENOZ": (d, [vp, i]),
+            "orc_rec1d_m_WENOZ": (d, [vp, i]),
             "orc_sr_p2c": (None, [d, vp, vp]),
             "orc_sr_c2p": (i, [d, vp, vp, d]),
             "orc_sr_flux": (None, [d, i, vp, vp, vp, vp]),
@@ -177,7 +179,7 @@ class OMesh:
 
 def set_fv_rec(mode):
     """reconstruction of the oracle's FV elements: "WENOm5_2" or "WENOm3_2" """
-    lib().orc_set_fv_rec({"WENOm5_2": 5, "WENOm3_2": 3}[mode])
+    lib().orc_set_fv_rec({"WENOm5_2": 5, "WENOm3_2": 3, "WENOmZ_2": 7}[mode])
 
 
 # ---------------------------------------------------------------------------
@@ -196,6 +198,16 @@ def rec_p(u, im):
 def rec_m(u, im):
     u = np.ascontiguousarray(u, dtype=np.float64)
     return lib().orc_rec1d_m_WENO5(_ptr(u), int(im))
+
+
+def rec_pz(u, im):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    return lib().orc_rec1d_p_WENOZ(_ptr(u), int(im))
+
+
+def rec_mz(u, im):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    return lib().orc_rec1d_m_WENOZ(_ptr(u), int(im))
 
 
 def sr_p2c(gam, P):

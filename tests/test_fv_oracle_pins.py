@@ -70,6 +70,36 @@ def test_weno5_is_essentially_non_oscillatory_at_a_step():
     assert orc.rec_p(np.full(6, 0.7), 2) == pytest.approx(0.7, abs=1e-16)
 
 
+@pytest.mark.parametrize("rec", [orc.rec_pz, orc.rec_mz])
+def test_wenoz_exact_on_quadratics_fifth_order_and_eno(rec):
+    """WENO-Z (rec1d.c:1627-1781): the same stencils as WENO5 with weights
+    g_k (1 + tau5 / (beta_k + eps)), tau5 = |beta0 - beta2|."""
+    rng = np.random.default_rng(6)
+    for _ in range(100):
+        c = rng.uniform(-2, 2, 3)
+        x = np.arange(-3, 4, dtype=float) + rng.uniform(-5, 5)
+        u = cell_avg_quadratic(c, x)
+        xf = x[3] + 1.0
+        exact = c[0] + c[1] * xf + c[2] * xf * xf
+        assert abs(rec(u, 3) - exact) <= 1e-12 * (1 + abs(exact))
+    e64, e128 = _sine_errors(64, rec), _sine_errors(128, rec)
+    assert e64 / e128 > 25.0
+    step = np.array([0., 0., 0., 1., 1., 1.])
+    if rec is orc.rec_pz:
+        assert abs(rec(step, 2)) < 1e-30
+    else:
+        assert abs(rec(step, 2) - 1.0) < 1e-15
+
+
+def test_wenoz_differs_from_weno5_only_in_the_weights():
+    # with beta0 = beta2 (tau5 = 0) the Z weights are the ideal ones exactly
+    u = np.array([1.0, 2.0, 2.5, 2.0, 1.0, 0.3])     # symmetric about index 2
+    lin = (2 * u[0] - 13 * u[1] + 47 * u[2] + 27 * u[3] - 3 * u[4]) / 60
+    assert abs(orc.rec_pz(u, 2) - lin) < 1e-14
+    # while WENO5-JS on the same data is not at the ideal weights
+    assert abs(orc.rec_p(u, 2) - lin) > 1e-6
+
+
 def test_weno5_m_mirrors_p():
     rng = np.random.default_rng(2)
     for _ in range(100):

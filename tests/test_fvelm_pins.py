@@ -95,7 +95,7 @@ def test_dnfn_extrap1_is_linear_extrapolation_from_the_fv_difference_locations()
         assert abs(df[0] - a) < 1e-13 and abs(df[-1] - (a + b * L)) < 1e-13
 
 
-@pytest.mark.parametrize("rec", ["WENOm3_2", "WENOm5_2"])
+@pytest.mark.parametrize("rec", ["WENOm3_2", "WENOm5_2", "WENOmZ_2"])
 @pytest.mark.parametrize("n", [5, 7])
 def test_fv_element_rhs_exact_for_linear_states(rec, n):
     orc.set_fv_rec(rec)

@@ -34,7 +34,7 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-@pytest.mark.parametrize("rec", ["WENOm3_2", "WENOm5_2"])
+@pytest.mark.parametrize("rec", ["WENOm3_2", "WENOm5_2", "WENOmZ_2"])
 @pytest.mark.parametrize("numflux", ["upwind", "LLF"])
 @pytest.mark.parametrize("luni,n", [(1, 5), (2, 8), (1, 10)])
 def test_fv_element_rhs_matches_oracle(rec, numflux, luni, n):
@@ -50,7 +50,7 @@ def test_fv_element_rhs_matches_oracle(rec, numflux, luni, n):
 
 
 @pytest.mark.parametrize("method", ["RK4", "sspRK3", "Euler"])
-@pytest.mark.parametrize("rec", ["WENOm3_2", "WENOm5_2"])
+@pytest.mark.parametrize("rec", ["WENOm3_2", "WENOm5_2", "WENOmZ_2"])
 def test_fv_element_steps_match_oracle(method, rec):
     print("fvelm steps", method, rec, flush=True)
     m, om = make(1, 6, rec)
