@@ -21,21 +21,29 @@ constexpr int kFaceT1 = 400, kFaceR1 = 100;
 constexpr int kFaceT4 = 800, kFaceR4 = 400;
 constexpr int kMaxFaceN = 10;
 
-// face descriptor, 8 ints per element face:
-//   [0] kind: 0 outer boundary, 1 conforming copy, 2 interpolation
-//   [1] nb, [2] base node in nb, [3] s1, [4] s2   (kind 1: nb node = base + a*s1 + b*s2)
-//   kind 2: [2] offset of the face's values in the face buffer ([v][n*n])
+// One element of an order bucket, everything its stage needs besides the
+// state: read with wave-uniform loads, so no load of the kernel waits on
+// another table lookup first.
+struct SlotRec {
+  long noff;            // first node of the element (state at 5 * noff)
+  double dXbdX[3];      // 2 / box widths
+  double bb[6];         // element box in patch coordinates
+  double s0, s1;        // CubedShell radii of the patch
+  long foff[6];         // kind 1: the neighbour's first state double + base
+                        // node; kind 2: the face's values in the face buffer
+  int fkind[6];         // 0 outer boundary, 1 conforming copy, 2 interpolated
+  int fs1[6], fs2[6];   // kind 1: neighbour node = base + a*fs1 + b*fs2
+  int dom;              // CubedShell patch direction and sign
+  int pad[9];
+};
+static_assert(sizeof(SlotRec) == 256, "SlotRec is 256 bytes");
+
 struct AmrArgs {
   const double *win = nullptr, *U = nullptr, *Ain = nullptr;
   double *Aout = nullptr, *Wout = nullptr, *R = nullptr;
-  const int *elist = nullptr;
+  const SlotRec *rec = nullptr;   // this bucket's elements
   int nlist = 0;
-  const long *soff = nullptr;     // [e] first state double of element e (5 * node offset)
-  const int *face = nullptr;      // [e][6][8]
   const double *aj = nullptr;     // interpolated face values
-  const double *dXbdX = nullptr;  // [e][3]
-  const double *egeo = nullptr;   // [e][8]
-  const int *edom = nullptr;      // [e]
   const double *dXdx = nullptr;   // element blocks of 9*np
   const double *xyz = nullptr;    // element blocks of 3*np
   const double *D = nullptr;      // this bucket's D[i*n + l]

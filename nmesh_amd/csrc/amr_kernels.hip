@@ -146,11 +146,20 @@ hipError_t launch_face_interp(const FaceArgs &a, bool multi, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Tables that no kernel writes are read through the constant address space:
+// wave-uniform reads from them become scalar loads issued together, instead
+// of a chain of vector loads each waiting for the last.
+#define NMX_CONST __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const NMX_CONST T *as_const(const T *p) {
+  return (const NMX_CONST T *)p;
+}
+
 // ---------------------------------------------------------------------------
 // stage kernel per order bucket
 // ---------------------------------------------------------------------------
 template <int NP, int GEOM, int NT>
-__global__ __launch_bounds__(NT, NMX_DG_WAVES) void dg_amr_kernel(AmrArgs a) {
+__global__ __launch_bounds__(NT, NMX_AMR_WAVES) void dg_amr_kernel(AmrArgs a) {
   constexpr int NP2 = NP * NP;
   constexpr int NP3 = NP * NP * NP;
   constexpr int NC = (NP3 + NT - 1) / NT;
@@ -178,9 +187,9 @@ __global__ __launch_bounds__(NT, NMX_DG_WAVES) void dg_amr_kernel(AmrArgs a) {
   }
 
   for (int it = i_first; it < i_hi; it += i_step) {
-    const int e = a.elist[it];
-    const long gbase = a.soff[e];
-    const int *fdesc = a.face + 48 * e;
+    const NMX_CONST SlotRec &R = *as_const(a.rec + it);
+    const long noff = R.noff;
+    const long gbase = 5 * noff;
     __syncthreads();
 
     // ---- phase 1: own fields and all six adjacent face planes ----
@@ -195,14 +204,13 @@ __global__ __launch_bounds__(NT, NMX_DG_WAVES) void dg_amr_kernel(AmrArgs a) {
     }
     for (int idx = tid; idx < 6 * NP2; idx += NT) {
       const int f = idx / NP2, q = idx % NP2;
-      const int *d = fdesc + 8 * f;
-      const int kind = d[0];
+      const int kind = R.fkind[f];
       if (kind == 1) {          // conforming copy from a same-order neighbour
-        const double *p = a.win + a.soff[d[1]] + d[2] + (q % NP) * d[3] + (q / NP) * d[4];
+        const double *p = a.win + R.foff[f] + (q % NP) * R.fs1[f] + (q / NP) * R.fs2[f];
 #pragma unroll
         for (int v = 0; v < 4; ++v) s_f[v][f][q] = p[v * NP3];
       } else if (kind == 2) {   // interpolated this stage by face_interp_kernel
-        const double *p = a.aj + d[2] + q;
+        const double *p = a.aj + R.foff[f] + q;
 #pragma unroll
         for (int v = 0; v < 4; ++v) s_f[v][f][q] = p[v * NP2];
       } else {                  // outer boundary: scalarwave1's exterior profile
@@ -214,7 +222,7 @@ __global__ __launch_bounds__(NT, NMX_DG_WAVES) void dg_amr_kernel(AmrArgs a) {
           const int jj = (dir == 1) ? pl : ((dir == 0) ? fa : fb);
           const int kk = (dir == 2) ? pl : fb;
           const int mynode = ii + NP * (jj + NP * kk);
-          const double *xp = a.xyz + 3 * (gbase / 5) + mynode;
+          const double *xp = a.xyz + 3 * noff + mynode;
           plane_wave(a.k, a.om, xp[0], xp[NP3], xp[2 * NP3], a.tdev[a.tidx], ua);
         }
 #pragma unroll
@@ -236,14 +244,14 @@ __global__ __launch_bounds__(NT, NMX_DG_WAVES) void dg_amr_kernel(AmrArgs a) {
     o.win = a.win; o.U = a.U; o.Ain = a.Ain; o.Aout = a.Aout; o.Wout = a.Wout; o.R = a.R;
     o.cA = a.cA; o.cW = a.cW; o.wend_inv = a.wend_inv; o.wmode = a.wmode; o.numflux = a.numflux;
     ElmGeo g;
-    g.dXbdX[0] = a.dXbdX[3 * e]; g.dXbdX[1] = a.dXbdX[3 * e + 1]; g.dXbdX[2] = a.dXbdX[3 * e + 2];
-    g.dXdx = a.dXdx + 9 * (gbase / 5);
+    g.dXbdX[0] = R.dXbdX[0]; g.dXbdX[1] = R.dXbdX[1]; g.dXbdX[2] = R.dXbdX[2];
+    g.dXdx = a.dXdx + 9 * noff;
     if (GEOM == 2) {
 #pragma unroll
-      for (int q = 0; q < 6; ++q) g.bb[q] = a.egeo[8 * e + q];
-      g.s0 = a.egeo[8 * e + 6];
-      g.s1 = a.egeo[8 * e + 7];
-      g.dom = a.edom[e];
+      for (int q = 0; q < 6; ++q) g.bb[q] = R.bb[q];
+      g.s0 = R.s0;
+      g.s1 = R.s1;
+      g.dom = R.dom;
     }
 #pragma unroll 1
     for (int c = 0; c < NC; ++c) {

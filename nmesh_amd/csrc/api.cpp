@@ -64,12 +64,15 @@ struct nmx_ctx {
   int geom = 0;          // 0 Cartesian, 1 stored dXdx, 2 recomputed CubedShell
   // batched (order-bucketed) mirror
   bool amr = false;
-  long *dsoff = nullptr;
-  int *dfaceB = nullptr, *djobs1 = nullptr, *djobs4 = nullptr, *dfirst4 = nullptr;
+  int *djobs1 = nullptr, *djobs4 = nullptr, *dfirst4 = nullptr;
   double *dBtab = nullptr, *dDall = nullptr, *dxball = nullptr, *daj = nullptr;
-  int nfint1 = 0, nfint4 = 0;   // interpolated faces with one / several neighbours
   // div: nodes per thread of this bucket's tile (0: the nmx_block parameter)
-  struct Bucket { int n; int count; int *dlist; long doff; long xoff; int div; };
+  // f1first/f1count: its elements' single-neighbour interpolated faces in
+  // djobs1; f4first/f4count: their faces with several neighbours in dfirst4
+  struct Bucket {
+    int n; int count; SlotRec *drec; long doff; long xoff; int div;
+    int f1first, f1count, f4first, f4count;
+  };
   std::vector<Bucket> buckets;
   std::vector<long> eoff;   // node offset per element
   // graph cache
@@ -83,11 +86,12 @@ struct nmx_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // helper streams for concurrent order buckets (fork/join inside graphs)
   static constexpr int kHelpers = 9;
+  static constexpr int kLanes = 4;     // stream lanes of the concurrent buckets
+  std::vector<int> border;             // buckets by decreasing node count
+  std::vector<int> blane;              // lane of each bucket
   hipStream_t helper[kHelpers] = {};
   hipEvent_t evfork = nullptr, evjoin[kHelpers] = {};
   // the faces with several neighbours run beside the single-neighbour ones
-  hipStream_t fhelper = nullptr;
-  hipEvent_t evface0 = nullptr, evface1 = nullptr;
   double last_ms = 0;
   bool tuned = false;
   // adjoint: lam register, neighbour face index per element face (plain
@@ -177,9 +181,6 @@ extern "C" int nmx_ctx_create(nmx_ctx **out, int device) {
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
   HIPCHK(hipEventCreateWithFlags(&c->evfork, hipEventDisableTiming));
-  HIPCHK(hipStreamCreateWithFlags(&c->fhelper, hipStreamNonBlocking));
-  HIPCHK(hipEventCreateWithFlags(&c->evface0, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&c->evface1, hipEventDisableTiming));
   for (int i = 0; i < nmx_ctx::kHelpers; ++i) {
     HIPCHK(hipStreamCreateWithFlags(&c->helper[i], hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&c->evjoin[i], hipEventDisableTiming));
@@ -195,14 +196,15 @@ static void free_mesh_dev(nmx_ctx *c) {
     if (*b) { (void)hipFree(*b); *b = nullptr; }
   if (c->dedom) { (void)hipFree(c->dedom); c->dedom = nullptr; }
   {
-    void **ptrs[] = {(void **)&c->dsoff, (void **)&c->dfaceB, (void **)&c->djobs1, (void **)&c->djobs4,
-                     (void **)&c->dfirst4, (void **)&c->dBtab, (void **)&c->dDall, (void **)&c->dxball,
-                     (void **)&c->daj};
+    void **ptrs[] = {(void **)&c->djobs1, (void **)&c->djobs4, (void **)&c->dfirst4,
+                     (void **)&c->dBtab, (void **)&c->dDall, (void **)&c->dxball, (void **)&c->daj};
     for (auto q : ptrs)
       if (*q) { (void)hipFree(*q); *q = nullptr; }
     for (auto &b : c->buckets)
-      if (b.dlist) (void)hipFree(b.dlist);
+      if (b.drec) (void)hipFree(b.drec);
     c->buckets.clear();
+    c->border.clear();
+    c->blane.clear();
   }
   if (c->dface) { (void)hipFree(c->dface); c->dface = nullptr; }
   if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
@@ -231,9 +233,6 @@ extern "C" int nmx_ctx_destroy(nmx_ctx *c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->evfork) (void)hipEventDestroy(c->evfork);
-  if (c->evface0) (void)hipEventDestroy(c->evface0);
-  if (c->evface1) (void)hipEventDestroy(c->evface1);
-  if (c->fhelper) (void)hipStreamDestroy(c->fhelper);
   for (int i = 0; i < nmx_ctx::kHelpers; ++i) {
     if (c->evjoin[i]) (void)hipEventDestroy(c->evjoin[i]);
     if (c->helper[i]) (void)hipStreamDestroy(c->helper[i]);
@@ -320,16 +319,21 @@ static int dev_upload(T **p, const std::vector<T> &h) {
   return NMX_OK;
 }
 
-// Tables of the batched path: per-element state offsets, face descriptors,
-// face interpolation jobs (single-neighbour faces and faces with several
-// neighbours in separate lists) with deduplicated 1D Lagrange rows, order
-// buckets.  Every gather a kernel will make is bounds-checked here.
+// Tables of the batched path: face descriptors, face interpolation jobs
+// (single-neighbour faces and faces with several neighbours in separate
+// lists, each grouped by the receiving element's order bucket) with
+// deduplicated 1D Lagrange rows, order buckets with one SlotRec per element.
+// Every gather a kernel will make is bounds-checked here.
 static int upload_amr_tables(nmx_ctx *c) {
   Mesh &M = c->mesh;
   const size_t ne = M.e.size();
   std::vector<long> soff(ne);
   for (size_t i = 0; i < ne; ++i) soff[i] = 5 * M.e[i].off;
-  std::vector<int> fd(48 * ne, 0), jobs1, jobs4, first4;
+  std::vector<int> fd(48 * ne, 0);
+  // jobs of each element's interpolated faces: single-neighbour faces, and
+  // the faces with several neighbours (each face's slots consecutive)
+  std::vector<std::vector<int>> ej1(ne);
+  std::vector<std::vector<std::vector<int>>> ej4(ne);
   std::vector<double> btab;
   std::map<std::vector<double>, int> bdedup;
   long ajlen = 0;
@@ -381,8 +385,11 @@ static int upload_amr_tables(nmx_ctx *c) {
         return fail(NMX_EUNSUP, "face interpolation exceeds the staging area");
       if (ajlen + 4L * n * n > (long)INT32_MAX) return fail(NMX_EUNSUP, "face buffer too large");
       d[0] = 2; d[2] = (int)ajlen;
-      std::vector<int> &jobs = ns == 1 ? jobs1 : jobs4;
-      if (ns > 1) first4.push_back((int)(jobs4.size() / kJobInts));
+      std::vector<int> *jobs = &ej1[i];
+      if (ns > 1) {
+        ej4[i].emplace_back();
+        jobs = &ej4[i].back();
+      }
       for (const InterpSlot &q : sl) {
         const Elm &N = M.e[q.nb];
         const int m = q.m;
@@ -402,24 +409,16 @@ static int upload_amr_tables(nmx_ctx *c) {
                              q.a0, q.a1, q.b0, q.b1, oa, ob, 0, 0};
         const long so = soff[q.nb];
         std::memcpy(rec + 14, &so, sizeof(long));
-        jobs.insert(jobs.end(), rec, rec + kJobInts);
+        jobs->insert(jobs->end(), rec, rec + kJobInts);
       }
       ajlen += 4L * n * n;
     }
   }
-  c->nfint1 = (int)(jobs1.size() / kJobInts);
-  c->nfint4 = (int)first4.size();
-  if (jobs1.empty()) jobs1.assign(kJobInts, 0);
-  if (jobs4.empty()) jobs4.assign(kJobInts, 0);
-  if (first4.empty()) first4.assign(1, 0);
   if (btab.empty()) btab.assign(1, 0.0);
   if (ajlen == 0) ajlen = 1;
   int r;
-  if ((r = dev_upload(&c->dsoff, soff)) || (r = dev_upload(&c->dfaceB, fd)) ||
-      (r = dev_upload(&c->djobs1, jobs1)) || (r = dev_upload(&c->djobs4, jobs4)) ||
-      (r = dev_upload(&c->dfirst4, first4)) || (r = dev_upload(&c->dBtab, btab)) ||
-      (r = dev_alloc(&c->daj, ajlen)))
-    return r;
+  if ((r = dev_upload(&c->dBtab, btab)) || (r = dev_alloc(&c->daj, ajlen))) return r;
+  std::vector<int> jobs1, jobs4, first4;
   // order buckets with their D matrices and nodes
   std::vector<double> dall, xball;
   for (int n = 2; n <= 10; ++n) {
@@ -430,17 +429,77 @@ static int upload_amr_tables(nmx_ctx *c) {
     nmx_ctx::Bucket b;
     b.n = n;
     b.count = (int)list.size();
-    b.dlist = nullptr;
+    b.drec = nullptr;
     b.div = 0;
-    if ((r = dev_upload(&b.dlist, list))) return r;
-    const Basis &bs = lgl_basis(n);
-    b.doff = (long)dall.size();
-    dall.insert(dall.end(), bs.D, bs.D + n * n);
-    b.xoff = (long)xball.size();
-    xball.insert(xball.end(), bs.x, bs.x + n);
+    std::vector<SlotRec> recs(list.size());
+    for (size_t it = 0; it < list.size(); ++it) {
+      const int e = list[it];
+      const Elm &E = M.e[e];
+      SlotRec &R = recs[it];
+      std::memset(&R, 0, sizeof R);
+      R.noff = E.off;
+      for (int d = 0; d < 3; ++d) R.dXbdX[d] = 2. / (E.bbox[2 * d + 1] - E.bbox[2 * d]);
+      for (int q = 0; q < 6; ++q) R.bb[q] = E.bbox[q];
+      R.s0 = M.pat[E.p].s0;
+      R.s1 = M.pat[E.p].s1;
+      R.dom = M.pat[E.p].dom;
+      for (int f = 0; f < 6; ++f) {
+        const int *d = &fd[48 * e + 8 * f];
+        R.fkind[f] = d[0];
+        if (d[0] == 1) {
+          R.foff[f] = soff[d[1]] + d[2];
+          R.fs1[f] = d[3];
+          R.fs2[f] = d[4];
+        } else if (d[0] == 2) {
+          R.foff[f] = d[2];
+        }
+      }
+    }
+    // the face jobs of this bucket's elements, in element order
+    b.f1first = (int)(jobs1.size() / kJobInts);
+    b.f4first = (int)first4.size();
+    for (int e : list) {
+      jobs1.insert(jobs1.end(), ej1[e].begin(), ej1[e].end());
+      for (const std::vector<int> &fj : ej4[e]) {
+        first4.push_back((int)(jobs4.size() / kJobInts));
+        jobs4.insert(jobs4.end(), fj.begin(), fj.end());
+      }
+    }
+    b.f1count = (int)(jobs1.size() / kJobInts) - b.f1first;
+    b.f4count = (int)first4.size() - b.f4first;
     c->buckets.push_back(b);
+    if ((r = dev_upload(&c->buckets.back().drec, recs))) return r;
+    const Basis &bs = lgl_basis(n);
+    c->buckets.back().doff = (long)dall.size();
+    dall.insert(dall.end(), bs.D, bs.D + n * n);
+    c->buckets.back().xoff = (long)xball.size();
+    xball.insert(xball.end(), bs.x, bs.x + n);
   }
   if ((r = dev_upload(&c->dDall, dall)) || (r = dev_upload(&c->dxball, xball))) return r;
+  // launch order and lanes of the concurrent buckets: largest first, each to
+  // the lane with the fewest nodes so far
+  {
+    const size_t nb = c->buckets.size();
+    auto nodes = [c](int i) { const long n = c->buckets[i].n; return (long)c->buckets[i].count * n * n * n; };
+    c->border.resize(nb);
+    for (size_t i = 0; i < nb; ++i) c->border[i] = (int)i;
+    std::stable_sort(c->border.begin(), c->border.end(), [&](int x, int y) { return nodes(x) > nodes(y); });
+    c->blane.assign(nb, 0);
+    long load[nmx_ctx::kLanes] = {};
+    for (int bi : c->border) {
+      int best = 0;
+      for (int l = 1; l < nmx_ctx::kLanes; ++l)
+        if (load[l] < load[best]) best = l;
+      c->blane[bi] = best;
+      load[best] += nodes(bi);
+    }
+  }
+  if (jobs1.empty()) jobs1.assign(kJobInts, 0);
+  if (jobs4.empty()) jobs4.assign(kJobInts, 0);
+  if (first4.empty()) first4.assign(1, 0);
+  if ((r = dev_upload(&c->djobs1, jobs1)) || (r = dev_upload(&c->djobs4, jobs4)) ||
+      (r = dev_upload(&c->dfirst4, first4)))
+    return r;
   return NMX_OK;
 }
 
@@ -843,7 +902,7 @@ static StageArgs base_args(nmx_ctx *c) {
 static AmrArgs amr_args(nmx_ctx *c, const StageArgs &sa) {
   AmrArgs a;
   a.win = sa.win; a.U = sa.U; a.Ain = sa.Ain; a.Aout = sa.Aout; a.Wout = sa.Wout; a.R = sa.R;
-  a.soff = c->dsoff; a.face = c->dfaceB; a.aj = c->daj; a.dXbdX = c->dXbdX; a.egeo = c->degeo; a.edom = c->dedom;
+  a.aj = c->daj;
   a.dXdx = c->dXdx; a.xyz = c->dxyz; a.tdev = sa.tdev; a.tidx = sa.tidx;
   a.cA = sa.cA; a.cW = sa.cW;
   for (int d = 0; d < 3; ++d) a.k[d] = sa.k[d];
@@ -857,7 +916,7 @@ static int launch_bucket(nmx_ctx *c, AmrArgs a, const nmx_ctx::Bucket &b, int di
   a.D = c->dDall + b.doff;
   a.xb = c->dxball + b.xoff;
   a.wend_inv = 1.0 / lgl_basis(b.n).w[0];
-  a.elist = b.dlist;
+  a.rec = b.drec;
   a.nlist = b.count;
   a.xcd_map = xcd_map;
   int grid = a.xcd_map ? 8 * ((b.count + 7) / 8) : b.count;
@@ -869,51 +928,48 @@ static int launch_bucket(nmx_ctx *c, AmrArgs a, const nmx_ctx::Bucket &b, int di
 static int launch_amr(nmx_ctx *c, const StageArgs &sa) {
   const bool perelm = (P(c, "nmx_path") == "perelement");
   AmrArgs a = amr_args(c, sa);
-  // interpolated faces of this stage first, all orders in one launch per list
-  {
-    FaceArgs fa;
-    fa.win = sa.win; fa.Btab = c->dBtab; fa.aj = c->daj;
-    const bool side = c->nfint4 > 0 && !perelm;
-    hipError_t err = hipSuccess;
-    if (side) {
-      HIPCHK(hipEventRecord(c->evface0, c->stream));
-      HIPCHK(hipStreamWaitEvent(c->fhelper, c->evface0, 0));
-      FaceArgs f4 = fa;
-      f4.jobs = c->djobs4; f4.first = c->dfirst4; f4.nfaces = c->nfint4;
-      err = launch_face_interp(f4, true, c->fhelper);
-      if (err == hipSuccess) HIPCHK(hipEventRecord(c->evface1, c->fhelper));
-    }
-    fa.jobs = c->djobs1; fa.first = nullptr; fa.nfaces = c->nfint1;
-    if (err == hipSuccess) err = launch_face_interp(fa, false, c->stream);
-    if (err == hipSuccess && !side) {
-      fa.jobs = c->djobs4; fa.first = c->dfirst4; fa.nfaces = c->nfint4;
-      err = launch_face_interp(fa, true, c->stream);
-    }
-    if (err != hipSuccess) return fail(NMX_EHIP, std::string("face launch: ") + hipGetErrorString(err));
-    if (side) HIPCHK(hipStreamWaitEvent(c->stream, c->evface1, 0));
-  }
   std::string blk = P(c, "nmx_block");
   const int div0 = blk == "half" ? 2 : blk == "quarter" ? 4 : 1;
-  // buckets run concurrently on helper streams forked from the context's
-  // stream (parallel branches when captured into a graph)
-  const bool fork = !perelm && c->buckets.size() > 1 && P(c, "nmx_concurrent") == "yes";
-  if (fork) HIPCHK(hipEventRecord(c->evfork, c->stream));
-  for (size_t bi = 0; bi < c->buckets.size(); ++bi) {
-    const auto &b = c->buckets[bi];
-    hipStream_t st = c->stream;
-    if (fork) {
-      st = c->helper[bi % nmx_ctx::kHelpers];
-      HIPCHK(hipStreamWaitEvent(st, c->evfork, 0));
+  // interpolated faces of one bucket's elements (single-neighbour faces,
+  // then faces with several neighbours), just before its stage kernel
+  auto faces = [&](const nmx_ctx::Bucket &b, hipStream_t st) -> int {
+    FaceArgs fa;
+    fa.win = sa.win; fa.Btab = c->dBtab; fa.aj = c->daj;
+    fa.jobs = c->djobs1 + (long)kJobInts * b.f1first; fa.first = nullptr; fa.nfaces = b.f1count;
+    hipError_t err = launch_face_interp(fa, false, st);
+    if (err == hipSuccess) {
+      fa.jobs = c->djobs4; fa.first = c->dfirst4 + b.f4first; fa.nfaces = b.f4count;
+      err = launch_face_interp(fa, true, st);
     }
+    if (err != hipSuccess) return fail(NMX_EHIP, std::string("face launch: ") + hipGetErrorString(err));
+    return NMX_OK;
+  };
+  // buckets run concurrently on kLanes stream lanes: lane 0 is the context's
+  // stream, the others helpers forked from it (parallel branches when
+  // captured into a graph); the largest buckets start first, one per lane,
+  // the rest queue on the least loaded lane (assigned at upload).  A
+  // bucket's faces read only the stage input, so each lane computes its own
+  // buckets' faces and the face work overlaps other lanes' buckets.
+  const bool fork = !perelm && c->buckets.size() > 1 && P(c, "nmx_concurrent") == "yes";
+  auto lane_stream = [c](int lane) { return lane == 0 ? c->stream : c->helper[lane - 1]; };
+  if (fork) {
+    HIPCHK(hipEventRecord(c->evfork, c->stream));
+    for (int l = 1; l < nmx_ctx::kLanes; ++l) HIPCHK(hipStreamWaitEvent(lane_stream(l), c->evfork, 0));
+  }
+  for (size_t k = 0; k < c->buckets.size(); ++k) {
+    const int bi = perelm ? (int)k : c->border[k];
+    const auto &b = c->buckets[bi];
+    hipStream_t st = fork ? lane_stream(c->blane[bi]) : c->stream;
     a.D = c->dDall + b.doff;
     a.xb = c->dxball + b.xoff;
     a.wend_inv = 1.0 / lgl_basis(b.n).w[0];
     const int div = b.div ? b.div : div0;
+    if (int r = faces(b, st)) return r;
     if (perelm) {
       a.xcd_map = 0;
       a.nlist = 1;
       for (int q = 0; q < b.count; ++q) {
-        a.elist = b.dlist + q;
+        a.rec = b.drec + q;
         hipError_t err = launch_dg_amr(b.n, c->geom, a, 1, div, c->stream);
         if (err != hipSuccess) return fail(NMX_EHIP, std::string("stage launch: ") + hipGetErrorString(err));
       }
@@ -921,11 +977,12 @@ static int launch_amr(nmx_ctx *c, const StageArgs &sa) {
     }
     int r = launch_bucket(c, a, b, div, sa.xcd_map, st);
     if (r) return r;
-    if (fork) HIPCHK(hipEventRecord(c->evjoin[bi % nmx_ctx::kHelpers], st));
   }
   if (fork)
-    for (size_t bi = 0; bi < c->buckets.size() && bi < (size_t)nmx_ctx::kHelpers; ++bi)
-      HIPCHK(hipStreamWaitEvent(c->stream, c->evjoin[bi], 0));
+    for (int l = 1; l < nmx_ctx::kLanes; ++l) {
+      HIPCHK(hipEventRecord(c->evjoin[l - 1], lane_stream(l)));
+      HIPCHK(hipStreamWaitEvent(c->stream, c->evjoin[l - 1], 0));
+    }
   return NMX_OK;
 }
 
