@@ -4,22 +4,23 @@
 
 namespace nmx {
 
-// face interpolation job, kJobInts ints per (face, neighbour slot), the
-// slots of one face consecutive:
+// face interpolation job, kJobInts ints per (face, neighbour):
 //   [0] n (my points per direction), [1] m (neighbour's), [2] offset of the
-//   face's values in the face buffer ([v][n*n]), [3] slots of this face,
-//   [4] base, [5] t1, [6] t2 (neighbour plane node = base + q1*t1 + q2*t2
-//   inside the neighbour), [7] swap, [8..11] a0,a1,b0,b1 covered point
-//   ranges, [12] offset of the rows for my a, [13] for my b, in Btab (rows of
-//   m doubles), [14..15] the neighbour's first state double (a long)
+//   job's region in the face buffer ([v][n*n]), [3] 1, [4] base, [5] t1,
+//   [6] t2 (neighbour plane node = base + q1*t1 + q2*t2 inside the
+//   neighbour), [7] swap, [8..11] a0,a1,b0,b1 covered point ranges, [12]
+//   offset of the rows for my a, [13] for my b, in Btab (rows of m doubles),
+//   [14..15] the neighbour's first state double (a long)
 constexpr int kJobInts = 16;
-// LDS of the face interpolation kernel per face, in doubles: first-pass
-// results (4 * covered rows * m per slot) and the rows of each pass (covered
-// rows or columns * m per slot); covered rows and columns of a slot <= kMaxFaceN.
-// Single-neighbour faces (T1, R1) and faces with up to 4 neighbours (T4, R4).
-constexpr int kFaceT1 = 400, kFaceR1 = 100;
-constexpr int kFaceT4 = 800, kFaceR4 = 400;
+// LDS of the face interpolation kernel, in doubles: first-pass results
+// (4 * covered rows * m) and the rows of each pass (covered rows or columns
+// * m); covered rows and columns <= kMaxFaceN.
+constexpr int kFaceT1 = 4 * 10 * 10, kFaceR1 = 10 * 10;
 constexpr int kMaxFaceN = 10;
+// Faces with several neighbours (a coarse face against finer ones): one
+// region per neighbour; mdesc[foff] = ns, then per neighbour {region, a0,
+// a1, b0, b1}.
+constexpr int kMaxFaceNb = 4;
 
 // One element of an order bucket, everything its stage needs besides the
 // state: read with wave-uniform loads, so no load of the kernel waits on
@@ -30,8 +31,10 @@ struct SlotRec {
   double bb[6];         // element box in patch coordinates
   double s0, s1;        // CubedShell radii of the patch
   long foff[6];         // kind 1: the neighbour's first state double + base
-                        // node; kind 2: the face's values in the face buffer
+                        // node; kind 2: the face's values in the face buffer;
+                        // kind 3: the face's entry in mdesc
   int fkind[6];         // 0 outer boundary, 1 conforming copy, 2 interpolated
+                        // from one neighbour, 3 from several
   int fs1[6], fs2[6];   // kind 1: neighbour node = base + a*fs1 + b*fs2
   int dom;              // CubedShell patch direction and sign
   int pad[9];
@@ -44,6 +47,7 @@ struct AmrArgs {
   const SlotRec *rec = nullptr;   // this bucket's elements
   int nlist = 0;
   const double *aj = nullptr;     // interpolated face values
+  const int *mdesc = nullptr;     // faces with several neighbours
   const double *dXdx = nullptr;   // element blocks of 9*np
   const double *xyz = nullptr;    // element blocks of 3*np
   const double *D = nullptr;      // this bucket's D[i*n + l]
@@ -59,14 +63,13 @@ struct AmrArgs {
 struct FaceArgs {
   const double *win = nullptr;
   const int *jobs = nullptr;      // [job][kJobInts]
-  const int *first = nullptr;     // [face] first job, or null: face i is job i
-  int nfaces = 0;
+  int nfaces = 0;                 // jobs
   const double *Btab = nullptr;
   double *aj = nullptr;
 };
 
 // div: nodes per thread (1, 2 or 4)
 hipError_t launch_dg_amr(int np, int geom, const AmrArgs &a, int grid, int div, hipStream_t s);
-hipError_t launch_face_interp(const FaceArgs &a, bool multi, hipStream_t s);
+hipError_t launch_face_interp(const FaceArgs &a, hipStream_t s);
 
 }  // namespace nmx

@@ -1,150 +1,22 @@
 // amr_kernels.hip -- batched DG RHS + RK stage for meshes with mixed
 // refinement levels and mixed element orders.
 //
-// Each RK stage is two steps:
-//   1. face_interp_kernel: one workgroup per non-conforming or mixed-order
-//      face computes ajsurf (the neighbours' values at this face's points)
-//      by 2D tensor Lagrange interpolation of each neighbour's face plane,
-//      averaged where several neighbours hold a point (surface.c:1081-1337,
-//      interpolate.c:296-363), into a compact face buffer;
-//   2. dg_amr_kernel, one launch per order bucket (buckets run concurrently):
-//      own fields to LDS, conforming faces copied from the neighbour
-//      (surface.c:825-1072), interpolated faces read from the face buffer,
-//      then the per-node RHS and stage update shared with the plain kernel.
+// Each RK stage, per order bucket (the buckets run concurrently on stream
+// lanes):
+//   1. face_interp_kernel: one wave per (interpolated face, neighbour)
+//      computes ajsurf (the neighbour's values at this face's points) by 2D
+//      tensor Lagrange interpolation of its face plane (surface.c:1081-1337,
+//      interpolate.c:296-363) into a compact face buffer;
+//   2. dg_amr_kernel: own fields to LDS, conforming faces copied from the
+//      neighbour (surface.c:825-1072), interpolated faces read from the face
+//      buffer (averaged where several neighbours hold a point), then the
+//      per-node RHS and stage update shared with the plain kernel.
 #include <hip/hip_runtime.h>
 
 #include "amr.h"
 #include "dg_common.h"
 
 namespace nmx {
-
-// ---------------------------------------------------------------------------
-// face interpolation, one wave per face:
-//   pass 1: T[v][x1][q2] = sum_q1 P[v](q1, q2) R1[x1][q1]; a lane owns one
-//           (slot, v, q2) line of the neighbour plane, requests all of it
-//           from L2 at once and keeps the x1 sums in registers;
-//   pass 2: V[v][a][b] = sum_q2 T[v][x1][q2] R2[x2][q2], (x1, x2) = (a, b),
-//           or (b, a) when the neighbour's first in-plane axis runs along my
-//           b; averaged over the neighbours that hold the point.
-// TC, RC: LDS capacity (doubles) for T and for the rows of one pass.
-// ---------------------------------------------------------------------------
-constexpr int kFaceThreads = 64;
-
-struct JobView {
-  int n, m, ajoff, nslot, base, t1, t2, sw, a0, a1, b0, b1, offA, offB;
-  long soff;
-  __device__ JobView(const int *j) {
-    n = j[0]; m = j[1]; ajoff = j[2]; nslot = j[3]; base = j[4]; t1 = j[5]; t2 = j[6];
-    sw = j[7]; a0 = j[8]; a1 = j[9]; b0 = j[10]; b1 = j[11]; offA = j[12]; offB = j[13];
-    soff = *reinterpret_cast<const long *>(j + 14);
-  }
-  __device__ int x1lo() const { return sw ? b0 : a0; }
-  __device__ int nx1() const { return (sw ? b1 - b0 : a1 - a0) + 1; }
-  __device__ int x2lo() const { return sw ? a0 : b0; }
-  __device__ int nx2() const { return (sw ? a1 - a0 : b1 - b0) + 1; }
-};
-
-template <int TC, int RC>
-__global__ __launch_bounds__(kFaceThreads) void face_interp_kernel(FaceArgs a) {
-  __shared__ double s_T[TC];      // [slot][v][x1][m]
-  __shared__ double s_R1[RC];     // [slot][x1][m]
-  __shared__ double s_R2[RC];     // [slot][x2][m]
-  const int tid = threadIdx.x;
-  // contiguous face ranges per XCD (faces are listed in element order, so
-  // neighbour planes gathered by nearby faces share that XCD's L2)
-  const int xcd = blockIdx.x & 7, g8 = gridDim.x >> 3;
-  const int chunk = (a.nfaces + 7) >> 3;
-  const int f_hi = min((xcd + 1) * chunk, a.nfaces);
-  for (int fi = xcd * chunk + (blockIdx.x >> 3); fi < f_hi; fi += g8) {
-    const int *job0 = a.jobs + kJobInts * (a.first ? a.first[fi] : fi);
-    const int n = job0[0], ns = job0[3], ajoff = job0[2];
-    const int n2 = n * n;
-    __syncthreads();
-    // rows of both passes to LDS
-    int ro1 = 0, ro2 = 0, total = 0;
-    for (int si = 0; si < ns; ++si) {
-      const JobView J(job0 + kJobInts * si);
-      const double *R1 = a.Btab + (J.sw ? J.offB : J.offA) + J.x1lo() * J.m;
-      const double *R2 = a.Btab + (J.sw ? J.offA : J.offB) + J.x2lo() * J.m;
-      for (int idx = tid; idx < J.nx1() * J.m; idx += kFaceThreads) s_R1[ro1 + idx] = R1[idx];
-      for (int idx = tid; idx < J.nx2() * J.m; idx += kFaceThreads) s_R2[ro2 + idx] = R2[idx];
-      ro1 += J.nx1() * J.m;
-      ro2 += J.nx2() * J.m;
-      total += 4 * J.m;
-    }
-    // pass 1 over the (slot, v, q2) lines of all slots
-    for (int idx0 = 0; idx0 < total; idx0 += kFaceThreads) {
-      const int idx = idx0 + tid;
-      const bool act = idx < total;
-      int si = 0, rem = act ? idx : 0, to = 0, r1 = 0;
-      for (;;) {
-        const JobView J(job0 + kJobInts * si);
-        if (rem < 4 * J.m || si == ns - 1) break;
-        rem -= 4 * J.m;
-        to += 4 * J.nx1() * J.m;
-        r1 += J.nx1() * J.m;
-        ++si;
-      }
-      const JobView J(job0 + kJobInts * si);
-      const int m = J.m, nx1 = J.nx1();
-      const int v = rem / m, q2 = rem % m;
-      const double *P = a.win + J.soff + J.base + (long)v * m * m * m + q2 * J.t2;
-      double p[kMaxFaceN];
-#pragma unroll
-      for (int q1 = 0; q1 < kMaxFaceN; ++q1) p[q1] = (act && q1 < m) ? P[q1 * J.t1] : 0.0;
-      __syncthreads();   // rows in LDS
-      if (act) {
-        double *T = s_T + to + v * nx1 * m + q2;
-        const double *R1 = s_R1 + r1;
-#pragma unroll
-        for (int x = 0; x < kMaxFaceN; ++x) {
-          if (x < nx1) {
-            double t = 0.0;
-#pragma unroll
-            for (int q1 = 0; q1 < kMaxFaceN; ++q1)
-              if (q1 < m) t += p[q1] * R1[x * m + q1];
-            T[x * m] = t;
-          }
-        }
-      }
-    }
-    __syncthreads();
-    // pass 2 and averaging
-    for (int idx = tid; idx < 4 * n2; idx += kFaceThreads) {
-      const int v = idx / n2, q = idx % n2;
-      const int fa = q % n, fb = q / n;
-      double sum = 0.0;
-      int cnt = 0, to = 0, ro = 0;
-      for (int si = 0; si < ns; ++si) {
-        const JobView J(job0 + kJobInts * si);
-        const int m = J.m, nx1 = J.nx1();
-        if (fa >= J.a0 && fa <= J.a1 && fb >= J.b0 && fb <= J.b1) {
-          const int x1 = (J.sw ? fb : fa) - J.x1lo(), x2 = (J.sw ? fa : fb) - J.x2lo();
-          const double *T = s_T + to + (v * nx1 + x1) * m;
-          const double *row = s_R2 + ro + x2 * m;
-          double sv = 0.0;
-          for (int q2 = 0; q2 < m; ++q2) sv += T[q2] * row[q2];
-          sum += sv;
-          ++cnt;
-        }
-        to += 4 * nx1 * m;
-        ro += J.nx2() * m;
-      }
-      if (cnt > 1) sum /= cnt;
-      a.aj[ajoff + idx] = sum;
-    }
-  }
-}
-
-hipError_t launch_face_interp(const FaceArgs &a, bool multi, hipStream_t s) {
-  if (a.nfaces == 0) return hipSuccess;
-  const int grid = 8 * ((a.nfaces + 7) / 8);   // a multiple of 8 (XCD mapping)
-  if (multi)
-    hipLaunchKernelGGL((face_interp_kernel<kFaceT4, kFaceR4>), dim3(grid), dim3(kFaceThreads), 0, s, a);
-  else
-    hipLaunchKernelGGL((face_interp_kernel<kFaceT1, kFaceR1>), dim3(grid), dim3(kFaceThreads), 0, s, a);
-  return hipGetLastError();
-}
 
 // Tables that no kernel writes are read through the constant address space:
 // wave-uniform reads from them become scalar loads issued together, instead
@@ -153,6 +25,103 @@ hipError_t launch_face_interp(const FaceArgs &a, bool multi, hipStream_t s) {
 template <class T>
 __device__ __forceinline__ const NMX_CONST T *as_const(const T *p) {
   return (const NMX_CONST T *)p;
+}
+
+// ---------------------------------------------------------------------------
+// face interpolation, one wave per (face, neighbour) job:
+//   pass 1: T[v][x1][q2] = sum_q1 P[v](q1, q2) R1[x1][q1]; a lane owns one
+//           (v, q2) line of the neighbour plane and keeps the x1 sums in
+//           registers;
+//   pass 2: V[v][a][b] = sum_q2 T[v][x1][q2] R2[x2][q2], (x1, x2) = (a, b),
+//           or (b, a) when the neighbour's first in-plane axis runs along my
+//           b, for the points this neighbour covers (0 elsewhere).
+// The plane lines and both passes' rows are requested together, so a job
+// waits on one round trip to memory.  A face with several neighbours has
+// one region per neighbour, averaged by the stage kernel.
+// ---------------------------------------------------------------------------
+constexpr int kFaceThreads = 64;
+
+struct JobView {
+  int n, m, ajoff, nslot, base, t1, t2, sw, a0, a1, b0, b1, offA, offB;
+  long soff;
+  __device__ JobView(const NMX_CONST int *j) {
+    n = j[0]; m = j[1]; ajoff = j[2]; nslot = j[3]; base = j[4]; t1 = j[5]; t2 = j[6];
+    sw = j[7]; a0 = j[8]; a1 = j[9]; b0 = j[10]; b1 = j[11]; offA = j[12]; offB = j[13];
+    soff = (long)(unsigned)j[14] | ((long)j[15] << 32);
+  }
+  __device__ int x1lo() const { return sw ? b0 : a0; }
+  __device__ int nx1() const { return (sw ? b1 - b0 : a1 - a0) + 1; }
+  __device__ int x2lo() const { return sw ? a0 : b0; }
+  __device__ int nx2() const { return (sw ? a1 - a0 : b1 - b0) + 1; }
+};
+
+__global__ __launch_bounds__(kFaceThreads) void face_interp_kernel(FaceArgs a) {
+  __shared__ double s_T[kFaceT1];    // [v][x1][m]
+  __shared__ double s_R1[kFaceR1];   // [x1][m]
+  __shared__ double s_R2[kFaceR1];   // [x2][m]
+  const int tid = threadIdx.x;
+  const int xcd = blockIdx.x & 7, g8 = gridDim.x >> 3;
+  const int chunk = (a.nfaces + 7) >> 3;
+  const int f_hi = min((xcd + 1) * chunk, a.nfaces);
+  for (int fi = xcd * chunk + (blockIdx.x >> 3); fi < f_hi; fi += g8) {
+    const JobView J(as_const(a.jobs) + kJobInts * fi);
+    const int n = J.n, m = J.m, nx1 = J.nx1(), nx2 = J.nx2();
+    const int n1 = nx1 * m, n2 = nx2 * m;
+    // every load of this face: rows (at most 100 doubles each, two per lane)
+    // and my (v, q2) line of the plane
+    const double *R1 = a.Btab + (J.sw ? J.offB : J.offA) + J.x1lo() * m;
+    const double *R2 = a.Btab + (J.sw ? J.offA : J.offB) + J.x2lo() * m;
+    const double r1a = tid < n1 ? R1[tid] : 0.0, r1b = tid + 64 < n1 ? R1[tid + 64] : 0.0;
+    const double r2a = tid < n2 ? R2[tid] : 0.0, r2b = tid + 64 < n2 ? R2[tid + 64] : 0.0;
+    const bool act = tid < 4 * m;
+    const int v = act ? tid / m : 0, q2 = act ? tid % m : 0;
+    const double *P = a.win + J.soff + J.base + (long)v * m * m * m + q2 * J.t2;
+    double p[kMaxFaceN];
+#pragma unroll
+    for (int q1 = 0; q1 < kMaxFaceN; ++q1) p[q1] = (act && q1 < m) ? P[q1 * J.t1] : 0.0;
+    __syncthreads();   // the previous face is done with LDS
+    if (tid < n1) s_R1[tid] = r1a;
+    if (tid + 64 < n1) s_R1[tid + 64] = r1b;
+    if (tid < n2) s_R2[tid] = r2a;
+    if (tid + 64 < n2) s_R2[tid + 64] = r2b;
+    __syncthreads();
+    if (act) {
+      double *T = s_T + v * nx1 * m + q2;
+#pragma unroll
+      for (int x = 0; x < kMaxFaceN; ++x) {
+        if (x < nx1) {
+          double t = 0.0;
+#pragma unroll
+          for (int q1 = 0; q1 < kMaxFaceN; ++q1)
+            if (q1 < m) t += p[q1] * s_R1[x * m + q1];
+          T[x * m] = t;
+        }
+      }
+    }
+    __syncthreads();
+    const int nn = n * n;
+    for (int idx = tid; idx < 4 * nn; idx += kFaceThreads) {
+      const int vv = idx / nn, q = idx % nn;
+      const int fa = q % n, fb = q / n;
+      double sum = 0.0;
+      if (fa >= J.a0 && fa <= J.a1 && fb >= J.b0 && fb <= J.b1) {
+        const int x1 = (J.sw ? fb : fa) - J.x1lo(), x2 = (J.sw ? fa : fb) - J.x2lo();
+        const double *T = s_T + (vv * nx1 + x1) * m;
+        const double *row = s_R2 + x2 * m;
+        double sv = 0.0;
+        for (int k = 0; k < m; ++k) sv += T[k] * row[k];
+        sum += sv;
+      }
+      a.aj[J.ajoff + idx] = sum;
+    }
+  }
+}
+
+hipError_t launch_face_interp(const FaceArgs &a, hipStream_t s) {
+  if (a.nfaces == 0) return hipSuccess;
+  const int grid = 8 * ((a.nfaces + 7) / 8);   // a multiple of 8 (XCD mapping)
+  hipLaunchKernelGGL(face_interp_kernel, dim3(grid), dim3(kFaceThreads), 0, s, a);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -213,6 +182,24 @@ __global__ __launch_bounds__(NT, NMX_AMR_WAVES) void dg_amr_kernel(AmrArgs a) {
         const double *p = a.aj + R.foff[f] + q;
 #pragma unroll
         for (int v = 0; v < 4; ++v) s_f[v][f][q] = p[v * NP2];
+      } else if (kind == 3) {   // several neighbours: averaged where several hold
+                                // the point (surface.c:1322-1337)
+        const int *md = a.mdesc + R.foff[f];
+        const int ns = md[0];
+        const int fa = q % NP, fb = q / NP;
+        double sum[4] = {0.0, 0.0, 0.0, 0.0};
+        int cnt = 0;
+        for (int si = 0; si < ns; ++si) {
+          const int *d = md + 1 + 5 * si;
+          if (fa >= d[1] && fa <= d[2] && fb >= d[3] && fb <= d[4]) {
+            const double *p = a.aj + d[0] + q;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sum[v] += p[v * NP2];
+            ++cnt;
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s_f[v][f][q] = cnt > 1 ? sum[v] / cnt : sum[v];
       } else {                  // outer boundary: scalarwave1's exterior profile
         double ua[4] = {0.0, 0.0, 0.0, 0.0};
         if (a.profile) {

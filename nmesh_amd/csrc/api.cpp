@@ -64,14 +64,14 @@ struct nmx_ctx {
   int geom = 0;          // 0 Cartesian, 1 stored dXdx, 2 recomputed CubedShell
   // batched (order-bucketed) mirror
   bool amr = false;
-  int *djobs1 = nullptr, *djobs4 = nullptr, *dfirst4 = nullptr;
+  int *djobs1 = nullptr, *dmdesc = nullptr;
   double *dBtab = nullptr, *dDall = nullptr, *dxball = nullptr, *daj = nullptr;
   // div: nodes per thread of this bucket's tile (0: the nmx_block parameter)
-  // f1first/f1count: its elements' single-neighbour interpolated faces in
-  // djobs1; f4first/f4count: their faces with several neighbours in dfirst4
+  // f1first/f1count: the face interpolation jobs of its elements in djobs1
   struct Bucket {
     int n; int count; SlotRec *drec; long doff; long xoff; int div;
-    int f1first, f1count, f4first, f4count;
+    int f1first, f1count;
+    float ms;   // tuned stage time alone (0: not tuned)
   };
   std::vector<Bucket> buckets;
   std::vector<long> eoff;   // node offset per element
@@ -196,7 +196,7 @@ static void free_mesh_dev(nmx_ctx *c) {
     if (*b) { (void)hipFree(*b); *b = nullptr; }
   if (c->dedom) { (void)hipFree(c->dedom); c->dedom = nullptr; }
   {
-    void **ptrs[] = {(void **)&c->djobs1, (void **)&c->djobs4, (void **)&c->dfirst4,
+    void **ptrs[] = {(void **)&c->djobs1, (void **)&c->dmdesc,
                      (void **)&c->dBtab, (void **)&c->dDall, (void **)&c->dxball, (void **)&c->daj};
     for (auto q : ptrs)
       if (*q) { (void)hipFree(*q); *q = nullptr; }
@@ -319,6 +319,46 @@ static int dev_upload(T **p, const std::vector<T> &h) {
   return NMX_OK;
 }
 
+// Launch order and lanes of the concurrent buckets, largest first.  Running
+// together, the largest buckets share the GPU and end near the same time,
+// so each of the kLanes-1 largest gets a lane of its own and the rest queue
+// on the last lane, unless together they outweigh the largest bucket: then
+// each goes to the lane with the least work so far.  Work is the bucket's
+// tuned stage time once the autotuner has run, its node count before.
+static void assign_lanes(nmx_ctx *c) {
+  const size_t nb = c->buckets.size();
+  bool timed = nb > 0;
+  for (const auto &b : c->buckets) timed = timed && b.ms > 0;
+  auto work = [c, timed](int i) {
+    const nmx_ctx::Bucket &b = c->buckets[i];
+    const double n = b.n;
+    return timed ? (double)b.ms : (double)b.count * n * n * n;
+  };
+  c->border.resize(nb);
+  for (size_t i = 0; i < nb; ++i) c->border[i] = (int)i;
+  std::stable_sort(c->border.begin(), c->border.end(), [&](int x, int y) { return work(x) > work(y); });
+  c->blane.assign(nb, 0);
+  constexpr int L = nmx_ctx::kLanes;
+  double load[L] = {};
+  double rest = 0;
+  for (size_t k = L - 1; k < nb; ++k) rest += work(c->border[k]);
+  const bool last_lane = nb >= (size_t)L && rest <= work(c->border[0]);
+  for (size_t k = 0; k < nb; ++k) {
+    const int bi = c->border[k];
+    int lane = 0;
+    if (k < (size_t)L - 1) {
+      lane = (int)k;
+    } else if (last_lane) {
+      lane = L - 1;
+    } else {
+      for (int l = 1; l < L; ++l)
+        if (load[l] < load[lane]) lane = l;
+    }
+    c->blane[bi] = lane;
+    load[lane] += work(bi);
+  }
+}
+
 // Tables of the batched path: face descriptors, face interpolation jobs
 // (single-neighbour faces and faces with several neighbours in separate
 // lists, each grouped by the receiving element's order bucket) with
@@ -330,10 +370,9 @@ static int upload_amr_tables(nmx_ctx *c) {
   std::vector<long> soff(ne);
   for (size_t i = 0; i < ne; ++i) soff[i] = 5 * M.e[i].off;
   std::vector<int> fd(48 * ne, 0);
-  // jobs of each element's interpolated faces: single-neighbour faces, and
-  // the faces with several neighbours (each face's slots consecutive)
+  // jobs of each element's interpolated faces, one per (face, neighbour)
   std::vector<std::vector<int>> ej1(ne);
-  std::vector<std::vector<std::vector<int>>> ej4(ne);
+  std::vector<int> mdesc;
   std::vector<double> btab;
   std::map<std::vector<double>, int> bdedup;
   long ajlen = 0;
@@ -367,30 +406,28 @@ static int upload_amr_tables(nmx_ctx *c) {
       }
       const std::vector<InterpSlot> &sl = M.fint[6 * i + f];
       const int ns = (int)sl.size();
-      if (ns == 0 || ns > 4) return fail(NMX_EUNSUP, "face with no or more than 4 interpolation neighbours");
-      // LDS of the face kernel: first-pass results (4 * covered rows * m
-      // per slot) and the rows of both passes (covered rows or columns * m)
-      int needT = 0, needR1 = 0, needR2 = 0;
+      if (ns == 0 || ns > kMaxFaceNb) return fail(NMX_EUNSUP, "face with no or more than 4 interpolation neighbours");
+      // the face kernel's LDS holds 4 * covered rows * m first-pass results
+      // and covered rows (or columns) * m doubles of each pass's rows
       for (const InterpSlot &q : sl) {
         const int nx1 = q.swap ? (q.b1 - q.b0 + 1) : (q.a1 - q.a0 + 1);
         const int nx2 = q.swap ? (q.a1 - q.a0 + 1) : (q.b1 - q.b0 + 1);
-        if (nx1 > kMaxFaceN || nx2 > kMaxFaceN || q.m > kMaxFaceN)
+        if (nx1 > kMaxFaceN || nx2 > kMaxFaceN || q.m > kMaxFaceN ||
+            4 * nx1 * q.m > kFaceT1 || nx1 * q.m > kFaceR1 || nx2 * q.m > kFaceR1)
           return fail(NMX_EUNSUP, "face order above 10");
-        needT += 4 * nx1 * q.m;
-        needR1 += nx1 * q.m;
-        needR2 += nx2 * q.m;
       }
-      const int capT = ns == 1 ? kFaceT1 : kFaceT4, capR = ns == 1 ? kFaceR1 : kFaceR4;
-      if (needT > capT || needR1 > capR || needR2 > capR)
-        return fail(NMX_EUNSUP, "face interpolation exceeds the staging area");
-      if (ajlen + 4L * n * n > (long)INT32_MAX) return fail(NMX_EUNSUP, "face buffer too large");
-      d[0] = 2; d[2] = (int)ajlen;
-      std::vector<int> *jobs = &ej1[i];
-      if (ns > 1) {
-        ej4[i].emplace_back();
-        jobs = &ej4[i].back();
+      // one region of 4*n*n values per neighbour; a face with several
+      // neighbours is averaged by its element from their regions
+      if (ajlen + 4L * n * n * ns > (long)INT32_MAX) return fail(NMX_EUNSUP, "face buffer too large");
+      if (ns == 1) {
+        d[0] = 2; d[2] = (int)ajlen;
+      } else {
+        d[0] = 3; d[2] = (int)mdesc.size();
+        mdesc.push_back(ns);
       }
-      for (const InterpSlot &q : sl) {
+      for (int si = 0; si < ns; ++si) {
+        const InterpSlot &q = sl[si];
+        const long region = ajlen + 4L * n * n * si;
         const Elm &N = M.e[q.nb];
         const int m = q.m;
         const long base = q.nbase - N.off;
@@ -405,20 +442,27 @@ static int upload_amr_tables(nmx_ctx *c) {
         if ((int)q.Ba.size() != n * m || (int)q.Bb.size() != n * m)
           return fail(NMX_EINVAL, "interpolation rows have the wrong size");
         int oa = brow(q.Ba), ob = brow(q.Bb);
-        int rec[kJobInts] = {n, m, (int)ajlen, ns, (int)base, q.t1, q.t2, q.swap,
+        int rec[kJobInts] = {n, m, (int)region, 1, (int)base, q.t1, q.t2, q.swap,
                              q.a0, q.a1, q.b0, q.b1, oa, ob, 0, 0};
         const long so = soff[q.nb];
         std::memcpy(rec + 14, &so, sizeof(long));
-        jobs->insert(jobs->end(), rec, rec + kJobInts);
+        ej1[i].insert(ej1[i].end(), rec, rec + kJobInts);
+        if (ns > 1) {
+          const int md[5] = {(int)region, q.a0, q.a1, q.b0, q.b1};
+          mdesc.insert(mdesc.end(), md, md + 5);
+        }
       }
-      ajlen += 4L * n * n;
+      ajlen += 4L * n * n * ns;
     }
   }
   if (btab.empty()) btab.assign(1, 0.0);
+  if (mdesc.empty()) mdesc.assign(1, 0);
   if (ajlen == 0) ajlen = 1;
   int r;
-  if ((r = dev_upload(&c->dBtab, btab)) || (r = dev_alloc(&c->daj, ajlen))) return r;
-  std::vector<int> jobs1, jobs4, first4;
+  if ((r = dev_upload(&c->dBtab, btab)) || (r = dev_alloc(&c->daj, ajlen)) ||
+      (r = dev_upload(&c->dmdesc, mdesc)))
+    return r;
+  std::vector<int> jobs1;
   // order buckets with their D matrices and nodes
   std::vector<double> dall, xball;
   for (int n = 2; n <= 10; ++n) {
@@ -431,6 +475,7 @@ static int upload_amr_tables(nmx_ctx *c) {
     b.count = (int)list.size();
     b.drec = nullptr;
     b.div = 0;
+    b.ms = 0.f;
     std::vector<SlotRec> recs(list.size());
     for (size_t it = 0; it < list.size(); ++it) {
       const int e = list[it];
@@ -450,23 +495,15 @@ static int upload_amr_tables(nmx_ctx *c) {
           R.foff[f] = soff[d[1]] + d[2];
           R.fs1[f] = d[3];
           R.fs2[f] = d[4];
-        } else if (d[0] == 2) {
+        } else if (d[0] >= 2) {
           R.foff[f] = d[2];
         }
       }
     }
     // the face jobs of this bucket's elements, in element order
     b.f1first = (int)(jobs1.size() / kJobInts);
-    b.f4first = (int)first4.size();
-    for (int e : list) {
-      jobs1.insert(jobs1.end(), ej1[e].begin(), ej1[e].end());
-      for (const std::vector<int> &fj : ej4[e]) {
-        first4.push_back((int)(jobs4.size() / kJobInts));
-        jobs4.insert(jobs4.end(), fj.begin(), fj.end());
-      }
-    }
+    for (int e : list) jobs1.insert(jobs1.end(), ej1[e].begin(), ej1[e].end());
     b.f1count = (int)(jobs1.size() / kJobInts) - b.f1first;
-    b.f4count = (int)first4.size() - b.f4first;
     c->buckets.push_back(b);
     if ((r = dev_upload(&c->buckets.back().drec, recs))) return r;
     const Basis &bs = lgl_basis(n);
@@ -476,31 +513,9 @@ static int upload_amr_tables(nmx_ctx *c) {
     xball.insert(xball.end(), bs.x, bs.x + n);
   }
   if ((r = dev_upload(&c->dDall, dall)) || (r = dev_upload(&c->dxball, xball))) return r;
-  // launch order and lanes of the concurrent buckets: largest first, each to
-  // the lane with the fewest nodes so far
-  {
-    const size_t nb = c->buckets.size();
-    auto nodes = [c](int i) { const long n = c->buckets[i].n; return (long)c->buckets[i].count * n * n * n; };
-    c->border.resize(nb);
-    for (size_t i = 0; i < nb; ++i) c->border[i] = (int)i;
-    std::stable_sort(c->border.begin(), c->border.end(), [&](int x, int y) { return nodes(x) > nodes(y); });
-    c->blane.assign(nb, 0);
-    long load[nmx_ctx::kLanes] = {};
-    for (int bi : c->border) {
-      int best = 0;
-      for (int l = 1; l < nmx_ctx::kLanes; ++l)
-        if (load[l] < load[best]) best = l;
-      c->blane[bi] = best;
-      load[best] += nodes(bi);
-    }
-  }
+  assign_lanes(c);
   if (jobs1.empty()) jobs1.assign(kJobInts, 0);
-  if (jobs4.empty()) jobs4.assign(kJobInts, 0);
-  if (first4.empty()) first4.assign(1, 0);
-  if ((r = dev_upload(&c->djobs1, jobs1)) || (r = dev_upload(&c->djobs4, jobs4)) ||
-      (r = dev_upload(&c->dfirst4, first4)))
-    return r;
-  return NMX_OK;
+  return dev_upload(&c->djobs1, jobs1);
 }
 
 // host -> device mirror of the mesh (element-block layout)
@@ -903,6 +918,7 @@ static AmrArgs amr_args(nmx_ctx *c, const StageArgs &sa) {
   AmrArgs a;
   a.win = sa.win; a.U = sa.U; a.Ain = sa.Ain; a.Aout = sa.Aout; a.Wout = sa.Wout; a.R = sa.R;
   a.aj = c->daj;
+  a.mdesc = c->dmdesc;
   a.dXdx = c->dXdx; a.xyz = c->dxyz; a.tdev = sa.tdev; a.tidx = sa.tidx;
   a.cA = sa.cA; a.cW = sa.cW;
   for (int d = 0; d < 3; ++d) a.k[d] = sa.k[d];
@@ -930,26 +946,21 @@ static int launch_amr(nmx_ctx *c, const StageArgs &sa) {
   AmrArgs a = amr_args(c, sa);
   std::string blk = P(c, "nmx_block");
   const int div0 = blk == "half" ? 2 : blk == "quarter" ? 4 : 1;
-  // interpolated faces of one bucket's elements (single-neighbour faces,
-  // then faces with several neighbours), just before its stage kernel
+  // interpolated faces of one bucket's elements, just before its stage kernel
   auto faces = [&](const nmx_ctx::Bucket &b, hipStream_t st) -> int {
     FaceArgs fa;
     fa.win = sa.win; fa.Btab = c->dBtab; fa.aj = c->daj;
-    fa.jobs = c->djobs1 + (long)kJobInts * b.f1first; fa.first = nullptr; fa.nfaces = b.f1count;
-    hipError_t err = launch_face_interp(fa, false, st);
-    if (err == hipSuccess) {
-      fa.jobs = c->djobs4; fa.first = c->dfirst4 + b.f4first; fa.nfaces = b.f4count;
-      err = launch_face_interp(fa, true, st);
-    }
+    fa.jobs = c->djobs1 + (long)kJobInts * b.f1first; fa.nfaces = b.f1count;
+    hipError_t err = launch_face_interp(fa, st);
     if (err != hipSuccess) return fail(NMX_EHIP, std::string("face launch: ") + hipGetErrorString(err));
     return NMX_OK;
   };
   // buckets run concurrently on kLanes stream lanes: lane 0 is the context's
   // stream, the others helpers forked from it (parallel branches when
-  // captured into a graph); the largest buckets start first, one per lane,
-  // the rest queue on the least loaded lane (assigned at upload).  A
-  // bucket's faces read only the stage input, so each lane computes its own
-  // buckets' faces and the face work overlaps other lanes' buckets.
+  // captured into a graph); the longest buckets start first, one per lane,
+  // the rest queue on the least loaded lane.  A bucket's faces read only the
+  // stage input, so each lane computes its own buckets' faces and the face
+  // work overlaps other lanes' buckets.
   const bool fork = !perelm && c->buckets.size() > 1 && P(c, "nmx_concurrent") == "yes";
   auto lane_stream = [c](int lane) { return lane == 0 ? c->stream : c->helper[lane - 1]; };
   if (fork) {
@@ -1135,7 +1146,9 @@ extern "C" int nmx_autotune(nmx_ctx *c) {
         if (ms < bms) { bms = ms; best_div = div; }
       }
       b.div = best_div;
+      b.ms = bms;
     }
+    assign_lanes(c);
   }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
