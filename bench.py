@@ -216,6 +216,33 @@ def c1_params():
             "evolve_method": "RK4", "dtfac": 0.25, "dt": "auto"}
 
 
+def filter_leg(nm, reps=100):
+    """The FILTER bin's exponential filter (alp 36, s 32, dn 0, Jacobian
+    power 1) of all 5 evolved fields on the 3072-element N=7 shell, in place,
+    reps back-to-back launches.  Algorithmic bytes per node: 5 doubles read
+    and 5 written (the Jacobian is recomputed): 80 B."""
+    m = nm.Mesh(dict(c2_params(), evolve_filter_all="yes"), device=0)
+    nm.amr_setup_mesh(m)
+    x = m.coords()
+    m.upload(shell_initial_data(x))
+    nm.evolve_filter_evosys_mesh(m)
+    m.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        nm.evolve_filter_evosys_mesh(m)
+    m.synchronize()
+    wall = time.perf_counter() - t0
+    per = wall / reps
+    gbs = 80.0 * m.nnodes / per / 1e9
+    out = {"workload": "exponential filter of pi, cx, cy, cz, phi on the 3072-element N=7 shell "
+                       "(1,572,864 nodes), alp 36, s 32, J-weighted",
+           "value": m.nnodes / per, "unit": "node-filterings/s", "us_per_launch": per * 1e6,
+           "bytes_per_node": 80.0, "achieved_GBps": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
+           "kernel": "filter_kernel<8,2,512>"}
+    del m
+    return out
+
+
 def adjoint_leg(nm, steps=256, every=16, reps=3):
     """Discrete adjoint cost against the forward step.  C5: the config-1 box
     (64 elements, N=4, 8,000 nodes), 256 RK4 steps, psi = exp(-r^2) with
@@ -307,6 +334,8 @@ def main():
     ap.add_argument("--only-fv", action="store_true", help="run only the FV leg")
     ap.add_argument("--fv-steps", type=int, default=50, help="sspRK3 steps of the FV leg")
     ap.add_argument("--only-adjoint", action="store_true", help="run only the adjoint-cost leg")
+    ap.add_argument("--only-filter", action="store_true", help="run only the exponential-filter leg")
+    ap.add_argument("--no-filter", action="store_true", help="skip the exponential-filter leg")
     ap.add_argument("--path", default="plain")
     ap.add_argument("--graph", default="yes", help="capture RK steps in hipGraphs (yes|no)")
     ap.add_argument("--geometry", default="analytic", help="dXdx recomputed (analytic) or streamed (stored)")
@@ -346,6 +375,9 @@ def main():
         return
     if args.only_adjoint:
         print(json.dumps(adjoint_leg(nm)), flush=True)
+        return
+    if args.only_filter:
+        print(json.dumps(filter_leg(nm)), flush=True)
         return
     if args.only_fv:
         print(json.dumps(fv_leg(args.fv_steps, cpu=not args.no_cpu)), flush=True)
@@ -427,6 +459,8 @@ def main():
         out["adjoint"] = adjoint_leg(nm)
     if not args.no_fv:
         out["fv_sr_euler"] = fv_leg(args.fv_steps, cpu=not args.no_cpu)
+    if not args.no_filter:
+        out["filter"] = filter_leg(nm)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x)
     print(json.dumps(out), flush=True)

@@ -60,7 +60,11 @@ int nmx_synchronize(nmx_ctx *ctx);
  * amr_mesh_xc, amr_Shell_rin, amr_Shell_rout, amr_hrefine_sphere_radius,
  * amr_hrefine_sphere_levels, dt (auto|number), dtfac, evolve_method
  * (RK4|sspRK3|Euler), scalarwave1_k, scalarwave1_numflux (upwind|LLF),
- * scalarwave1_profile (sin), plus library knobs prefixed nmx_ (nmx_path:
+ * scalarwave1_profile (sin), the FILTER bin's evolve_filter_all (yes|no),
+ * evolve_filter_varlist, evolve_filter_alp, evolve_filter_s, evolve_filter_dn
+ * (nmesh_evolve.c:35-39), basis_expfilter_JacobianPower (1|0|-1) and
+ * basis_filter_fv (yes|no) (nmesh_basis.c:22-24), plus library knobs
+ * prefixed nmx_ (nmx_path:
  * plain|batched|perelement, nmx_graph: yes|no, nmx_graph_steps, nmx_geometry:
  * stored|analytic, nmx_block: full|half|quarter, nmx_xcd: yes|no,
  * nmx_autotune: yes|no). */
@@ -116,6 +120,12 @@ int nmx_setrhs(nmx_ctx *ctx, int reg_out, int reg_in, double t);
 /* nsteps full time steps with evolve_method (evolve_myln, evolve.c:25-176;
  * RungeKutta.c:19-188).  Advances NMX_REG_U and the mesh time. */
 int nmx_evolve(nmx_ctx *ctx, int nsteps);
+/* The FILTER bin's exponential filter, in place on register reg, with the
+ * evolve_filter_* and basis_expfilter_* parameters (evolve_filter_evosys_mesh,
+ * evolve.c:656-710; expfilter_var, filter.c:75-92).  nmx_evolve applies it
+ * after every full step when evolve_filter_all or evolve_filter_varlist asks
+ * for it (main.c:628-630).  No filter requested: does nothing. */
+int nmx_filter(nmx_ctx *ctx, int reg);
 
 /* device pointer and length (doubles) of a register, for zero-copy use from
  * frameworks; valid until the mesh is rebuilt */

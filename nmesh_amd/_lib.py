@@ -49,6 +49,7 @@ def _proto(lib):
         "nmx_setrhs": (i, [vp, i, i, d]),
         "nmx_autotune": (i, [vp]),
         "nmx_evolve": (i, [vp, i]),
+        "nmx_filter": (i, [vp, i]),
         "nmx_state_device_ptr": (i, [vp, i, P(vp), P(l)]),
         "nmx_adjoint_setrhs": (i, [vp, i, i]),
         "nmx_adjoint_evolve": (i, [vp, i]),

@@ -68,8 +68,21 @@ struct FaceArgs {
   double *aj = nullptr;
 };
 
+// The exponential filter, in place, over a list of elements of one order.
+struct FilterArgs {
+  double *u = nullptr;            // register filtered in place
+  const SlotRec *rec = nullptr;
+  int nlist = 0;
+  const double *F = nullptr;      // F[i*np + l]: the filter along one direction
+  const double *xb = nullptr;     // LGL nodes of this order
+  const double *dXdx = nullptr;   // element blocks of 9*np (stored geometry)
+  int varmask = 0;                // bit v: pi, cx, cy, cz, phi
+  int jpow = 0;                   // filter u J^jpow, J = 1/det(dXb/dx)
+};
+
 // div: nodes per thread (1, 2 or 4)
 hipError_t launch_dg_amr(int np, int geom, const AmrArgs &a, int grid, int div, hipStream_t s);
 hipError_t launch_face_interp(const FaceArgs &a, hipStream_t s);
+hipError_t launch_filter(int np, int geom, const FilterArgs &a, hipStream_t s);
 
 }  // namespace nmx

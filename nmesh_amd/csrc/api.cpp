@@ -65,6 +65,14 @@ struct nmx_ctx {
   // batched (order-bucketed) mirror
   bool amr = false;
   int *djobs1 = nullptr, *dmdesc = nullptr;
+  // exponential filter: matrices per order, the plain path's element list,
+  // the variables of its passes (evolve_filter_all, then the varlist)
+  double *dFilt = nullptr;      // order n at (n-2)*kMaxN*kMaxN: F[j*n + l]
+  SlotRec *dfrec = nullptr;
+  std::string filt_key;
+  int filt_mask[2] = {0, 0};
+  int filt_jpow = 0;
+  bool filt_on = false;
   double *dBtab = nullptr, *dDall = nullptr, *dxball = nullptr, *daj = nullptr;
   // div: nodes per thread of this bucket's tile (0: the nmx_block parameter)
   // f1first/f1count: the face interpolation jobs of its elements in djobs1
@@ -119,7 +127,7 @@ extern "C" const char *nmx_last_error(void) { return g_err.c_str(); }
 
 static void set_defaults(nmx_ctx *c) {
   // defaults of the reference parameters (nmesh_amr.c:31-121, nmesh_main.c:29-35,
-  // nmesh_evolve.c:26, nmesh_scalarwave1.c:31-34)
+  // nmesh_evolve.c:26,35-39, nmesh_scalarwave1.c:31-34, nmesh_basis.c:22-24)
   c->par = {
       {"amr_mesh_type", "BoxMesh"},
       {"amr_n0", "5"},
@@ -145,6 +153,13 @@ static void set_defaults(nmx_ctx *c) {
       {"scalarwave1_k", "0.8 -0.5 0.33166247903554"},
       {"scalarwave1_numflux", "upwind"},
       {"scalarwave1_profile", "sin"},
+      {"evolve_filter_all", "no"},
+      {"evolve_filter_varlist", ""},
+      {"evolve_filter_alp", "36"},
+      {"evolve_filter_s", "32"},
+      {"evolve_filter_dn", "0"},
+      {"basis_expfilter_JacobianPower", "1"},
+      {"basis_filter_fv", "yes"},
       {"nmx_path", "plain"},
       {"nmx_graph", "yes"},
       {"nmx_graph_steps", "8"},
@@ -196,7 +211,7 @@ static void free_mesh_dev(nmx_ctx *c) {
     if (*b) { (void)hipFree(*b); *b = nullptr; }
   if (c->dedom) { (void)hipFree(c->dedom); c->dedom = nullptr; }
   {
-    void **ptrs[] = {(void **)&c->djobs1, (void **)&c->dmdesc,
+    void **ptrs[] = {(void **)&c->djobs1, (void **)&c->dmdesc, (void **)&c->dFilt, (void **)&c->dfrec,
                      (void **)&c->dBtab, (void **)&c->dDall, (void **)&c->dxball, (void **)&c->daj};
     for (auto q : ptrs)
       if (*q) { (void)hipFree(*q); *q = nullptr; }
@@ -205,6 +220,7 @@ static void free_mesh_dev(nmx_ctx *c) {
     c->buckets.clear();
     c->border.clear();
     c->blane.clear();
+    c->filt_key.clear();
   }
   if (c->dface) { (void)hipFree(c->dface); c->dface = nullptr; }
   if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
@@ -276,6 +292,10 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
   if ((k == "fv_flux" && v != "fnum") || (k == "fv_divf_extrap" && v != "dnfn_extrap1") ||
       (k == "fv_divf_use_only_right_flux" && v != "yes") || (k == "fv_WENOm3_opt_weightratio" && v != "2"))
     return fail(NMX_EUNSUP, k + ": only nmesh's default is built");
+  if ((k == "evolve_filter_all" || k == "basis_filter_fv") && v != "yes" && v != "no")
+    return fail(NMX_EINVAL, k + " must be yes or no");
+  if (k == "basis_expfilter_JacobianPower" && v != "1" && v != "0" && v != "-1")
+    return fail(NMX_EINVAL, "Jpower must be 1, -1 or 0");
   if (k == "nmx_block" && v != "full" && v != "half" && v != "quarter")
     return fail(NMX_EINVAL, "nmx_block must be full, half or quarter");
   c->par[k] = v;
@@ -317,6 +337,18 @@ static int dev_upload(T **p, const std::vector<T> &h) {
   if (r) return r;
   HIPCHK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
   return NMX_OK;
+}
+
+// An element's state offset and geometry in its SlotRec (faces left empty).
+static void slot_geometry(const Mesh &M, int e, SlotRec &R) {
+  const Elm &E = M.e[e];
+  std::memset(&R, 0, sizeof R);
+  R.noff = E.off;
+  for (int d = 0; d < 3; ++d) R.dXbdX[d] = 2. / (E.bbox[2 * d + 1] - E.bbox[2 * d]);
+  for (int q = 0; q < 6; ++q) R.bb[q] = E.bbox[q];
+  R.s0 = M.pat[E.p].s0;
+  R.s1 = M.pat[E.p].s1;
+  R.dom = M.pat[E.p].dom;
 }
 
 // Launch order and lanes of the concurrent buckets, largest first.  Running
@@ -481,13 +513,7 @@ static int upload_amr_tables(nmx_ctx *c) {
       const int e = list[it];
       const Elm &E = M.e[e];
       SlotRec &R = recs[it];
-      std::memset(&R, 0, sizeof R);
-      R.noff = E.off;
-      for (int d = 0; d < 3; ++d) R.dXbdX[d] = 2. / (E.bbox[2 * d + 1] - E.bbox[2 * d]);
-      for (int q = 0; q < 6; ++q) R.bb[q] = E.bbox[q];
-      R.s0 = M.pat[E.p].s0;
-      R.s1 = M.pat[E.p].s1;
-      R.dom = M.pat[E.p].dom;
+      slot_geometry(M, e, R);
       for (int f = 0; f < 6; ++f) {
         const int *d = &fd[48 * e + 8 * f];
         R.fkind[f] = d[0];
@@ -608,6 +634,12 @@ static int upload_mesh(nmx_ctx *c) {
     if ((r = dev_upload(&c->dwc, wc))) return r;
   }
   if (c->amr) return upload_amr_tables(c);
+  {
+    // every element, for the filter
+    std::vector<SlotRec> fr(ne);
+    for (size_t i = 0; i < ne; ++i) slot_geometry(M, (int)i, fr[i]);
+    if ((r = dev_upload(&c->dfrec, fr))) return r;
+  }
   // face gather descriptors {nb, local base, s1, s2}
   std::vector<int4> hf(6 * ne);
   for (size_t i = 0; i < ne; ++i)
@@ -1046,6 +1078,82 @@ static int method_id(nmx_ctx *c) {
   return m == "RK4" ? 0 : m == "sspRK3" ? 1 : 2;
 }
 
+// The FILTER bin's exponential filter (evolve_filter_evosys_mesh,
+// evolve.c:656-710): evolve_filter_all filters every evolved variable, then
+// evolve_filter_varlist the ones it names (first components; a variable in
+// both is filtered twice, as in the reference).  Names of variables this
+// build does not hold are skipped, as MeshVarIndLax does for unknown names.
+// Builds the per-order matrices when the parameters change; on = false when
+// nothing is filtered.
+static int filter_prepare(nmx_ctx *c, bool &on) {
+  on = false;
+  c->filt_mask[0] = P(c, "evolve_filter_all") == "yes" ? 31 : 0;
+  c->filt_mask[1] = 0;
+  std::istringstream vl(P(c, "evolve_filter_varlist"));
+  for (std::string name; vl >> name;) {
+    if (name == "scalarwave1_pi") c->filt_mask[1] |= 1;
+    else if (name == "scalarwave1_cx") c->filt_mask[1] |= 2 | 4 | 8;
+    else if (name == "scalarwave1_phi") c->filt_mask[1] |= 16;
+  }
+  if (!c->filt_mask[0] && !c->filt_mask[1]) return NMX_OK;
+  if (c->fvelm) {
+    if (P(c, "basis_filter_fv") == "yes")
+      return fail(NMX_EUNSUP, "the filter on FV elements is not built (set basis_filter_fv = no)");
+    return NMX_OK;   // FV elements are left unfiltered (filter.c:79-80)
+  }
+  c->filt_jpow = Pi(c, "basis_expfilter_JacobianPower");
+  const double alp = Pd(c, "evolve_filter_alp"), sf = Pd(c, "evolve_filter_s");
+  const int dn = Pi(c, "evolve_filter_dn");
+  char key[96];
+  std::snprintf(key, sizeof key, "%.17g %.17g %d", alp, sf, dn);
+  if (c->filt_key != key) {
+    constexpr int kF = kMaxN * kMaxN;
+    std::vector<double> F(9 * kF, 0.0);
+    for (int n = 2; n <= 10; ++n) filter_matrix(n, alp, sf, dn, F.data() + (n - 2) * kF);
+    if (c->dFilt) { (void)hipFree(c->dFilt); c->dFilt = nullptr; }
+    int r = dev_upload(&c->dFilt, F);
+    if (r) return r;
+    c->filt_key = key;
+  }
+  on = true;
+  return NMX_OK;
+}
+
+// the filter passes on register reg (after filter_prepare)
+static int launch_filters(nmx_ctx *c, double *reg) {
+  constexpr int kF = kMaxN * kMaxN;
+  for (int p = 0; p < 2; ++p) {
+    if (!c->filt_mask[p]) continue;
+    FilterArgs fa;
+    fa.u = reg; fa.dXdx = c->dXdx; fa.varmask = c->filt_mask[p]; fa.jpow = c->filt_jpow;
+    hipError_t err = hipSuccess;
+    if (c->amr) {
+      for (const auto &b : c->buckets) {
+        fa.rec = b.drec; fa.nlist = b.count;
+        fa.F = c->dFilt + (b.n - 2) * kF; fa.xb = c->dxball + b.xoff;
+        err = launch_filter(b.n, c->geom, fa, c->stream);
+        if (err != hipSuccess) break;
+      }
+    } else {
+      fa.rec = c->dfrec; fa.nlist = (int)c->mesh.e.size();
+      fa.F = c->dFilt + (c->np1 - 2) * kF; fa.xb = c->dxb;
+      err = launch_filter(c->np1, c->geom, fa, c->stream);
+    }
+    if (err != hipSuccess) return fail(NMX_EHIP, std::string("filter launch: ") + hipGetErrorString(err));
+  }
+  return NMX_OK;
+}
+
+extern "C" int nmx_filter(nmx_ctx *c, int reg) {
+  NEED_DEV(c);
+  double *u = reg_ptr(c, reg);
+  if (!u) return fail(NMX_EINVAL, "bad register");
+  bool on = false;
+  int r = filter_prepare(c, on);
+  if (r || !on) return r;
+  return launch_filters(c, u);
+}
+
 // issue the stage launches of one step; stage times at dT[tbase + s]
 static int issue_step(nmx_ctx *c, int method, double dt, int tbase, bool swap_parity) {
   double *U = swap_parity ? c->dA : c->dU;
@@ -1074,6 +1182,8 @@ static int issue_step(nmx_ctx *c, int method, double dt, int tbase, bool swap_pa
     a.win = U; a.U = U; a.Aout = A; a.cA = dt; a.wmode = 0; a.tidx = tbase;
     if ((r = launch(c, a))) return r;
   }
+  // FILTER bin after the full step (main.c:628-630)
+  if (c->filt_on && (r = launch_filters(c, A))) return r;
   return NMX_OK;
 }
 
@@ -1166,6 +1276,12 @@ extern "C" int nmx_evolve(nmx_ctx *c, int nsteps) {
   NEED_DEV(c);
   if (nsteps < 0) return fail(NMX_EINVAL, "nsteps < 0");
   HIPCHK(hipSetDevice(c->device));
+  {
+    bool on = false;
+    int r = filter_prepare(c, on);
+    if (r) return r;
+    c->filt_on = on;
+  }
   if (!c->tuned && P(c, "nmx_autotune") == "yes" && nsteps > 0) {
     int r = nmx_autotune(c);
     if (r) return r;
@@ -1267,6 +1383,9 @@ static AdjArgs adj_base_args(nmx_ctx *c) {
     NEED_DEV(c);                                                                         \
     if ((c)->amr || !(c)->dnbf || (c)->fvelm)                                            \
       return fail(NMX_EUNSUP, "the adjoint needs a conforming DG mesh of one element order"); \
+    bool filt_ = false;                                                                  \
+    if (int r_ = filter_prepare((c), filt_)) return r_;                                  \
+    if (filt_) return fail(NMX_EUNSUP, "the adjoint of the exponential filter is not built"); \
   } while (0)
 
 static int launch_adj(nmx_ctx *c, const AdjArgs &a) {

@@ -119,6 +119,44 @@ const Basis &lgl_basis(int n) {
   return cache[n];
 }
 
+// The exponential filter along one direction on n LGL nodes, F = S diag(f)
+// A: the normalized Legendre analysis A_il = w_l P_i(x_l)/sqrt(c_i) and
+// synthesis S_ji = P_i(x_j)/sqrt(c_i), c_i = 2/(2i+1) except c_N = 2/N
+// (LGL_AT_ST_matrices, src/main/basis/Legendre.c:257-307, with P_i by the
+// recurrence of Legendre.c:14-38), around f(i) = exp(-alp (i/N)^s),
+// N = n-1-dn, 1 if not positive (expfilter_coeff_array, filter.c:20-55).
+// F[j*n + l].
+void filter_matrix(int n, double alp, double s, int dn, double *F) {
+  const Basis &b = lgl_basis(n);
+  const int N = n - 1;
+  std::vector<double> A(n * n), S(n * n), f(n);
+  double Nf = n - 1 - dn;
+  if (Nf <= 0.) Nf = 1.;
+  for (int i = 0; i < n; ++i) {
+    f[i] = std::exp(-alp * std::pow(i / Nf, s));
+    double ooci = 0.5 * (2 * i + 1);
+    if (i == N && N != 0) ooci = 0.5 * i;
+    const double oosci = std::sqrt(ooci);
+    for (int j = 0; j < n; ++j) {
+      const double x = b.x[j];
+      double P0 = 1.0, P1 = x, Pk = (i == 0) ? 1.0 : x;
+      for (int k = 2; k <= i; ++k) {
+        Pk = (x * (2 * k - 1) * P1 - (k - 1) * P0) / k;
+        P0 = P1;
+        P1 = Pk;
+      }
+      S[j * n + i] = Pk * oosci;
+      A[i * n + j] = b.w[j] * S[j * n + i];
+    }
+  }
+  for (int j = 0; j < n; ++j)
+    for (int l = 0; l < n; ++l) {
+      double sum = 0.0;
+      for (int i = 0; i < n; ++i) sum += S[j * n + i] * f[i] * A[i * n + l];
+      F[j * n + l] = sum;
+    }
+}
+
 static double lagrange(const Basis &b, int q, double x) {
   double prod = 1.0;
   for (int m = 0; m < b.n; ++m)

@@ -49,6 +49,7 @@ struct Basis {
   double D[kMaxN * kMaxN];                // D[i*n + l] = l_l'(x_i)
 };
 const Basis &lgl_basis(int n);
+void filter_matrix(int n, double alp, double s, int dn, double *F);
 // uniform points incl. both faces (x only)
 const Basis &uniform_basis(int n);
 

@@ -236,6 +236,14 @@ def autotune(mesh):
     check(lib().nmx_autotune(mesh.ctx))
 
 
+def evolve_filter_evosys_mesh(mesh, reg=REG_U):
+    """The FILTER bin's exponential filter of the evolved variables, in place
+    (evolve_filter_evosys_mesh, evolve.c:656-710); nmx_evolve applies it after
+    every step when evolve_filter_all / evolve_filter_varlist ask for it."""
+    check(lib().nmx_filter(mesh.ctx, int(reg)))
+    return 0
+
+
 def evolve_myln(mesh, nsteps=1):
     """Step(s) with the method named by evolve_method (evolve.c:25-176)."""
     check(lib().nmx_evolve(mesh.ctx, int(nsteps)))

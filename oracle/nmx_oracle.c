@@ -1593,3 +1593,105 @@ int orc_num_threads(void)
   return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------ */
+/* exponential filter (FILTER bin, run after each full step)           */
+/* ------------------------------------------------------------------ */
+
+/* P_l(x) by the recurrence P_l = (x (2l-1) P_{l-1} - (l-1) P_{l-2}) / l
+   (src/main/basis/Legendre.c:14-38) */
+static double legendre_P(int l, double x)
+{
+  double P0 = 1.0, P1 = x, Pk = 0.0;
+  if (l == 0) return P0;
+  if (l == 1) return P1;
+  for (int k = 2; k <= l; k++) {
+    Pk = (x * (2 * k - 1) * P1 - (k - 1) * P0) / k;
+    P0 = P1;
+    P1 = Pk;
+  }
+  return Pk;
+}
+
+/* normalized Legendre analysis and synthesis on LGL nodes, stored
+   transposed: coefficient i = sum_j AT[j + i n] u_j, value j = sum_i
+   ST[i + j n] c_i, with c_i = 2/(2i+1) except c_N = 2/N, the discrete norm
+   of the top mode (LGL_AT_ST_matrices, src/main/basis/Legendre.c:257-307) */
+static void lgl_at_st(int n, const double *x, const double *w, double *AT, double *ST)
+{
+  int N = n - 1;
+  for (int i = 0; i < n; i++) {
+    double ooci = 0.5 * (2 * i + 1);
+    if (i == N && N != 0) ooci = 0.5 * i;
+    double oosci = sqrt(ooci);
+    for (int j = 0; j < n; j++) {
+      ST[i + j * n] = legendre_P(i, x[j]) * oosci;
+      AT[j + i * n] = w[j] * ST[i + j * n];
+    }
+  }
+}
+
+/* out_i = sum_l MT[l + n i] in_l along direction dir (mm_array_indir,
+   src/main/amr/array.c:100-300) */
+static void contract1(int n, const double *MT, int dir, const double *in, double *out)
+{
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        int ii = (dir == 0) ? i : (dir == 1) ? j : k;
+        double sum = 0.0;
+        for (int l = 0; l < n; l++) {
+          int src = (dir == 0) ? l + n * (j + n * k) : (dir == 1) ? i + n * (l + n * k)
+                                                                  : i + n * (j + n * l);
+          sum += MT[l + n * ii] * in[src];
+        }
+        out[i + n * (j + n * k)] = sum;
+      }
+}
+
+/* Filter the variables in varmask (bit v: pi, cx, cy, cz, phi) of every
+   element: u -> u J^jpow, coefficients c = A_z A_y A_x u, c_ijk *= f(i) f(j)
+   f(k) with f(i) = exp(-alp (i/N)^s), N = n-1-dn (1 if not positive), then
+   u = S_z S_y S_x c and u -> u J^-jpow; J = 1/det(dXb/dx)
+   (expfilter_coeff_array, expfilter_array, expfilter_var,
+   src/main/basis/filter.c:20-92; var_to_var_times_JtoPower,
+   src/main/coordinates/utilities.c:885-909; basis_array_analysis3 and
+   synthesis3, src/main/basis/basis.c:183-196). */
+void orc_filter(const OMesh *M, double *u, int varmask, double alp, double s, int dn, int jpow)
+{
+  long nn = M->nnodes;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int e = 0; e < M->nelem; e++) {
+    const OElm *E = &M->e[e];
+    int n = E->n, np = elm_np(E);
+    OBasis *b = basis(n);
+    double AT[ORC_MAXN * ORC_MAXN], ST[ORC_MAXN * ORC_MAXN], f[ORC_MAXN];
+    lgl_at_st(n, b->Xb, b->Wq, AT, ST);
+    double N = n - 1 - dn;
+    if (N <= 0.) N = 1.;
+    for (int i = 0; i < n; i++) f[i] = exp(-alp * pow(i / N, s));
+    double *c = malloc(sizeof(double) * np), *t = malloc(sizeof(double) * np);
+    const double *ooJ = M->det + E->off;
+    for (int v = 0; v < 5; v++) {
+      if (!(varmask >> v & 1)) continue;
+      double *uv = u + v * nn + E->off;
+      if (jpow == 1) for (int q = 0; q < np; q++) uv[q] = uv[q] / ooJ[q];
+      else if (jpow == -1) for (int q = 0; q < np; q++) uv[q] = uv[q] * ooJ[q];
+      contract1(n, AT, 0, uv, c);
+      contract1(n, AT, 1, c, t);
+      contract1(n, AT, 2, t, c);
+      for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++)
+          for (int i = 0; i < n; i++) {
+            int q = i + n * (j + n * k);
+            c[q] = c[q] * f[i] * f[j] * f[k];
+          }
+      contract1(n, ST, 0, c, t);
+      contract1(n, ST, 1, t, c);
+      contract1(n, ST, 2, c, uv);
+      if (jpow == 1) for (int q = 0; q < np; q++) uv[q] = uv[q] * ooJ[q];
+      else if (jpow == -1) for (int q = 0; q < np; q++) uv[q] = uv[q] / ooJ[q];
+    }
+    free(c); free(t);
+  }
+}

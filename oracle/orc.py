@@ -48,6 +48,7 @@ def lib():
             "orc_set_profile": (None, [vp, vp, d, vp, vp]),
             "orc_rhs": (None, [vp, i, vp, vp, d, vp]),
             "orc_evolve": (d, [vp, i, vp, i, d, i, d, vp]),
+            "orc_filter": (None, [vp, vp, i, d, d, i, i]),
             "orc_num_threads": (i, []),
             "orc_mesh_set_fv": (None, [vp, i]),
             "orc_set_fv_rec": (None, [i]),
@@ -175,6 +176,13 @@ class OMesh:
         t = lib().orc_evolve(self.h, NUMFLUX[numflux], _p(k), METHOD[method], float(dt),
                              int(nsteps), float(t0), _p(u))
         return u, t
+
+    def filter(self, u, varmask=31, alp=36.0, s=32.0, dn=0, jpow=1):
+        """exponential filter of the variables in varmask (bit v of pi, cx,
+        cy, cz, phi) in every element (filter.c:20-92)"""
+        u = np.array(u, dtype=np.float64, copy=True)
+        lib().orc_filter(self.h, _p(u), int(varmask), float(alp), float(s), int(dn), int(jpow))
+        return u
 
 
 def set_fv_rec(mode):
