@@ -216,6 +216,42 @@ def c1_params():
             "evolve_method": "RK4", "dtfac": 0.25, "dt": "auto"}
 
 
+def fvelm_leg(nm, steps=20):
+    """scalarwave1 on FV elements inside the mesh (fv_divf): a BoxMesh
+    [-8,8]^3 of 16^3 elements, all FV, n = 8 uniform points per direction
+    (2,097,152 cells), WENOm5_2, upwind fnum, RK4.  Bytes per cell-update as
+    the DG stage on Cartesian elements: 160 B (stage average of the 15-25
+    state doubles read and written)."""
+    p = {"amr_mesh_type": "BoxMesh", "amr_BoxMesh_dout": 8.0, "amr_luni": 4,
+         "amr_n0": 8, "amr_n1": 8, "amr_n2": 8, "amr_uniform_p": "0", "amr_fv_p": "0",
+         "fv_rec": "WENOm5_2", "scalarwave1_k": "0 0 0", "evolve_method": "RK4"}
+    m = nm.Mesh(p, device=0)
+    nm.amr_setup_mesh(m)
+    x = m.coords()
+    r2 = (x ** 2).sum(0)
+    u = np.zeros((5, x.shape[1]))
+    u[4] = np.exp(-r2)
+    u[1:4] = -2.0 * x * u[4]
+    m.upload(u.ravel())
+    nm.evolve_myln(m, 2)
+    m.synchronize()
+    t0 = time.perf_counter()
+    nm.evolve_myln(m, steps)
+    m.synchronize()
+    wall = time.perf_counter() - t0
+    dev_s = m.last_evolve_ms() * 1e-3
+    nu = 4.0 * steps * m.nnodes
+    gbs = MIN_BYTES_PER_NU * nu / dev_s / 1e9
+    out = {"workload": "scalarwave1 on FV elements (fv_divf, WENOm5_2, upwind), BoxMesh 16^3 "
+                       "elements of 8^3 uniform points (%d cells), RK4" % m.nnodes,
+           "value": nu / wall, "unit": "cell-updates/s", "steps": steps,
+           "ms_per_step": 1e3 * wall / steps, "bytes_per_cell_update": MIN_BYTES_PER_NU,
+           "achieved_GBps": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
+           "kernel": "fvelm_stage_kernel<8,5,*>"}
+    del m
+    return out
+
+
 def filter_leg(nm, reps=100):
     """The FILTER bin's exponential filter (alp 36, s 32, dn 0, Jacobian
     power 1) of all 5 evolved fields on the 3072-element N=7 shell, in place,
@@ -335,6 +371,8 @@ def main():
     ap.add_argument("--fv-steps", type=int, default=50, help="sspRK3 steps of the FV leg")
     ap.add_argument("--only-adjoint", action="store_true", help="run only the adjoint-cost leg")
     ap.add_argument("--only-filter", action="store_true", help="run only the exponential-filter leg")
+    ap.add_argument("--only-fvelm", action="store_true", help="run only the FV-element leg")
+    ap.add_argument("--no-fvelm", action="store_true", help="skip the FV-element leg")
     ap.add_argument("--no-filter", action="store_true", help="skip the exponential-filter leg")
     ap.add_argument("--path", default="plain")
     ap.add_argument("--graph", default="yes", help="capture RK steps in hipGraphs (yes|no)")
@@ -378,6 +416,9 @@ def main():
         return
     if args.only_filter:
         print(json.dumps(filter_leg(nm)), flush=True)
+        return
+    if args.only_fvelm:
+        print(json.dumps(fvelm_leg(nm)), flush=True)
         return
     if args.only_fv:
         print(json.dumps(fv_leg(args.fv_steps, cpu=not args.no_cpu)), flush=True)
@@ -461,6 +502,8 @@ def main():
         out["fv_sr_euler"] = fv_leg(args.fv_steps, cpu=not args.no_cpu)
     if not args.no_filter:
         out["filter"] = filter_leg(nm)
+    if not args.no_fvelm:
+        out["fv_elements"] = fvelm_leg(nm)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x)
     print(json.dumps(out), flush=True)

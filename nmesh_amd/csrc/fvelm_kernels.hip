@@ -28,23 +28,27 @@ namespace nmx {
 constexpr double kWeno3Eps = 1e-6;    // WENO3_epsilon (rec1d.c:35), u_scale = 1
 constexpr double kWeno5EpsE = 1e-40;  // WENO5_epsilon (rec1d.c:1193)
 
+// The reconstructions below normalise their nonlinear weights with one
+// division: a weight d_k / b_k^p over their sum equals (d_k prod_{j!=k}
+// b_j^p) over the sum of those products.  Only rounding differs from the
+// reference's per-weight divisions.
+
 // WENOm3 (rec1d_p_WENO3 with the uniform WENOm3 weights)
 __device__ __forceinline__ double wm3_p(double um1, double u0, double up1) {
   const double d1 = u0 - um1, d2 = up1 - u0;
   const double b1 = d1 * d1 + kWeno3Eps, b2 = d2 * d2 + kWeno3Eps;
-  const double o1 = 1. / (b1 * b1), o2 = 2. / (b2 * b2);
-  const double s = o1 + o2;
-  return (o1 / s) * (-0.5 * um1 + 1.5 * u0) + (o2 / s) * (0.5 * u0 + 0.5 * up1);
+  const double p1 = b2 * b2, p2 = 2. * (b1 * b1);   // weights 1/b1^2, 2/b2^2
+  return (p1 * (-0.5 * um1 + 1.5 * u0) + p2 * (0.5 * u0 + 0.5 * up1)) / (p1 + p2);
 }
 __device__ __forceinline__ double wm3_m(double u0, double up1, double up2) {
   const double d1 = up1 - u0, d2 = up2 - up1;
   const double b1 = d1 * d1 + kWeno3Eps, b2 = d2 * d2 + kWeno3Eps;
-  const double o1 = 2. / (b1 * b1), o2 = 1. / (b2 * b2);
-  const double s = o1 + o2;
-  return (o1 / s) * (0.5 * u0 + 0.5 * up1) + (o2 / s) * (1.5 * up1 - 0.5 * up2);
+  const double p1 = 2. * (b2 * b2), p2 = b1 * b1;   // weights 2/b1^2, 1/b2^2
+  return (p1 * (0.5 * u0 + 0.5 * up1) + p2 * (1.5 * up1 - 0.5 * up2)) / (p1 + p2);
 }
 
-// WENO5 (rec1d_p/m_WENO5 with uniform weights), the reference's arithmetic
+// WENO5 (rec1d_p/m_WENO5 with uniform weights): ideal weights (1, 6, 3)
+// over the squared smoothness indicators plus 1e-40
 __device__ __forceinline__ double w5_p(double a, double b, double c, double d, double e) {
   const double s01 = a - 2. * b + c, s02 = a - 4. * b + 3. * c;
   const double s11 = b - 2. * c + d, s12 = b - d;
@@ -52,13 +56,13 @@ __device__ __forceinline__ double w5_p(double a, double b, double c, double d, d
   const double e0 = (13. / 12.) * s01 * s01 + 0.25 * s02 * s02 + kWeno5EpsE;
   const double e1 = (13. / 12.) * s11 * s11 + 0.25 * s12 * s12 + kWeno5EpsE;
   const double e2 = (13. / 12.) * s21 * s21 + 0.25 * s22 * s22 + kWeno5EpsE;
-  const double w0 = 1. / (e0 * e0), w1 = 6. / (e1 * e1), w2 = 3. / (e2 * e2);
-  const double inv = 1. / (w0 + w1 + w2);
+  const double E0 = e0 * e0, E1 = e1 * e1, E2 = e2 * e2;
+  const double p0 = E1 * E2, p1 = 6. * (E0 * E2), p2 = 3. * (E0 * E1);
   const double o6 = 1. / 6.;
   const double q0 = (2. * o6) * a + (-7. * o6) * b + (11. * o6) * c;
   const double q1 = (-o6) * b + (5. * o6) * c + (2. * o6) * d;
   const double q2 = (2. * o6) * c + (5. * o6) * d + (-o6) * e;
-  return (w0 * inv) * q0 + (w1 * inv) * q1 + (w2 * inv) * q2;
+  return (p0 * q0 + p1 * q1 + p2 * q2) / (p0 + p1 + p2);
 }
 __device__ __forceinline__ double w5_m(double a, double b, double c, double d, double e) {
   const double s21 = c - 2. * d + e, s22 = 3. * c - 4. * d + e;
@@ -67,49 +71,49 @@ __device__ __forceinline__ double w5_m(double a, double b, double c, double d, d
   const double e2 = (13. / 12.) * s21 * s21 + 0.25 * s22 * s22 + kWeno5EpsE;
   const double e1 = (13. / 12.) * s11 * s11 + 0.25 * s12 * s12 + kWeno5EpsE;
   const double e0 = (13. / 12.) * s01 * s01 + 0.25 * s02 * s02 + kWeno5EpsE;
-  const double w2 = 1. / (e2 * e2), w1 = 6. / (e1 * e1), w0 = 3. / (e0 * e0);
-  const double inv = 1. / (w0 + w1 + w2);
+  const double E0 = e0 * e0, E1 = e1 * e1, E2 = e2 * e2;
+  const double p2 = E0 * E1, p1 = 6. * (E0 * E2), p0 = 3. * (E1 * E2);
   const double o6 = 1. / 6.;
   const double q2 = (11. * o6) * c + (-7. * o6) * d + (2. * o6) * e;
   const double q1 = (2. * o6) * b + (5. * o6) * c + (-o6) * d;
   const double q0 = (-o6) * a + (5. * o6) * b + (2. * o6) * c;
-  return (w0 * inv) * q0 + (w1 * inv) * q1 + (w2 * inv) * q2;
+  return (p0 * q0 + p1 * q1 + p2 * q2) / (p0 + p1 + p2);
 }
 
-// WENO-Z (rec1d_p/m_WENOZ with uniform weights, rec1d.c:1627-1781)
+// WENO-Z (rec1d_p/m_WENOZ with uniform weights, rec1d.c:1627-1781): weights
+// d_k (1 + tau5 / (beta_k + 1e-40)), tau5 = |beta0 - beta2|
 __device__ __forceinline__ double wz_p(double a, double b, double c, double d, double e) {
   const double s01 = a - 2. * b + c, s02 = a - 4. * b + 3. * c;
   const double s11 = b - 2. * c + d, s12 = b - d;
   const double s21 = c - 2. * d + e, s22 = 3. * c - 4. * d + e;
-  const double beta0 = (13. / 12.) * s01 * s01 + 0.25 * s02 * s02;
-  const double beta1 = (13. / 12.) * s11 * s11 + 0.25 * s12 * s12;
-  const double beta2 = (13. / 12.) * s21 * s21 + 0.25 * s22 * s22;
-  const double tau5 = fabs(beta0 - beta2);
-  const double w0 = 1. * (1. + tau5 / (beta0 + kWeno5EpsE)), w1 = 6. * (1. + tau5 / (beta1 + kWeno5EpsE)),
-               w2 = 3. * (1. + tau5 / (beta2 + kWeno5EpsE));
-  const double inv = 1. / (w0 + w1 + w2);
+  const double g0 = (13. / 12.) * s01 * s01 + 0.25 * s02 * s02 + kWeno5EpsE;
+  const double g1 = (13. / 12.) * s11 * s11 + 0.25 * s12 * s12 + kWeno5EpsE;
+  const double g2 = (13. / 12.) * s21 * s21 + 0.25 * s22 * s22 + kWeno5EpsE;
+  const double tau5 = fabs(g0 - g2);
+  // d_k (g_k + tau5) / g_k over the sum, times g0 g1 g2
+  const double p0 = (g0 + tau5) * (g1 * g2), p1 = 6. * ((g1 + tau5) * (g0 * g2)),
+               p2 = 3. * ((g2 + tau5) * (g0 * g1));
   const double o6 = 1. / 6.;
   const double q0 = (2. * o6) * a + (-7. * o6) * b + (11. * o6) * c;
   const double q1 = (-o6) * b + (5. * o6) * c + (2. * o6) * d;
   const double q2 = (2. * o6) * c + (5. * o6) * d + (-o6) * e;
-  return (w0 * inv) * q0 + (w1 * inv) * q1 + (w2 * inv) * q2;
+  return (p0 * q0 + p1 * q1 + p2 * q2) / (p0 + p1 + p2);
 }
 __device__ __forceinline__ double wz_m(double a, double b, double c, double d, double e) {
   const double s21 = c - 2. * d + e, s22 = 3. * c - 4. * d + e;
   const double s11 = b - 2. * c + d, s12 = b - d;
   const double s01 = a - 2. * b + c, s02 = a - 4. * b + 3. * c;
-  const double beta2 = (13. / 12.) * s21 * s21 + 0.25 * s22 * s22;
-  const double beta1 = (13. / 12.) * s11 * s11 + 0.25 * s12 * s12;
-  const double beta0 = (13. / 12.) * s01 * s01 + 0.25 * s02 * s02;
-  const double tau5 = fabs(beta0 - beta2);
-  const double w2 = 1. * (1. + tau5 / (beta2 + kWeno5EpsE)), w1 = 6. * (1. + tau5 / (beta1 + kWeno5EpsE)),
-               w0 = 3. * (1. + tau5 / (beta0 + kWeno5EpsE));
-  const double inv = 1. / (w0 + w1 + w2);
+  const double g2 = (13. / 12.) * s21 * s21 + 0.25 * s22 * s22 + kWeno5EpsE;
+  const double g1 = (13. / 12.) * s11 * s11 + 0.25 * s12 * s12 + kWeno5EpsE;
+  const double g0 = (13. / 12.) * s01 * s01 + 0.25 * s02 * s02 + kWeno5EpsE;
+  const double tau5 = fabs(g0 - g2);
+  const double p2 = (g2 + tau5) * (g0 * g1), p1 = 6. * ((g1 + tau5) * (g0 * g2)),
+               p0 = 3. * ((g0 + tau5) * (g1 * g2));
   const double o6 = 1. / 6.;
   const double q2 = (11. * o6) * c + (-7. * o6) * d + (2. * o6) * e;
   const double q1 = (2. * o6) * b + (5. * o6) * c + (-o6) * d;
   const double q0 = (-o6) * a + (5. * o6) * b + (2. * o6) * c;
-  return (w0 * inv) * q0 + (w1 * inv) * q1 + (w2 * inv) * q2;
+  return (p0 * q0 + p1 * q1 + p2 * q2) / (p0 + p1 + p2);
 }
 
 // both sides of midpoint im of a line q[0..NP-1]: WENOm5_2 (REC 5),
@@ -142,62 +146,127 @@ __device__ __forceinline__ void fnum_dir(int numflux, double sig, const double u
   fn[3] = (DIR == 2 ? -ui[0] * sig : 0.0) + F[3];
 }
 
-template <int NP, int REC, int DIR>
-__device__ __forceinline__ void fv_line(const StageArgs &a, const double *wc, int numflux, double g_dir,
-                                        double ooJ, int la, int lb, const double (*s_u)[NP * NP * NP],
-                                        const double (*s_f)[6][NP * NP], double (*s_df)[NP * NP * NP]) {
+// the neighbour's (or the exterior's) values at face f, point (fa, fb), of
+// element e: the DG kernel's phase-1 gather for one point
+template <int NP>
+__device__ __forceinline__ void face_value(const StageArgs &a, int e, int f, int fa, int fb,
+                                           double ua[4]) {
+  constexpr int NP3 = NP * NP * NP;
+  const int4 fd = a.face[6 * e + f];
+  if (fd.x >= 0) {
+    const double *nw = a.win + 5L * NP3 * fd.x + fd.y + fa * fd.z + fb * fd.w;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ua[v] = nw[v * NP3];
+  } else if (a.profile) {
+    const int dir = f >> 1, pl = (f & 1) * (NP - 1);
+    const int ii = (dir == 0) ? pl : fa;
+    const int jj = (dir == 1) ? pl : ((dir == 0) ? fa : fb);
+    const int kk = (dir == 2) ? pl : fb;
+    const double *xp = a.xyz + 3L * NP3 * e + ii + NP * (jj + NP * kk);
+    plane_wave(a.k, a.om, xp[0], xp[NP3], xp[2 * NP3], a.tdev[a.tidx], ua);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ua[v] = 0.0;
+  }
+}
+
+// One line of direction DIR at (la, lb) (along Dir1_norm, Dir2_norm of DIR)
+// of element e, streamed from HBM through a window of six points per
+// variable (points i0-2 .. i0+3 at midpoint i0) with the next point
+// requested one midpoint ahead.  Adds df * ooJ of the line to acc (FIRST:
+// writes it), with dnfn_extrap1 at both ends.
+template <int NP, int REC, int DIR, bool FIRST>
+__device__ __forceinline__ void fv_line(const StageArgs &a, int e, const double *wc, double g_dir,
+                                        double ooJ, int la, int lb, double (*acc)[NP * NP * NP]) {
   constexpr int NP2 = NP * NP;
+  constexpr int NP3 = NP * NP * NP;
   const int stride = DIR == 0 ? 1 : DIR == 1 ? NP : NP2;
-  // line base: la, lb run along Dir1_norm, Dir2_norm of DIR
   const int base = DIR == 0 ? (la * NP + lb * NP2) : DIR == 1 ? (la + lb * NP2) : (la + lb * NP);
-  const int JK = la + NP * lb;
-  double q[4][NP];
+  const double *g = a.win + 5L * NP3 * e + base;
+  double lo[4], hi[4];
+  face_value<NP>(a, e, 2 * DIR, la, lb, lo);
+  face_value<NP>(a, e, 2 * DIR + 1, la, lb, hi);
+  double q[4][6], nx[4];
 #pragma unroll
-  for (int v = 0; v < 4; ++v)
+  for (int v = 0; v < 4; ++v) {
 #pragma unroll
-    for (int i = 0; i < NP; ++i) q[v][i] = s_u[v][base + i * stride];
-  double fR[4], fL[4], df[4][NP];
+    for (int s = 0; s < 6; ++s) {
+      const int i = s - 2;
+      q[v][s] = (i >= 0 && i < NP) ? g[v * NP3 + i * stride] : 0.0;
+    }
+    nx[v] = (4 < NP) ? g[v * NP3 + 4 * stride] : 0.0;
+  }
+  auto put = [&](int i, int l, double d) {
+    if (FIRST) acc[l][base + i * stride] = d * ooJ;
+    else acc[l][base + i * stride] += d * ooJ;
+  };
+  double fR[4], fL[4], d0[4], d1[4], dm2[4], dm1[4];
+  {
+    double ui[4];
 #pragma unroll
+    for (int v = 0; v < 4; ++v) ui[v] = q[v][2];
+    fnum_dir<DIR>(a.numflux, -1.0, ui, lo, fL);
+  }
+#pragma unroll 1
   for (int i0 = 0; i0 < NP; ++i0) {
+    const int im = i0;
     const double JR = g_dir / (ooJ * wc[i0]), JL = JR;
     if (i0 > 0) {
 #pragma unroll
       for (int l = 0; l < 4; ++l) fL[l] = -fR[l];
-    } else {
-      double ui[4], ua[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) { ui[v] = q[v][0]; ua[v] = s_f[v][2 * DIR][JK]; }
-      fnum_dir<DIR>(numflux, -1.0, ui, ua, fL);
     }
     double ui[4], ua[4];
     if (i0 < NP - 1) {
+      // both sides of midpoint im (WENOm5_2, WENOmZ_2 or WENOm3_2): the
+      // window holds points im-2 .. im+3 at q[.][0..5]
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        double up, um;
-        rec_mid<NP, REC>(q[v], i0, up, um);
-        ui[v] = up;
-        ua[v] = um;
+        const double *w = q[v];
+        if (REC == 5 && im >= 2 && im <= NP - 3) ui[v] = w5_p(w[0], w[1], w[2], w[3], w[4]);
+        else if (REC == 7 && im >= 2 && im <= NP - 3) ui[v] = wz_p(w[0], w[1], w[2], w[3], w[4]);
+        else if (im >= 1) ui[v] = wm3_p(w[1], w[2], w[3]);
+        else ui[v] = 0.5 * w[2] + 0.5 * w[3];
+        if (REC == 5 && im >= 1 && im <= NP - 4) ua[v] = w5_m(w[1], w[2], w[3], w[4], w[5]);
+        else if (REC == 7 && im >= 1 && im <= NP - 4) ua[v] = wz_m(w[1], w[2], w[3], w[4], w[5]);
+        else if (im <= NP - 3) ua[v] = wm3_m(w[2], w[3], w[4]);
+        else ua[v] = 0.5 * w[2] + 0.5 * w[3];
       }
     } else {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) { ui[v] = q[v][NP - 1]; ua[v] = s_f[v][2 * DIR + 1][JK]; }
+      for (int v = 0; v < 4; ++v) { ui[v] = q[v][2]; ua[v] = hi[v]; }
     }
-    fnum_dir<DIR>(numflux, 1.0, ui, ua, fR);
+    fnum_dir<DIR>(a.numflux, 1.0, ui, ua, fR);
 #pragma unroll
-    for (int l = 0; l < 4; ++l) df[l][i0] = JR * fR[l] + JL * fL[l];
+    for (int l = 0; l < 4; ++l) {
+      const double d = JR * fR[l] + JL * fL[l];
+      if (NP > 2 && i0 == 0) d0[l] = d;
+      else if (NP > 2 && i0 == 1) d1[l] = d;
+      else if (NP > 2 && i0 == NP - 2) dm2[l] = d;
+      else if (NP > 2 && i0 == NP - 1) dm1[l] = d;
+      else put(i0, l, d);
+    }
+    // slide the window by one point; request the one after next
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+#pragma unroll
+      for (int s = 0; s < 5; ++s) q[v][s] = q[v][s + 1];
+      q[v][5] = nx[v];
+      nx[v] = (i0 + 5 < NP) ? g[v * NP3 + (i0 + 5) * stride] : 0.0;
+    }
   }
   if (NP > 2) {
+    // dnfn_extrap1 (rec1d.c:542-626): weights 4/3 and -1/3 at both ends
     const double w0 = 1. / 0.75, w1 = 1. - w0;
 #pragma unroll
     for (int l = 0; l < 4; ++l) {
-      df[l][0] = w0 * df[l][0] + w1 * df[l][1];
-      df[l][NP - 1] = w0 * df[l][NP - 1] + w1 * df[l][NP - 2];
+      put(0, l, w0 * d0[l] + w1 * d1[l]);
+      put(NP - 1, l, w0 * dm1[l] + w1 * dm2[l]);
+      if (NP > 3) {
+        put(1, l, d1[l]);
+        put(NP - 2, l, dm2[l]);
+      }
     }
   }
-#pragma unroll
-  for (int l = 0; l < 4; ++l)
-#pragma unroll
-    for (int i = 0; i < NP; ++i) s_df[l][base + i * stride] = df[l][i] * ooJ;
 }
 
 template <int NP, int REC, int NT>
@@ -205,9 +274,7 @@ __global__ __launch_bounds__(NT) void fvelm_stage_kernel(StageArgs a, const doub
   constexpr int NP2 = NP * NP;
   constexpr int NP3 = NP * NP * NP;
   constexpr int NC = (NP3 + NT - 1) / NT;
-  __shared__ double s_u[4][NP3];
-  __shared__ double s_f[4][6][NP2];
-  __shared__ double s_df[3][4][NP3];
+  __shared__ double acc[4][NP3];   // divf, summed over x, then y, then z
   __shared__ double s_wc[NP];
   const int tid = threadIdx.x;
   for (int q = tid; q < NP; q += NT) s_wc[q] = wc[q];
@@ -228,49 +295,17 @@ __global__ __launch_bounds__(NT) void fvelm_stage_kernel(StageArgs a, const doub
 
   for (int e = e_first; e < e_hi; e += e_step) {
     const long ebase = 5L * NP3 * e;
-    __syncthreads();
-    // ---- phase 1 (as in dg_stage_kernel) ----
-    for (int idx = tid; idx < 4 * NP3; idx += NT) {
-      const int v = idx / NP3, node = idx % NP3;
-      s_u[v][node] = a.win[ebase + node + v * NP3];
-    }
-    for (int idx = tid; idx < 6 * NP2; idx += NT) {
-      const int f = idx / NP2, q = idx % NP2;
-      const int fa = q % NP, fb = q / NP;
-      const int4 fd = a.face[6 * e + f];
-      if (fd.x >= 0) {
-        const double *nw = a.win + 5L * NP3 * fd.x + fd.y + fa * fd.z + fb * fd.w;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) s_f[v][f][q] = nw[v * NP3];
-      } else if (a.profile) {
-        const int dir = f >> 1, pl = (f & 1) * (NP - 1);
-        const int ii = (dir == 0) ? pl : fa;
-        const int jj = (dir == 1) ? pl : ((dir == 0) ? fa : fb);
-        const int kk = (dir == 2) ? pl : fb;
-        const int mynode = ii + NP * (jj + NP * kk);
-        const double *xp = a.xyz + 3L * NP3 * e + mynode;
-        double ua[4];
-        plane_wave(a.k, a.om, xp[0], xp[NP3], xp[2 * NP3], a.tdev[a.tidx], ua);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) s_f[v][f][q] = ua[v];
-      } else {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) s_f[v][f][q] = 0.0;
-      }
-    }
     const double g0 = a.dXbdX[3 * e], g1 = a.dXbdX[3 * e + 1], g2 = a.dXbdX[3 * e + 2];
     const double ooJ = g0 * g1 * g2;
+    const int la = tid % NP, lb = tid / NP;
+    __syncthreads();   // the previous element's epilogue is done with acc
+    if (tid < NP2) fv_line<NP, REC, 0, true>(a, e, s_wc, g0, ooJ, la, lb, acc);
     __syncthreads();
-    // ---- phase 2: lines of all three directions ----
-    for (int task = tid; task < 3 * NP2; task += NT) {
-      const int dir = task / NP2, l = task % NP2;
-      const int la = l % NP, lb = l / NP;
-      if (dir == 0) fv_line<NP, REC, 0>(a, s_wc, a.numflux, g0, ooJ, la, lb, s_u, s_f, s_df[0]);
-      else if (dir == 1) fv_line<NP, REC, 1>(a, s_wc, a.numflux, g1, ooJ, la, lb, s_u, s_f, s_df[1]);
-      else fv_line<NP, REC, 2>(a, s_wc, a.numflux, g2, ooJ, la, lb, s_u, s_f, s_df[2]);
-    }
+    if (tid < NP2) fv_line<NP, REC, 1, false>(a, e, s_wc, g1, ooJ, la, lb, acc);
     __syncthreads();
-    // ---- epilogue ----
+    if (tid < NP2) fv_line<NP, REC, 2, false>(a, e, s_wc, g2, ooJ, la, lb, acc);
+    __syncthreads();
+    // ---- epilogue: r_{pi,c} = -divf, r_phi = pi, the RK stage ----
 #pragma unroll 1
     for (int c = 0; c < NC; ++c) {
       const int node = tid + c * NT;
@@ -278,8 +313,8 @@ __global__ __launch_bounds__(NT) void fvelm_stage_kernel(StageArgs a, const doub
       const long gi = ebase + node;
       double r[5];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) r[v] = -((s_df[0][v][node] + s_df[1][v][node]) + s_df[2][v][node]);
-      r[4] = s_u[0][node];
+      for (int v = 0; v < 4; ++v) r[v] = -acc[v][node];
+      r[4] = a.win[gi];
       double Ubuf[5], Abuf[5];
       if (a.wmode) {
 #pragma unroll
@@ -302,9 +337,8 @@ __global__ __launch_bounds__(NT) void fvelm_stage_kernel(StageArgs a, const doub
 #pragma unroll
         for (int v = 0; v < 5; ++v) a.Wout[gi + v * NP3] = Ubuf[v] + a.cW * r[v];
       } else if (a.wmode == 2) {
-        const double wv[5] = {s_u[0][node], s_u[1][node], s_u[2][node], s_u[3][node], a.win[gi + 4 * NP3]};
 #pragma unroll
-        for (int v = 0; v < 5; ++v) a.Wout[gi + v * NP3] = 0.75 * Ubuf[v] + 0.25 * (wv[v] + a.cW * r[v]);
+        for (int v = 0; v < 5; ++v) a.Wout[gi + v * NP3] = 0.75 * Ubuf[v] + 0.25 * (a.win[gi + v * NP3] + a.cW * r[v]);
       }
     }
   }
@@ -313,8 +347,7 @@ __global__ __launch_bounds__(NT) void fvelm_stage_kernel(StageArgs a, const doub
 template <int NP, int REC>
 static hipError_t launch_fvelm_t(const StageArgs &a, const double *wc, int grid, hipStream_t s) {
   constexpr int NP3 = NP * NP * NP;
-  constexpr int NT0 = ((NP3 + 1) / 2 + 63) / 64 * 64;
-  constexpr int NT = NT0 < 3 * NP * NP ? ((3 * NP * NP + 63) / 64 * 64) : NT0;
+  constexpr int NT = (NP * NP + 63) / 64 * 64;   // one thread per line
   hipLaunchKernelGGL((fvelm_stage_kernel<NP, REC, NT>), dim3(grid), dim3(NT), 0, s, a, wc);
   return hipGetLastError();
 }
