@@ -63,7 +63,10 @@ int nmx_synchronize(nmx_ctx *ctx);
  * scalarwave1_profile (sin), the FILTER bin's evolve_filter_all (yes|no),
  * evolve_filter_varlist, evolve_filter_alp, evolve_filter_s, evolve_filter_dn
  * (nmesh_evolve.c:35-39), basis_expfilter_JacobianPower (1|0|-1) and
- * basis_filter_fv (yes|no) (nmesh_basis.c:22-24), plus library knobs
+ * basis_filter_fv (yes|no) (nmesh_basis.c:22-24), the FV elements'
+ * amr_uniform_p, amr_fv_p, fv_rec (WENOm3_2|WENOm5_2|WENOmZ_2) and
+ * coordinates_midpoint_data (no|yes|yes all, nmesh_coordinates.c:84),
+ * plus library knobs
  * prefixed nmx_ (nmx_path:
  * plain|batched|perelement, nmx_graph: yes|no, nmx_graph_steps, nmx_geometry:
  * stored|analytic, nmx_block: full|half|quarter, nmx_xcd: yes|no,

@@ -160,6 +160,7 @@ static void set_defaults(nmx_ctx *c) {
       {"evolve_filter_dn", "0"},
       {"basis_expfilter_JacobianPower", "1"},
       {"basis_filter_fv", "yes"},
+      {"coordinates_midpoint_data", "no"},
       {"nmx_path", "plain"},
       {"nmx_graph", "yes"},
       {"nmx_graph_steps", "8"},
@@ -292,6 +293,8 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
   if ((k == "fv_flux" && v != "fnum") || (k == "fv_divf_extrap" && v != "dnfn_extrap1") ||
       (k == "fv_divf_use_only_right_flux" && v != "yes") || (k == "fv_WENOm3_opt_weightratio" && v != "2"))
     return fail(NMX_EUNSUP, k + ": only nmesh's default is built");
+  if (k == "coordinates_midpoint_data" && v != "no" && v != "yes" && v != "yes all")
+    return fail(NMX_EINVAL, "coordinates_midpoint_data must be no, yes or yes all");
   if ((k == "evolve_filter_all" || k == "basis_filter_fv") && v != "yes" && v != "no")
     return fail(NMX_EINVAL, k + " must be yes or no");
   if (k == "basis_expfilter_JacobianPower" && v != "1" && v != "0" && v != "-1")
@@ -679,10 +682,12 @@ static int upload_mesh(nmx_ctx *c) {
   if ((r = dev_upload(&c->dnbf, hnbf))) return r;
   const Basis &b = lgl_basis(c->np1);
   std::vector<double> hD(b.D, b.D + c->np1 * c->np1);
+  // xb: the element's points per direction (uniform ones on FV elements)
+  const double *xbp = c->fvelm ? uniform_basis(c->np1).x : b.x;
   if ((r = dev_alloc(&c->dD, hD.size()))) return r;
   HIPCHK(hipMemcpy(c->dD, hD.data(), hD.size() * 8, hipMemcpyHostToDevice));
   if ((r = dev_alloc(&c->dxb, c->np1))) return r;
-  HIPCHK(hipMemcpy(c->dxb, b.x, c->np1 * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dxb, xbp, c->np1 * 8, hipMemcpyHostToDevice));
   return NMX_OK;
 }
 
@@ -723,7 +728,8 @@ extern "C" int nmx_mesh_build(nmx_ctx *c) {
       for (auto &E : c->mesh.e)
         if (!E.fv) return fail(NMX_EUNSUP, "meshes mixing DG and FV elements are not built");
       for (auto &Pt : c->mesh.pat)
-        if (Pt.type != PAT_BOX) return fail(NMX_EUNSUP, "FV elements are built on Cartesian (BoxMesh) patches");
+        if (Pt.type == PAT_CUBEDSHELL && (Pt.xc[0] != 0 || Pt.xc[1] != 0 || Pt.xc[2] != 0))
+          return fail(NMX_EUNSUP, "FV elements need the shell centre at the origin");
     }
   }
   c->have_mesh = true;
@@ -1036,7 +1042,8 @@ static int launch(nmx_ctx *c, StageArgs a) {
     int grid = a.xcd_map ? 8 * ((nel + 7) / 8) : nel;
     const std::string &fr = P(c, "fv_rec");
     int rec = fr == "WENOm5_2" ? 5 : fr == "WENOmZ_2" ? 7 : 3;
-    hipError_t err = launch_fvelm_stage(c->np1, rec, a, c->dwc, grid, c->stream);
+    a.fv_mid = P(c, "coordinates_midpoint_data") != "no";
+    hipError_t err = launch_fvelm_stage(c->np1, rec, c->geom == 0 ? 0 : 2, a, c->dwc, grid, c->stream);
     if (err != hipSuccess) return fail(NMX_EHIP, std::string("fv stage launch: ") + hipGetErrorString(err));
     return NMX_OK;
   }

@@ -68,6 +68,13 @@ __device__ __forceinline__ void shell_dXdx(int dom, double s0, double s1, const 
   }
 }
 
+// det of a 3x3 matrix (cofactor expansion along the first row)
+__device__ __forceinline__ double det3(const double M[3][3]) {
+  return M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+         M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+         M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+}
+
 // Everything the per-node update needs besides LDS.
 struct NodeOut {
   const double *win, *U, *Ain;

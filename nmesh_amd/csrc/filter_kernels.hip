@@ -23,13 +23,6 @@ namespace nmx {
 
 #define NMX_CONST __attribute__((address_space(4)))
 
-// det(dX/dx) of a 3x3 matrix (cofactor expansion along the first row)
-__device__ __forceinline__ double det3(const double M[3][3]) {
-  return M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
-         M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
-         M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
-}
-
 template <int NP, int GEOM, int NT>
 __global__ __launch_bounds__(NT) void filter_kernel(FilterArgs a) {
   constexpr int NP2 = NP * NP;

@@ -1,5 +1,5 @@
 // fvelm_kernels.hip -- scalarwave1 RHS + RK stage on FV elements (uniform
-// points including both faces) of a Cartesian mesh, gfx950.
+// points including both faces) of a Cartesian or CubedShell mesh, gfx950.
 //
 // nmesh's fv_divf (src/main/dg/fv.c:115-624) with scalarwave1's callbacks
 // (src/tests/scalarwave1/scalarwave1.c:211-237,755-790) and its default
@@ -17,7 +17,8 @@
 //            (ooJ dXb_i), dnfn_extrap1 at both ends (rec1d.c:542-626) and
 //            divf_d = df ooJ, one LDS slab per direction;
 //   epilogue: r_{pi,c} = -(divf_x + divf_y + divf_z), r_phi = pi, RK stage.
-// Cartesian elements only: their midpoint metric equals the node metric.
+// On shell elements the metric factors and normals are recomputed at each
+// node and midpoint of a line (no stored geometry is read).
 #include <hip/hip_runtime.h>
 
 #include "dg_common.h"
@@ -170,16 +171,72 @@ __device__ __forceinline__ void face_value(const StageArgs &a, int e, int f, int
   }
 }
 
+// fnum = f(ui).n + F(ui - ua) for a unit normal n (curvilinear elements)
+__device__ __forceinline__ void fnum_n(int numflux, const double n[3], const double ui[4],
+                                       const double ua[4], double fn[4]) {
+  const double dc[3] = {ui[1] - ua[1], ui[2] - ua[2], ui[3] - ua[3]};
+  double F[4];
+  face_flux(numflux, n, ui[0] - ua[0], dc, F);
+  fn[0] = -(ui[1] * n[0] + ui[2] * n[1] + ui[3] * n[2]) + F[0];
+  fn[1] = -ui[0] * n[0] + F[1];
+  fn[2] = -ui[0] * n[1] + F[2];
+  fn[3] = -ui[0] * n[2] + F[3];
+}
+
+// Geometry of one element for its FV lines.  GEOM 0 (Cartesian): dXbdX and
+// ooJ = det(dXb/dx) are constant.  GEOM 2 (CubedShell): the metric is
+// recomputed at each point from the element's box in patch coordinates.
+struct FvGeo {
+  double dXbdX[3];
+  double detb;          // dXbdX[0] dXbdX[1] dXbdX[2]
+  double bb[6];         // GEOM 2: element box in patch coordinates
+  double s0, s1;
+  int dom;
+};
+
+// sqrtgdiag / det(dXb/dx) along DIR, det(dXb/dx) and the unit normal (row
+// DIR of dX/dx over its length) at patch point X of a shell element, flat
+// metric (coordinates.c:133-224,284-415; coordinates_set_sqrtgdiag_array
+// coordinates.c:728-802; node_normal_from_idXdx_at_ijk utilities.c:48-81)
+template <int DIR>
+__device__ __forceinline__ void shell_metric(const FvGeo &g, const double X[3], double &soj,
+                                             double &ooJ, double nrm[3]) {
+  double M[3][3];
+  shell_dXdx(g.dom, g.s0, g.s1, X, M);
+  ooJ = g.detb * det3(M);
+  double s = sqrt(M[DIR][0] * M[DIR][0] + M[DIR][1] * M[DIR][1] + M[DIR][2] * M[DIR][2]);
+  soj = g.dXbdX[DIR] * s / ooJ;
+  if (s == 0.0) s = 1.0;
+  nrm[0] = M[DIR][0] / s;
+  nrm[1] = M[DIR][1] / s;
+  nrm[2] = M[DIR][2] / s;
+}
+
+// patch coordinate d of element point Xb (XYZ_of_XbYbZb, get_coords.c:384-394)
+__device__ __forceinline__ double patch_X(const FvGeo &g, int d, double xb) {
+  return 0.5 * ((g.bb[2 * d + 1] - g.bb[2 * d]) * xb + (g.bb[2 * d + 1] + g.bb[2 * d]));
+}
+
 // One line of direction DIR at (la, lb) (along Dir1_norm, Dir2_norm of DIR)
 // of element e, streamed from HBM through a window of six points per
 // variable (points i0-2 .. i0+3 at midpoint i0) with the next point
 // requested one midpoint ahead.  Adds df * ooJ of the line to acc (FIRST:
 // writes it), with dnfn_extrap1 at both ends.
-template <int NP, int REC, int DIR, bool FIRST>
-__device__ __forceinline__ void fv_line(const StageArgs &a, int e, const double *wc, double g_dir,
-                                        double ooJ, int la, int lb, double (*acc)[NP * NP * NP]) {
+//
+// Metric factors (fv.c:347-401): Jgdow_R = sqrtgdiag / (ooJ wc) at the
+// midpoint right of i0 (the node itself at the right end), Jgdow_L the same
+// at the midpoint left of i0 (the node itself at the left end); the flux at
+// a midpoint takes the midpoint's normal, the end faces the end node's.
+// With coordinates_midpoint_data = no (a.fv_mid = 0) the metric and normal
+// of midpoint i0+1/2 are node i0's (coordinates.c:467-483).  On Cartesian
+// elements all of these are the constant dXbdX[DIR] / ooJ and +-e_DIR.
+template <int NP, int REC, int DIR, bool FIRST, int GEOM>
+__device__ __forceinline__ void fv_line(const StageArgs &a, int e, const double *wc, const double *xb,
+                                        const FvGeo &G, int la, int lb, double (*acc)[NP * NP * NP]) {
   constexpr int NP2 = NP * NP;
   constexpr int NP3 = NP * NP * NP;
+  constexpr int D1 = DIR == 0 ? 1 : 0;   // Dir1_norm: la runs along it
+  constexpr int D2 = DIR == 2 ? 1 : 2;   // Dir2_norm: lb runs along it
   const int stride = DIR == 0 ? 1 : DIR == 1 ? NP : NP2;
   const int base = DIR == 0 ? (la * NP + lb * NP2) : DIR == 1 ? (la + lb * NP2) : (la + lb * NP);
   const double *g = a.win + 5L * NP3 * e + base;
@@ -196,24 +253,53 @@ __device__ __forceinline__ void fv_line(const StageArgs &a, int e, const double 
     }
     nx[v] = (4 < NP) ? g[v * NP3 + 4 * stride] : 0.0;
   }
-  auto put = [&](int i, int l, double d) {
-    if (FIRST) acc[l][base + i * stride] = d * ooJ;
-    else acc[l][base + i * stride] += d * ooJ;
+  // patch coordinates of the line off its direction
+  double X[3] = {0.0, 0.0, 0.0};
+  if (GEOM == 2) {
+    X[D1] = patch_X(G, D1, xb[la]);
+    X[D2] = patch_X(G, D2, xb[lb]);
+  }
+  auto put = [&](int i, int l, double d, double oj) {
+    if (FIRST) acc[l][base + i * stride] = d * oj;
+    else acc[l][base + i * stride] += d * oj;
   };
   double fR[4], fL[4], d0[4], d1[4], dm2[4], dm1[4];
-  {
-    double ui[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) ui[v] = q[v][2];
-    fnum_dir<DIR>(a.numflux, -1.0, ui, lo, fL);
-  }
+  double oj0 = G.detb, oj1 = G.detb, ojm2 = G.detb, ojm1 = G.detb;
+  double sojL = G.dXbdX[DIR] / G.detb;   // sqrtgdiag / ooJ left of i0
 #pragma unroll 1
   for (int i0 = 0; i0 < NP; ++i0) {
     const int im = i0;
-    const double JR = g_dir / (ooJ * wc[i0]), JL = JR;
+    // node metric; the right midpoint's metric and normal
+    double ojn = G.detb, sojn = G.dXbdX[DIR] / G.detb, nn[3] = {0.0, 0.0, 0.0};
+    double sojR, nR[3];
+    nn[DIR] = 1.0;
+    if (GEOM == 2) {
+      X[DIR] = patch_X(G, DIR, xb[i0]);
+      shell_metric<DIR>(G, X, sojn, ojn, nn);
+    }
+    sojR = sojn;
+    nR[0] = nn[0]; nR[1] = nn[1]; nR[2] = nn[2];
+    if (GEOM == 2 && a.fv_mid && i0 < NP - 1) {
+      double ojm;
+      X[DIR] = patch_X(G, DIR, 0.5 * (xb[i0] + xb[i0 + 1]));
+      shell_metric<DIR>(G, X, sojR, ojm, nR);
+    }
+    if (i0 == 0) sojL = sojn;
+    const double JR = sojR / wc[i0], JL = sojL / wc[i0];
+    sojL = sojR;
     if (i0 > 0) {
 #pragma unroll
       for (int l = 0; l < 4; ++l) fL[l] = -fR[l];
+    } else {
+      double ui[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) ui[v] = q[v][2];
+      if (GEOM == 0) {
+        fnum_dir<DIR>(a.numflux, -1.0, ui, lo, fL);
+      } else {
+        const double nL[3] = {-nn[0], -nn[1], -nn[2]};
+        fnum_n(a.numflux, nL, ui, lo, fL);
+      }
     }
     double ui[4], ua[4];
     if (i0 < NP - 1) {
@@ -235,7 +321,12 @@ __device__ __forceinline__ void fv_line(const StageArgs &a, int e, const double 
 #pragma unroll
       for (int v = 0; v < 4; ++v) { ui[v] = q[v][2]; ua[v] = hi[v]; }
     }
-    fnum_dir<DIR>(a.numflux, 1.0, ui, ua, fR);
+    if (GEOM == 0) fnum_dir<DIR>(a.numflux, 1.0, ui, ua, fR);
+    else fnum_n(a.numflux, nR, ui, ua, fR);
+    if (i0 == 0) oj0 = ojn;
+    else if (i0 == 1) oj1 = ojn;
+    if (i0 == NP - 2) ojm2 = ojn;
+    else if (i0 == NP - 1) ojm1 = ojn;
 #pragma unroll
     for (int l = 0; l < 4; ++l) {
       const double d = JR * fR[l] + JL * fL[l];
@@ -243,7 +334,7 @@ __device__ __forceinline__ void fv_line(const StageArgs &a, int e, const double 
       else if (NP > 2 && i0 == 1) d1[l] = d;
       else if (NP > 2 && i0 == NP - 2) dm2[l] = d;
       else if (NP > 2 && i0 == NP - 1) dm1[l] = d;
-      else put(i0, l, d);
+      else put(i0, l, d, ojn);
     }
     // slide the window by one point; request the one after next
 #pragma unroll
@@ -259,25 +350,29 @@ __device__ __forceinline__ void fv_line(const StageArgs &a, int e, const double 
     const double w0 = 1. / 0.75, w1 = 1. - w0;
 #pragma unroll
     for (int l = 0; l < 4; ++l) {
-      put(0, l, w0 * d0[l] + w1 * d1[l]);
-      put(NP - 1, l, w0 * dm1[l] + w1 * dm2[l]);
+      put(0, l, w0 * d0[l] + w1 * d1[l], oj0);
+      put(NP - 1, l, w0 * dm1[l] + w1 * dm2[l], ojm1);
       if (NP > 3) {
-        put(1, l, d1[l]);
-        put(NP - 2, l, dm2[l]);
+        put(1, l, d1[l], oj1);
+        put(NP - 2, l, dm2[l], ojm2);
       }
     }
   }
 }
 
-template <int NP, int REC, int NT>
+template <int NP, int REC, int NT, int GEOM>
 __global__ __launch_bounds__(NT) void fvelm_stage_kernel(StageArgs a, const double *wc) {
   constexpr int NP2 = NP * NP;
   constexpr int NP3 = NP * NP * NP;
   constexpr int NC = (NP3 + NT - 1) / NT;
   __shared__ double acc[4][NP3];   // divf, summed over x, then y, then z
   __shared__ double s_wc[NP];
+  __shared__ double s_xb[NP];
   const int tid = threadIdx.x;
-  for (int q = tid; q < NP; q += NT) s_wc[q] = wc[q];
+  for (int q = tid; q < NP; q += NT) {
+    s_wc[q] = wc[q];
+    s_xb[q] = GEOM == 2 ? a.xb[q] : 0.0;
+  }
 
   int e_hi, e_step, e_first;
   if (a.xcd_map) {
@@ -295,15 +390,25 @@ __global__ __launch_bounds__(NT) void fvelm_stage_kernel(StageArgs a, const doub
 
   for (int e = e_first; e < e_hi; e += e_step) {
     const long ebase = 5L * NP3 * e;
-    const double g0 = a.dXbdX[3 * e], g1 = a.dXbdX[3 * e + 1], g2 = a.dXbdX[3 * e + 2];
-    const double ooJ = g0 * g1 * g2;
+    FvGeo G;
+    G.dXbdX[0] = a.dXbdX[3 * e];
+    G.dXbdX[1] = a.dXbdX[3 * e + 1];
+    G.dXbdX[2] = a.dXbdX[3 * e + 2];
+    G.detb = G.dXbdX[0] * G.dXbdX[1] * G.dXbdX[2];
+    if (GEOM == 2) {
+#pragma unroll
+      for (int d = 0; d < 6; ++d) G.bb[d] = a.egeo[8 * e + d];
+      G.s0 = a.egeo[8 * e + 6];
+      G.s1 = a.egeo[8 * e + 7];
+      G.dom = a.edom[e];
+    }
     const int la = tid % NP, lb = tid / NP;
     __syncthreads();   // the previous element's epilogue is done with acc
-    if (tid < NP2) fv_line<NP, REC, 0, true>(a, e, s_wc, g0, ooJ, la, lb, acc);
+    if (tid < NP2) fv_line<NP, REC, 0, true, GEOM>(a, e, s_wc, s_xb, G, la, lb, acc);
     __syncthreads();
-    if (tid < NP2) fv_line<NP, REC, 1, false>(a, e, s_wc, g1, ooJ, la, lb, acc);
+    if (tid < NP2) fv_line<NP, REC, 1, false, GEOM>(a, e, s_wc, s_xb, G, la, lb, acc);
     __syncthreads();
-    if (tid < NP2) fv_line<NP, REC, 2, false>(a, e, s_wc, g2, ooJ, la, lb, acc);
+    if (tid < NP2) fv_line<NP, REC, 2, false, GEOM>(a, e, s_wc, s_xb, G, la, lb, acc);
     __syncthreads();
     // ---- epilogue: r_{pi,c} = -divf, r_phi = pi, the RK stage ----
 #pragma unroll 1
@@ -345,19 +450,22 @@ __global__ __launch_bounds__(NT) void fvelm_stage_kernel(StageArgs a, const doub
 }
 
 template <int NP, int REC>
-static hipError_t launch_fvelm_t(const StageArgs &a, const double *wc, int grid, hipStream_t s) {
-  constexpr int NP3 = NP * NP * NP;
+static hipError_t launch_fvelm_t(int geom, const StageArgs &a, const double *wc, int grid, hipStream_t s) {
   constexpr int NT = (NP * NP + 63) / 64 * 64;   // one thread per line
-  hipLaunchKernelGGL((fvelm_stage_kernel<NP, REC, NT>), dim3(grid), dim3(NT), 0, s, a, wc);
+  if (geom == 0)
+    hipLaunchKernelGGL((fvelm_stage_kernel<NP, REC, NT, 0>), dim3(grid), dim3(NT), 0, s, a, wc);
+  else
+    hipLaunchKernelGGL((fvelm_stage_kernel<NP, REC, NT, 2>), dim3(grid), dim3(NT), 0, s, a, wc);
   return hipGetLastError();
 }
 
-hipError_t launch_fvelm_stage(int np, int rec, const StageArgs &a, const double *wc, int grid, hipStream_t s) {
+hipError_t launch_fvelm_stage(int np, int rec, int geom, const StageArgs &a, const double *wc, int grid,
+                              hipStream_t s) {
 #define NMX_CASE(N)                                                                 \
   case N:                                                                           \
-    return rec == 5   ? launch_fvelm_t<N, 5>(a, wc, grid, s)                        \
-           : rec == 7 ? launch_fvelm_t<N, 7>(a, wc, grid, s)                        \
-                      : launch_fvelm_t<N, 3>(a, wc, grid, s);
+    return rec == 5   ? launch_fvelm_t<N, 5>(geom, a, wc, grid, s)                  \
+           : rec == 7 ? launch_fvelm_t<N, 7>(geom, a, wc, grid, s)                  \
+                      : launch_fvelm_t<N, 3>(geom, a, wc, grid, s);
   switch (np) {
     NMX_CASE(3)
     NMX_CASE(4)

@@ -33,6 +33,7 @@ struct StageArgs {
   double om = 0;
   int e0 = 0, e1 = 0;
   int wmode = 0, numflux = 0, profile = 0;
+  int fv_mid = 0;                // FV elements: metric and normal at the midpoints
 };
 
 // Arguments of one fused adjoint RHS + adjoint RK stage launch: y = L^T lam
@@ -61,7 +62,10 @@ hipError_t launch_dg_adjoint(int np, int geom, const AdjArgs &a, int grid, hipSt
 // scalarwave1 on FV elements (fv_divf); rec 5 = WENOm5_2, 7 = WENOmZ_2,
 // 3 = WENOm3_2;
 // wc[np]: distances between the midpoints around each uniform point
-hipError_t launch_fvelm_stage(int np, int rec, const StageArgs &a, const double *wc, int grid, hipStream_t s);
+// geom: 0 Cartesian, 2 CubedShell (metric recomputed at nodes and midpoints
+// from egeo, edom and the uniform points in xb)
+hipError_t launch_fvelm_stage(int np, int rec, int geom, const StageArgs &a, const double *wc, int grid,
+                              hipStream_t s);
 
 // flat [v][node] <-> element blocks [e][v][np_e] on the device
 // (to_blocks: 1 flat -> blocks, 0 blocks -> flat)

@@ -1388,6 +1388,56 @@ void orc_dnfn_extrap1(int n, double *df)
 }
 
 /* r of one FV element: divf from fv_divf, r_{pi,c} = -divf, r_phi = pi */
+/* unit normal of face 2*dir+side at grid point g: +-row dir of dXdx over
+   its length (node_normal_at_ijk via node_normal_from_idXdx_at_ijk,
+   utilities.c:48-81) */
+static void node_normal(const OMesh *M, long g, int dir, int side, double nrm[3])
+{
+  long nn = M->nnodes;
+  double sig = side ? 1.0 : -1.0;
+  for (int d = 0; d < 3; d++) nrm[d] = M->dXdx[(3 * dir + d) * nn + g];
+  double smag = sig * sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+  if (smag == 0.0) smag = sig;
+  for (int d = 0; d < 3; d++) nrm[d] /= smag;
+}
+
+/* coordinates_midpoint_data: 0 = no (the default: midpoint metric and
+   normal are those of the grid point left of the midpoint), 1 = yes
+   (computed at the midpoints, coordinates.c:284-415) */
+static int g_fv_midpt = 0;
+void orc_set_fv_midpoint(int on) { g_fv_midpt = on; }
+
+/* metric of element E at element point Xb for direction dir, as
+   coordinates_init_node sets it at a midpoint (coordinates.c:284-415):
+   sg = dXbdX[dir] sqrt(sum_l (dX^dir/dx^l)^2) (flat metric,
+   coordinates_set_sqrtgdiag_array coordinates.c:728-802), det =
+   det(dXb/dx), nrm = row dir of dX/dx over its length
+   (node_normal_from_idXdx_at_ijk, utilities.c:48-81) */
+static void point_metric(const OMesh *M, const OElm *E, const double Xb[3], int dir, double *sg,
+                         double *det, double nrm[3])
+{
+  const OPat *P = &M->pat[E->p];
+  double dXbdX[3], X[3], x[3], D[3][3] = {{0}};
+  elm_dXbdX(E, dXbdX);
+  elm_X_of_Xb(E, Xb, X);
+  if (P->type == PAT_CUBEDSHELL) {
+    cubsph_dXdx(P, X, x, D);
+  } else {
+    for (int d = 0; d < 3; d++) D[d][d] = 1.;
+  }
+  double detb = dXbdX[0] * dXbdX[1] * dXbdX[2];
+  double det3 = D[0][0] * (D[1][1] * D[2][2] - D[1][2] * D[2][1]) -
+                D[0][1] * (D[1][0] * D[2][2] - D[1][2] * D[2][0]) +
+                D[0][2] * (D[1][0] * D[2][1] - D[1][1] * D[2][0]);
+  *det = (P->type == PAT_CUBEDSHELL) ? detb * det3 : detb;
+  double gam = 0.0;
+  for (int l = 0; l < 3; l++) gam += D[dir][l] * D[dir][l];
+  *sg = dXbdX[dir] * sqrt(gam);
+  double smag = sqrt(gam);
+  if (smag == 0.0) smag = 1.0;
+  for (int l = 0; l < 3; l++) nrm[l] = D[dir][l] / smag;
+}
+
 static void fv_elm_rhs(const OMesh *M, const OSW *sw, const OElm *E, double t, const double *u,
                        const double *planes, double *r)
 {
@@ -1427,8 +1477,6 @@ static void fv_elm_rhs(const OMesh *M, const OSW *sw, const OElm *E, double t, c
   for (int i = 1; i < n - 1; i++) dXb[i] = Xbm[i] - Xbm[i - 1];
   for (int dir = 0; dir < 3; dir++) {
     const int d1 = (dir == 0), d2 = (dir == 2) ? 1 : 2;
-    const double nR[3] = {dir == 0, dir == 1, dir == 2};
-    const double nL[3] = {-(double)(dir == 0), -(double)(dir == 1), -(double)(dir == 2)};
     for (int b = 0; b < n; b++)
       for (int a = 0; a < n; a++) {
         double qc[NV][ORC_MAXN], df[4][ORC_MAXN];
@@ -1441,16 +1489,44 @@ static void fv_elm_rhs(const OMesh *M, const OSW *sw, const OElm *E, double t, c
         }
         double fnumR[5] = {0, 0, 0, 0, 0}, fnumL[5];
         const int JK = a + n * b;
+        /* metric factor sqrtgdiag / det(dXb/dx) and unit normal at each
+           midpoint i0+1/2 (fv.c:347-401, node_normal_from_DGinfo
+           dg.c:468-479, node_normal_at_midpt_nextto_ijk utilities.c:
+           105-181).  With coordinates_midpoint_data = no these are node
+           i0's (coordinates.c:467-483: the midpoint variables alias the
+           node's); with yes they are computed at the midpoint. */
+        double SoJm[ORC_MAXN], nm_[ORC_MAXN][3];
+        for (int i0 = 0; i0 < n - 1; i0++) {
+          if (g_fv_midpt) {
+            const double *xb = elm_Xb1(E);
+            double Xbp[3], sg, det;
+            Xbp[dir] = Xbm[i0]; Xbp[d1] = xb[a]; Xbp[d2] = xb[b];
+            point_metric(M, E, Xbp, dir, &sg, &det, nm_[i0]);
+            SoJm[i0] = sg / det;
+          } else {
+            const long g = gl[i0];
+            SoJm[i0] = M->sqrtgdiag[dir * nn + g] / M->det[g];
+            node_normal(M, g, dir, 1, nm_[i0]);
+          }
+        }
         for (int i0 = 0; i0 < n; i0++) {
           const long g = gl[i0];
-          const double sgd = M->sqrtgdiag[dir * nn + g], ooJ = M->det[g];
+          /* Jgdow_R from the midpoint right of i0 (the node itself at the
+             right end), Jgdow_L from the midpoint left of i0 (the node
+             itself at the left end), both over wc of i0; the end faces
+             take the end node's normal */
           const double wc = dXb[i0];
-          /* midpoint metric = node metric on Cartesian elements */
-          double JR = sgd / (ooJ * wc), JL = sgd / (ooJ * wc);
+          const double SoJn = M->sqrtgdiag[dir * nn + g] / M->det[g];
+          const double JR = (i0 < n - 1 ? SoJm[i0] : SoJn) / wc;
+          const double JL = (i0 > 0 ? SoJm[i0 - 1] : SoJn) / wc;
+          double nR[3];
+          if (i0 < n - 1) memcpy(nR, nm_[i0], sizeof(nR));
+          else node_normal(M, g, dir, 1, nR);
           if (i0 > 0) {
             for (int l = 0; l < NV; l++) fnumL[l] = -fnumR[l];
           } else {
-            double ui[5], ua[5];
+            double ui[5], ua[5], nL[3];
+            node_normal(M, g, dir, 0, nL);
             for (int v = 0; v < NV; v++) { ui[v] = qc[v][0]; ua[v] = aj[2 * dir][v][JK]; }
             sw_fnum(sw, nL, ui, ua, fnumL);
           }

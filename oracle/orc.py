@@ -52,6 +52,7 @@ def lib():
             "orc_num_threads": (i, []),
             "orc_mesh_set_fv": (None, [vp, i]),
             "orc_set_fv_rec": (None, [i]),
+            "orc_set_fv_midpoint": (None, [i]),
             "orc_dnfn_extrap1": (None, [i, vp]),
             "orc_rec1d_p_WENOm3": (d, [vp, i]),
             "orc_rec1d_m_WENOm3": (d, [vp, i]),
@@ -183,6 +184,12 @@ class OMesh:
         u = np.array(u, dtype=np.float64, copy=True)
         lib().orc_filter(self.h, _p(u), int(varmask), float(alp), float(s), int(dn), int(jpow))
         return u
+
+
+def set_fv_midpoint(on):
+    """coordinates_midpoint_data of the oracle's FV elements: False (the
+    default, "no") or True ("yes")"""
+    lib().orc_set_fv_midpoint(1 if on else 0)
 
 
 def set_fv_rec(mode):

@@ -12,7 +12,23 @@ elements.
   closure, face flux, metric factor and the dnfn_extrap1 weights (4/3, -1/3)
   are exact on linear data, so an index, sign, weight or factor slip fails.
 - Uniform points as uniform_x places them (Legendre.c:626-663); coordinates,
-  tables and dt of the product host bit-exact to the oracle.
+  tables and dt of the product host bit-exact to the oracle, Cartesian and
+  cubed-sphere shell.
+- Curvilinear FV elements (cubed-sphere shell), the plane-wave RHS against
+  its exact time derivative:
+  - coordinates_midpoint_data = yes (metric and normals at the midpoints):
+    deep inside the elements the error falls at third order or better, and
+    everywhere at first order, as on the Cartesian box (the closures at the
+    element faces);
+  - coordinates_midpoint_data = no (the reference's default: a midpoint
+    takes the metric of the node on its left, coordinates.c:467-483) is
+    first order deep inside (the metric sits half a spacing off) and does
+    not converge at the nodes next to element faces, where the half-width
+    end cell takes the metric of its neighbour node (fv.c:386-391).
+  A midpoint metric evaluated at the wrong place, along the wrong direction
+  or without det(dXb/dx) fails the first; the second fixes which node the
+  default mode takes.
+  On a Cartesian box both modes give the identical RHS.
 """
 import numpy as np
 import pytest
@@ -138,12 +154,70 @@ def test_fv_element_rhs_converges_for_a_plane_wave():
     orc.set_fv_rec("WENOm3_2")
 
 
-@pytest.mark.parametrize("luni,n", [(1, 5), (2, 6)])
-def test_host_tables_of_fv_meshes_match_the_oracle(luni, n):
-    m = nm.Mesh({"amr_mesh_type": "BoxMesh", "amr_luni": luni, "amr_n0": n, "amr_n1": n, "amr_n2": n,
-                 "amr_BoxMesh_dout": 8.0, "amr_uniform_p": "0", "amr_fv_p": "0"}, device=-1)
+K = [0.8, -0.5, 0.33166247903554]
+
+
+def shell_errors(n, mid):
+    """max |RHS - du/dt| of the plane wave on FV shell elements: over all
+    nodes, and over the nodes at least n/4 points from every element face"""
+    om = OMesh("shell", 1, n, rin=2.0, rout=6.0, fv=True)
+    u, du = om.profile(K, 0.2)
+    orc.set_fv_midpoint(mid)
+    try:
+        r = om.rhs(u, 0.2, K, "upwind")
+    finally:
+        orc.set_fv_midpoint(False)
+    e = np.abs(r - du).reshape(5, om.nelem, n, n, n)[:4]
+    q = n // 4
+    return e.max(), e[:, :, q:-q, q:-q, q:-q].max()
+
+
+def order(e0, e1, n0, n1):
+    return np.log(e0 / e1) / np.log((n1 - 1) / (n0 - 1))
+
+
+def test_shell_fv_with_midpoint_metric_converges():
+    orc.set_fv_rec("WENOm5_2")
+    ns = (12, 16, 20)
+    errs = [shell_errors(n, True) for n in ns]
+    for (a0, d0), (a1, d1), n0, n1 in zip(errs[:-1], errs[1:], ns[:-1], ns[1:]):
+        assert order(d0, d1, n0, n1) > 2.4, (n0, n1, d0, d1)
+        assert 0.8 < order(a0, a1, n0, n1) < 1.6, (n0, n1, a0, a1)
+    orc.set_fv_rec("WENOm3_2")
+
+
+def test_shell_fv_default_metric_first_order_inside_inconsistent_at_faces():
+    orc.set_fv_rec("WENOm5_2")
+    ns = (12, 16, 20)
+    errs = [shell_errors(n, False) for n in ns]
+    for (a0, d0), (a1, d1), n0, n1 in zip(errs[:-1], errs[1:], ns[:-1], ns[1:]):
+        assert 0.7 < order(d0, d1, n0, n1) < 1.4, (n0, n1, d0, d1)
+        assert a1 > 1.0 and a1 > 0.95 * a0, (n0, n1, a0, a1)
+    orc.set_fv_rec("WENOm3_2")
+
+
+def test_midpoint_metric_on_cartesian_elements_is_the_node_metric():
+    om = OMesh("box", 1, 7, dout=2.0, fv=True)
+    u, _ = om.profile(K, 0.1)
+    u = u + 0.1 * np.random.default_rng(3).uniform(-1, 1, u.shape)
+    r0 = om.rhs(u, 0.1, K, "upwind")
+    orc.set_fv_midpoint(True)
+    try:
+        r1 = om.rhs(u, 0.1, K, "upwind")
+    finally:
+        orc.set_fv_midpoint(False)
+    assert np.array_equal(r0, r1)
+
+
+@pytest.mark.parametrize("kind,luni,n", [("box", 1, 5), ("box", 2, 6), ("shell", 1, 6)])
+def test_host_tables_of_fv_meshes_match_the_oracle(kind, luni, n):
+    m = nm.Mesh({"amr_mesh_type": "BoxMesh" if kind == "box" else "Shell", "amr_luni": luni,
+                 "amr_n0": n, "amr_n1": n, "amr_n2": n, "amr_BoxMesh_dout": 8.0,
+                 "amr_Shell_rin": 2.0, "amr_Shell_rout": 20.0,
+                 "amr_uniform_p": "0" if kind == "box" else "0 1 2 3 4 5",
+                 "amr_fv_p": "0" if kind == "box" else "5 4 3 2 1 0"}, device=-1)
     nm.amr_setup_mesh(m)
-    om = OMesh("box", luni, n, dout=8.0, fv=True)
+    om = OMesh(kind, luni, n, dout=8.0, rin=2.0, rout=20.0, fv=True)
     X, x = om.coords()
     assert np.array_equal(m.coords(), x)
     assert m.dt == om.dt()

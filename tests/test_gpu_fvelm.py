@@ -5,6 +5,8 @@ The kernel and the oracle evaluate the same reconstructions and metric
 factors; the numerical flux is the closed form on the device and the
 R-matrix / LLF form in the oracle, so the RHS agrees to ~1e-14 relative.
 The bar is 1e-12 relative to the field maximum, per RHS and per step.
+Cartesian boxes and cubed-sphere shells (metric recomputed on the device at
+nodes and midpoints), coordinates_midpoint_data no and yes.
 """
 import numpy as np
 import pytest
@@ -18,15 +20,21 @@ K = [0.8, -0.5, 0.33166247903554]
 TOL = 1e-12
 
 
-def make(luni, n, rec, numflux="upwind", k=K, graph="yes"):
-    pars = {"amr_mesh_type": "BoxMesh", "amr_luni": luni, "amr_n0": n, "amr_n1": n, "amr_n2": n,
-            "amr_BoxMesh_dout": 2.0, "amr_uniform_p": "0", "amr_fv_p": "0", "fv_rec": rec,
+def make(luni, n, rec, numflux="upwind", k=K, graph="yes", kind="box", mid="no", extra=None):
+    pars = {"amr_mesh_type": "BoxMesh" if kind == "box" else "Shell", "amr_luni": luni,
+            "amr_n0": n, "amr_n1": n, "amr_n2": n, "amr_BoxMesh_dout": 2.0,
+            "amr_Shell_rin": 2.0, "amr_Shell_rout": 6.0,
+            "amr_uniform_p": "0" if kind == "box" else "0 1 2 3 4 5",
+            "amr_fv_p": "0" if kind == "box" else "0 1 2 3 4 5", "fv_rec": rec,
+            "coordinates_midpoint_data": mid,
             "scalarwave1_numflux": numflux, "scalarwave1_k": "%.17g %.17g %.17g" % tuple(k),
             "nmx_graph": graph, "nmx_graph_steps": 2}
+    pars.update(extra or {})
     m = nm.Mesh(pars, device=0)
     nm.amr_setup_mesh(m)
     orc.set_fv_rec(rec)
-    om = OMesh("box", luni, n, dout=2.0, fv=True)
+    orc.set_fv_midpoint(mid != "no")
+    om = OMesh(kind, luni, n, dout=2.0, rin=2.0, rout=6.0, fv=True)
     return m, om
 
 
@@ -70,12 +78,49 @@ def test_fv_element_steps_match_oracle(method, rec):
     assert np.array_equal(m2.download(), m.download())
 
 
+@pytest.mark.parametrize("mid", ["no", "yes"])
+@pytest.mark.parametrize("rec", ["WENOm3_2", "WENOm5_2", "WENOmZ_2"])
+@pytest.mark.parametrize("numflux,n,geometry", [("upwind", 6, "analytic"), ("LLF", 9, "stored")])
+def test_fv_shell_rhs_matches_oracle(mid, rec, numflux, n, geometry):
+    print("fvelm shell rhs", mid, rec, numflux, n, geometry, flush=True)
+    m, om = make(1, n, rec, numflux, kind="shell", mid=mid, extra={"nmx_geometry": geometry})
+    u, _ = om.profile(K, 0.2)
+    u = u + 0.1 * np.random.default_rng(n).uniform(-1, 1, u.shape)
+    m.upload(u)
+    r = nm.evolve_setrhs_mesh(m, t=0.2)
+    r_o = om.rhs(u, 0.2, K, numflux)
+    orc.set_fv_midpoint(False)
+    assert rel(r, r_o) <= TOL
+
+
+@pytest.mark.parametrize("mid", ["no", "yes"])
+def test_fv_shell_steps_match_oracle(mid):
+    """RK4 on FV shell elements, checked every step; the midpoint mode is
+    read at each launch, so switching it on an existing mesh takes effect"""
+    print("fvelm shell steps", mid, flush=True)
+    m, om = make(1, 7, "WENOm5_2", kind="shell", mid="no" if mid == "yes" else "yes")
+    m.set_par("coordinates_midpoint_data", mid)
+    orc.set_fv_midpoint(mid == "yes")
+    u, _ = om.profile(K, 0.0)
+    m.upload(u)
+    t = 0.0
+    try:
+        for s in range(3):
+            u, t = om.evolve(u, m.dt, 1, t, K, "RK4")
+            nm.evolve_myln(m, 1)
+            assert rel(m.download(), u) <= TOL, "step %d" % s
+    finally:
+        orc.set_fv_midpoint(False)
+
+
 def test_fv_elements_refuse_unbuilt_configurations():
     pars = {"amr_mesh_type": "Shell", "amr_luni": 1, "amr_n0": 5, "amr_n1": 5, "amr_n2": 5,
-            "amr_uniform_p": "0", "amr_fv_p": "0"}
+            "amr_uniform_p": "0 2 4", "amr_fv_p": "0 2 4"}
     m = nm.Mesh(pars, device=0)
     with pytest.raises(nm.NmxError):
-        nm.amr_setup_mesh(m)          # FV on curvilinear patches
+        nm.amr_setup_mesh(m)          # DG and FV elements side by side
+    with pytest.raises(nm.NmxError):
+        m.set_par("coordinates_midpoint_data", "maybe")
     m = nm.Mesh({"amr_mesh_type": "BoxMesh", "amr_uniform_p": "0"}, device=0)
     with pytest.raises(nm.NmxError):
         nm.amr_setup_mesh(m)          # DG on uniform points
