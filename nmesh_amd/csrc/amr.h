@@ -58,6 +58,7 @@ struct AmrArgs {
   double k[3] = {0, 0, 0};
   double om = 0;
   int wmode = 0, numflux = 0, profile = 0, xcd_map = 0;
+  int fv_mid = 0;                 // FV buckets: metric and normal at the midpoints
 };
 
 struct FaceArgs {
@@ -84,5 +85,10 @@ struct FilterArgs {
 hipError_t launch_dg_amr(int np, int geom, const AmrArgs &a, int grid, int div, hipStream_t s);
 hipError_t launch_face_interp(const FaceArgs &a, hipStream_t s);
 hipError_t launch_filter(int np, int geom, const FilterArgs &a, hipStream_t s);
+// an FV order bucket (fv_divf on uniform points): rec 5 = WENOm5_2, 7 =
+// WENOmZ_2, 3 = WENOm3_2; geom 0 Cartesian, 2 CubedShell; wc[np] the
+// distances between the midpoints around each point; a.xb uniform points
+hipError_t launch_fvelm_amr(int np, int rec, int geom, const AmrArgs &a, const double *wc, int grid,
+                            hipStream_t s);
 
 }  // namespace nmx

@@ -80,6 +80,8 @@ struct nmx_ctx {
     int n; int count; SlotRec *drec; long doff; long xoff; int div;
     int f1first, f1count;
     float ms;   // tuned stage time alone (0: not tuned)
+    int fv;     // 1: FV elements on uniform points (fv_divf)
+    long woff;  // FV: the bucket's midpoint distances in dwcall
   };
   std::vector<Bucket> buckets;
   std::vector<long> eoff;   // node offset per element
@@ -106,9 +108,11 @@ struct nmx_ctx {
   // path), its own graph cache, and the checkpoint pool
   double *dL = nullptr;
   int *dnbf = nullptr;
-  // FV elements (fv_divf): every element FV on uniform points
-  bool fvelm = false;
+  // FV elements (fv_divf) on uniform points: every element (fvelm, plain
+  // path) or some (anyfv; FV order buckets of the batched path)
+  bool fvelm = false, anyfv = false;
   double *dwc = nullptr;    // distances between midpoints around each point
+  double *dwcall = nullptr; // the same per FV bucket
   long *deoff = nullptr;
   int *den = nullptr;
   hipGraph_t agraph = nullptr;
@@ -147,6 +151,8 @@ static void set_defaults(nmx_ctx *c) {
       {"fv_divf_extrap", "dnfn_extrap1"},
       {"fv_divf_use_only_right_flux", "yes"},
       {"fv_WENOm3_opt_weightratio", "2"},
+      {"fv_surface_interp", "linear"},
+      {"fv2dg_interp_use_extrap1", "no"},
       {"dt", "auto"},
       {"dtfac", "0.25"},
       {"evolve_method", "RK4"},
@@ -229,7 +235,7 @@ static void free_mesh_dev(nmx_ctx *c) {
   c->graph_steps = 0;
   {
     void **ptrs[] = {(void **)&c->dL, (void **)&c->dnbf, (void **)&c->deoff, (void **)&c->den,
-                     (void **)&c->dCk, (void **)&c->dwc};
+                     (void **)&c->dCk, (void **)&c->dwc, (void **)&c->dwcall};
     for (auto q : ptrs)
       if (*q) { (void)hipFree(*q); *q = nullptr; }
   }
@@ -291,7 +297,8 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
   if (k == "fv_rec" && v != "WENOm3_2" && v != "WENOm5_2" && v != "WENOmZ_2")
     return fail(NMX_EUNSUP, "fv_rec: WENOm3_2, WENOm5_2 and WENOmZ_2 are built");
   if ((k == "fv_flux" && v != "fnum") || (k == "fv_divf_extrap" && v != "dnfn_extrap1") ||
-      (k == "fv_divf_use_only_right_flux" && v != "yes") || (k == "fv_WENOm3_opt_weightratio" && v != "2"))
+      (k == "fv_divf_use_only_right_flux" && v != "yes") || (k == "fv_WENOm3_opt_weightratio" && v != "2") ||
+      (k == "fv_surface_interp" && v != "linear") || (k == "fv2dg_interp_use_extrap1" && v != "no"))
     return fail(NMX_EUNSUP, k + ": only nmesh's default is built");
   if (k == "coordinates_midpoint_data" && v != "no" && v != "yes" && v != "yes all")
     return fail(NMX_EINVAL, "coordinates_midpoint_data must be no, yes or yes all");
@@ -392,6 +399,18 @@ static void assign_lanes(nmx_ctx *c) {
     c->blane[bi] = lane;
     load[lane] += work(bi);
   }
+}
+
+// Distances between the midpoints around each of n uniform points, from the
+// face to the first midpoint at the ends (get_coords.c:1749-1860).
+static std::vector<double> fv_cell_widths(int n) {
+  const Basis &ub = uniform_basis(n);
+  std::vector<double> xm(n), wc(n);
+  for (int i = 0; i < n - 1; ++i) xm[i] = 0.5 * (ub.x[i] + ub.x[i + 1]);
+  wc[0] = std::fabs(xm[0] - (-1.));
+  wc[n - 1] = std::fabs(xm[n - 2] - 1.);
+  for (int i = 1; i < n - 1; ++i) wc[i] = xm[i] - xm[i - 1];
+  return wc;
 }
 
 // Tables of the batched path: face descriptors, face interpolation jobs
@@ -499,18 +518,26 @@ static int upload_amr_tables(nmx_ctx *c) {
     return r;
   std::vector<int> jobs1;
   // order buckets with their D matrices and nodes
-  std::vector<double> dall, xball;
-  for (int n = 2; n <= 10; ++n) {
+  std::vector<double> dall, xball, wcall;
+  for (int n = 2; n <= 10; ++n)
+  for (int fv = 0; fv <= 1; ++fv) {
     std::vector<int> list;
     for (size_t i = 0; i < ne; ++i)
-      if (M.e[i].n == n) list.push_back((int)i);
+      if (M.e[i].n == n && (M.e[i].fv != 0) == (fv != 0)) list.push_back((int)i);
     if (list.empty()) continue;
     nmx_ctx::Bucket b;
     b.n = n;
     b.count = (int)list.size();
     b.drec = nullptr;
-    b.div = 0;
+    b.div = fv ? 1 : 0;
     b.ms = 0.f;
+    b.fv = fv;
+    b.woff = 0;
+    if (fv) {
+      b.woff = (long)wcall.size();
+      const std::vector<double> wc = fv_cell_widths(n);
+      wcall.insert(wcall.end(), wc.begin(), wc.end());
+    }
     std::vector<SlotRec> recs(list.size());
     for (size_t it = 0; it < list.size(); ++it) {
       const int e = list[it];
@@ -539,9 +566,13 @@ static int upload_amr_tables(nmx_ctx *c) {
     c->buckets.back().doff = (long)dall.size();
     dall.insert(dall.end(), bs.D, bs.D + n * n);
     c->buckets.back().xoff = (long)xball.size();
-    xball.insert(xball.end(), bs.x, bs.x + n);
+    const Basis &bx = fv ? uniform_basis(n) : bs;   // the bucket's points
+    xball.insert(xball.end(), bx.x, bx.x + n);
   }
-  if ((r = dev_upload(&c->dDall, dall)) || (r = dev_upload(&c->dxball, xball))) return r;
+  if (wcall.empty()) wcall.assign(1, 0.0);
+  if ((r = dev_upload(&c->dDall, dall)) || (r = dev_upload(&c->dxball, xball)) ||
+      (r = dev_upload(&c->dwcall, wcall)))
+    return r;
   assign_lanes(c);
   if (jobs1.empty()) jobs1.assign(kJobInts, 0);
   return dev_upload(&c->djobs1, jobs1);
@@ -621,20 +652,18 @@ static int upload_mesh(nmx_ctx *c) {
   c->tcap = 4 * 64;
   if ((r = dev_alloc(&c->dT, c->tcap))) return r;
   if (!c->hT) HIPCHK(hipHostMalloc((void **)&c->hT, c->tcap * sizeof(double), hipHostMallocDefault));
-  c->fvelm = M.e[0].fv != 0;
+  c->fvelm = true;
+  c->anyfv = false;
+  for (auto &E : M.e) {
+    if (E.fv) c->anyfv = true;
+    else c->fvelm = false;
+    if (E.fv && E.n < 3) return fail(NMX_EUNSUP, "FV elements need n >= 3 points per direction");
+  }
+  if (c->anyfv && P(c, "nmx_path") == "perelement")
+    return fail(NMX_EUNSUP, "FV elements run on the plain or batched path");
+  if (c->fvelm && c->amr) c->fvelm = false;   // all FV, but in order buckets
   if (c->fvelm) {
-    if (c->amr || c->np1 < 3) return fail(NMX_EUNSUP, "FV elements need a conforming mesh of one order n >= 3");
-    if (P(c, "nmx_path") != "plain") return fail(NMX_EUNSUP, "FV elements run on the plain path");
-    // midpoints between the uniform points and the distances around each
-    // point, face to first midpoint at the ends (get_coords.c:1749-1860)
-    const Basis &ub = uniform_basis(c->np1);
-    const int n = c->np1;
-    std::vector<double> xm(n), wc(n);
-    for (int i = 0; i < n - 1; ++i) xm[i] = 0.5 * (ub.x[i] + ub.x[i + 1]);
-    wc[0] = std::fabs(xm[0] - (-1.));
-    wc[n - 1] = std::fabs(xm[n - 2] - 1.);
-    for (int i = 1; i < n - 1; ++i) wc[i] = xm[i] - xm[i - 1];
-    if ((r = dev_upload(&c->dwc, wc))) return r;
+    if ((r = dev_upload(&c->dwc, fv_cell_widths(c->np1)))) return r;
   }
   if (c->amr) return upload_amr_tables(c);
   {
@@ -725,8 +754,6 @@ extern "C" int nmx_mesh_build(nmx_ctx *c) {
     if (!fp.empty()) {
       err = set_fv_patches(c->mesh, fp);
       if (!err.empty()) return fail(NMX_EINVAL, err);
-      for (auto &E : c->mesh.e)
-        if (!E.fv) return fail(NMX_EUNSUP, "meshes mixing DG and FV elements are not built");
       for (auto &Pt : c->mesh.pat)
         if (Pt.type == PAT_CUBEDSHELL && (Pt.xc[0] != 0 || Pt.xc[1] != 0 || Pt.xc[2] != 0))
           return fail(NMX_EUNSUP, "FV elements need the shell centre at the origin");
@@ -974,7 +1001,15 @@ static int launch_bucket(nmx_ctx *c, AmrArgs a, const nmx_ctx::Bucket &b, int di
   a.nlist = b.count;
   a.xcd_map = xcd_map;
   int grid = a.xcd_map ? 8 * ((b.count + 7) / 8) : b.count;
-  hipError_t err = launch_dg_amr(b.n, c->geom, a, grid, div, st);
+  hipError_t err;
+  if (b.fv) {
+    const std::string &fr = P(c, "fv_rec");
+    const int rec = fr == "WENOm5_2" ? 5 : fr == "WENOmZ_2" ? 7 : 3;
+    a.fv_mid = P(c, "coordinates_midpoint_data") != "no";
+    err = launch_fvelm_amr(b.n, rec, c->geom == 0 ? 0 : 2, a, c->dwcall + b.woff, grid, st);
+  } else {
+    err = launch_dg_amr(b.n, c->geom, a, grid, div, st);
+  }
   if (err != hipSuccess) return fail(NMX_EHIP, std::string("stage launch: ") + hipGetErrorString(err));
   return NMX_OK;
 }
@@ -1015,6 +1050,7 @@ static int launch_amr(nmx_ctx *c, const StageArgs &sa) {
     const int div = b.div ? b.div : div0;
     if (int r = faces(b, st)) return r;
     if (perelm) {
+      if (b.fv) return fail(NMX_EUNSUP, "FV elements run on the plain or batched path");
       a.xcd_map = 0;
       a.nlist = 1;
       for (int q = 0; q < b.count; ++q) {
@@ -1103,10 +1139,10 @@ static int filter_prepare(nmx_ctx *c, bool &on) {
     else if (name == "scalarwave1_phi") c->filt_mask[1] |= 16;
   }
   if (!c->filt_mask[0] && !c->filt_mask[1]) return NMX_OK;
-  if (c->fvelm) {
+  if (c->anyfv) {
     if (P(c, "basis_filter_fv") == "yes")
       return fail(NMX_EUNSUP, "the filter on FV elements is not built (set basis_filter_fv = no)");
-    return NMX_OK;   // FV elements are left unfiltered (filter.c:79-80)
+    if (c->fvelm) return NMX_OK;   // FV elements are left unfiltered (filter.c:79-80)
   }
   c->filt_jpow = Pi(c, "basis_expfilter_JacobianPower");
   const double alp = Pd(c, "evolve_filter_alp"), sf = Pd(c, "evolve_filter_s");
@@ -1136,6 +1172,7 @@ static int launch_filters(nmx_ctx *c, double *reg) {
     hipError_t err = hipSuccess;
     if (c->amr) {
       for (const auto &b : c->buckets) {
+        if (b.fv) continue;   // left unfiltered (basis_filter_fv = no, filter.c:79-80)
         fa.rec = b.drec; fa.nlist = b.count;
         fa.F = c->dFilt + (b.n - 2) * kF; fa.xb = c->dxball + b.xoff;
         err = launch_filter(b.n, c->geom, fa, c->stream);
@@ -1251,7 +1288,8 @@ extern "C" int nmx_autotune(nmx_ctx *c) {
     for (auto &b : c->buckets) {
       int best_div = 0;
       float bms = 1e30f;
-      for (int div : {1, 2, 4}) {
+      const std::vector<int> divs = b.fv ? std::vector<int>{1} : std::vector<int>{1, 2, 4};
+      for (int div : divs) {
         if ((r = launch_bucket(c, a, b, div, xcd, c->stream))) return r;   // warm-up
         HIPCHK(hipEventRecord(t0, c->stream));
         for (int s = 0; s < 5 && !r; ++s) r = launch_bucket(c, a, b, div, xcd, c->stream);

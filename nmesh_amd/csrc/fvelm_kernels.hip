@@ -21,10 +21,13 @@
 // node and midpoint of a line (no stored geometry is read).
 #include <hip/hip_runtime.h>
 
+#include "amr.h"
 #include "dg_common.h"
 #include "kernels.h"
 
 namespace nmx {
+
+#define NMX_CONST __attribute__((address_space(4)))
 
 constexpr double kWeno3Eps = 1e-6;    // WENO3_epsilon (rec1d.c:35), u_scale = 1
 constexpr double kWeno5EpsE = 1e-40;  // WENO5_epsilon (rec1d.c:1193)
@@ -230,19 +233,22 @@ __device__ __forceinline__ double patch_X(const FvGeo &g, int d, double xb) {
 // With coordinates_midpoint_data = no (a.fv_mid = 0) the metric and normal
 // of midpoint i0+1/2 are node i0's (coordinates.c:467-483).  On Cartesian
 // elements all of these are the constant dXbdX[DIR] / ooJ and +-e_DIR.
-template <int NP, int REC, int DIR, bool FIRST, int GEOM>
-__device__ __forceinline__ void fv_line(const StageArgs &a, int e, const double *wc, const double *xb,
-                                        const FvGeo &G, int la, int lb, double (*acc)[NP * NP * NP]) {
+// eu: the element's state (5 variables of NP^3 points); face(f, fa, fb, ua)
+// gives the adjacent values at point (fa, fb) of face f.
+template <int NP, int REC, int DIR, bool FIRST, int GEOM, class Face>
+__device__ __forceinline__ void fv_line(const double *eu, const Face &face, int numflux, int fv_mid,
+                                        const double *wc, const double *xb, const FvGeo &G, int la,
+                                        int lb, double (*acc)[NP * NP * NP]) {
   constexpr int NP2 = NP * NP;
   constexpr int NP3 = NP * NP * NP;
   constexpr int D1 = DIR == 0 ? 1 : 0;   // Dir1_norm: la runs along it
   constexpr int D2 = DIR == 2 ? 1 : 2;   // Dir2_norm: lb runs along it
   const int stride = DIR == 0 ? 1 : DIR == 1 ? NP : NP2;
   const int base = DIR == 0 ? (la * NP + lb * NP2) : DIR == 1 ? (la + lb * NP2) : (la + lb * NP);
-  const double *g = a.win + 5L * NP3 * e + base;
+  const double *g = eu + base;
   double lo[4], hi[4];
-  face_value<NP>(a, e, 2 * DIR, la, lb, lo);
-  face_value<NP>(a, e, 2 * DIR + 1, la, lb, hi);
+  face(2 * DIR, la, lb, lo);
+  face(2 * DIR + 1, la, lb, hi);
   double q[4][6], nx[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
@@ -279,7 +285,7 @@ __device__ __forceinline__ void fv_line(const StageArgs &a, int e, const double 
     }
     sojR = sojn;
     nR[0] = nn[0]; nR[1] = nn[1]; nR[2] = nn[2];
-    if (GEOM == 2 && a.fv_mid && i0 < NP - 1) {
+    if (GEOM == 2 && fv_mid && i0 < NP - 1) {
       double ojm;
       X[DIR] = patch_X(G, DIR, 0.5 * (xb[i0] + xb[i0 + 1]));
       shell_metric<DIR>(G, X, sojR, ojm, nR);
@@ -295,10 +301,10 @@ __device__ __forceinline__ void fv_line(const StageArgs &a, int e, const double 
 #pragma unroll
       for (int v = 0; v < 4; ++v) ui[v] = q[v][2];
       if (GEOM == 0) {
-        fnum_dir<DIR>(a.numflux, -1.0, ui, lo, fL);
+        fnum_dir<DIR>(numflux, -1.0, ui, lo, fL);
       } else {
         const double nL[3] = {-nn[0], -nn[1], -nn[2]};
-        fnum_n(a.numflux, nL, ui, lo, fL);
+        fnum_n(numflux, nL, ui, lo, fL);
       }
     }
     double ui[4], ua[4];
@@ -321,8 +327,8 @@ __device__ __forceinline__ void fv_line(const StageArgs &a, int e, const double 
 #pragma unroll
       for (int v = 0; v < 4; ++v) { ui[v] = q[v][2]; ua[v] = hi[v]; }
     }
-    if (GEOM == 0) fnum_dir<DIR>(a.numflux, 1.0, ui, ua, fR);
-    else fnum_n(a.numflux, nR, ui, ua, fR);
+    if (GEOM == 0) fnum_dir<DIR>(numflux, 1.0, ui, ua, fR);
+    else fnum_n(numflux, nR, ui, ua, fR);
     if (i0 == 0) oj0 = ojn;
     else if (i0 == 1) oj1 = ojn;
     if (i0 == NP - 2) ojm2 = ojn;
@@ -360,11 +366,75 @@ __device__ __forceinline__ void fv_line(const StageArgs &a, int e, const double 
   }
 }
 
-template <int NP, int REC, int NT, int GEOM>
-__global__ __launch_bounds__(NT) void fvelm_stage_kernel(StageArgs a, const double *wc) {
+// Outputs of the RK stage (as StageArgs / AmrArgs name them).
+struct FvOut {
+  const double *win, *U, *Ain;
+  double *Aout, *Wout, *R;
+  double cA, cW;
+  int wmode;
+};
+
+// One FV element whose state starts at eu = win + gbase: the three sweeps
+// into acc, then r_{pi,c} = -divf, r_phi = pi and the RK stage.
+template <int NP, int REC, int NT, int GEOM, class Face>
+__device__ __forceinline__ void fv_element(const FvOut &o, long gbase, const Face &face, int numflux,
+                                           int fv_mid, const double *s_wc, const double *s_xb,
+                                           const FvGeo &G, double (*acc)[NP * NP * NP]) {
   constexpr int NP2 = NP * NP;
   constexpr int NP3 = NP * NP * NP;
   constexpr int NC = (NP3 + NT - 1) / NT;
+  const int tid = threadIdx.x;
+  const double *eu = o.win + gbase;
+  const int la = tid % NP, lb = tid / NP;
+  __syncthreads();   // the previous element's epilogue is done with acc
+  if (tid < NP2) fv_line<NP, REC, 0, true, GEOM>(eu, face, numflux, fv_mid, s_wc, s_xb, G, la, lb, acc);
+  __syncthreads();
+  if (tid < NP2) fv_line<NP, REC, 1, false, GEOM>(eu, face, numflux, fv_mid, s_wc, s_xb, G, la, lb, acc);
+  __syncthreads();
+  if (tid < NP2) fv_line<NP, REC, 2, false, GEOM>(eu, face, numflux, fv_mid, s_wc, s_xb, G, la, lb, acc);
+  __syncthreads();
+  // ---- epilogue: r_{pi,c} = -divf, r_phi = pi, the RK stage ----
+#pragma unroll 1
+  for (int c = 0; c < NC; ++c) {
+    const int node = tid + c * NT;
+    if (node >= NP3) continue;
+    const long gi = gbase + node;
+    double r[5];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) r[v] = -acc[v][node];
+    r[4] = o.win[gi];
+    double Ubuf[5], Abuf[5];
+    if (o.wmode) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Ubuf[v] = o.U[gi + v * NP3];
+    }
+    if (o.Aout) {
+      const double *bas = o.Ain ? o.Ain : o.win;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Abuf[v] = bas[gi + v * NP3];
+    }
+    if (o.R) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) o.R[gi + v * NP3] = r[v];
+    }
+    if (o.Aout) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) o.Aout[gi + v * NP3] = Abuf[v] + o.cA * r[v];
+    }
+    if (o.wmode == 1) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) o.Wout[gi + v * NP3] = Ubuf[v] + o.cW * r[v];
+    } else if (o.wmode == 2) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) o.Wout[gi + v * NP3] = 0.75 * Ubuf[v] + 0.25 * (o.win[gi + v * NP3] + o.cW * r[v]);
+    }
+  }
+}
+
+// element-range kernel of meshes that are FV throughout (one order)
+template <int NP, int REC, int NT, int GEOM>
+__global__ __launch_bounds__(NT) void fvelm_stage_kernel(StageArgs a, const double *wc) {
+  constexpr int NP3 = NP * NP * NP;
   __shared__ double acc[4][NP3];   // divf, summed over x, then y, then z
   __shared__ double s_wc[NP];
   __shared__ double s_xb[NP];
@@ -387,9 +457,9 @@ __global__ __launch_bounds__(NT) void fvelm_stage_kernel(StageArgs a, const doub
     e_first = a.e0 + blockIdx.x;
     e_step = gridDim.x;
   }
+  const FvOut o = {a.win, a.U, a.Ain, a.Aout, a.Wout, a.R, a.cA, a.cW, a.wmode};
 
   for (int e = e_first; e < e_hi; e += e_step) {
-    const long ebase = 5L * NP3 * e;
     FvGeo G;
     G.dXbdX[0] = a.dXbdX[3 * e];
     G.dXbdX[1] = a.dXbdX[3 * e + 1];
@@ -402,50 +472,98 @@ __global__ __launch_bounds__(NT) void fvelm_stage_kernel(StageArgs a, const doub
       G.s1 = a.egeo[8 * e + 7];
       G.dom = a.edom[e];
     }
-    const int la = tid % NP, lb = tid / NP;
-    __syncthreads();   // the previous element's epilogue is done with acc
-    if (tid < NP2) fv_line<NP, REC, 0, true, GEOM>(a, e, s_wc, s_xb, G, la, lb, acc);
-    __syncthreads();
-    if (tid < NP2) fv_line<NP, REC, 1, false, GEOM>(a, e, s_wc, s_xb, G, la, lb, acc);
-    __syncthreads();
-    if (tid < NP2) fv_line<NP, REC, 2, false, GEOM>(a, e, s_wc, s_xb, G, la, lb, acc);
-    __syncthreads();
-    // ---- epilogue: r_{pi,c} = -divf, r_phi = pi, the RK stage ----
-#pragma unroll 1
-    for (int c = 0; c < NC; ++c) {
-      const int node = tid + c * NT;
-      if (node >= NP3) continue;
-      const long gi = ebase + node;
-      double r[5];
+    auto face = [&](int f, int fa, int fb, double ua[4]) { face_value<NP>(a, e, f, fa, fb, ua); };
+    fv_element<NP, REC, NT, GEOM>(o, 5L * NP3 * e, face, a.numflux, a.fv_mid, s_wc, s_xb, G, acc);
+  }
+}
+
+// Order-bucket kernel of meshes with DG and FV elements (or AMR): the
+// bucket's elements from their SlotRec; faces copied from a same-kind
+// neighbour (kind 1), interpolated this stage by face_interp_kernel (kind
+// 2), averaged over several neighbours (kind 3), or the exterior (kind 0),
+// as the DG bucket kernel gathers them.
+template <int NP, int REC, int NT, int GEOM>
+__global__ __launch_bounds__(NT) void fvelm_amr_kernel(AmrArgs a, const double *wc) {
+  constexpr int NP2 = NP * NP;
+  constexpr int NP3 = NP * NP * NP;
+  __shared__ double acc[4][NP3];
+  __shared__ double s_wc[NP];
+  __shared__ double s_xb[NP];
+  const int tid = threadIdx.x;
+  for (int q = tid; q < NP; q += NT) {
+    s_wc[q] = wc[q];
+    s_xb[q] = GEOM == 2 ? a.xb[q] : 0.0;
+  }
+  int i_hi, i_step, i_first;
+  if (a.xcd_map) {
+    const int xcd = blockIdx.x & 7, q = blockIdx.x >> 3, g8 = gridDim.x >> 3;
+    const int chunk = (a.nlist + 7) >> 3;
+    const int lo = xcd * chunk;
+    i_hi = min(lo + chunk, a.nlist);
+    i_first = lo + q;
+    i_step = g8;
+  } else {
+    i_hi = a.nlist;
+    i_first = blockIdx.x;
+    i_step = gridDim.x;
+  }
+  const FvOut o = {a.win, a.U, a.Ain, a.Aout, a.Wout, a.R, a.cA, a.cW, a.wmode};
+
+  for (int it = i_first; it < i_hi; it += i_step) {
+    const NMX_CONST SlotRec &R = *(const NMX_CONST SlotRec *)(a.rec + it);
+    const long noff = R.noff;
+    FvGeo G;
+    G.dXbdX[0] = R.dXbdX[0];
+    G.dXbdX[1] = R.dXbdX[1];
+    G.dXbdX[2] = R.dXbdX[2];
+    G.detb = G.dXbdX[0] * G.dXbdX[1] * G.dXbdX[2];
+    if (GEOM == 2) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) r[v] = -acc[v][node];
-      r[4] = a.win[gi];
-      double Ubuf[5], Abuf[5];
-      if (a.wmode) {
-#pragma unroll
-        for (int v = 0; v < 5; ++v) Ubuf[v] = a.U[gi + v * NP3];
-      }
-      if (a.Aout) {
-        const double *bas = a.Ain ? a.Ain : a.win;
-#pragma unroll
-        for (int v = 0; v < 5; ++v) Abuf[v] = bas[gi + v * NP3];
-      }
-      if (a.R) {
-#pragma unroll
-        for (int v = 0; v < 5; ++v) a.R[gi + v * NP3] = r[v];
-      }
-      if (a.Aout) {
-#pragma unroll
-        for (int v = 0; v < 5; ++v) a.Aout[gi + v * NP3] = Abuf[v] + a.cA * r[v];
-      }
-      if (a.wmode == 1) {
-#pragma unroll
-        for (int v = 0; v < 5; ++v) a.Wout[gi + v * NP3] = Ubuf[v] + a.cW * r[v];
-      } else if (a.wmode == 2) {
-#pragma unroll
-        for (int v = 0; v < 5; ++v) a.Wout[gi + v * NP3] = 0.75 * Ubuf[v] + 0.25 * (a.win[gi + v * NP3] + a.cW * r[v]);
-      }
+      for (int d = 0; d < 6; ++d) G.bb[d] = R.bb[d];
+      G.s0 = R.s0;
+      G.s1 = R.s1;
+      G.dom = R.dom;
     }
+    auto face = [&](int f, int fa, int fb, double ua[4]) {
+      const int kind = R.fkind[f];
+      const int q = fa + NP * fb;
+      if (kind == 1) {
+        const double *p = a.win + R.foff[f] + fa * R.fs1[f] + fb * R.fs2[f];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) ua[v] = p[v * NP3];
+      } else if (kind == 2) {
+        const double *p = a.aj + R.foff[f] + q;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) ua[v] = p[v * NP2];
+      } else if (kind == 3) {
+        const int *md = a.mdesc + R.foff[f];
+        const int ns = md[0];
+        double sum[4] = {0.0, 0.0, 0.0, 0.0};
+        int cnt = 0;
+        for (int si = 0; si < ns; ++si) {
+          const int *d = md + 1 + 5 * si;
+          if (fa >= d[1] && fa <= d[2] && fb >= d[3] && fb <= d[4]) {
+            const double *p = a.aj + d[0] + q;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sum[v] += p[v * NP2];
+            ++cnt;
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) ua[v] = cnt > 1 ? sum[v] / cnt : sum[v];
+      } else if (a.profile) {
+        const int dir = f >> 1, pl = (f & 1) * (NP - 1);
+        const int ii = (dir == 0) ? pl : fa;
+        const int jj = (dir == 1) ? pl : ((dir == 0) ? fa : fb);
+        const int kk = (dir == 2) ? pl : fb;
+        const double *xp = a.xyz + 3 * noff + ii + NP * (jj + NP * kk);
+        plane_wave(a.k, a.om, xp[0], xp[NP3], xp[2 * NP3], a.tdev[a.tidx], ua);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) ua[v] = 0.0;
+      }
+    };
+    fv_element<NP, REC, NT, GEOM>(o, 5 * noff, face, a.numflux, a.fv_mid, s_wc, s_xb, G, acc);
   }
 }
 
@@ -459,6 +577,16 @@ static hipError_t launch_fvelm_t(int geom, const StageArgs &a, const double *wc,
   return hipGetLastError();
 }
 
+template <int NP, int REC>
+static hipError_t launch_fvelm_amr_t(int geom, const AmrArgs &a, const double *wc, int grid, hipStream_t s) {
+  constexpr int NT = (NP * NP + 63) / 64 * 64;
+  if (geom == 0)
+    hipLaunchKernelGGL((fvelm_amr_kernel<NP, REC, NT, 0>), dim3(grid), dim3(NT), 0, s, a, wc);
+  else
+    hipLaunchKernelGGL((fvelm_amr_kernel<NP, REC, NT, 2>), dim3(grid), dim3(NT), 0, s, a, wc);
+  return hipGetLastError();
+}
+
 hipError_t launch_fvelm_stage(int np, int rec, int geom, const StageArgs &a, const double *wc, int grid,
                               hipStream_t s) {
 #define NMX_CASE(N)                                                                 \
@@ -466,6 +594,29 @@ hipError_t launch_fvelm_stage(int np, int rec, int geom, const StageArgs &a, con
     return rec == 5   ? launch_fvelm_t<N, 5>(geom, a, wc, grid, s)                  \
            : rec == 7 ? launch_fvelm_t<N, 7>(geom, a, wc, grid, s)                  \
                       : launch_fvelm_t<N, 3>(geom, a, wc, grid, s);
+  switch (np) {
+    NMX_CASE(3)
+    NMX_CASE(4)
+    NMX_CASE(5)
+    NMX_CASE(6)
+    NMX_CASE(7)
+    NMX_CASE(8)
+    NMX_CASE(9)
+    NMX_CASE(10)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef NMX_CASE
+}
+
+hipError_t launch_fvelm_amr(int np, int rec, int geom, const AmrArgs &a, const double *wc, int grid,
+                            hipStream_t s) {
+  if (a.nlist == 0) return hipSuccess;
+#define NMX_CASE(N)                                                                 \
+  case N:                                                                           \
+    return rec == 5   ? launch_fvelm_amr_t<N, 5>(geom, a, wc, grid, s)              \
+           : rec == 7 ? launch_fvelm_amr_t<N, 7>(geom, a, wc, grid, s)              \
+                      : launch_fvelm_amr_t<N, 3>(geom, a, wc, grid, s);
   switch (np) {
     NMX_CASE(3)
     NMX_CASE(4)

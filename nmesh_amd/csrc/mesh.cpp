@@ -164,6 +164,29 @@ static double lagrange(const Basis &b, int q, double x) {
   return prod * b.wb[q];
 }
 
+// piecewise linear basis function q on the points of b (basis_pw_linear,
+// src/main/basis/interpolate.c:53-102): the end intervals reach past the
+// ends; a point on an inner node belongs to the interval on its right
+static double pw_linear(const Basis &b, int q, double x) {
+  const int np = b.n;
+  const double *xp = b.x;
+  if (np <= 1) return 1.0;
+  if (x < xp[1]) {
+    if (q == 0) return (x - xp[1]) / (xp[0] - xp[1]);
+    if (q == 1) return (x - xp[0]) / (xp[1] - xp[0]);
+    return 0.0;
+  }
+  if (x >= xp[np - 2]) {
+    const int n0 = np - 2, n1 = np - 1;
+    if (q == n0) return (x - xp[n1]) / (xp[n0] - xp[n1]);
+    if (q == n1) return (x - xp[n0]) / (xp[n1] - xp[n0]);
+    return 0.0;
+  }
+  if (q < np - 1 && x >= xp[q] && x < xp[q + 1]) return (x - xp[q + 1]) / (xp[q] - xp[q + 1]);
+  if (q > 0 && x >= xp[q - 1] && x < xp[q]) return (x - xp[q - 1]) / (xp[q] - xp[q - 1]);
+  return 0.0;
+}
+
 // ------------------------------------------------------------------------
 // coordinate maps: Cartesian box and CubedShell with constant radii
 // (src/main/coordinates/coordtrans_CubedSphere.c:38-509)
@@ -608,7 +631,10 @@ static std::string face_tables(Mesh &M) {
           }
         }
       }
-      // interpolation from every neighbour: separable 1D Lagrange matrices
+      // interpolation from every neighbour: separable 1D matrices, Lagrange
+      // rows from a DG neighbour, piecewise linear ones from an FV
+      // neighbour on uniform points (fv_surface_interp = linear,
+      // surface.c:1182-1245)
       int dir = f / 2, d1, d2;
       dirs_of(dir, &d1, &d2);
       std::vector<InterpSlot> &slots = M.fint[6 * ei + f];
@@ -617,7 +643,8 @@ static std::string face_tables(Mesh &M) {
         int nf = E.nbf[f][ni];
         int od = nf / 2, od1, od2;
         dirs_of(od, &od1, &od2);
-        const Basis &bn = lgl_basis(N.n);
+        const Basis &bn = N.fv ? uniform_basis(N.n) : lgl_basis(N.n);
+        auto row = [&](int q, double x) { return N.fv ? pw_linear(bn, q, x) : lagrange(bn, q, x); };
         // neighbour in-plane Xb coordinates of all my face points
         std::vector<double> C0(n * n), C1(n * n);
         std::vector<char> in(n * n);
@@ -681,15 +708,15 @@ static std::string face_tables(Mesh &M) {
         if (depends_only_on_a(C0) && depends_only_on_b(C1)) {
           s.swap = 0;
           for (int a = s.a0; a <= s.a1; ++a)
-            for (int q = 0; q < m; ++q) s.Ba[a * m + q] = lagrange(bn, q, C0[a + n * s.b0]);
+            for (int q = 0; q < m; ++q) s.Ba[a * m + q] = row(q, C0[a + n * s.b0]);
           for (int b = s.b0; b <= s.b1; ++b)
-            for (int q = 0; q < m; ++q) s.Bb[b * m + q] = lagrange(bn, q, C1[s.a0 + n * b]);
+            for (int q = 0; q < m; ++q) s.Bb[b * m + q] = row(q, C1[s.a0 + n * b]);
         } else if (depends_only_on_b(C0) && depends_only_on_a(C1)) {
           s.swap = 1;   // my a indexes the neighbour's q2 axis
           for (int a = s.a0; a <= s.a1; ++a)
-            for (int q = 0; q < m; ++q) s.Ba[a * m + q] = lagrange(bn, q, C1[a + n * s.b0]);
+            for (int q = 0; q < m; ++q) s.Ba[a * m + q] = row(q, C1[a + n * s.b0]);
           for (int b = s.b0; b <= s.b1; ++b)
-            for (int q = 0; q < m; ++q) s.Bb[b * m + q] = lagrange(bn, q, C0[s.a0 + n * b]);
+            for (int q = 0; q < m; ++q) s.Bb[b * m + q] = row(q, C0[s.a0 + n * b]);
         } else {
           return "non-separable face interpolation is not supported";
         }

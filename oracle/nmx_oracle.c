@@ -121,6 +121,28 @@ static double lagrange_of_x(int k, double x, int np, const double *xp,
   return prod * wi[k];
 }
 
+/* piecewise linear basis function k on points xp (basis_pw_linear,
+   src/main/basis/interpolate.c:53-102): the two end intervals extend past
+   the ends, x in [xp[k], xp[k+1]) belongs to the interval on the right */
+static double pw_linear_of_x(int k, double x, int np, const double *xp)
+{
+  if (np <= 1) return 1.;
+  if (x < xp[1]) {
+    if (k == 0) return (x - xp[1]) / (xp[0] - xp[1]);
+    if (k == 1) return (x - xp[0]) / (xp[1] - xp[0]);
+    return 0.;
+  }
+  if (x >= xp[np - 2]) {
+    int n0 = np - 2, n1 = np - 1;
+    if (k == n0) return (x - xp[n1]) / (xp[n0] - xp[n1]);
+    if (k == n1) return (x - xp[n0]) / (xp[n1] - xp[n0]);
+    return 0.;
+  }
+  if (k < np - 1 && x >= xp[k] && x < xp[k + 1]) return (x - xp[k + 1]) / (xp[k] - xp[k + 1]);
+  if (k > 0 && x >= xp[k - 1] && x < xp[k]) return (x - xp[k - 1]) / (xp[k] - xp[k - 1]);
+  return 0.;
+}
+
 static OBasis *basis(int n)
 {
   OBasis *b = &g_basis[n];
@@ -813,6 +835,15 @@ void orc_mesh_set_fv(OMesh *M, int use_fv)
   mesh_geometry(M);
 }
 
+/* FV on uniform points in the patches of bit mask patmask, DG elsewhere
+   (amr_uniform_p = amr_fv_p naming some patches, mesh.c:40-80) */
+void orc_mesh_set_fv_patches(OMesh *M, int patmask)
+{
+  for (int e = 0; e < M->nelem; e++) M->e[e].use_fv = (patmask >> M->e[e].p) & 1;
+  set_ajmodes(M);
+  mesh_geometry(M);
+}
+
 void orc_mesh_free(OMesh *M)
 {
   if (!M) return;
@@ -1181,13 +1212,20 @@ static void build_ajsurf(const OMesh *M, const OElm *E, int f, const double *pla
                      (N->bbox[2 * nd1 + 1] - N->bbox[2 * nd1]);
         double Cb1 = (2. * XN[nd2] - (N->bbox[2 * nd2 + 1] + N->bbox[2 * nd2])) /
                      (N->bbox[2 * nd2 + 1] - N->bbox[2 * nd2]);
-        /* 2D tensor Lagrange interpolation (src/main/basis/interpolate.c:296-363) */
+        /* 2D tensor interpolation (src/main/basis/interpolate.c:296-363):
+           Lagrange from a DG neighbour, piecewise linear from an FV one on
+           uniform points (fv_surface_interp = linear, surface.c:1182-1245) */
         int nn_ = N->n;
         OBasis *bn = basis(nn_);
         double B1[ORC_MAXN], B2[ORC_MAXN];
         for (int q = 0; q < nn_; q++) {
-          B1[q] = lagrange_of_x(q, Cb0, nn_, bn->Xb, bn->WL);
-          B2[q] = lagrange_of_x(q, Cb1, nn_, bn->Xb, bn->WL);
+          if (N->use_fv) {
+            B1[q] = pw_linear_of_x(q, Cb0, nn_, bn->Xu);
+            B2[q] = pw_linear_of_x(q, Cb1, nn_, bn->Xu);
+          } else {
+            B1[q] = lagrange_of_x(q, Cb0, nn_, bn->Xb, bn->WL);
+            B2[q] = lagrange_of_x(q, Cb1, nn_, bn->Xb, bn->WL);
+          }
         }
         const double *np_ = planes + v * nf + M->foff[6 * E->fnb[f][ni] + nbf];
         double s = 0.;
@@ -1199,6 +1237,19 @@ static void build_ajsurf(const OMesh *M, const OElm *E, int f, const double *pla
       if (cnt > 1) sum /= cnt;
       aj[a + n * b] = sum;
     }
+}
+
+/* the adjacent values ajsurf of variable v on face f of element e for state
+   u (set_ajsurf_vl, surface.c:799-1358): out[a + n b] */
+void orc_face_values(const OMesh *M, const double *u, int e, int f, int v, double *out)
+{
+  double *planes = malloc(sizeof(double) * NV * M->nface);
+  for (int e2 = 0; e2 < M->nelem; e2++)
+    for (int f2 = 0; f2 < 6; f2++)
+      for (int v2 = 0; v2 < NV; v2++)
+        set_mysurf(M, &M->e[e2], f2, u, v2, planes + v2 * M->nface + M->foff[6 * e2 + f2]);
+  build_ajsurf(M, &M->e[e], f, planes, v, out);
+  free(planes);
 }
 
 /* surface part: numerical flux and lift on each face node, then r -= divf
@@ -1739,6 +1790,7 @@ void orc_filter(const OMesh *M, double *u, int varmask, double alp, double s, in
 #pragma omp parallel for schedule(dynamic, 4)
   for (int e = 0; e < M->nelem; e++) {
     const OElm *E = &M->e[e];
+    if (E->use_fv) continue;   /* basis_filter_fv = no (filter.c:79-80) */
     int n = E->n, np = elm_np(E);
     OBasis *b = basis(n);
     double AT[ORC_MAXN * ORC_MAXN], ST[ORC_MAXN * ORC_MAXN], f[ORC_MAXN];

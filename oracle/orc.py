@@ -51,6 +51,8 @@ def lib():
             "orc_filter": (None, [vp, vp, i, d, d, i, i]),
             "orc_num_threads": (i, []),
             "orc_mesh_set_fv": (None, [vp, i]),
+            "orc_mesh_set_fv_patches": (None, [vp, i]),
+            "orc_face_values": (None, [vp, vp, i, i, i, vp]),
             "orc_set_fv_rec": (None, [i]),
             "orc_set_fv_midpoint": (None, [i]),
             "orc_dnfn_extrap1": (None, [i, vp]),
@@ -89,7 +91,8 @@ METHOD = {"RK4": 0, "sspRK3": 1, "Euler": 2}
 class OMesh:
     """Oracle mesh built from nmesh parameter values."""
 
-    def __init__(self, kind, luni, n, dout=1.0, rin=2.0, rout=20.0, radius=10.0, levels=0, fv=False):
+    def __init__(self, kind, luni, n, dout=1.0, rin=2.0, rout=20.0, radius=10.0, levels=0, fv=False,
+                 fv_patches=None):
         L = lib()
         if kind == "box":
             self.h = L.orc_mesh_box(int(luni), int(n), float(dout), None)
@@ -102,6 +105,8 @@ class OMesh:
             raise ValueError(kind)
         if fv:
             L.orc_mesh_set_fv(self.h, 1)
+        elif fv_patches:
+            L.orc_mesh_set_fv_patches(self.h, sum(1 << p for p in fv_patches))
 
     def __del__(self):
         try:

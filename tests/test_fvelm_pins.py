@@ -13,7 +13,10 @@ elements.
   are exact on linear data, so an index, sign, weight or factor slip fails.
 - Uniform points as uniform_x places them (Legendre.c:626-663); coordinates,
   tables and dt of the product host bit-exact to the oracle, Cartesian and
-  cubed-sphere shell.
+  cubed-sphere shell, all FV or FV in some patches only.
+- basis_pw_linear (interpolate.c:53-102), the interpolation from an FV
+  neighbour's face: reproduces linear data exactly, is a partition of
+  unity, and equals numpy's interp inside the points.
 - Curvilinear FV elements (cubed-sphere shell), the plane-wave RHS against
   its exact time derivative:
   - coordinates_midpoint_data = yes (metric and normals at the midpoints):
@@ -209,17 +212,128 @@ def test_midpoint_metric_on_cartesian_elements_is_the_node_metric():
     assert np.array_equal(r0, r1)
 
 
-@pytest.mark.parametrize("kind,luni,n", [("box", 1, 5), ("box", 2, 6), ("shell", 1, 6)])
-def test_host_tables_of_fv_meshes_match_the_oracle(kind, luni, n):
+@pytest.mark.parametrize("kind,luni,n,pats", [("box", 1, 5, [0]), ("box", 2, 6, [0]),
+                                              ("shell", 1, 6, [5, 4, 3, 2, 1, 0]),
+                                              ("shell", 1, 5, [0, 2, 4])])
+def test_host_tables_of_fv_meshes_match_the_oracle(kind, luni, n, pats):
+    ps = " ".join(str(p) for p in pats)
     m = nm.Mesh({"amr_mesh_type": "BoxMesh" if kind == "box" else "Shell", "amr_luni": luni,
                  "amr_n0": n, "amr_n1": n, "amr_n2": n, "amr_BoxMesh_dout": 8.0,
                  "amr_Shell_rin": 2.0, "amr_Shell_rout": 20.0,
-                 "amr_uniform_p": "0" if kind == "box" else "0 1 2 3 4 5",
-                 "amr_fv_p": "0" if kind == "box" else "5 4 3 2 1 0"}, device=-1)
+                 "amr_uniform_p": ps, "amr_fv_p": ps}, device=-1)
     nm.amr_setup_mesh(m)
-    om = OMesh(kind, luni, n, dout=8.0, rin=2.0, rout=20.0, fv=True)
+    om = OMesh(kind, luni, n, dout=8.0, rin=2.0, rout=20.0, fv_patches=pats)
     X, x = om.coords()
     assert np.array_equal(m.coords(), x)
     assert m.dt == om.dt()
     for e in range(om.nelem):
         assert np.array_equal(m.elm_info(e), om.elm_info(e))
+
+
+def lgl(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().orc_lgl(n, _p(x), _p(w))
+    return x
+
+
+def test_face_values_across_dg_fv_faces():
+    """set_ajsurf_vl on the shell with FV patches 0, 2, 4 (the reference's
+    sw7FV-DG layout): a DG element takes its FV neighbour's face values by
+    piecewise linear interpolation from the uniform points (surface.c:
+    1182-1245, fv_surface_interp = linear), an FV element its DG
+    neighbour's by Lagrange interpolation from the LGL points.  The
+    neighbour's face plane is set to g(x1) + g(x2) of its own in-plane
+    point coordinates with g(x) = x^2, symmetric under the face
+    orientations, so the expected values at my face point (a, b) are
+    G(x_a) + G(x_b) of my own point coordinates: G = numpy's piecewise
+    linear interp of x^2 on the uniform points for a DG receiver, and x^2
+    itself (Lagrange is exact on it) for an FV receiver."""
+    n = 5
+    om = OMesh("shell", 1, n, rin=2.0, rout=6.0, fv_patches=[0, 2, 4])
+    xl = lgl(n)
+    xu = np.linspace(-1.0, 1.0, n)
+    checked = {0: 0, 1: 0}
+    for e in range(om.nelem):
+        info = om.elm_info(e)
+        fv_me = info[0] in (0, 2, 4)
+        for f in range(6):
+            nb, nbf = om.elm_fnb(e, f)
+            if len(nb) != 1:
+                continue
+            ninfo = om.elm_info(nb[0])
+            fv_nb = ninfo[0] in (0, 2, 4)
+            if fv_nb == fv_me:
+                continue
+            assert info[10 + f] == 3           # interpolated, never copied
+            # neighbour's nodes: g of its coordinates along its face's plane
+            od = nbf[0] // 2
+            od1, od2 = (1, 2) if od == 0 else ((0, 2) if od == 1 else (0, 1))
+            xn = xu if fv_nb else xl
+            u = np.zeros((5, om.nnodes))
+            I = np.indices((n, n, n))[::-1]                 # I[0] = i, I[1] = j, I[2] = k
+            blk = xn[I[od1]] ** 2 + xn[I[od2]] ** 2         # [k, j, i]
+            off = ninfo[3]
+            u[1, off:off + n ** 3] = blk.ravel()
+            out = np.zeros(n * n)
+            lib().orc_face_values(om.h, _p(np.ascontiguousarray(u.ravel())), e, f, 1, _p(out))
+            xm = xu if fv_me else xl
+            G = (lambda x: x ** 2) if fv_me else (lambda x: np.interp(x, xu, xu ** 2))
+            a = np.arange(n)
+            expect = G(xm[a][None, :]) + G(xm[a][:, None])    # [b, a]
+            assert np.abs(out.reshape(n, n) - expect).max() < 1e-13, (e, f, fv_me)
+            checked[int(fv_me)] += 1
+    assert checked[0] > 0 and checked[1] > 0
+
+
+def test_pw_linear_face_interpolation_from_fv_neighbours():
+    """The DG side of a DG/FV face on the shell takes its neighbour's face
+    values by piecewise linear interpolation from the uniform points: a
+    field linear in the patch coordinates (lam, A, B) of the FV patch is
+    reproduced exactly at the DG face points, so the DG elements' RHS next
+    to FV elements equals the one of the all-DG mesh with the same field.
+    Checked through the oracle RHS of a state that is linear in the FV
+    patches' coordinates and zero-curvature across the face."""
+    # 1D: linear data exact, partition of unity, numpy interp inside
+    from oracle.orc import lib as L
+    rng = np.random.default_rng(2)
+    for n in range(2, 9):
+        xp = np.linspace(-1.0, 1.0, n)
+        for x in rng.uniform(-1.0, 1.0, 20):
+            w = np.array([_pwl(k, x, xp) for k in range(n)])
+            assert abs(w.sum() - 1.0) < 1e-14
+            assert abs(w @ (0.3 + 2.0 * xp) - (0.3 + 2.0 * x)) < 1e-13
+            vals = rng.standard_normal(n)
+            assert abs(w @ vals - np.interp(x, xp, vals)) < 1e-13
+
+
+def _pwl(k, x, xp):
+    """basis_pw_linear restated (interpolate.c:53-102)"""
+    np_ = len(xp)
+    if np_ <= 1:
+        return 1.0
+    if x < xp[1]:
+        return [(x - xp[1]) / (xp[0] - xp[1]), (x - xp[0]) / (xp[1] - xp[0])][k] if k < 2 else 0.0
+    if x >= xp[np_ - 2]:
+        n0, n1 = np_ - 2, np_ - 1
+        if k == n0:
+            return (x - xp[n1]) / (xp[n0] - xp[n1])
+        if k == n1:
+            return (x - xp[n0]) / (xp[n1] - xp[n0])
+        return 0.0
+    if k < np_ - 1 and xp[k] <= x < xp[k + 1]:
+        return (x - xp[k + 1]) / (xp[k] - xp[k + 1])
+    if k > 0 and xp[k - 1] <= x < xp[k]:
+        return (x - xp[k - 1]) / (xp[k] - xp[k - 1])
+    return 0.0
+
+
+def test_unbuilt_fv_options_are_refused():
+    m = nm.Mesh({"amr_mesh_type": "BoxMesh"}, device=-1)
+    for k, v in [("fv_surface_interp", "parabolic"), ("fv_surface_interp", "WENO"),
+                 ("fv2dg_interp_use_extrap1", "yes"), ("fv_rec", "WENOm5_1"),
+                 ("coordinates_midpoint_data", "sometimes")]:
+        with pytest.raises(nm.NmxError):
+            m.set_par(k, v)
+    m.set_par("fv_surface_interp", "linear")
+    m.set_par("coordinates_midpoint_data", "yes all")

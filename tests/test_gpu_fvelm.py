@@ -6,7 +6,9 @@ factors; the numerical flux is the closed form on the device and the
 R-matrix / LLF form in the oracle, so the RHS agrees to ~1e-14 relative.
 The bar is 1e-12 relative to the field maximum, per RHS and per step.
 Cartesian boxes and cubed-sphere shells (metric recomputed on the device at
-nodes and midpoints), coordinates_midpoint_data no and yes.
+nodes and midpoints), coordinates_midpoint_data no and yes; meshes with FV
+elements in some patches and DG in the others (order buckets of both kinds,
+DG/FV faces interpolated), h-refined ones among them.
 """
 import numpy as np
 import pytest
@@ -113,12 +115,104 @@ def test_fv_shell_steps_match_oracle(mid):
         orc.set_fv_midpoint(False)
 
 
+def make_mixed(n, rec, mid="yes", numflux="upwind", levels=0, pats=(0, 2, 4), extra=None):
+    """the shell with FV elements in patches pats and DG elsewhere (the
+    reference's sw7FV-DG layout for 0 2 4), optionally h-refined"""
+    ps = " ".join(str(p) for p in pats)
+    pars = {"amr_mesh_type": "Shell", "amr_luni": 1, "amr_n0": n, "amr_n1": n, "amr_n2": n,
+            "amr_Shell_rin": 2.0, "amr_Shell_rout": 6.0, "amr_hrefine_sphere_radius": 4,
+            "amr_hrefine_sphere_levels": levels, "amr_uniform_p": ps, "amr_fv_p": ps,
+            "fv_rec": rec, "coordinates_midpoint_data": mid, "scalarwave1_numflux": numflux,
+            "scalarwave1_k": "%.17g %.17g %.17g" % tuple(K), "nmx_graph_steps": 2}
+    pars.update(extra or {})
+    m = nm.Mesh(pars, device=0)
+    nm.amr_setup_mesh(m)
+    orc.set_fv_rec(rec)
+    orc.set_fv_midpoint(mid != "no")
+    om = OMesh("shell", 1, n, rin=2.0, rout=6.0, radius=4, levels=levels, fv_patches=list(pats))
+    return m, om
+
+
+@pytest.mark.parametrize("mid", ["no", "yes"])
+@pytest.mark.parametrize("rec,numflux,n,geometry", [("WENOm3_2", "upwind", 6, "analytic"),
+                                                    ("WENOm5_2", "upwind", 7, "analytic"),
+                                                    ("WENOmZ_2", "LLF", 9, "stored")])
+def test_mixed_dg_fv_rhs_matches_oracle(mid, rec, numflux, n, geometry):
+    """DG and FV elements side by side: FV order buckets next to DG ones,
+    DG/FV faces interpolated each stage (Lagrange into FV elements,
+    piecewise linear into DG elements)"""
+    print("mixed dg/fv rhs", mid, rec, numflux, n, geometry, flush=True)
+    m, om = make_mixed(n, rec, mid, numflux, extra={"nmx_geometry": geometry})
+    u, _ = om.profile(K, 0.2)
+    u = u + 0.1 * np.random.default_rng(n).uniform(-1, 1, u.shape)
+    m.upload(u)
+    r = nm.evolve_setrhs_mesh(m, t=0.2)
+    r_o = om.rhs(u, 0.2, K, numflux)
+    orc.set_fv_midpoint(False)
+    assert rel(r, r_o) <= TOL
+
+
+@pytest.mark.parametrize("case", ["refined", "all_fv_batched", "fv_in_1_3_5"])
+def test_fv_buckets_rhs_matches_oracle(case):
+    """FV order buckets elsewhere: an h-refined mixed shell (FV/FV and DG/FV
+    faces with several neighbours), every element FV on the batched path,
+    and the other half of the patches FV"""
+    print("fv buckets rhs", case, flush=True)
+    if case == "refined":
+        m, om = make_mixed(5, "WENOm5_2", levels=1)
+    elif case == "all_fv_batched":
+        m, om = make_mixed(6, "WENOm5_2", pats=(0, 1, 2, 3, 4, 5), extra={"nmx_path": "batched"})
+    else:
+        m, om = make_mixed(6, "WENOm3_2", mid="no", pats=(1, 3, 5))
+    u, _ = om.profile(K, 0.2)
+    u = u + 0.1 * np.random.default_rng(5).uniform(-1, 1, u.shape)
+    m.upload(u)
+    r = nm.evolve_setrhs_mesh(m, t=0.2)
+    r_o = om.rhs(u, 0.2, K, "upwind")
+    orc.set_fv_midpoint(False)
+    assert rel(r, r_o) <= TOL
+
+
+def test_mixed_dg_fv_steps_match_oracle():
+    """RK4 on the mixed shell, every step against the oracle; a captured
+    multi-step graph (order buckets on stream lanes) gives the same states
+    bitwise; the filter with basis_filter_fv = no smooths the DG elements
+    only"""
+    print("mixed dg/fv steps", flush=True)
+    m, om = make_mixed(6, "WENOm5_2")
+    u, _ = om.profile(K, 0.0)
+    m.upload(u)
+    t = 0.0
+    try:
+        for s in range(3):
+            u, t = om.evolve(u, m.dt, 1, t, K, "RK4")
+            nm.evolve_myln(m, 1)
+            assert rel(m.download(), u) <= TOL, "step %d" % s
+        m2, _ = make_mixed(6, "WENOm5_2")
+        m2.upload(om.profile(K, 0.0)[0])
+        nm.evolve_myln(m2, 3)
+        assert np.array_equal(m2.download(), m.download())
+        # filter: refused on FV elements unless basis_filter_fv = no
+        m.set_par("evolve_filter_all", "yes")
+        m.set_par("evolve_filter_alp", "4")
+        m.set_par("evolve_filter_s", "6")
+        with pytest.raises(nm.NmxError):
+            nm.evolve_filter_evosys_mesh(m)
+        m.set_par("basis_filter_fv", "no")
+        v = u + 0.05 * np.random.default_rng(6).uniform(-1, 1, u.shape)
+        m.upload(v)
+        nm.evolve_filter_evosys_mesh(m)
+        assert rel(m.download(), om.filter(v, 31, 4.0, 6.0, 0, 1)) <= TOL
+    finally:
+        orc.set_fv_midpoint(False)
+
+
 def test_fv_elements_refuse_unbuilt_configurations():
     pars = {"amr_mesh_type": "Shell", "amr_luni": 1, "amr_n0": 5, "amr_n1": 5, "amr_n2": 5,
-            "amr_uniform_p": "0 2 4", "amr_fv_p": "0 2 4"}
+            "amr_uniform_p": "0 2 4", "amr_fv_p": "0 2 4", "nmx_path": "perelement"}
     m = nm.Mesh(pars, device=0)
     with pytest.raises(nm.NmxError):
-        nm.amr_setup_mesh(m)          # DG and FV elements side by side
+        nm.amr_setup_mesh(m)          # FV elements on the per-element path
     with pytest.raises(nm.NmxError):
         m.set_par("coordinates_midpoint_data", "maybe")
     m = nm.Mesh({"amr_mesh_type": "BoxMesh", "amr_uniform_p": "0"}, device=0)
