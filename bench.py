@@ -279,6 +279,42 @@ def filter_leg(nm, reps=100):
     return out
 
 
+def crc_leg(nm, reps=50):
+    """The checkpoint's "vars" CRC-64 of the state register on the device
+    (crc_elm_kernel + crc_xor_kernel), on the config-4 AMR shell (19,200
+    elements, N in 4..10, 8,213,159 nodes: 328.5 MB of doubles) and the
+    3072-element N=7 shell (62.9 MB).  Algorithmic bytes: the state once,
+    40 B per node.  Wall time per call includes the launch, the stream
+    synchronisation and the 8-byte read-back; the kernel's own duration is in
+    the rocprofv3 summary.  The host CRC of the same bytes (table-driven,
+    one core) is timed beside it on a 64 MB sample."""
+    from nmesh_amd import checkpoint as ck
+    out = {}
+    for name, m in (("amr_shell_19200", c4_mesh(nm, "batched")),
+                    ("shell_3072_n7", nm.Mesh(c2_params(), device=0))):
+        if name == "shell_3072_n7":
+            nm.amr_setup_mesh(m)
+        m.upload(np.random.default_rng(9).standard_normal(5 * m.nnodes))
+        ck.vars_crc(m)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            c = ck.vars_crc(m)
+        per = (time.perf_counter() - t0) / reps
+        nbytes = 40.0 * m.nnodes
+        out[name] = {"nodes": m.nnodes, "bytes": nbytes, "us_per_call": per * 1e6,
+                     "GBps_wall": nbytes / per / 1e9, "crc": "%016x" % c}
+        del m
+    buf = np.random.default_rng(10).integers(0, 256, 64 << 20, dtype=np.uint8).tobytes()
+    t0 = time.perf_counter()
+    ck.crc64(buf)
+    host = time.perf_counter() - t0
+    out["host_crc64_GBps_1core"] = len(buf) / host / 1e9
+    out["workload"] = ("vars CRC-64 (reflected ECMA-182) of a state register on the device, "
+                       "nmesh's checkpoint order (per element phi, pi, cx, cy, cz)")
+    out["kernel"] = "crc_elm_kernel"
+    return out
+
+
 def adjoint_leg(nm, steps=256, every=16, reps=3):
     """Discrete adjoint cost against the forward step.  C5: the config-1 box
     (64 elements, N=4, 8,000 nodes), 256 RK4 steps, psi = exp(-r^2) with
@@ -372,6 +408,8 @@ def main():
     ap.add_argument("--only-adjoint", action="store_true", help="run only the adjoint-cost leg")
     ap.add_argument("--only-filter", action="store_true", help="run only the exponential-filter leg")
     ap.add_argument("--only-fvelm", action="store_true", help="run only the FV-element leg")
+    ap.add_argument("--only-crc", action="store_true", help="run only the checkpoint CRC leg")
+    ap.add_argument("--no-crc", action="store_true", help="skip the checkpoint CRC leg")
     ap.add_argument("--no-fvelm", action="store_true", help="skip the FV-element leg")
     ap.add_argument("--no-filter", action="store_true", help="skip the exponential-filter leg")
     ap.add_argument("--path", default="plain")
@@ -419,6 +457,9 @@ def main():
         return
     if args.only_fvelm:
         print(json.dumps(fvelm_leg(nm)), flush=True)
+        return
+    if args.only_crc:
+        print(json.dumps(crc_leg(nm)), flush=True)
         return
     if args.only_fv:
         print(json.dumps(fv_leg(args.fv_steps, cpu=not args.no_cpu)), flush=True)
@@ -504,6 +545,8 @@ def main():
         out["filter"] = filter_leg(nm)
     if not args.no_fvelm:
         out["fv_elements"] = fvelm_leg(nm)
+    if not args.no_crc:
+        out["checkpoint_crc"] = crc_leg(nm)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x)
     print(json.dumps(out), flush=True)

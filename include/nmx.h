@@ -19,9 +19,15 @@
  *   - steppers: evolve_myln / evolve_RK4_mesh / evolve_sspRK3_mesh /
  *     evolve_Euler_mesh (src/main/evolve/evolve.c:25, RungeKutta.c:19-188).
  *   - time step: update_mesh_myelms_elm_eid_dt (src/main/amr/storage.c:1370-1386).
+ *   - checkpoint variables and CRCs: checkpoint_save_EvoVars /
+ *     checkpoint_load_Vars (src/main/checkpoint/checkpoint_save.c:337-468,
+ *     checkpoint_load.c:396-700), nmesh_CRCs (src/main/main/norms.c:480-520).
  */
 #ifndef NMX_H
 #define NMX_H
+
+#include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -34,6 +40,7 @@ extern "C" {
 #define NMX_EHIP (-3)     /* HIP runtime error */
 #define NMX_ENOMEM (-4)   /* allocation failed */
 #define NMX_EUNSUP (-5)   /* configuration not supported */
+#define NMX_EIO (-6)      /* a file could not be opened or written */
 
 /* state registers */
 #define NMX_REG_U 0       /* evolved variables u (scalarwave1 vlu) */
@@ -129,6 +136,29 @@ int nmx_evolve(nmx_ctx *ctx, int nsteps);
  * after every full step when evolve_filter_all or evolve_filter_varlist asks
  * for it (main.c:628-630).  No filter requested: does nothing. */
 int nmx_filter(nmx_ctx *ctx, int reg);
+
+/* Checkpoint variables file and CRCs.
+ * nmx_checkpoint_save_vars writes register reg as nmesh's variables.bin:
+ * scalarwave1's evolved variables (phi, pi, cx, cy, cz) of every element,
+ * little-endian doubles, under nmesh's element names
+ * (checkpoint_save_EvoVars / checkpoint_write_vl, checkpoint_save.c:337-468).
+ * nmx_checkpoint_load_vars reads one into register reg
+ * (checkpoint_load_Vars, checkpoint_load.c:396-700): every element of the
+ * mesh must be present with its five variables and point count; variables
+ * of other names are skipped; on any error (NMX_EINVAL) the register is
+ * left unchanged.  Only the variables: time, mesh and parameters stay the
+ * caller's (nmx_set_time).
+ * nmx_vars_crc computes on the device the "vars" CRC of nmesh_CRCs for
+ * register reg (norms.c:440-458 crc_VarList_local; what
+ * checkpoint_save_CRCs writes as "vars:"), fewer than 2^32 nodes.
+ * nmx_crc64 and nmx_crc64_combine are nmesh's crc64_continue and
+ * crc64_0start_combine (crc64.c:153-256): reflected CRC-64, ECMA-182
+ * polynomial, pre- and post-inverted. */
+int nmx_checkpoint_save_vars(nmx_ctx *ctx, int reg, const char *path);
+int nmx_checkpoint_load_vars(nmx_ctx *ctx, int reg, const char *path);
+int nmx_vars_crc(nmx_ctx *ctx, int reg, uint64_t *crc);
+uint64_t nmx_crc64(uint64_t crc, const void *buf, size_t nbytes);
+uint64_t nmx_crc64_combine(uint64_t crc1, uint64_t crc2, uint64_t nbytes2);
 
 /* device pointer and length (doubles) of a register, for zero-copy use from
  * frameworks; valid until the mesh is rebuilt */

@@ -15,6 +15,7 @@
 
 #include "../../include/nmx.h"
 #include "amr.h"
+#include "ckpt.h"
 #include "errors.h"
 #include "kernels.h"
 #include "mesh.h"
@@ -113,6 +114,9 @@ struct nmx_ctx {
   bool fvelm = false, anyfv = false;
   double *dwc = nullptr;    // distances between midpoints around each point
   double *dwcall = nullptr; // the same per FV bucket
+  // checkpoint CRC: device tables, per-element contributions, result
+  uint64_t *dcrctab = nullptr, *dcrcvtab = nullptr, *dcrcklane = nullptr, *dcrcaw = nullptr,
+           *dcrcypow = nullptr, *dcrcpart = nullptr, *dcrcout = nullptr;
   long *deoff = nullptr;
   int *den = nullptr;
   hipGraph_t agraph = nullptr;
@@ -235,7 +239,10 @@ static void free_mesh_dev(nmx_ctx *c) {
   c->graph_steps = 0;
   {
     void **ptrs[] = {(void **)&c->dL, (void **)&c->dnbf, (void **)&c->deoff, (void **)&c->den,
-                     (void **)&c->dCk, (void **)&c->dwc, (void **)&c->dwcall};
+                     (void **)&c->dCk, (void **)&c->dwc, (void **)&c->dwcall,
+                     (void **)&c->dcrctab, (void **)&c->dcrcvtab, (void **)&c->dcrcklane,
+                     (void **)&c->dcrcaw, (void **)&c->dcrcypow,
+                     (void **)&c->dcrcpart, (void **)&c->dcrcout};
     for (auto q : ptrs)
       if (*q) { (void)hipFree(*q); *q = nullptr; }
   }
@@ -1196,6 +1203,71 @@ extern "C" int nmx_filter(nmx_ctx *c, int reg) {
   int r = filter_prepare(c, on);
   if (r || !on) return r;
   return launch_filters(c, u);
+}
+
+// ---------------------------------------------------------------------------
+// checkpoint: variables.bin and the "vars" CRC of nmesh_CRCs
+// ---------------------------------------------------------------------------
+
+extern "C" uint64_t nmx_crc64(uint64_t crc, const void *buf, size_t nbytes) {
+  return crc64(crc, buf, nbytes);
+}
+
+extern "C" uint64_t nmx_crc64_combine(uint64_t crc1, uint64_t crc2, uint64_t nbytes2) {
+  return crc64_combine(crc1, crc2, nbytes2);
+}
+
+extern "C" int nmx_vars_crc(nmx_ctx *c, int reg, uint64_t *crc) {
+  NEED_DEV(c);
+  double *u = reg_ptr(c, reg);
+  if (!u || !crc) return fail(NMX_EINVAL, "bad register or NULL result");
+  if (c->mesh.nnodes >= (1L << 32)) return fail(NMX_EUNSUP, "the device CRC takes fewer than 2^32 nodes");
+  HIPCHK(hipSetDevice(c->device));
+  int r;
+  if (!c->dcrctab) {
+    std::vector<uint64_t> tab, vtab, klane, aw, ypow;
+    crc_device_tables(tab, vtab, klane, aw, ypow);
+    if ((r = dev_upload(&c->dcrctab, tab)) || (r = dev_upload(&c->dcrcvtab, vtab)) ||
+        (r = dev_upload(&c->dcrcklane, klane)) || (r = dev_upload(&c->dcrcaw, aw)) ||
+        (r = dev_upload(&c->dcrcypow, ypow)) || (r = dev_alloc(&c->dcrcpart, c->mesh.e.size())) ||
+        (r = dev_alloc(&c->dcrcout, 1)))
+      return r;
+  }
+  CrcArgs a;
+  a.u = u; a.eoff = c->deoff; a.en = c->den; a.nelem = (int)c->mesh.e.size(); a.nnodes = c->mesh.nnodes;
+  a.tab = c->dcrctab; a.vtab = c->dcrcvtab; a.klane = c->dcrcklane; a.aw = c->dcrcaw;
+  a.ypow = c->dcrcypow; a.part = c->dcrcpart;
+  hipError_t err = launch_vars_crc(a, c->dcrcout, c->stream);
+  if (err != hipSuccess) return fail(NMX_EHIP, std::string("crc launch: ") + hipGetErrorString(err));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(crc, c->dcrcout, sizeof(uint64_t), hipMemcpyDeviceToHost));
+  return NMX_OK;
+}
+
+extern "C" int nmx_checkpoint_save_vars(nmx_ctx *c, int reg, const char *path) {
+  NEED_DEV(c);
+  double *u = reg_ptr(c, reg);
+  if (!u || !path) return fail(NMX_EINVAL, "bad register or NULL path");
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<double> h(c->nstate);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(h.data(), u, c->nstate * sizeof(double), hipMemcpyDeviceToHost));
+  std::string err = write_variables_bin(c->mesh, h.data(), path);
+  if (!err.empty()) return fail(NMX_EIO, err);
+  return NMX_OK;
+}
+
+extern "C" int nmx_checkpoint_load_vars(nmx_ctx *c, int reg, const char *path) {
+  NEED_DEV(c);
+  double *u = reg_ptr(c, reg);
+  if (!u || !path) return fail(NMX_EINVAL, "bad register or NULL path");
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<double> h(c->nstate, 0.0);
+  std::string err = read_variables_bin(c->mesh, h.data(), path);
+  if (!err.empty()) return fail(NMX_EINVAL, err);   // the register is left as it was
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(u, h.data(), c->nstate * sizeof(double), hipMemcpyHostToDevice));
+  return NMX_OK;
 }
 
 // issue the stage launches of one step; stage times at dT[tbase + s]

@@ -1,6 +1,7 @@
 // kernels.h -- launch interface between the libnmx host code and its kernels.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 
 namespace nmx {
 
@@ -66,6 +67,23 @@ hipError_t launch_dg_adjoint(int np, int geom, const AdjArgs &a, int grid, hipSt
 // from egeo, edom and the uniform points in xb)
 hipError_t launch_fvelm_stage(int np, int rec, int geom, const StageArgs &a, const double *wc, int grid,
                               hipStream_t s);
+
+// The checkpoint's "vars" CRC-64 of a register (element blocks [e][5][np_e]).
+struct CrcArgs {
+  const double *u = nullptr;
+  const long *eoff = nullptr;      // first node per element
+  const int *en = nullptr;         // points per direction per element
+  int nelem = 0;
+  long nnodes = 0;
+  const uint64_t *tab = nullptr;   // [8][256] slicing-by-8 tables: S(w) = w x^64 mod P
+  const uint64_t *vtab = nullptr;  // [8][256]: (byte b at position j) x^4096 mod P
+  const uint64_t *klane = nullptr; // [n - 2][64]: x^(64 (5 n^3 - 1 - lane l's last word)) mod P
+  const uint64_t *aw = nullptr;    // [n - 2]: (~0) x^(64 * 5 n^3) mod P
+  const uint64_t *ypow = nullptr;  // [4][256]: x^(320 * d * 256^j) mod P
+  uint64_t *part = nullptr;        // [nelem] element contributions
+};
+// out[0] = CRC (device memory)
+hipError_t launch_vars_crc(const CrcArgs &a, uint64_t *out, hipStream_t s);
 
 // flat [v][node] <-> element blocks [e][v][np_e] on the device
 // (to_blocks: 1 flat -> blocks, 0 blocks -> flat)

@@ -18,6 +18,7 @@
  * (src/tests/scalarwave1/scalarwave1.c:670-672).
  */
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1822,4 +1823,130 @@ void orc_filter(const OMesh *M, double *u, int varmask, double alp, double s, in
     }
     free(c); free(t);
   }
+}
+
+/* ------------------------------------------------------------------ */
+/* Checkpoint: CRC-64 and variables.bin (src/main/main/crc64.c,        */
+/* src/main/main/norms.c:331-520, src/main/checkpoint/                 */
+/* checkpoint_save.c:337-475)                                          */
+/* ------------------------------------------------------------------ */
+
+/* reflected CRC-64 with the ECMA-182 polynomial: 0x42f0e1eba9ea3693
+   bit-reversed is 0xc96c5795d7870f42 (crc64.c:9-17) */
+#define ORC_CRC64_POLY 0xc96c5795d7870f42ULL
+static uint64_t g_crc_tab[256];
+static int g_crc_tab_set = 0;
+
+/* table of the byte-wise reflected CRC: entry b is b shifted through 8
+   steps of the polynomial division */
+static void crc_table(void)
+{
+  if (g_crc_tab_set) return;
+  for (int b = 0; b < 256; b++) {
+    uint64_t r = (uint64_t)b;
+    for (int k = 0; k < 8; k++) r = (r & 1) ? (r >> 1) ^ ORC_CRC64_POLY : (r >> 1);
+    g_crc_tab[b] = r;
+  }
+  g_crc_tab_set = 1;
+}
+
+/* crc64_continue (crc64.c:153-165): pre- and post-inverted */
+uint64_t orc_crc64(uint64_t crc, const void *buf, size_t nbytes)
+{
+  crc_table();
+  const uint8_t *d = (const uint8_t *)buf;
+  crc = ~crc;
+  while (nbytes--) crc = g_crc_tab[(crc ^ *d++) & 0xff] ^ (crc >> 8);
+  return ~crc;
+}
+
+/* a*b mod P in the reflected representation (bit 63 is x^0) */
+static uint64_t gf2_mulmod(uint64_t a, uint64_t b)
+{
+  uint64_t r = 0;
+  for (int i = 63; i >= 0; i--) {
+    if ((a >> i) & 1) r ^= b;
+    b = (b & 1) ? (b >> 1) ^ ORC_CRC64_POLY : (b >> 1);
+  }
+  return r;
+}
+
+/* x^n mod P, reflected, by square-and-multiply */
+static uint64_t gf2_xpow(uint64_t n)
+{
+  uint64_t r = 1ULL << 63, sq = 1ULL << 62;   /* x^0, x^1 */
+  while (n) {
+    if (n & 1) r = gf2_mulmod(r, sq);
+    sq = gf2_mulmod(sq, sq);
+    n >>= 1;
+  }
+  return r;
+}
+
+/* crc64_0start_combine (crc64.c:251-256): the CRC of the concatenation of
+   two buffers whose CRCs both started from 0, the second nbytes2 long */
+uint64_t orc_crc64_combine(uint64_t crc1, uint64_t crc2, uint64_t nbytes2)
+{
+  return crc2 ^ gf2_mulmod(crc1, gf2_xpow(8 * nbytes2));
+}
+
+/* scalarwave1's evolved variables in the reference's variable-index order
+   (AddEvoVar order, nmesh_scalarwave1.c:20-22): phi, pi, cx, cy, cz; the
+   state here is indexed pi, cx, cy, cz, phi */
+static const int g_ckpt_var[5] = {4, 0, 1, 2, 3};
+static const char *g_ckpt_name[5] = {"scalarwave1_phi", "scalarwave1_pi", "scalarwave1_cx",
+                                     "scalarwave1_cy", "scalarwave1_cz"};
+
+/* the "vars" CRC of nmesh_CRCs (norms.c:440-458 crc_VarList_local): every
+   element in order, each variable of the list in order, its doubles as
+   stored */
+uint64_t orc_vars_crc(const OMesh *M, const double *u)
+{
+  long nn = M->nnodes;
+  uint64_t crc = 0;
+  for (int e = 0; e < M->nelem; e++) {
+    const OElm *E = &M->e[e];
+    for (int k = 0; k < 5; k++)
+      crc = orc_crc64(crc, u + g_ckpt_var[k] * nn + E->off, sizeof(double) * elm_np(E));
+  }
+  return crc;
+}
+
+/* variables.bin as checkpoint_write_vl writes it (checkpoint_save.c:
+   412-468): the variable list, then per element "{", its name
+   "<patch>_<location digits>" (elmname, connections.c:819-866), np, and per
+   variable its list index and np little-endian doubles; "EndOfFile." */
+int orc_write_variables_bin(const OMesh *M, const double *u, const char *path)
+{
+  FILE *fp = fopen(path, "wb");
+  if (!fp) return -1;
+  long nn = M->nnodes;
+  fprintf(fp, "number of variables and variable list in this file:\n");
+  fprintf(fp, "%d\n", 5);
+  for (int k = 0; k < 5; k++) fprintf(fp, "%s\n", g_ckpt_name[k]);
+  fprintf(fp, "\n\n");
+  fprintf(fp, "variable-list-data:\n\n");
+  for (int e = 0; e < M->nelem; e++) {
+    const OElm *E = &M->e[e];
+    int np = elm_np(E);
+    fprintf(fp, "{\n");
+    fprintf(fp, "%d_%.*s\n", E->p, E->l, E->loc);
+    fprintf(fp, "%d\n", np);
+    for (int k = 0; k < 5; k++) {
+      fprintf(fp, "%d\n", k);
+      const double *v = u + g_ckpt_var[k] * nn + E->off;
+      for (int q = 0; q < np; q++) {
+        uint64_t bits;
+        memcpy(&bits, v + q, 8);
+        unsigned char b[8];
+        for (int i = 0; i < 8; i++) b[i] = (unsigned char)(bits >> (8 * i));
+        fwrite(b, 1, 8, fp);
+      }
+      fprintf(fp, "\n");
+    }
+    fprintf(fp, "}\n");
+    fprintf(fp, "\n");
+  }
+  fprintf(fp, "EndOfFile.\n");
+  return fclose(fp) ? -1 : 0;
 }

@@ -26,6 +26,7 @@ def lib():
             build()
         L = C.CDLL(LIB)
         vp, i, l, d = C.c_void_p, C.c_int, C.c_long, C.c_double
+        u64, sz, cp = C.c_uint64, C.c_size_t, C.c_char_p
         sig = {
             "orc_lgl": (None, [i, vp, vp]),
             "orc_dt_matrix": (None, [i, vp]),
@@ -53,6 +54,10 @@ def lib():
             "orc_mesh_set_fv": (None, [vp, i]),
             "orc_mesh_set_fv_patches": (None, [vp, i]),
             "orc_face_values": (None, [vp, vp, i, i, i, vp]),
+            "orc_crc64": (u64, [u64, vp, sz]),
+            "orc_crc64_combine": (u64, [u64, u64, u64]),
+            "orc_vars_crc": (u64, [vp, vp]),
+            "orc_write_variables_bin": (i, [vp, vp, cp]),
             "orc_set_fv_rec": (None, [i]),
             "orc_set_fv_midpoint": (None, [i]),
             "orc_dnfn_extrap1": (None, [i, vp]),
@@ -182,6 +187,17 @@ class OMesh:
         t = lib().orc_evolve(self.h, NUMFLUX[numflux], _p(k), METHOD[method], float(dt),
                              int(nsteps), float(t0), _p(u))
         return u, t
+
+    def vars_crc(self, u):
+        """the reference's "vars" CRC-64 of state u (norms.c:440-458)"""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        return int(lib().orc_vars_crc(self.h, _p(u)))
+
+    def write_variables_bin(self, u, path):
+        """variables.bin of state u (checkpoint_save.c:412-468)"""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if lib().orc_write_variables_bin(self.h, _p(u), str(path).encode()) != 0:
+            raise OSError("orc_write_variables_bin failed: %s" % path)
 
     def filter(self, u, varmask=31, alp=36.0, s=32.0, dn=0, jpow=1):
         """exponential filter of the variables in varmask (bit v of pi, cx,
