@@ -85,9 +85,24 @@ int nmx_get_param(nmx_ctx *ctx, const char *name, char *buf, int buflen);
  * parameters (replaces amr_setup_mesh, mesh.c:181-319, and coordinates_init,
  * coordinates.c:133-443).  Sets mesh time 0 and dt per the dt parameter. */
 int nmx_mesh_build(nmx_ctx *ctx);
-/* p-refine: set points per direction of every element (n[nelem]) and rebuild
- * tables (replaces update_node_n_pt_typ in src/main/amr/storage.c). */
+/* Set points per direction of every element (n[nelem]) at set-up and rebuild
+ * tables; the registers start from zero (element orders chosen before the
+ * initial data, as mesh.c:181-319 does with amr_n). */
 int nmx_mesh_set_elm_n(nmx_ctx *ctx, const int *n, int nelem);
+/* Adapt the mesh between steps with the evolved state (NMX_REG_U)
+ * interpolated onto the new elements; the other registers start from zero,
+ * time is kept and dt follows the dt parameter.  DG meshes (no FV elements).
+ * nmx_mesh_hadapt: flag > 0 replaces an element by its 8 children of the
+ *   same n, interpolated from it (hp_refine_elms_if_rflag with PARENT_n,
+ *   src/main/amr/refine.c:174-276; make_child_elm, storage.c:493-585);
+ *   8 siblings in a row all with flag < 0 become their parent, whose n is
+ *   the largest sibling n and whose points average the interpolants of the
+ *   children that hold them (remove_elms_if_rflag, refine.c:359-506;
+ *   make_parent_elm, storage.c:627-760).
+ * nmx_mesh_padapt: new points per direction n[nelem], the state Lagrange-
+ *   interpolated per element (update_node_n_pt_typ, storage.c:858-989). */
+int nmx_mesh_hadapt(nmx_ctx *ctx, const int *flag, int nelem);
+int nmx_mesh_padapt(nmx_ctx *ctx, const int *n, int nelem);
 
 int nmx_mesh_nelem(nmx_ctx *ctx, int *nelem);
 int nmx_mesh_nnodes(nmx_ctx *ctx, long *nnodes);

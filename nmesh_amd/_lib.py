@@ -30,6 +30,8 @@ def _proto(lib):
         "nmx_get_param": (i, [vp, C.c_char_p, C.c_char_p, i]),
         "nmx_mesh_build": (i, [vp]),
         "nmx_mesh_set_elm_n": (i, [vp, vp, i]),
+        "nmx_mesh_hadapt": (i, [vp, vp, i]),
+        "nmx_mesh_padapt": (i, [vp, vp, i]),
         "nmx_mesh_nelem": (i, [vp, P(i)]),
         "nmx_mesh_nnodes": (i, [vp, P(l)]),
         "nmx_mesh_elm_info": (i, [vp, i, vp]),

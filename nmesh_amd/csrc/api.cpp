@@ -774,6 +774,168 @@ extern "C" int nmx_mesh_build(nmx_ctx *c) {
   return upload_mesh(c);
 }
 
+// ---------------------------------------------------------------------------
+// mesh adaptation with the evolved state interpolated onto the new elements
+// ---------------------------------------------------------------------------
+
+// 1D Lagrange rows l_s(x_i) of the LGL basis of order ns at nd points
+static void lagrange_rows(int ns, const std::vector<double> &x, std::vector<double> &tab, int *off) {
+  const Basis &b = lgl_basis(ns);
+  *off = (int)tab.size();
+  for (double xi : x)
+    for (int s = 0; s < ns; ++s) {
+      double prod = 1.0;
+      for (int m = 0; m < ns; ++m)
+        if (m != s) prod *= (xi - b.x[m]);
+      tab.push_back(prod * b.wb[s]);
+    }
+}
+
+// jobs moving the state of Mold (register uold) onto c->mesh, then the launch
+static int adapt_state(nmx_ctx *c, const Mesh &Mold, const double *uold) {
+  const Mesh &M = c->mesh;
+  std::map<std::string, int> where;
+  for (size_t e = 0; e < Mold.e.size(); ++e) where[elm_name(Mold.e[e])] = (int)e;
+  std::vector<AdaptJob> jobs;
+  std::vector<int> jfirst;
+  std::vector<double> atab;
+  for (const Elm &E : M.e) {
+    jfirst.push_back((int)jobs.size());
+    const int nd = E.n;
+    const Basis &bd = lgl_basis(nd);
+    std::vector<double> xd(bd.x, bd.x + nd);
+    AdaptJob J;
+    J.dst = E.off; J.nd = nd;
+    for (int d = 0; d < 3; ++d) { J.lo[d] = 0; J.hi[d] = nd - 1; }
+    const std::string name = elm_name(E);
+    auto it = where.find(name);
+    if (it != where.end()) {                        // same place
+      const Elm &S = Mold.e[it->second];
+      J.src = S.off; J.ns = S.n;
+      if (S.n == nd) {
+        J.a0 = J.a1 = J.a2 = -1;
+      } else {
+        lagrange_rows(S.n, xd, atab, &J.a0);
+        J.a1 = J.a2 = J.a0;
+      }
+      jobs.push_back(J);
+      continue;
+    }
+    if (E.l > 0 && (it = where.find(name.substr(0, name.size() - 1))) != where.end()) {   // child
+      const Elm &S = Mold.e[it->second];
+      J.src = S.off; J.ns = S.n;
+      const int ch = E.path[E.l - 1];
+      int *ao[3] = {&J.a0, &J.a1, &J.a2};
+      for (int d = 0; d < 3; ++d) {
+        std::vector<double> xp(nd);
+        for (int i = 0; i < nd; ++i) xp[i] = 0.5 * xd[i] + (((ch >> d) & 1) ? 0.5 : -0.5);
+        lagrange_rows(S.n, xp, atab, ao[d]);
+      }
+      jobs.push_back(J);
+      continue;
+    }
+    for (int ch = 0; ch < 8; ++ch) {               // parent of 8 old children
+      it = where.find(name + std::string(1, (char)('0' + ch)));
+      if (it == where.end()) return fail(NMX_EINVAL, "adapted element " + name + " has no source");
+      const Elm &S = Mold.e[it->second];
+      J.src = S.off; J.ns = S.n;
+      int *ao[3] = {&J.a0, &J.a1, &J.a2};
+      for (int d = 0; d < 3; ++d) {
+        std::vector<double> xc(nd);
+        int lo = nd, hi = -1;
+        for (int i = 0; i < nd; ++i) {
+          xc[i] = 2. * xd[i] - (((ch >> d) & 1) ? 1. : -1.);
+          if (xc[i] >= -1. - 1e-10 && xc[i] <= 1. + 1e-10) { lo = std::min(lo, i); hi = std::max(hi, i); }
+        }
+        J.lo[d] = lo; J.hi[d] = hi;
+        lagrange_rows(S.n, xc, atab, ao[d]);
+      }
+      jobs.push_back(J);
+    }
+  }
+  jfirst.push_back((int)jobs.size());
+  // every job reads inside the old register and writes inside the new one
+  const long oldlen = 5 * Mold.nnodes, newlen = 5 * M.nnodes;
+  for (const AdaptJob &J : jobs) {
+    if (J.nd < 2 || J.nd > 10 || J.ns < 2 || J.ns > 10) return fail(NMX_EINVAL, "adapt: order out of range");
+    if (J.src < 0 || 5 * J.src + 5L * J.ns * J.ns * J.ns > oldlen ||
+        J.dst < 0 || 5 * J.dst + 5L * J.nd * J.nd * J.nd > newlen)
+      return fail(NMX_EINVAL, "adapt: job out of range");
+    if (J.a0 >= 0 && (J.a0 + J.nd * J.ns > (long)atab.size() || J.a1 + J.nd * J.ns > (long)atab.size() ||
+                      J.a2 + J.nd * J.ns > (long)atab.size()))
+      return fail(NMX_EINVAL, "adapt: rows out of range");
+    if (J.a0 < 0 && J.ns != J.nd) return fail(NMX_EINVAL, "adapt: copy between orders");
+  }
+  if (atab.empty()) atab.assign(1, 0.0);
+  AdaptJob *djobs = nullptr;
+  int *djf = nullptr;
+  double *dat = nullptr;
+  int r;
+  if ((r = dev_upload(&djobs, jobs)) || (r = dev_upload(&djf, jfirst)) || (r = dev_upload(&dat, atab))) {
+    (void)hipFree(djobs); (void)hipFree(djf); (void)hipFree(dat);
+    return r;
+  }
+  AdaptArgs a;
+  a.uold = uold; a.unew = c->dU; a.jobs = djobs; a.jfirst = djf; a.atab = dat;
+  a.nelem = (int)M.e.size();
+  hipError_t err = launch_adapt(a, c->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+  (void)hipFree(djobs); (void)hipFree(djf); (void)hipFree(dat);
+  if (err != hipSuccess) return fail(NMX_EHIP, std::string("adapt launch: ") + hipGetErrorString(err));
+  return NMX_OK;
+}
+
+// the mesh changed on the host: new device mirror, the state interpolated.
+// If the new mirror cannot be made, the old mesh and state are restored.
+static int remirror_with_state(nmx_ctx *c, const Mesh &old) {
+  const double dt_old = c->dt;
+  if (P(c, "dt") == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"));
+  if (c->device < 0) return NMX_OK;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double *uold = c->dU;
+  c->dU = nullptr;                 // kept across the re-upload
+  int r = upload_mesh(c);
+  if (!r) {
+    r = adapt_state(c, old, uold);
+    (void)hipFree(uold);
+    return r;
+  }
+  const std::string msg = nmx_last_error();
+  c->mesh = old;
+  c->dt = dt_old;
+  if (upload_mesh(c) == NMX_OK)
+    (void)hipMemcpy(c->dU, uold, c->nstate * sizeof(double), hipMemcpyDeviceToDevice);
+  (void)hipFree(uold);
+  return fail(r, msg);
+}
+
+extern "C" int nmx_mesh_hadapt(nmx_ctx *c, const int *flag, int nelem) {
+  if (!c || !flag) return fail(NMX_EINVAL, "NULL argument");
+  if (!c->have_mesh) return fail(NMX_ESTATE, "no mesh");
+  if (nelem != (int)c->mesh.e.size()) return fail(NMX_EINVAL, "need one flag per element");
+  for (const Elm &E : c->mesh.e)
+    if (E.fv) return fail(NMX_EUNSUP, "adapting meshes with FV elements is not built");
+  Mesh old = c->mesh;
+  std::string err = hadapt(c->mesh, std::vector<int>(flag, flag + nelem));
+  if (!err.empty()) { c->mesh = old; return fail(NMX_EINVAL, err); }
+  return remirror_with_state(c, old);
+}
+
+extern "C" int nmx_mesh_padapt(nmx_ctx *c, const int *n, int nelem) {
+  if (!c || !n) return fail(NMX_EINVAL, "NULL argument");
+  if (!c->have_mesh) return fail(NMX_ESTATE, "no mesh");
+  if (nelem != (int)c->mesh.e.size()) return fail(NMX_EINVAL, "need one n per element");
+  for (const Elm &E : c->mesh.e)
+    if (E.fv) return fail(NMX_EUNSUP, "adapting meshes with FV elements is not built");
+  for (int i = 0; i < nelem; ++i)
+    if (n[i] < 2 || n[i] > 10) return fail(NMX_EUNSUP, "the device kernels take 2 to 10 points per direction");
+  Mesh old = c->mesh;
+  std::string err = set_elm_n(c->mesh, std::vector<int>(n, n + nelem));
+  if (!err.empty()) { c->mesh = old; return fail(NMX_EINVAL, err); }
+  return remirror_with_state(c, old);
+}
+
 extern "C" int nmx_mesh_set_elm_n(nmx_ctx *c, const int *n, int nelem) {
   if (!c || !n) return fail(NMX_EINVAL, "NULL argument");
   if (!c->have_mesh) return fail(NMX_ESTATE, "no mesh");

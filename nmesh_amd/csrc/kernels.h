@@ -85,6 +85,26 @@ struct CrcArgs {
 // out[0] = CRC (device memory)
 hipError_t launch_vars_crc(const CrcArgs &a, uint64_t *out, hipStream_t s);
 
+// One source of a new element's state after mesh adaptation: the element's
+// points in [lo, hi] (per direction) take A2 (x) A1 (x) A0 applied to the
+// source element (rows of ns doubles at a0, a1, a2 in atab).  a0 < 0: copy
+// the source unchanged (the only job of its element).
+struct AdaptJob {
+  long dst, src;        // first node of the new / old element
+  int nd, ns;           // points per direction, new / old
+  int a0, a1, a2;
+  int lo[3], hi[3];
+};
+struct AdaptArgs {
+  const double *uold = nullptr;  // old register, element blocks
+  double *unew = nullptr;        // new register, element blocks
+  const AdaptJob *jobs = nullptr;
+  const int *jfirst = nullptr;   // [nelem + 1]: jobs of new element e
+  const double *atab = nullptr;
+  int nelem = 0;                 // new elements
+};
+hipError_t launch_adapt(const AdaptArgs &a, hipStream_t s);
+
 // flat [v][node] <-> element blocks [e][v][np_e] on the device
 // (to_blocks: 1 flat -> blocks, 0 blocks -> flat)
 hipError_t launch_layout(const double *src, double *dst, const long *eoff, const int *en, int nelem,

@@ -803,6 +803,60 @@ std::string set_elm_n(Mesh &M, const std::vector<int> &n) {
   return face_tables(M);
 }
 
+// the run of 8 elements from e is the 8 children of one parent in child
+// order (amr_elms_are_siblings, child0 first)
+static bool are_siblings(const Mesh &M, size_t e) {
+  if (e + 8 > M.e.size()) return false;
+  const Elm &E0 = M.e[e];
+  if (E0.l < 1) return false;
+  for (int c = 0; c < 8; ++c) {
+    const Elm &E = M.e[e + c];
+    if (E.p != E0.p || E.l != E0.l || E.path[E0.l - 1] != c) return false;
+    if (std::memcmp(E.path, E0.path, E0.l - 1) != 0) return false;
+  }
+  return true;
+}
+
+std::string hadapt(Mesh &M, const std::vector<int> &flag) {
+  if (flag.size() != M.e.size()) return "need one flag per element";
+  // refinement: children of the same n in place of the parent
+  std::vector<char> ref(M.e.size());
+  std::vector<char> cf;
+  for (size_t i = 0; i < M.e.size(); ++i) {
+    ref[i] = flag[i] > 0;
+    if (ref[i] && M.e[i].l + 1 >= (int)sizeof(M.e[i].path)) return "refinement level too deep";
+  }
+  for (size_t i = 0; i < M.e.size(); ++i) {
+    if (ref[i]) cf.insert(cf.end(), 8, 0);
+    else cf.push_back(flag[i] < 0);
+  }
+  refine(M, ref);
+  // coarsening: 8 flagged siblings in a row become their parent, with the
+  // largest sibling n (refine_sib_with_max_np_pt_typ)
+  std::vector<Elm> ne;
+  ne.reserve(M.e.size());
+  for (size_t e = 0; e < M.e.size();) {
+    bool all = are_siblings(M, e);
+    for (int c = 0; all && c < 8; ++c) all = cf[e + c] != 0;
+    if (!all) { ne.push_back(M.e[e]); ++e; continue; }
+    const Elm &C0 = M.e[e];
+    Elm P;
+    P.p = C0.p;
+    P.l = C0.l - 1;
+    std::memcpy(P.path, C0.path, sizeof(P.path));
+    P.path[P.l] = 0;
+    for (int d = 0; d < 3; ++d) P.ic[d] = C0.ic[d] >> 1;
+    P.n = 0;
+    for (int c = 0; c < 8; ++c) P.n = std::max(P.n, M.e[e + c].n);
+    P.fv = C0.fv;
+    elm_bbox(M.pat[P.p], P);
+    ne.push_back(P);
+    e += 8;
+  }
+  M.e = std::move(ne);
+  return finish(M);
+}
+
 std::string set_fv_patches(Mesh &M, const std::vector<int> &patches) {
   for (auto &E : M.e) {
     E.fv = 0;

@@ -87,6 +87,12 @@ std::string build_box(Mesh &M, int luni, int n, double dout, const double xc[3])
 std::string build_shell(Mesh &M, int luni, int n, double rin, double rout,
                         double sph_radius, int sph_levels);
 std::string set_elm_n(Mesh &M, const std::vector<int> &n);
+// h-adaptation (hp_refine_elms_if_rflag with PARENT_n, refine.c:174-276;
+// remove_elms_if_rflag, refine.c:359-506): flag > 0 replaces an element by
+// its 8 children of the same n, in place; 8 siblings in a row that all have
+// flag < 0 become their parent, with the largest sibling n.  Tables and
+// geometry are rebuilt.
+std::string hadapt(Mesh &M, const std::vector<int> &flag);
 // put every element of the listed patches in FV mode on uniform points
 // (amr_uniform_p, amr_fv_p; src/main/amr/mesh.c:40-80) and rebuild
 std::string set_fv_patches(Mesh &M, const std::vector<int> &patches);

@@ -130,6 +130,19 @@ class Mesh:
         n = np.ascontiguousarray(n, dtype=np.int32)
         check(lib().nmx_mesh_set_elm_n(self._ctx, _p(n), int(n.size)))
 
+    def hadapt(self, flag):
+        """h-adapt between steps, the state interpolated (refine.c:174-506):
+        flag > 0 refines an element, 8 siblings in a row with flag < 0
+        coarsen into their parent"""
+        f = np.ascontiguousarray(flag, dtype=np.int32)
+        check(lib().nmx_mesh_hadapt(self._ctx, _p(f), int(f.size)))
+
+    def padapt(self, n):
+        """new points per direction per element, the state interpolated
+        (update_node_n_pt_typ, storage.c:858-989)"""
+        n = np.ascontiguousarray(n, dtype=np.int32)
+        check(lib().nmx_mesh_padapt(self._ctx, _p(n), int(n.size)))
+
     # state (Vard over all elements, flat [var][node])
     def upload(self, u, reg=REG_U):
         u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)

@@ -1950,3 +1950,212 @@ int orc_write_variables_bin(const OMesh *M, const double *u, const char *path)
   fprintf(fp, "EndOfFile.\n");
   return fclose(fp) ? -1 : 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* Mesh adaptation with the state interpolated onto the new elements    */
+/* (src/main/amr/refine.c:156-537, storage.c:493-1000)                  */
+/* ------------------------------------------------------------------ */
+
+/* a copy of M (elements, patches; geometry rebuilt) */
+OMesh *orc_mesh_clone(const OMesh *M)
+{
+  OMesh *C = calloc(1, sizeof(OMesh));
+  C->npat = M->npat;
+  memcpy(C->pat, M->pat, sizeof(M->pat));
+  C->nelem = M->nelem;
+  C->e = malloc(sizeof(OElm) * M->nelem);
+  memcpy(C->e, M->e, sizeof(OElm) * M->nelem);
+  mesh_finish(C);
+  return C;
+}
+
+/* is the run of 8 elements from e the 8 children of one parent, in child
+   order (amr_elms_are_siblings, child0 first) */
+static int are_siblings(const OMesh *M, int e)
+{
+  if (e + 8 > M->nelem) return 0;
+  const OElm *E0 = &M->e[e];
+  if (E0->l < 1) return 0;
+  for (int c = 0; c < 8; c++) {
+    const OElm *E = &M->e[e + c];
+    if (E->p != E0->p || E->l != E0->l) return 0;
+    if (memcmp(E->loc, E0->loc, E0->l - 1) != 0) return 0;
+    if (E->loc[E0->l - 1] != '0' + c) return 0;
+  }
+  return 1;
+}
+
+/* h-adaptation by flag (hp_refine_elms_if_rflag with PARENT_n, refine.c:
+   174-276; remove_elms_if_rflag, refine.c:359-506): flag > 0 replaces an
+   element by its 8 children of the same n; 8 siblings in a row that all have
+   flag < 0 are replaced by their parent, whose n is that of the sibling with
+   the most points (refine_sib_with_max_np_pt_typ, refine.c:314-357).
+   Refinement first, then coarsening, as two passes. */
+void orc_mesh_hadapt(OMesh *M, const int *flag)
+{
+  int *f = malloc(sizeof(int) * M->nelem);
+  int *ref = calloc(M->nelem, sizeof(int));
+  for (int e = 0; e < M->nelem; e++) { f[e] = flag[e]; ref[e] = flag[e] > 0; }
+  /* coarsening flags survive refinement only on elements not refined */
+  int old_n = M->nelem;
+  refine_flagged(M, ref);
+  int *cf = calloc(M->nelem, sizeof(int));
+  for (int e = 0, k = 0; e < old_n; e++) {
+    if (ref[e]) { k += 8; continue; }
+    cf[k++] = f[e] < 0;
+  }
+  /* coarsen */
+  OElm *ne = malloc(sizeof(OElm) * M->nelem);
+  int k = 0;
+  for (int e = 0; e < M->nelem;) {
+    int all = are_siblings(M, e);
+    for (int c = 0; all && c < 8; c++) all = cf[e + c];
+    if (!all) { ne[k++] = M->e[e]; e++; continue; }
+    OElm P;
+    memset(&P, 0, sizeof(P));
+    P.p = M->e[e].p;
+    P.l = M->e[e].l - 1;
+    memcpy(P.loc, M->e[e].loc, sizeof(P.loc));
+    P.loc[P.l] = 0;
+    int nmax = 0;
+    for (int c = 0; c < 8; c++) if (M->e[e + c].n > nmax) nmax = M->e[e + c].n;
+    P.n = nmax;
+    P.use_fv = M->e[e].use_fv;
+    elm_bbox_from_loc(&M->pat[P.p], &P);
+    ne[k++] = P;
+    e += 8;
+  }
+  free(M->e);
+  M->e = ne;
+  M->nelem = k;
+  free(f); free(ref); free(cf);
+  mesh_finish(M);
+}
+
+/* 1D rows A[i*ns + s] = l_s(x_i) of the Lagrange basis on the ns points xs
+   (basis_interp_toIpoints with Lagrange_of_x at amr_Lagrange_interp_order
+   n, interpolate.c:540-600) */
+static void lagrange_rows(int nd, const double *x, int ns, double *A)
+{
+  OBasis *b = basis(ns);
+  for (int i = 0; i < nd; i++)
+    for (int s = 0; s < ns; s++) A[i * ns + s] = lagrange_of_x(s, x[i], ns, b->Xb, b->WL);
+}
+
+/* out[k][j][i] (nd^3) = sum A2[k][c] A1[j][b] A0[i][a] in[c][b][a] (ns^3) */
+static void tensor_apply(int nd, int ns, const double *A0, const double *A1, const double *A2,
+                         const double *in, double *out)
+{
+  double *t1 = malloc(sizeof(double) * ns * ns * nd), *t2 = malloc(sizeof(double) * ns * nd * nd);
+  for (int c = 0; c < ns; c++)
+    for (int b = 0; b < ns; b++)
+      for (int i = 0; i < nd; i++) {
+        double s = 0.;
+        for (int a = 0; a < ns; a++) s += A0[i * ns + a] * in[a + ns * (b + ns * c)];
+        t1[i + nd * (b + ns * c)] = s;
+      }
+  for (int c = 0; c < ns; c++)
+    for (int j = 0; j < nd; j++)
+      for (int i = 0; i < nd; i++) {
+        double s = 0.;
+        for (int b = 0; b < ns; b++) s += A1[j * ns + b] * t1[i + nd * (b + ns * c)];
+        t2[i + nd * (j + nd * c)] = s;
+      }
+  for (int k = 0; k < nd; k++)
+    for (int j = 0; j < nd; j++)
+      for (int i = 0; i < nd; i++) {
+        double s = 0.;
+        for (int c = 0; c < ns; c++) s += A2[k * ns + c] * t2[i + nd * (j + nd * c)];
+        out[i + nd * (j + nd * k)] = s;
+      }
+  free(t1); free(t2);
+}
+
+static int find_loc(const OMesh *M, int p, int l, const char *loc)
+{
+  for (int e = 0; e < M->nelem; e++)
+    if (M->e[e].p == p && M->e[e].l == l && memcmp(M->e[e].loc, loc, l) == 0) return e;
+  return -1;
+}
+
+/* The state of Mnew from that of Mold (update_node_n_pt_typ_return_node_old
+   storage.c:858-931, make_child_elm storage.c:493-585, make_parent_elm
+   storage.c:627-760): an element at the same place keeps its values (same
+   n) or is Lagrange-interpolated to its new points; a child is interpolated
+   from its old parent; a parent takes at each of its points the average
+   over the old children that hold the point of their interpolants.  DG
+   elements (LGL points) only. */
+int orc_adapt_state(const OMesh *Mold, const double *uold, const OMesh *Mnew, double *unew)
+{
+  const long nno = Mold->nnodes, nnn = Mnew->nnodes;
+  for (int e = 0; e < Mnew->nelem; e++) {
+    const OElm *E = &Mnew->e[e];
+    if (E->use_fv) return -1;
+    const int nd = E->n, npd = elm_np(E);
+    const double *xd = basis(nd)->Xb;
+    double A[3][ORC_MAXN * ORC_MAXN];
+    int s = find_loc(Mold, E->p, E->l, E->loc);
+    if (s >= 0) {                                  /* same place */
+      const OElm *S = &Mold->e[s];
+      if (S->n == nd) {
+        for (int v = 0; v < NV; v++) memcpy(unew + v * nnn + E->off, uold + v * nno + S->off, sizeof(double) * npd);
+        continue;
+      }
+      for (int d = 0; d < 3; d++) lagrange_rows(nd, xd, S->n, A[d]);
+      for (int v = 0; v < NV; v++)
+        tensor_apply(nd, S->n, A[0], A[1], A[2], uold + v * nno + S->off, unew + v * nnn + E->off);
+      continue;
+    }
+    if (E->l > 0 && (s = find_loc(Mold, E->p, E->l - 1, E->loc)) >= 0) {   /* child of an old parent */
+      const OElm *S = &Mold->e[s];
+      const int c = E->loc[E->l - 1] - '0';
+      for (int d = 0; d < 3; d++) {
+        double xp[ORC_MAXN];
+        for (int i = 0; i < nd; i++) xp[i] = 0.5 * xd[i] + (((c >> d) & 1) ? 0.5 : -0.5);
+        lagrange_rows(nd, xp, S->n, A[d]);
+      }
+      for (int v = 0; v < NV; v++)
+        tensor_apply(nd, S->n, A[0], A[1], A[2], uold + v * nno + S->off, unew + v * nnn + E->off);
+      continue;
+    }
+    /* parent of 8 old children */
+    char cl[32];
+    memcpy(cl, E->loc, sizeof(cl));
+    double *sum = calloc((size_t)NV * npd, sizeof(double));
+    int *cnt = calloc(npd, sizeof(int));
+    double *tmp = malloc(sizeof(double) * npd);
+    for (int c = 0; c < 8; c++) {
+      cl[E->l] = '0' + c;
+      int sc = find_loc(Mold, E->p, E->l + 1, cl);
+      if (sc < 0) { free(sum); free(cnt); free(tmp); return -2; }
+      const OElm *S = &Mold->e[sc];
+      /* my points in the child's Xb; a point is the child's if it lies in
+         [-1, 1] there (array_find_XYZ_in_node), boundaries included */
+      int in[3][ORC_MAXN];
+      for (int d = 0; d < 3; d++) {
+        double xc[ORC_MAXN];
+        for (int i = 0; i < nd; i++) {
+          xc[i] = 2. * xd[i] - (((c >> d) & 1) ? 1. : -1.);
+          in[d][i] = (xc[i] >= -1. - 1e-10 && xc[i] <= 1. + 1e-10);
+        }
+        lagrange_rows(nd, xc, S->n, A[d]);
+      }
+      for (int v = 0; v < NV; v++) {
+        tensor_apply(nd, S->n, A[0], A[1], A[2], uold + v * nno + S->off, tmp);
+        for (int k = 0; k < nd; k++)
+          for (int j = 0; j < nd; j++)
+            for (int i = 0; i < nd; i++)
+              if (in[0][i] && in[1][j] && in[2][k]) {
+                int q = i + nd * (j + nd * k);
+                sum[v * npd + q] += tmp[q];
+                if (v == 0) cnt[q]++;
+              }
+      }
+    }
+    for (int v = 0; v < NV; v++)
+      for (int q = 0; q < npd; q++)
+        unew[v * nnn + E->off + q] = cnt[q] > 1 ? sum[v * npd + q] / cnt[q] : sum[v * npd + q];
+    free(sum); free(cnt); free(tmp);
+  }
+  return 0;
+}

@@ -58,6 +58,9 @@ def lib():
             "orc_crc64_combine": (u64, [u64, u64, u64]),
             "orc_vars_crc": (u64, [vp, vp]),
             "orc_write_variables_bin": (i, [vp, vp, cp]),
+            "orc_mesh_clone": (vp, [vp]),
+            "orc_mesh_hadapt": (None, [vp, vp]),
+            "orc_adapt_state": (i, [vp, vp, vp, vp]),
             "orc_set_fv_rec": (None, [i]),
             "orc_set_fv_midpoint": (None, [i]),
             "orc_dnfn_extrap1": (None, [i, vp]),
@@ -187,6 +190,24 @@ class OMesh:
         t = lib().orc_evolve(self.h, NUMFLUX[numflux], _p(k), METHOD[method], float(dt),
                              int(nsteps), float(t0), _p(u))
         return u, t
+
+    def adapted(self, u, hflag=None, n=None):
+        """(new mesh, new state): h-adaptation by hflag (> 0 refine, < 0
+        coarsen 8 flagged siblings), then per-element n (p-change), the state
+        interpolated as nmesh's refine and update_node_n_pt_typ do"""
+        new = OMesh.__new__(OMesh)
+        new.h = lib().orc_mesh_clone(self.h)
+        if hflag is not None:
+            f = np.ascontiguousarray(hflag, dtype=np.int32)
+            lib().orc_mesh_hadapt(new.h, _p(f))
+        if n is not None:
+            new.set_n(n)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros(5 * new.nnodes)
+        r = lib().orc_adapt_state(self.h, _p(u), new.h, _p(out))
+        if r != 0:
+            raise ValueError("orc_adapt_state failed (%d)" % r)
+        return new, out
 
     def vars_crc(self, u):
         """the reference's "vars" CRC-64 of state u (norms.c:440-458)"""
