@@ -103,6 +103,17 @@ int nmx_mesh_set_elm_n(nmx_ctx *ctx, const int *n, int nelem);
  *   interpolated per element (update_node_n_pt_typ, storage.c:858-989). */
 int nmx_mesh_hadapt(nmx_ctx *ctx, const int *flag, int nelem);
 int nmx_mesh_padapt(nmx_ctx *ctx, const int *n, int nelem);
+/* Switch elements between DG on LGL points (to_fv[e] = 0) and FV on uniform
+ * points of the same n (to_fv[e] = 1; n >= 3), the state interpolated: LGL
+ * to uniform by Lagrange (INTERP_LGL_TO_n_UNIFORM, LGL_to_nUNIt) and back by
+ * its inverse, the Lagrange interpolation from the uniform points
+ * (INTERP_UNIFORM_TO_n_LGL, UNI_to_nLGLt with basis_BackInterpMatrix =
+ * PseudoInv; gridpoints.c:158-290).  The mechanism of
+ * evolve_switch_troubled_nodes_mesh / evolve_switch_nontroubled_nodes_mesh
+ * (src/main/evolve/trouble.c:186-330); which elements switch is the
+ * caller's choice (scalarwave1 registers no TROUBLE function).  Meshes
+ * mixing both kinds run on the batched path. */
+int nmx_mesh_switch_fv(nmx_ctx *ctx, const int *to_fv, int nelem);
 
 int nmx_mesh_nelem(nmx_ctx *ctx, int *nelem);
 int nmx_mesh_nnodes(nmx_ctx *ctx, long *nnodes);

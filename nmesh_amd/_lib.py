@@ -32,6 +32,7 @@ def _proto(lib):
         "nmx_mesh_set_elm_n": (i, [vp, vp, i]),
         "nmx_mesh_hadapt": (i, [vp, vp, i]),
         "nmx_mesh_padapt": (i, [vp, vp, i]),
+        "nmx_mesh_switch_fv": (i, [vp, vp, i]),
         "nmx_mesh_nelem": (i, [vp, P(i)]),
         "nmx_mesh_nnodes": (i, [vp, P(l)]),
         "nmx_mesh_elm_info": (i, [vp, i, vp]),

@@ -778,16 +778,26 @@ extern "C" int nmx_mesh_build(nmx_ctx *c) {
 // mesh adaptation with the evolved state interpolated onto the new elements
 // ---------------------------------------------------------------------------
 
-// 1D Lagrange rows l_s(x_i) of the LGL basis of order ns at nd points
-static void lagrange_rows(int ns, const std::vector<double> &x, std::vector<double> &tab, int *off) {
-  const Basis &b = lgl_basis(ns);
+// 1D Lagrange rows l_s(x_i) on the ns points of an element (LGL, or
+// uniform for an FV element) at nd points
+static void lagrange_rows(int ns, const std::vector<double> &x, std::vector<double> &tab, int *off,
+                          bool uni = false) {
+  const Basis &b = uni ? uniform_basis(ns) : lgl_basis(ns);
+  double w[kMaxN];
+  for (int s = 0; s < ns; ++s) {
+    if (!uni) { w[s] = b.wb[s]; continue; }
+    double prod = 1.0;
+    for (int m = 0; m < ns; ++m)
+      if (m != s) prod *= (b.x[s] - b.x[m]);
+    w[s] = 1.0 / prod;
+  }
   *off = (int)tab.size();
   for (double xi : x)
     for (int s = 0; s < ns; ++s) {
       double prod = 1.0;
       for (int m = 0; m < ns; ++m)
         if (m != s) prod *= (xi - b.x[m]);
-      tab.push_back(prod * b.wb[s]);
+      tab.push_back(prod * w[s]);
     }
 }
 
@@ -802,7 +812,7 @@ static int adapt_state(nmx_ctx *c, const Mesh &Mold, const double *uold) {
   for (const Elm &E : M.e) {
     jfirst.push_back((int)jobs.size());
     const int nd = E.n;
-    const Basis &bd = lgl_basis(nd);
+    const Basis &bd = E.fv ? uniform_basis(nd) : lgl_basis(nd);
     std::vector<double> xd(bd.x, bd.x + nd);
     AdaptJob J;
     J.dst = E.off; J.nd = nd;
@@ -812,15 +822,17 @@ static int adapt_state(nmx_ctx *c, const Mesh &Mold, const double *uold) {
     if (it != where.end()) {                        // same place
       const Elm &S = Mold.e[it->second];
       J.src = S.off; J.ns = S.n;
-      if (S.n == nd) {
+      if (S.n == nd && S.fv == E.fv) {
         J.a0 = J.a1 = J.a2 = -1;
       } else {
-        lagrange_rows(S.n, xd, atab, &J.a0);
+        if (S.fv && S.n != nd) return fail(NMX_EUNSUP, "p-changes of FV elements are not built");
+        lagrange_rows(S.n, xd, atab, &J.a0, S.fv != 0);
         J.a1 = J.a2 = J.a0;
       }
       jobs.push_back(J);
       continue;
     }
+    if (E.fv) return fail(NMX_EUNSUP, "h-adaptation of FV elements is not built");
     if (E.l > 0 && (it = where.find(name.substr(0, name.size() - 1))) != where.end()) {   // child
       const Elm &S = Mold.e[it->second];
       J.src = S.off; J.ns = S.n;
@@ -932,6 +944,18 @@ extern "C" int nmx_mesh_padapt(nmx_ctx *c, const int *n, int nelem) {
     if (n[i] < 2 || n[i] > 10) return fail(NMX_EUNSUP, "the device kernels take 2 to 10 points per direction");
   Mesh old = c->mesh;
   std::string err = set_elm_n(c->mesh, std::vector<int>(n, n + nelem));
+  if (!err.empty()) { c->mesh = old; return fail(NMX_EINVAL, err); }
+  return remirror_with_state(c, old);
+}
+
+extern "C" int nmx_mesh_switch_fv(nmx_ctx *c, const int *to_fv, int nelem) {
+  if (!c || !to_fv) return fail(NMX_EINVAL, "NULL argument");
+  if (!c->have_mesh) return fail(NMX_ESTATE, "no mesh");
+  if (nelem != (int)c->mesh.e.size()) return fail(NMX_EINVAL, "need one flag per element");
+  for (int i = 0; i < nelem; ++i)
+    if (to_fv[i] && c->mesh.e[i].n < 3) return fail(NMX_EUNSUP, "FV elements need n >= 3 points per direction");
+  Mesh old = c->mesh;
+  std::string err = set_fv_elems(c->mesh, std::vector<int>(to_fv, to_fv + nelem));
   if (!err.empty()) { c->mesh = old; return fail(NMX_EINVAL, err); }
   return remirror_with_state(c, old);
 }

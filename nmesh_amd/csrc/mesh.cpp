@@ -857,6 +857,13 @@ std::string hadapt(Mesh &M, const std::vector<int> &flag) {
   return finish(M);
 }
 
+std::string set_fv_elems(Mesh &M, const std::vector<int> &fv) {
+  if (fv.size() != M.e.size()) return "need one flag per element";
+  for (size_t i = 0; i < fv.size(); ++i) M.e[i].fv = fv[i] != 0;
+  geometry(M);
+  return face_tables(M);
+}
+
 std::string set_fv_patches(Mesh &M, const std::vector<int> &patches) {
   for (auto &E : M.e) {
     E.fv = 0;

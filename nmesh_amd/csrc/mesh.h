@@ -93,6 +93,9 @@ std::string set_elm_n(Mesh &M, const std::vector<int> &n);
 // flag < 0 become their parent, with the largest sibling n.  Tables and
 // geometry are rebuilt.
 std::string hadapt(Mesh &M, const std::vector<int> &flag);
+// FV on uniform points where fv[e] is set, DG on LGL points elsewhere;
+// geometry and tables rebuilt
+std::string set_fv_elems(Mesh &M, const std::vector<int> &fv);
 // put every element of the listed patches in FV mode on uniform points
 // (amr_uniform_p, amr_fv_p; src/main/amr/mesh.c:40-80) and rebuild
 std::string set_fv_patches(Mesh &M, const std::vector<int> &patches);

@@ -137,6 +137,13 @@ class Mesh:
         f = np.ascontiguousarray(flag, dtype=np.int32)
         check(lib().nmx_mesh_hadapt(self._ctx, _p(f), int(f.size)))
 
+    def switch_fv(self, to_fv):
+        """elements to FV on uniform points (1) or DG on LGL points (0),
+        the state interpolated (evolve_switch_troubled_nodes_mesh and
+        evolve_switch_nontroubled_nodes_mesh, trouble.c:186-330)"""
+        f = np.ascontiguousarray(to_fv, dtype=np.int32)
+        check(lib().nmx_mesh_switch_fv(self._ctx, _p(f), int(f.size)))
+
     def padapt(self, n):
         """new points per direction per element, the state interpolated
         (update_node_n_pt_typ, storage.c:858-989)"""

@@ -836,6 +836,15 @@ void orc_mesh_set_fv(OMesh *M, int use_fv)
   mesh_geometry(M);
 }
 
+/* FV on uniform points where use_fv[e] is set, DG elsewhere (the element
+   switch of evolve_switch_troubled_nodes_mesh, trouble.c:186-261) */
+void orc_mesh_set_fv_elems(OMesh *M, const int *use_fv)
+{
+  for (int e = 0; e < M->nelem; e++) M->e[e].use_fv = use_fv[e] != 0;
+  set_ajmodes(M);
+  mesh_geometry(M);
+}
+
 /* FV on uniform points in the patches of bit mask patmask, DG elsewhere
    (amr_uniform_p = amr_fv_p naming some patches, mesh.c:40-80) */
 void orc_mesh_set_fv_patches(OMesh *M, int patmask)
@@ -2032,14 +2041,25 @@ void orc_mesh_hadapt(OMesh *M, const int *flag)
   mesh_finish(M);
 }
 
-/* 1D rows A[i*ns + s] = l_s(x_i) of the Lagrange basis on the ns points xs
-   (basis_interp_toIpoints with Lagrange_of_x at amr_Lagrange_interp_order
-   n, interpolate.c:540-600) */
-static void lagrange_rows(int nd, const double *x, int ns, double *A)
+/* 1D rows A[i*ns + s] = l_s(x_i) of the Lagrange basis on the ns points of
+   an element: LGL, or uniform (uni) for an FV element (basis_interp_toIpoints
+   with Lagrange_of_x at amr_Lagrange_interp_order n, interpolate.c:540-600;
+   between the point types of one n, LGL_to_nUNIt and its PseudoInv inverse
+   UNI_to_nLGLt, gridpoints.c:158-290, which for a square matrix is the
+   Lagrange interpolation back) */
+static void lagrange_rows_on(int nd, const double *x, int ns, int uni, double *A)
 {
   OBasis *b = basis(ns);
+  const double *xs = uni ? b->Xu : b->Xb;
+  double w[ORC_MAXN];
+  lagrange_winterp(ns, xs, w);
   for (int i = 0; i < nd; i++)
-    for (int s = 0; s < ns; s++) A[i * ns + s] = lagrange_of_x(s, x[i], ns, b->Xb, b->WL);
+    for (int s = 0; s < ns; s++) A[i * ns + s] = lagrange_of_x(s, x[i], ns, xs, w);
+}
+
+static void lagrange_rows(int nd, const double *x, int ns, double *A)
+{
+  lagrange_rows_on(nd, x, ns, 0, A);
 }
 
 /* out[k][j][i] (nd^3) = sum A2[k][c] A1[j][b] A0[i][a] in[c][b][a] (ns^3) */
@@ -2090,22 +2110,23 @@ int orc_adapt_state(const OMesh *Mold, const double *uold, const OMesh *Mnew, do
   const long nno = Mold->nnodes, nnn = Mnew->nnodes;
   for (int e = 0; e < Mnew->nelem; e++) {
     const OElm *E = &Mnew->e[e];
-    if (E->use_fv) return -1;
     const int nd = E->n, npd = elm_np(E);
-    const double *xd = basis(nd)->Xb;
+    const double *xd = E->use_fv ? basis(nd)->Xu : basis(nd)->Xb;
     double A[3][ORC_MAXN * ORC_MAXN];
     int s = find_loc(Mold, E->p, E->l, E->loc);
     if (s >= 0) {                                  /* same place */
       const OElm *S = &Mold->e[s];
-      if (S->n == nd) {
+      if (S->n == nd && S->use_fv == E->use_fv) {
         for (int v = 0; v < NV; v++) memcpy(unew + v * nnn + E->off, uold + v * nno + S->off, sizeof(double) * npd);
         continue;
       }
-      for (int d = 0; d < 3; d++) lagrange_rows(nd, xd, S->n, A[d]);
+      if (S->use_fv && S->n != nd) return -1;    /* WENO p-interpolation: not built */
+      for (int d = 0; d < 3; d++) lagrange_rows_on(nd, xd, S->n, S->use_fv, A[d]);
       for (int v = 0; v < NV; v++)
         tensor_apply(nd, S->n, A[0], A[1], A[2], uold + v * nno + S->off, unew + v * nnn + E->off);
       continue;
     }
+    if (E->use_fv) return -1;                    /* FV h-adaptation: not built */
     if (E->l > 0 && (s = find_loc(Mold, E->p, E->l - 1, E->loc)) >= 0) {   /* child of an old parent */
       const OElm *S = &Mold->e[s];
       const int c = E->loc[E->l - 1] - '0';

@@ -61,6 +61,7 @@ def lib():
             "orc_mesh_clone": (vp, [vp]),
             "orc_mesh_hadapt": (None, [vp, vp]),
             "orc_adapt_state": (i, [vp, vp, vp, vp]),
+            "orc_mesh_set_fv_elems": (None, [vp, vp]),
             "orc_set_fv_rec": (None, [i]),
             "orc_set_fv_midpoint": (None, [i]),
             "orc_dnfn_extrap1": (None, [i, vp]),
@@ -191,7 +192,7 @@ class OMesh:
                              int(nsteps), float(t0), _p(u))
         return u, t
 
-    def adapted(self, u, hflag=None, n=None):
+    def adapted(self, u, hflag=None, n=None, fv=None):
         """(new mesh, new state): h-adaptation by hflag (> 0 refine, < 0
         coarsen 8 flagged siblings), then per-element n (p-change), the state
         interpolated as nmesh's refine and update_node_n_pt_typ do"""
@@ -202,6 +203,9 @@ class OMesh:
             lib().orc_mesh_hadapt(new.h, _p(f))
         if n is not None:
             new.set_n(n)
+        if fv is not None:
+            f = np.ascontiguousarray(fv, dtype=np.int32)
+            lib().orc_mesh_set_fv_elems(new.h, _p(f))
         u = np.ascontiguousarray(u, dtype=np.float64)
         out = np.zeros(5 * new.nnodes)
         r = lib().orc_adapt_state(self.h, _p(u), new.h, _p(out))

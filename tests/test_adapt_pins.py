@@ -144,3 +144,28 @@ def test_product_host_tables_after_adaptation_match_the_oracle():
         assert np.array_equal(m.elm_info(e), om4.elm_info(e)), e
     assert np.array_equal(m.coords(), om4.coords()[1])
     assert m.dt == om4.dt()
+
+
+def test_switch_to_fv_and_back():
+    """DG -> FV on uniform points of the same n (Lagrange from LGL) and back
+    (Lagrange from the uniform points, the inverse of the first matrix):
+    exact on polynomials of degree <= n-1, and the round trip returns any
+    state; the host tables after a switch equal the oracle's"""
+    import nmesh_amd as nm
+    om = OMesh("box", 1, 5, dout=2.0)
+    u = state(om, 4)
+    to_fv = np.array([1, 0, 1, 1, 0, 0, 1, 0], np.int32)
+    fvm, ufv = om.adapted(u, fv=to_fv)
+    check(fvm, ufv, 4)                  # the polynomial at the uniform points
+    v = np.random.default_rng(8).standard_normal(5 * om.nnodes)
+    fvm, vf = om.adapted(v, fv=to_fv)
+    back, vb = fvm.adapted(vf, fv=np.zeros(8, np.int32))
+    assert np.abs(vb - v).max() <= 1e-12 * np.abs(v).max()
+    m = nm.Mesh({"amr_mesh_type": "BoxMesh", "amr_luni": 1, "amr_n0": 5, "amr_n1": 5, "amr_n2": 5,
+                 "amr_BoxMesh_dout": 2.0}, device=-1)
+    nm.amr_setup_mesh(m)
+    m.switch_fv(to_fv)
+    for e in range(8):
+        assert np.array_equal(m.elm_info(e), fvm.elm_info(e)), e
+    assert np.array_equal(m.coords(), fvm.coords()[1])
+    assert m.dt == fvm.dt()

@@ -1,5 +1,6 @@
 """GPU parity of mesh adaptation with the state interpolated on the device
-(libnmx adapt_kernel behind nmx_mesh_hadapt / nmx_mesh_padapt) against the
+(libnmx adapt_kernel behind nmx_mesh_hadapt / nmx_mesh_padapt /
+nmx_mesh_switch_fv) against the
 oracle (orc_mesh_hadapt, orc_adapt_state).  The device applies the three
 1D Lagrange matrices one direction at a time, the oracle the same products
 in its own loop order, so the bar is 1e-12 relative to the field maximum.
@@ -9,7 +10,9 @@ in its own loop order, so the bar is 1e-12 relative to the field maximum.
 - refine then coarsen gives back the state;
 - RK4 steps on the adapted mesh (batched path, interpolated faces) match the
   oracle's steps on its adapted mesh;
-- FV meshes and wrong sizes are refused, the mesh left as it was.
+- FV meshes and wrong sizes are refused, the mesh left as it was;
+- the DG <-> FV element switch (nmx_mesh_switch_fv): the state after the
+  switch, RK4 steps on the mixed mesh, the switch back, and a round trip.
 """
 import numpy as np
 import pytest
@@ -117,3 +120,40 @@ def test_refusals_leave_the_mesh():
     nm.amr_setup_mesh(fv)
     with pytest.raises(nm.NmxError):
         fv.hadapt(np.ones(8, np.int32))
+
+
+@pytest.mark.parametrize("rec", ["WENOm5_2", "WENOm3_2"])
+def test_switch_to_fv_and_back_then_steps(rec):
+    """DG -> FV switch of some elements with the state interpolated on the
+    device, against the oracle; RK4 steps on the mixed mesh; the switch
+    back; a round trip restores the state"""
+    print("switch fv", rec, flush=True)
+    from oracle import orc
+    m, om = shell(6)
+    m.set_par("fv_rec", rec)
+    orc.set_fv_rec(rec)
+    orc.set_fv_midpoint(False)
+    u, _ = om.profile(K, 0.0)
+    m.upload(u)
+    m.time = 0.0
+    to_fv = np.zeros(om.nelem, np.int32)
+    to_fv[np.random.default_rng(5).permutation(om.nelem)[:20]] = 1
+    m.switch_fv(to_fv)
+    om2, u2 = om.adapted(u, fv=to_fv)
+    same_tables(m, om2)
+    assert rel(m.download(), u2) <= TOL
+    t = 0.0
+    for s in range(2):
+        u2, t = om2.evolve(u2, m.dt, 1, t, K, "RK4")
+        nm.evolve_myln(m, 1)
+        assert rel(m.download(), u2) <= TOL, "step %d" % s
+    m.switch_fv(np.zeros(om.nelem, np.int32))
+    om3, u3 = om2.adapted(u2, fv=np.zeros(om.nelem, np.int32))
+    same_tables(m, om3)
+    assert rel(m.download(), u3) <= TOL
+    # round trip of an arbitrary state
+    v = np.random.default_rng(6).standard_normal(5 * om.nnodes)
+    m.upload(v)
+    m.switch_fv(to_fv)
+    m.switch_fv(np.zeros(om.nelem, np.int32))
+    assert rel(m.download(), v) <= TOL
