@@ -114,6 +114,11 @@ struct nmx_ctx {
   bool fvelm = false, anyfv = false;
   double *dwc = nullptr;    // distances between midpoints around each point
   double *dwcall = nullptr; // the same per FV bucket
+  // MRS limiter: neighbour lists, h^1.5, weights, per-element data
+  int *dnbr = nullptr;
+  double *dlimh = nullptr, *dwqall = nullptr, *dldat = nullptr;
+  bool lim_on = false;
+  double lim_alpha = 0;
   // checkpoint CRC: device tables, per-element contributions, result
   uint64_t *dcrctab = nullptr, *dcrcvtab = nullptr, *dcrcklane = nullptr, *dcrcaw = nullptr,
            *dcrcypow = nullptr, *dcrcpart = nullptr, *dcrcout = nullptr;
@@ -171,6 +176,10 @@ static void set_defaults(nmx_ctx *c) {
       {"basis_expfilter_JacobianPower", "1"},
       {"basis_filter_fv", "yes"},
       {"coordinates_midpoint_data", "no"},
+      {"scalarwave1_limiter", "none"},
+      {"limiter_alpha", "5"},
+      {"limiter_beta", "0.5"},
+      {"limiter_scaleBound", "no"},
       {"nmx_path", "plain"},
       {"nmx_graph", "yes"},
       {"nmx_graph_steps", "8"},
@@ -242,7 +251,8 @@ static void free_mesh_dev(nmx_ctx *c) {
                      (void **)&c->dCk, (void **)&c->dwc, (void **)&c->dwcall,
                      (void **)&c->dcrctab, (void **)&c->dcrcvtab, (void **)&c->dcrcklane,
                      (void **)&c->dcrcaw, (void **)&c->dcrcypow,
-                     (void **)&c->dcrcpart, (void **)&c->dcrcout};
+                     (void **)&c->dcrcpart, (void **)&c->dcrcout, (void **)&c->dnbr,
+                     (void **)&c->dlimh, (void **)&c->dwqall, (void **)&c->dldat};
     for (auto q : ptrs)
       if (*q) { (void)hipFree(*q); *q = nullptr; }
   }
@@ -307,6 +317,9 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
       (k == "fv_divf_use_only_right_flux" && v != "yes") || (k == "fv_WENOm3_opt_weightratio" && v != "2") ||
       (k == "fv_surface_interp" && v != "linear") || (k == "fv2dg_interp_use_extrap1" && v != "no"))
     return fail(NMX_EUNSUP, k + ": only nmesh's default is built");
+  if (k == "scalarwave1_limiter" && v != "none" && v != "MRS")
+    return fail(v == "minmodB" ? NMX_EUNSUP : NMX_EINVAL,
+                "scalarwave1_limiter: none and MRS are built (minmodB is not)");
   if (k == "coordinates_midpoint_data" && v != "no" && v != "yes" && v != "yes all")
     return fail(NMX_EINVAL, "coordinates_midpoint_data must be no, yes or yes all");
   if ((k == "evolve_filter_all" || k == "basis_filter_fv") && v != "yes" && v != "no")
@@ -1456,34 +1469,85 @@ extern "C" int nmx_checkpoint_load_vars(nmx_ctx *c, int reg, const char *path) {
   return NMX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// MRS limiter (scalarwave1_limiter = MRS)
+// ---------------------------------------------------------------------------
+
+// limiter on or off from the parameters; its tables on first use
+static int limiter_prepare(nmx_ctx *c) {
+  c->lim_on = P(c, "scalarwave1_limiter") == "MRS";
+  if (!c->lim_on) return NMX_OK;
+  if (c->anyfv) return fail(NMX_EUNSUP, "the MRS limiter on FV elements is not built");
+  c->lim_alpha = Pd(c, "limiter_alpha");
+  if (c->dnbr) return NMX_OK;
+  const Mesh &M = c->mesh;
+  const size_t ne = M.e.size();
+  std::vector<int> nbr(24 * ne, -1);
+  std::vector<double> h15(ne), wq(9 * 10, 0.0);
+  for (size_t e = 0; e < ne; ++e) {
+    const Elm &E = M.e[e];
+    int k = 0;
+    for (int f = 0; f < 6; ++f)
+      for (int ni = 0; ni < E.nfnb[f]; ++ni) {
+        if (k >= 24) return fail(NMX_EUNSUP, "more than 24 face neighbours");
+        nbr[24 * e + k++] = E.fnb[f][ni];
+      }
+    const double *bb = E.bbox;
+    const double h = std::max(bb[1] - bb[0], std::max(bb[3] - bb[2], bb[5] - bb[4]));
+    h15[e] = std::pow(h, 1.5);
+  }
+  for (int n = 2; n <= 10; ++n) {
+    const Basis &b = lgl_basis(n);
+    for (int i = 0; i < n; ++i) wq[(n - 2) * 10 + i] = b.w[i];
+  }
+  int r;
+  if ((r = dev_upload(&c->dnbr, nbr)) || (r = dev_upload(&c->dlimh, h15)) || (r = dev_upload(&c->dwqall, wq)) ||
+      (r = dev_alloc(&c->dldat, 15 * ne)))
+    return r;
+  return NMX_OK;
+}
+
+static int launch_limiter_on(nmx_ctx *c, double *reg) {
+  LimArgs a;
+  a.u = reg; a.eoff = c->deoff; a.en = c->den; a.nbr = c->dnbr; a.h15 = c->dlimh; a.wq = c->dwqall;
+  a.ldat = c->dldat; a.alpha = c->lim_alpha; a.nelem = (int)c->mesh.e.size();
+  hipError_t err = launch_limiter(a, c->stream);
+  if (err != hipSuccess) return fail(NMX_EHIP, std::string("limiter launch: ") + hipGetErrorString(err));
+  return NMX_OK;
+}
+
 // issue the stage launches of one step; stage times at dT[tbase + s]
 static int issue_step(nmx_ctx *c, int method, double dt, int tbase, bool swap_parity) {
   double *U = swap_parity ? c->dA : c->dU;
   double *A = swap_parity ? c->dU : c->dA;
   StageArgs a = base_args(c);
   int r;
+  // the MRS limiter acts on every later stage's input w (RungeKutta.c:45,
+  // 56, 67, 161, 173) and on the final u (evolve.c:143-144)
+  auto lim = [&](double *reg) { return c->lim_on ? launch_limiter_on(c, reg) : NMX_OK; };
   if (method == 0) {   // evolve_RK4_mesh (RungeKutta.c:19-82)
     a.win = U; a.U = U; a.Ain = nullptr; a.Aout = A; a.cA = dt / 6.; a.Wout = c->dW0; a.wmode = 1;
     a.cW = dt / 2.; a.tidx = tbase + 0;
-    if ((r = launch(c, a))) return r;
+    if ((r = launch(c, a)) || (r = lim(c->dW0))) return r;
     a.win = c->dW0; a.Ain = A; a.cA = dt / 3.; a.Wout = c->dW1; a.cW = dt / 2.; a.tidx = tbase + 1;
-    if ((r = launch(c, a))) return r;
+    if ((r = launch(c, a)) || (r = lim(c->dW1))) return r;
     a.win = c->dW1; a.cA = dt / 3.; a.Wout = c->dW0; a.cW = dt; a.tidx = tbase + 2;
-    if ((r = launch(c, a))) return r;
+    if ((r = launch(c, a)) || (r = lim(c->dW0))) return r;
     a.win = c->dW0; a.cA = dt / 6.; a.Wout = nullptr; a.wmode = 0; a.tidx = tbase + 3;
     if ((r = launch(c, a))) return r;
   } else if (method == 1) {   // evolve_sspRK3_mesh (RungeKutta.c:130-188)
     a.win = U; a.U = U; a.Aout = A; a.cA = dt / 6.; a.Wout = c->dW0; a.wmode = 1; a.cW = dt;
     a.tidx = tbase + 0;
-    if ((r = launch(c, a))) return r;
+    if ((r = launch(c, a)) || (r = lim(c->dW0))) return r;
     a.win = c->dW0; a.Ain = A; a.cA = dt / 6.; a.Wout = c->dW1; a.wmode = 2; a.cW = dt; a.tidx = tbase + 1;
-    if ((r = launch(c, a))) return r;
+    if ((r = launch(c, a)) || (r = lim(c->dW1))) return r;
     a.win = c->dW1; a.cA = dt * 2. / 3.; a.Wout = nullptr; a.wmode = 0; a.tidx = tbase + 2;
     if ((r = launch(c, a))) return r;
   } else {   // evolve_Euler_mesh (RungeKutta.c:85-115)
     a.win = U; a.U = U; a.Aout = A; a.cA = dt; a.wmode = 0; a.tidx = tbase;
     if ((r = launch(c, a))) return r;
   }
+  if ((r = lim(A))) return r;
   // FILTER bin after the full step (main.c:628-630)
   if (c->filt_on && (r = launch_filters(c, A))) return r;
   return NMX_OK;
@@ -1501,6 +1565,7 @@ static void stage_times(int method, double t, double dt, double *ts) {
 extern "C" int nmx_autotune(nmx_ctx *c) {
   NEED_DEV(c);
   HIPCHK(hipSetDevice(c->device));
+  if (int r = limiter_prepare(c)) return r;
   if (P(c, "nmx_path") == "perelement" || c->fvelm) { c->tuned = true; return NMX_OK; }
   double *backup = nullptr;
   int r;
@@ -1584,6 +1649,7 @@ extern "C" int nmx_evolve(nmx_ctx *c, int nsteps) {
     int r = filter_prepare(c, on);
     if (r) return r;
     c->filt_on = on;
+    if ((r = limiter_prepare(c))) return r;
   }
   if (!c->tuned && P(c, "nmx_autotune") == "yes" && nsteps > 0) {
     int r = nmx_autotune(c);
@@ -1689,6 +1755,8 @@ static AdjArgs adj_base_args(nmx_ctx *c) {
     bool filt_ = false;                                                                  \
     if (int r_ = filter_prepare((c), filt_)) return r_;                                  \
     if (filt_) return fail(NMX_EUNSUP, "the adjoint of the exponential filter is not built"); \
+    if (P((c), "scalarwave1_limiter") != "none")                                          \
+      return fail(NMX_EUNSUP, "the adjoint of the limiter is not built");                \
   } while (0)
 
 static int launch_adj(nmx_ctx *c, const AdjArgs &a) {

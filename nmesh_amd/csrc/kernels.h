@@ -105,6 +105,20 @@ struct AdaptArgs {
 };
 hipError_t launch_adapt(const AdaptArgs &a, hipStream_t s);
 
+// The MRS limiter in place on a register (element blocks).
+struct LimArgs {
+  double *u = nullptr;
+  const long *eoff = nullptr;
+  const int *en = nullptr;
+  const int *nbr = nullptr;        // [e][24]: face neighbours (-1 none)
+  const double *h15 = nullptr;     // [e]: (largest box width in patch coordinates)^1.5
+  const double *wq = nullptr;      // [n - 2][10]: LGL quadrature weights
+  double *ldat = nullptr;          // [e][5][3]: average, min, max
+  double alpha = 0;
+  int nelem = 0;
+};
+hipError_t launch_limiter(const LimArgs &a, hipStream_t s);
+
 // flat [v][node] <-> element blocks [e][v][np_e] on the device
 // (to_blocks: 1 flat -> blocks, 0 blocks -> flat)
 hipError_t launch_layout(const double *src, double *dst, const long *eoff, const int *en, int nelem,

@@ -1675,6 +1675,107 @@ static void vladdto(long N, double *r, double ca, const double *a)
 }
 
 /* method: 0 RK4, 1 sspRK3, 2 Euler; u advanced in place; returns final t */
+/* ------------------------------------------------------------------ */
+/* MRS limiter (src/main/limiter/limiter.c:56-196, arXiv:1507.03024) as  */
+/* scalarwave1 registers it with scalarwave1_limiter = MRS               */
+/* (scalarwave1.c:687-691), applied by evolve_limiter_mesh               */
+/* (evolve.c:430-541)                                                    */
+/* ------------------------------------------------------------------ */
+
+static double g_lim_alpha = -1.;   /* < 0: no limiter */
+void orc_set_limiter_MRS(double alpha) { g_lim_alpha = alpha; }
+
+/* var_nodeaverage (basis.c:462-510): the LGL quadrature along x, then y,
+   then z, over the Xb volume 8 */
+static double elm_average(const OElm *E, const double *q)
+{
+  const int n = E->n;
+  const double *w = basis(n)->Wq;
+  double s2[ORC_MAXN];
+  for (int k = 0; k < n; k++) {
+    double s1[ORC_MAXN];
+    for (int j = 0; j < n; j++) {
+      double s = 0.;
+      for (int i = 0; i < n; i++) s += w[i] * q[i + n * (j + n * k)];
+      s1[j] = s;
+    }
+    double s = 0.;
+    for (int j = 0; j < n; j++) s += w[j] * s1[j];
+    s2[k] = s;
+  }
+  double s = 0.;
+  for (int k = 0; k < n; k++) s += w[k] * s2[k];
+  return s * 0.125;
+}
+
+static double mrs_phi(double y)
+{
+  if (y < 0.) return 0.;
+  return fmin(y / 1.1, 1.);
+}
+
+static double mrs_theta(double Mi, double wbar, double wMi)
+{
+  double den = wMi - wbar;
+  if (den == 0.) return 1.;
+  return mrs_phi((Mi - wbar) / den);
+}
+
+/* limdata_MRS on every element, then limiter_MRS on every element */
+void orc_limit_MRS(const OMesh *M, double *u)
+{
+  long nn = M->nnodes;
+  double *ld = malloc(sizeof(double) * 15 * M->nelem);
+  for (int e = 0; e < M->nelem; e++) {
+    const OElm *E = &M->e[e];
+    int np = elm_np(E);
+    for (int v = 0; v < NV; v++) {
+      const double *q = u + v * nn + E->off;
+      double mn = q[0], mx = q[0];
+      for (int i = 1; i < np; i++) { if (q[i] < mn) mn = q[i]; if (q[i] > mx) mx = q[i]; }
+      ld[15 * e + 3 * v + 0] = elm_average(E, q);
+      ld[15 * e + 3 * v + 1] = mn;
+      ld[15 * e + 3 * v + 2] = mx;
+    }
+  }
+  for (int e = 0; e < M->nelem; e++) {
+    const OElm *E = &M->e[e];
+    const double *bb = E->bbox;
+    double h = fmax(bb[1] - bb[0], fmax(bb[3] - bb[2], bb[5] - bb[4]));
+    double ah = g_lim_alpha * pow(h, 1.5);
+    double thM = 1.7976931348623157e308, thm = 1.7976931348623157e308;
+    for (int v = 0; v < NV; v++) {
+      double wbar = ld[15 * e + 3 * v], wmi = ld[15 * e + 3 * v + 1], wMi = ld[15 * e + 3 * v + 2];
+      double Mi = -1.7976931348623157e308, mi = 1.7976931348623157e308;
+      for (int f = 0; f < 6; f++)
+        for (int ni = 0; ni < E->nfnb[f]; ni++) {
+          int nb = E->fnb[f][ni];
+          double ma = ld[15 * nb + 3 * v + 1], Ma = ld[15 * nb + 3 * v + 2];
+          if (ma < mi) mi = ma;
+          if (Ma > Mi) Mi = Ma;
+        }
+      mi = fmin(wbar - ah, mi);
+      Mi = fmax(wbar + ah, Mi);
+      double a = mrs_theta(Mi, wbar, wMi), b = mrs_theta(mi, wbar, wmi);
+      if (a < thM) thM = a;
+      if (b < thm) thm = b;
+    }
+    double th = fmin(1., fmin(thm, thM));
+    int np = elm_np(E);
+    for (int v = 0; v < NV; v++) {
+      double *q = u + v * nn + E->off;
+      double qbar = elm_average(E, q);
+      for (int i = 0; i < np; i++) q[i] = qbar + th * (q[i] - qbar);
+    }
+  }
+  free(ld);
+}
+
+static void limit_if_on(const OMesh *M, double *u)
+{
+  if (g_lim_alpha >= 0.) orc_limit_MRS(M, u);
+}
+
 double orc_evolve(const OMesh *M, int numflux, const double *k, int method, double dt,
                   int nsteps, double t, double *u)
 {
@@ -1691,12 +1792,15 @@ double orc_evolve(const OMesh *M, int numflux, const double *k, int method, doub
       memcpy(u, w, sizeof(double) * N);
       vladdto(N, u, dt / 6., r);
       vladd(N, w, 1., up, dt / 2., r);
+      limit_if_on(M, w);
       orc_rhs(M, numflux, k, w, t + 0.5 * dt, r);
       vladdto(N, u, dt / 3., r);
       vladd(N, w, 1., up, dt / 2., r);
+      limit_if_on(M, w);
       orc_rhs(M, numflux, k, w, t + 0.5 * dt, r);
       vladdto(N, u, dt / 3., r);
       vladd(N, w, 1., up, dt, r);
+      limit_if_on(M, w);
       orc_rhs(M, numflux, k, w, t + dt, r);
       vladdto(N, u, dt / 6., r);
     } else if (method == 1) {
@@ -1704,10 +1808,12 @@ double orc_evolve(const OMesh *M, int numflux, const double *k, int method, doub
       memcpy(u, w, sizeof(double) * N);
       vladdto(N, u, dt / 6., r);
       vladd(N, w, 1., up, dt, r);
+      limit_if_on(M, w);
       orc_rhs(M, numflux, k, w, t + dt, r);
       vladdto(N, u, dt / 6., r);
       vladdto(N, w, dt, r);
       vladd(N, w, 0.75, up, 0.25, w);
+      limit_if_on(M, w);
       orc_rhs(M, numflux, k, w, t + 0.5 * dt, r);
       vladdto(N, u, dt * 2. / 3., r);
     } else {
@@ -1715,6 +1821,8 @@ double orc_evolve(const OMesh *M, int numflux, const double *k, int method, doub
       memcpy(u, w, sizeof(double) * N);
       vladdto(N, u, dt, r);
     }
+    /* the final u is limited in evolve_myln (evolve.c:143-144) */
+    limit_if_on(M, u);
     te += dt;
     t = (double)(Te + te);
   }

@@ -62,6 +62,8 @@ def lib():
             "orc_mesh_hadapt": (None, [vp, vp]),
             "orc_adapt_state": (i, [vp, vp, vp, vp]),
             "orc_mesh_set_fv_elems": (None, [vp, vp]),
+            "orc_set_limiter_MRS": (None, [d]),
+            "orc_limit_MRS": (None, [vp, vp]),
             "orc_set_fv_rec": (None, [i]),
             "orc_set_fv_midpoint": (None, [i]),
             "orc_dnfn_extrap1": (None, [i, vp]),
@@ -213,6 +215,17 @@ class OMesh:
             raise ValueError("orc_adapt_state failed (%d)" % r)
         return new, out
 
+    def limit_MRS(self, u, alpha):
+        """one application of the MRS limiter to state u (limdata_MRS then
+        limiter_MRS on every element, limiter.c:56-196)"""
+        u = np.array(u, dtype=np.float64, copy=True)
+        set_limiter_MRS(alpha)
+        try:
+            lib().orc_limit_MRS(self.h, _p(u))
+        finally:
+            set_limiter_MRS(None)
+        return u
+
     def vars_crc(self, u):
         """the reference's "vars" CRC-64 of state u (norms.c:440-458)"""
         u = np.ascontiguousarray(u, dtype=np.float64)
@@ -230,6 +243,12 @@ class OMesh:
         u = np.array(u, dtype=np.float64, copy=True)
         lib().orc_filter(self.h, _p(u), int(varmask), float(alp), float(s), int(dn), int(jpow))
         return u
+
+
+def set_limiter_MRS(alpha):
+    """scalarwave1_limiter = MRS with limiter_alpha = alpha in the oracle's
+    RK steps (None: no limiter)"""
+    lib().orc_set_limiter_MRS(-1.0 if alpha is None else float(alpha))
 
 
 def set_fv_midpoint(on):
