@@ -116,9 +116,11 @@ struct nmx_ctx {
   double *dwcall = nullptr; // the same per FV bucket
   // MRS limiter: neighbour lists, h^1.5, weights, per-element data
   int *dnbr = nullptr;
-  double *dlimh = nullptr, *dwqall = nullptr, *dldat = nullptr;
+  double *dlimh = nullptr, *dlimhw = nullptr, *dwqall = nullptr, *dldat = nullptr, *damat = nullptr,
+         *dsmat = nullptr;
   bool lim_on = false;
-  double lim_alpha = 0;
+  int lim_kind = 0, lim_scale = 0;
+  double lim_alpha = 0, lim_beta = 0.5;
   // checkpoint CRC: device tables, per-element contributions, result
   uint64_t *dcrctab = nullptr, *dcrcvtab = nullptr, *dcrcklane = nullptr, *dcrcaw = nullptr,
            *dcrcypow = nullptr, *dcrcpart = nullptr, *dcrcout = nullptr;
@@ -252,7 +254,8 @@ static void free_mesh_dev(nmx_ctx *c) {
                      (void **)&c->dcrctab, (void **)&c->dcrcvtab, (void **)&c->dcrcklane,
                      (void **)&c->dcrcaw, (void **)&c->dcrcypow,
                      (void **)&c->dcrcpart, (void **)&c->dcrcout, (void **)&c->dnbr,
-                     (void **)&c->dlimh, (void **)&c->dwqall, (void **)&c->dldat};
+                     (void **)&c->dlimh, (void **)&c->dwqall, (void **)&c->dldat, (void **)&c->dlimhw,
+                     (void **)&c->damat, (void **)&c->dsmat};
     for (auto q : ptrs)
       if (*q) { (void)hipFree(*q); *q = nullptr; }
   }
@@ -317,9 +320,10 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
       (k == "fv_divf_use_only_right_flux" && v != "yes") || (k == "fv_WENOm3_opt_weightratio" && v != "2") ||
       (k == "fv_surface_interp" && v != "linear") || (k == "fv2dg_interp_use_extrap1" && v != "no"))
     return fail(NMX_EUNSUP, k + ": only nmesh's default is built");
-  if (k == "scalarwave1_limiter" && v != "none" && v != "MRS")
-    return fail(v == "minmodB" ? NMX_EUNSUP : NMX_EINVAL,
-                "scalarwave1_limiter: none and MRS are built (minmodB is not)");
+  if (k == "scalarwave1_limiter" && v != "none" && v != "MRS" && v != "minmodB")
+    return fail(NMX_EINVAL, "scalarwave1_limiter must be none, MRS or minmodB");
+  if (k == "limiter_scaleBound" && v != "yes" && v != "no")
+    return fail(NMX_EINVAL, "limiter_scaleBound must be yes or no");
   if (k == "coordinates_midpoint_data" && v != "no" && v != "yes" && v != "yes all")
     return fail(NMX_EINVAL, "coordinates_midpoint_data must be no, yes or yes all");
   if ((k == "evolve_filter_all" || k == "basis_filter_fv") && v != "yes" && v != "no")
@@ -1475,42 +1479,48 @@ extern "C" int nmx_checkpoint_load_vars(nmx_ctx *c, int reg, const char *path) {
 
 // limiter on or off from the parameters; its tables on first use
 static int limiter_prepare(nmx_ctx *c) {
-  c->lim_on = P(c, "scalarwave1_limiter") == "MRS";
+  const std::string &lim = P(c, "scalarwave1_limiter");
+  c->lim_on = lim == "MRS" || lim == "minmodB";
   if (!c->lim_on) return NMX_OK;
-  if (c->anyfv) return fail(NMX_EUNSUP, "the MRS limiter on FV elements is not built");
+  if (c->anyfv) return fail(NMX_EUNSUP, "the limiters on FV elements are not built");
+  c->lim_kind = lim == "minmodB" ? 1 : 0;
   c->lim_alpha = Pd(c, "limiter_alpha");
+  c->lim_beta = Pd(c, "limiter_beta");
+  c->lim_scale = P(c, "limiter_scaleBound") == "yes";
   if (c->dnbr) return NMX_OK;
   const Mesh &M = c->mesh;
   const size_t ne = M.e.size();
   std::vector<int> nbr(24 * ne, -1);
-  std::vector<double> h15(ne), wq(9 * 10, 0.0);
+  std::vector<double> h15(ne), hw(ne), wq(9 * 10, 0.0), am(9 * 100, 0.0), sm(9 * 100, 0.0);
   for (size_t e = 0; e < ne; ++e) {
     const Elm &E = M.e[e];
-    int k = 0;
-    for (int f = 0; f < 6; ++f)
-      for (int ni = 0; ni < E.nfnb[f]; ++ni) {
-        if (k >= 24) return fail(NMX_EUNSUP, "more than 24 face neighbours");
-        nbr[24 * e + k++] = E.fnb[f][ni];
-      }
+    for (int f = 0; f < 6; ++f) {
+      if (E.nfnb[f] > 4) return fail(NMX_EUNSUP, "more than 4 neighbours on a face");
+      for (int ni = 0; ni < E.nfnb[f]; ++ni) nbr[24 * e + 4 * f + ni] = E.fnb[f][ni];
+    }
     const double *bb = E.bbox;
     const double h = std::max(bb[1] - bb[0], std::max(bb[3] - bb[2], bb[5] - bb[4]));
     h15[e] = std::pow(h, 1.5);
+    hw[e] = h;
   }
   for (int n = 2; n <= 10; ++n) {
     const Basis &b = lgl_basis(n);
     for (int i = 0; i < n; ++i) wq[(n - 2) * 10 + i] = b.w[i];
+    legendre_analysis_synthesis(n, am.data() + (n - 2) * 100, sm.data() + (n - 2) * 100);
   }
   int r;
-  if ((r = dev_upload(&c->dnbr, nbr)) || (r = dev_upload(&c->dlimh, h15)) || (r = dev_upload(&c->dwqall, wq)) ||
-      (r = dev_alloc(&c->dldat, 15 * ne)))
+  if ((r = dev_upload(&c->dnbr, nbr)) || (r = dev_upload(&c->dlimh, h15)) || (r = dev_upload(&c->dlimhw, hw)) ||
+      (r = dev_upload(&c->dwqall, wq)) || (r = dev_upload(&c->damat, am)) || (r = dev_upload(&c->dsmat, sm)) ||
+      (r = dev_alloc(&c->dldat, 20 * ne)))
     return r;
   return NMX_OK;
 }
 
 static int launch_limiter_on(nmx_ctx *c, double *reg) {
   LimArgs a;
-  a.u = reg; a.eoff = c->deoff; a.en = c->den; a.nbr = c->dnbr; a.h15 = c->dlimh; a.wq = c->dwqall;
-  a.ldat = c->dldat; a.alpha = c->lim_alpha; a.nelem = (int)c->mesh.e.size();
+  a.u = reg; a.eoff = c->deoff; a.en = c->den; a.nbr = c->dnbr; a.h15 = c->dlimh; a.hw = c->dlimhw;
+  a.wq = c->dwqall; a.amat = c->damat; a.smat = c->dsmat; a.ldat = c->dldat; a.alpha = c->lim_alpha;
+  a.beta = c->lim_beta; a.kind = c->lim_kind; a.scale_bound = c->lim_scale; a.nelem = (int)c->mesh.e.size();
   hipError_t err = launch_limiter(a, c->stream);
   if (err != hipSuccess) return fail(NMX_EHIP, std::string("limiter launch: ") + hipGetErrorString(err));
   return NMX_OK;

@@ -105,16 +105,21 @@ struct AdaptArgs {
 };
 hipError_t launch_adapt(const AdaptArgs &a, hipStream_t s);
 
-// The MRS limiter in place on a register (element blocks).
+// The MRS or minmodB limiter in place on a register (element blocks).
 struct LimArgs {
   double *u = nullptr;
   const long *eoff = nullptr;
   const int *en = nullptr;
-  const int *nbr = nullptr;        // [e][24]: face neighbours (-1 none)
-  const double *h15 = nullptr;     // [e]: (largest box width in patch coordinates)^1.5
+  const int *nbr = nullptr;        // [e][6][4]: face neighbours in order (-1 none)
+  const double *h15 = nullptr;     // [e]: largest box width h in patch coordinates ^1.5 (MRS)
+  const double *hw = nullptr;      // [e]: h (minmodB)
   const double *wq = nullptr;      // [n - 2][10]: LGL quadrature weights
-  double *ldat = nullptr;          // [e][5][3]: average, min, max
-  double alpha = 0;
+  const double *amat = nullptr;    // [n - 2][100]: normalized Legendre analysis A[i*n + j]
+  const double *smat = nullptr;    // [n - 2][100]: synthesis S[j*n + i]
+  double *ldat = nullptr;          // [e][5][4]: MRS average, min, max; minmodB c000, c100, c010, c001
+  double alpha = 0, beta = 0.5;
+  int kind = 0;                    // 0 MRS, 1 minmodB
+  int scale_bound = 0;             // minmodB: bound scaled by the neighbours' mean |c000|
   int nelem = 0;
 };
 hipError_t launch_limiter(const LimArgs &a, hipStream_t s);

@@ -126,6 +126,27 @@ const Basis &lgl_basis(int n) {
 // recurrence of Legendre.c:14-38), around f(i) = exp(-alp (i/N)^s),
 // N = n-1-dn, 1 if not positive (expfilter_coeff_array, filter.c:20-55).
 // F[j*n + l].
+void legendre_analysis_synthesis(int n, double *A, double *S) {
+  const Basis &b = lgl_basis(n);
+  const int N = n - 1;
+  for (int i = 0; i < n; ++i) {
+    double ooci = 0.5 * (2 * i + 1);
+    if (i == N && N != 0) ooci = 0.5 * i;
+    const double oosci = std::sqrt(ooci);
+    for (int j = 0; j < n; ++j) {
+      const double x = b.x[j];
+      double P0 = 1.0, P1 = x, Pk = (i == 0) ? 1.0 : x;
+      for (int k = 2; k <= i; ++k) {
+        Pk = (x * (2 * k - 1) * P1 - (k - 1) * P0) / k;
+        P0 = P1;
+        P1 = Pk;
+      }
+      S[j * n + i] = Pk * oosci;
+      A[i * n + j] = b.w[j] * S[j * n + i];
+    }
+  }
+}
+
 void filter_matrix(int n, double alp, double s, int dn, double *F) {
   const Basis &b = lgl_basis(n);
   const int N = n - 1;

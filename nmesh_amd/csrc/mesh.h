@@ -50,6 +50,10 @@ struct Basis {
 };
 const Basis &lgl_basis(int n);
 void filter_matrix(int n, double alp, double s, int dn, double *F);
+// normalized Legendre analysis A[i*n + j] = w_j P_i(x_j)/sqrt(c_i) and
+// synthesis S[j*n + i] = P_i(x_j)/sqrt(c_i) on the n LGL nodes, c_N = 2/N
+// (LGL_AT_ST_matrices, src/main/basis/Legendre.c:257-307)
+void legendre_analysis_synthesis(int n, double *A, double *S);
 // uniform points incl. both faces (x only)
 const Basis &uniform_basis(int n);
 

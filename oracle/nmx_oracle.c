@@ -1684,6 +1684,96 @@ static void vladdto(long N, double *r, double ca, const double *a)
 
 static double g_lim_alpha = -1.;   /* < 0: no limiter */
 void orc_set_limiter_MRS(double alpha) { g_lim_alpha = alpha; }
+/* minmodB (limiter.c:200-380, arXiv:1506.06140): alpha < 0 is off */
+static double g_mm_alpha = -1., g_mm_beta = 0.5;
+static int g_mm_scale = 0;
+void orc_set_limiter_minmodB(double alpha, double beta, int scaleBound)
+{
+  g_mm_alpha = alpha; g_mm_beta = beta; g_mm_scale = scaleBound;
+}
+static void lgl_at_st(int n, const double *x, const double *w, double *AT, double *ST);
+static void contract1(int n, const double *MT, int dir, const double *in, double *out);
+
+static double minmod3(double a, double b, double c)
+{
+  if (a * b >= 0 && b * c >= 0)
+    return (a > 0. ? 1. : -1.) * fmin(fabs(a), fmin(fabs(b), fabs(c)));
+  return 0.;
+}
+
+static double minmod3B(double a, double b, double c, double aBound)
+{
+  if (fabs(a) <= aBound) return a;
+  return minmod3(a, b, c);
+}
+
+#define ORC_DEQUAL(a, b) (!((a) < (b) - 1e-10 || (b) < (a) - 1e-10))
+
+/* limdata_c000_100_010_001 on every element (the normalized Legendre
+   analysis along x, y, z: LGL_AT_ST_matrices), then limiter_minmodB on
+   every element: the three slopes limited against the neighbours' face
+   averages; a variable whose slopes change is replaced by its mean and
+   limited slopes */
+void orc_limit_minmodB(const OMesh *M, double *u)
+{
+  long nn = M->nnodes;
+  double *ld = malloc(sizeof(double) * 20 * M->nelem);
+  for (int e = 0; e < M->nelem; e++) {
+    const OElm *E = &M->e[e];
+    int n = E->n, np = elm_np(E);
+    OBasis *b = basis(n);
+    double AT[ORC_MAXN * ORC_MAXN], ST[ORC_MAXN * ORC_MAXN];
+    lgl_at_st(n, b->Xb, b->Wq, AT, ST);
+    double *c = malloc(sizeof(double) * np), *t = malloc(sizeof(double) * np);
+    for (int v = 0; v < NV; v++) {
+      contract1(n, AT, 0, u + v * nn + E->off, t);
+      contract1(n, AT, 1, t, c);
+      contract1(n, AT, 2, c, t);
+      ld[20 * e + 4 * v + 0] = t[0];
+      ld[20 * e + 4 * v + 1] = n > 1 ? t[1] : 0.;
+      ld[20 * e + 4 * v + 2] = n > 1 ? t[n] : 0.;
+      ld[20 * e + 4 * v + 3] = n > 1 ? t[n * n] : 0.;
+    }
+    free(c); free(t);
+  }
+  const double bos3 = g_mm_beta / sqrt(3.);
+  for (int e = 0; e < M->nelem; e++) {
+    const OElm *E = &M->e[e];
+    int n = E->n, np = elm_np(E);
+    const double *bb = E->bbox;
+    double h = fmax(bb[1] - bb[0], fmax(bb[3] - bb[2], bb[5] - bb[4]));
+    double Mt_h = g_mm_alpha * h;
+    for (int v = 0; v < NV; v++) {
+      const double *my = ld + 20 * e + 4 * v;
+      double w0[6], w0L1 = 0.;
+      for (int f = 0; f < 6; f++) {
+        double av = 0.;
+        for (int ni = 0; ni < E->nfnb[f]; ni++) av += ld[20 * E->fnb[f][ni] + 4 * v];
+        if (E->nfnb[f] > 0) av = av / E->nfnb[f];
+        w0[f] = av;
+        w0L1 += fabs(av) * 0.166666666666667;
+      }
+      double bound = g_mm_scale ? Mt_h * w0L1 : Mt_h;
+      double w0c = my[0];
+      double l1 = minmod3B(my[1], bos3 * (w0c - w0[0]), bos3 * (w0[1] - w0c), bound);
+      double l2 = minmod3B(my[2], bos3 * (w0c - w0[2]), bos3 * (w0[3] - w0c), bound);
+      double l3 = minmod3B(my[3], bos3 * (w0c - w0[4]), bos3 * (w0[5] - w0c), bound);
+      if (ORC_DEQUAL(l1, my[1]) && ORC_DEQUAL(l2, my[2]) && ORC_DEQUAL(l3, my[3])) continue;
+      OBasis *b = basis(n);
+      double AT[ORC_MAXN * ORC_MAXN], ST[ORC_MAXN * ORC_MAXN];
+      lgl_at_st(n, b->Xb, b->Wq, AT, ST);
+      double *c = calloc(np, sizeof(double)), *t = malloc(sizeof(double) * np);
+      c[0] = w0c;
+      if (n > 1) { c[1] = l1; c[n] = l2; c[n * n] = l3; }
+      double *q = u + v * nn + E->off;
+      contract1(n, ST, 0, c, t);
+      contract1(n, ST, 1, t, c);
+      contract1(n, ST, 2, c, q);
+      free(c); free(t);
+    }
+  }
+  free(ld);
+}
 
 /* var_nodeaverage (basis.c:462-510): the LGL quadrature along x, then y,
    then z, over the Xb volume 8 */
@@ -1774,6 +1864,7 @@ void orc_limit_MRS(const OMesh *M, double *u)
 static void limit_if_on(const OMesh *M, double *u)
 {
   if (g_lim_alpha >= 0.) orc_limit_MRS(M, u);
+  if (g_mm_alpha >= 0.) orc_limit_minmodB(M, u);
 }
 
 double orc_evolve(const OMesh *M, int numflux, const double *k, int method, double dt,

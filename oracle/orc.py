@@ -64,6 +64,8 @@ def lib():
             "orc_mesh_set_fv_elems": (None, [vp, vp]),
             "orc_set_limiter_MRS": (None, [d]),
             "orc_limit_MRS": (None, [vp, vp]),
+            "orc_set_limiter_minmodB": (None, [d, d, i]),
+            "orc_limit_minmodB": (None, [vp, vp]),
             "orc_set_fv_rec": (None, [i]),
             "orc_set_fv_midpoint": (None, [i]),
             "orc_dnfn_extrap1": (None, [i, vp]),
@@ -226,6 +228,16 @@ class OMesh:
             set_limiter_MRS(None)
         return u
 
+    def limit_minmodB(self, u, alpha, beta=0.5, scale_bound=False):
+        """one application of the minmodB limiter (limiter.c:200-380)"""
+        u = np.array(u, dtype=np.float64, copy=True)
+        set_limiter_minmodB(alpha, beta, scale_bound)
+        try:
+            lib().orc_limit_minmodB(self.h, _p(u))
+        finally:
+            set_limiter_minmodB(None)
+        return u
+
     def vars_crc(self, u):
         """the reference's "vars" CRC-64 of state u (norms.c:440-458)"""
         u = np.ascontiguousarray(u, dtype=np.float64)
@@ -249,6 +261,13 @@ def set_limiter_MRS(alpha):
     """scalarwave1_limiter = MRS with limiter_alpha = alpha in the oracle's
     RK steps (None: no limiter)"""
     lib().orc_set_limiter_MRS(-1.0 if alpha is None else float(alpha))
+
+
+def set_limiter_minmodB(alpha, beta=0.5, scale_bound=False):
+    """scalarwave1_limiter = minmodB with limiter_alpha, limiter_beta and
+    limiter_scaleBound in the oracle's RK steps (alpha None: off)"""
+    lib().orc_set_limiter_minmodB(-1.0 if alpha is None else float(alpha), float(beta),
+                                  1 if scale_bound else 0)
 
 
 def set_fv_midpoint(on):

@@ -1,6 +1,7 @@
-"""GPU parity of the MRS limiter (scalarwave1_limiter = MRS; libnmx
-lim_data_kernel / lim_apply_kernel) inside the RK steps against the oracle's
-steps with the same limiter (orc_limit_MRS in orc_evolve): every later
+"""GPU parity of the limiters (scalarwave1_limiter = MRS: libnmx
+lim_data_kernel / lim_apply_kernel; minmodB: mm_data_kernel /
+mm_apply_kernel) inside the RK steps against the oracle's steps with the
+same limiter (orc_limit_MRS / orc_limit_minmodB in orc_evolve): every later
 stage's input and the final u of each step are limited (RungeKutta.c:19-188,
 evolve.c:143-144).  The limiter's averages, theta and update are evaluated
 in the reference's operation order on both sides; the RHS differs at
@@ -22,11 +23,11 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-def make(path, alpha, method, levels=0, extra=None):
+def make(path, alpha, method, levels=0, extra=None, limiter="MRS"):
     pars = {"amr_mesh_type": "Shell", "amr_luni": 1, "amr_n0": 5, "amr_n1": 5, "amr_n2": 5,
             "amr_Shell_rin": 2.0, "amr_Shell_rout": 6.0, "amr_hrefine_sphere_radius": 4,
             "amr_hrefine_sphere_levels": levels, "scalarwave1_k": "%.17g %.17g %.17g" % tuple(K),
-            "scalarwave1_limiter": "MRS", "limiter_alpha": alpha, "evolve_method": method,
+            "scalarwave1_limiter": limiter, "limiter_alpha": alpha, "evolve_method": method,
             "nmx_path": path, "nmx_graph_steps": 2}
     pars.update(extra or {})
     m = nm.Mesh(pars, device=0)
@@ -106,7 +107,7 @@ def test_limiter_refusals():
     print("limiter refusals", flush=True)
     m, om = make("plain", 0.3, "RK4")
     with pytest.raises(nm.NmxError):
-        m.set_par("scalarwave1_limiter", "minmodB")
+        m.set_par("scalarwave1_limiter", "TVB")
     import torch
     from nmesh_amd import adjoint
     lam = torch.zeros(5 * om.nnodes, dtype=torch.float64, device="cuda")
@@ -117,3 +118,41 @@ def test_limiter_refusals():
     nm.amr_setup_mesh(fv)
     with pytest.raises(nm.NmxError):
         nm.evolve_myln(fv, 1)
+
+
+@pytest.mark.parametrize("method", ["RK4", "sspRK3"])
+@pytest.mark.parametrize("alpha,beta,scale", [(0.05, 0.5, "no"), (0.5, 1.0, "yes")])
+def test_minmodB_steps_match_oracle(method, alpha, beta, scale):
+    print("minmodB", method, alpha, beta, scale, flush=True)
+    m, om = make("plain", alpha, method, limiter="minmodB",
+                 extra={"limiter_beta": beta, "limiter_scaleBound": scale})
+    u = noisy(om, 4)
+    m.upload(u)
+    m.time = 0.0
+    orc.set_limiter_minmodB(alpha, beta, scale == "yes")
+    try:
+        t = 0.0
+        v = u.copy()
+        for s in range(3):
+            v, t = om.evolve(v, m.dt, 1, t, K, method)
+            nm.evolve_myln(m, 1)
+            assert rel(m.download(), v) <= TOL, "step %d" % s
+    finally:
+        orc.set_limiter_minmodB(None)
+    w, _ = om.evolve(u, m.dt, 3, 0.0, K, method)
+    assert rel(w, v) > 1e-6
+
+
+def test_minmodB_batched_refined():
+    print("minmodB batched", flush=True)
+    m, om = make("batched", 0.05, "RK4", levels=1, limiter="minmodB")
+    u = noisy(om, 5)
+    m.upload(u)
+    m.time = 0.0
+    orc.set_limiter_minmodB(0.05, 0.5, False)
+    try:
+        v, _ = om.evolve(u, m.dt, 4, 0.0, K, "RK4")
+    finally:
+        orc.set_limiter_minmodB(None)
+    nm.evolve_myln(m, 4)
+    assert rel(m.download(), v) <= TOL
