@@ -73,6 +73,8 @@ int nmx_synchronize(nmx_ctx *ctx);
  * basis_filter_fv (yes|no) (nmesh_basis.c:22-24), the FV elements'
  * amr_uniform_p, amr_fv_p, fv_rec (WENOm3_2|WENOm5_2|WENOmZ_2) and
  * coordinates_midpoint_data (no|yes|yes all, nmesh_coordinates.c:84),
+ * fv_surface_interp (linear|parabolic, nmesh_dg.c:53; set on a built mesh
+ * it rebuilds the face tables and keeps the state),
  * scalarwave1_limiter (none|MRS|minmodB; nmesh_scalarwave1.c:34) with
  * limiter_alpha, limiter_beta, limiter_scaleBound (nmesh_limiter.c:19-23):
  * the limiter on every later RK stage's input

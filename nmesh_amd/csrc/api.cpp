@@ -298,6 +298,8 @@ extern "C" int nmx_synchronize(nmx_ctx *c) {
   return NMX_OK;
 }
 
+static int remirror_with_state(nmx_ctx *c, const Mesh &old);
+
 extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
   if (!c || !name || !value) return fail(NMX_EINVAL, "NULL argument");
   std::string k(name), v(value);
@@ -318,7 +320,8 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
     return fail(NMX_EUNSUP, "fv_rec: WENOm3_2, WENOm5_2 and WENOmZ_2 are built");
   if ((k == "fv_flux" && v != "fnum") || (k == "fv_divf_extrap" && v != "dnfn_extrap1") ||
       (k == "fv_divf_use_only_right_flux" && v != "yes") || (k == "fv_WENOm3_opt_weightratio" && v != "2") ||
-      (k == "fv_surface_interp" && v != "linear") || (k == "fv2dg_interp_use_extrap1" && v != "no"))
+      (k == "fv_surface_interp" && v != "linear" && v != "parabolic") ||
+      (k == "fv2dg_interp_use_extrap1" && v != "no"))
     return fail(NMX_EUNSUP, k + ": only nmesh's default is built");
   if (k == "scalarwave1_limiter" && v != "none" && v != "MRS" && v != "minmodB")
     return fail(NMX_EINVAL, "scalarwave1_limiter must be none, MRS or minmodB");
@@ -332,6 +335,7 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
     return fail(NMX_EINVAL, "Jpower must be 1, -1 or 0");
   if (k == "nmx_block" && v != "full" && v != "half" && v != "quarter")
     return fail(NMX_EINVAL, "nmx_block must be full, half or quarter");
+  const std::string prev = c->par[k];
   c->par[k] = v;
   if (k == "nmx_block" || k == "nmx_xcd") c->tuned = true;   // explicit choice wins
   if (k == "nmx_block")
@@ -342,6 +346,17 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
   if (c->agexec) { (void)hipGraphExecDestroy(c->agexec); c->agexec = nullptr; }
   if (c->agraph) { (void)hipGraphDestroy(c->agraph); c->agraph = nullptr; }
   c->agraph_steps = 0;
+  if (k == "fv_surface_interp" && c->have_mesh) {
+    // new face tables for faces taken from FV neighbours; the state stays
+    const int mode = v == "parabolic" ? 1 : 0;
+    if (mode != c->mesh.fv_surface_interp) {
+      Mesh old = c->mesh;
+      std::string err = set_fv_surface_interp(c->mesh, mode);
+      if (!err.empty()) { c->mesh = old; c->par[k] = prev; return fail(NMX_EINVAL, err); }
+      int r = remirror_with_state(c, old);
+      if (r) { c->par[k] = prev; return r; }
+    }
+  }
   if (k == "dt" && c->have_mesh) {
     if (v == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"));
     else c->dt = std::atof(v.c_str());
@@ -565,7 +580,6 @@ static int upload_amr_tables(nmx_ctx *c) {
     std::vector<SlotRec> recs(list.size());
     for (size_t it = 0; it < list.size(); ++it) {
       const int e = list[it];
-      const Elm &E = M.e[e];
       SlotRec &R = recs[it];
       slot_geometry(M, e, R);
       for (int f = 0; f < 6; ++f) {
@@ -764,6 +778,7 @@ extern "C" int nmx_mesh_build(nmx_ctx *c) {
     return fail(NMX_EUNSUP, "amr_mesh_type must be BoxMesh or Shell");
   }
   if (!err.empty()) return fail(NMX_EINVAL, err);
+  c->mesh.fv_surface_interp = P(c, "fv_surface_interp") == "parabolic" ? 1 : 0;
   {
     // FV elements: amr_fv_p patches on uniform points (amr_uniform_p)
     std::vector<int> up, fp;

@@ -208,6 +208,63 @@ static double pw_linear(const Basis &b, int q, double x) {
   return 0.0;
 }
 
+// piecewise parabolic basis function q on the points of b (basis_pw_parab,
+// src/main/basis/interpolate.c:104-224): between the midpoints around node
+// m the parabola through m-1, m, m+1 (so basis q lives on the intervals
+// around q-1, q and q+1); the parabola through the first (last) three points
+// below the first (above the last) midpoint; one parabola for 3 points
+static double parab3(const double *xp, int a, int b, int c, double x) {
+  return (x - xp[b]) * (x - xp[c]) / ((xp[a] - xp[b]) * (xp[a] - xp[c]));
+}
+
+static double pw_parab(const Basis &b, int q, double x) {
+  const int np = b.n;
+  const double *xp = b.x;
+  if (np < 3) return pw_linear(b, q, x);
+  if (np == 3 || x < 0.5 * (xp[0] + xp[1])) {
+    switch (q) {
+      case 0: return parab3(xp, 0, 1, 2, x);
+      case 1: return parab3(xp, 1, 0, 2, x);
+      case 2: return parab3(xp, 2, 0, 1, x);
+      default: return 0.0;
+    }
+  }
+  if (x >= 0.5 * (xp[np - 2] + xp[np - 1])) {
+    const int a = np - 3, l = q - a;
+    switch (l) {
+      case 0: return parab3(xp, a, a + 1, a + 2, x);
+      case 1: return parab3(xp, a + 1, a, a + 2, x);
+      case 2: return parab3(xp, a + 2, a, a + 1, x);
+      default: return 0.0;
+    }
+  }
+  // interior x: basis q is nonzero on the cells around q - 1, q, q + 1,
+  // whose bounds at the ends of the point set are mirrored outwards
+  const int m = q <= 0 ? 0 : (q >= np - 1 ? np - 1 : q);
+  double lo, hi;   // cell of node m
+  if (m == 0) {
+    hi = 0.5 * (xp[0] + xp[1]);
+    lo = xp[0] - (hi - xp[0]);
+  } else if (m == np - 1) {
+    lo = 0.5 * (xp[m - 1] + xp[m]);
+    hi = xp[m] + (xp[m] - lo);
+  } else {
+    lo = 0.5 * (xp[m - 1] + xp[m]);
+    hi = 0.5 * (xp[m] + xp[m + 1]);
+  }
+  const double llo = (m >= 2) ? 0.5 * (xp[m - 2] + xp[m - 1]) : lo - (hi - lo);
+  const double hhi = (m + 2 < np) ? 0.5 * (xp[m + 1] + xp[m + 2]) : hi + (hi - lo);
+  if (x >= lo && x < hi) {
+    // the parabola of cell m: through m-1, m, m+1 (one-sided at the ends)
+    if (m == 0) return parab3(xp, 0, 1, 2, x);
+    if (m == np - 1) return parab3(xp, m, m - 2, m - 1, x);
+    return parab3(xp, m, m - 1, m + 1, x);
+  }
+  if (x >= llo && x < lo) return (m >= 2) ? parab3(xp, m, m - 2, m - 1, x) : 0.0;
+  if (x >= hi && x < hhi) return (m + 2 < np) ? parab3(xp, m, m + 1, m + 2, x) : 0.0;
+  return 0.0;
+}
+
 // ------------------------------------------------------------------------
 // coordinate maps: Cartesian box and CubedShell with constant radii
 // (src/main/coordinates/coordtrans_CubedSphere.c:38-509)
@@ -653,9 +710,9 @@ static std::string face_tables(Mesh &M) {
         }
       }
       // interpolation from every neighbour: separable 1D matrices, Lagrange
-      // rows from a DG neighbour, piecewise linear ones from an FV
-      // neighbour on uniform points (fv_surface_interp = linear,
-      // surface.c:1182-1245)
+      // rows from a DG neighbour, piecewise linear or parabolic ones from an
+      // FV neighbour on uniform points (fv_surface_interp = linear or
+      // parabolic, surface.c:1182-1245)
       int dir = f / 2, d1, d2;
       dirs_of(dir, &d1, &d2);
       std::vector<InterpSlot> &slots = M.fint[6 * ei + f];
@@ -665,7 +722,10 @@ static std::string face_tables(Mesh &M) {
         int od = nf / 2, od1, od2;
         dirs_of(od, &od1, &od2);
         const Basis &bn = N.fv ? uniform_basis(N.n) : lgl_basis(N.n);
-        auto row = [&](int q, double x) { return N.fv ? pw_linear(bn, q, x) : lagrange(bn, q, x); };
+        auto row = [&](int q, double x) {
+          if (!N.fv) return lagrange(bn, q, x);
+          return M.fv_surface_interp == 1 ? pw_parab(bn, q, x) : pw_linear(bn, q, x);
+        };
         // neighbour in-plane Xb coordinates of all my face points
         std::vector<double> C0(n * n), C1(n * n);
         std::vector<char> in(n * n);
@@ -892,6 +952,12 @@ std::string set_fv_patches(Mesh &M, const std::vector<int> &patches) {
       if (E.p == p) E.fv = 1;
   }
   geometry(M);
+  return face_tables(M);
+}
+
+std::string set_fv_surface_interp(Mesh &M, int mode) {
+  if (mode != 0 && mode != 1) return "fv_surface_interp must be linear or parabolic";
+  M.fv_surface_interp = mode;
   return face_tables(M);
 }
 

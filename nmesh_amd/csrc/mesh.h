@@ -84,6 +84,7 @@ struct Mesh {
   std::vector<FaceCopy> fcopy;      // 6 per element
   std::vector<std::vector<InterpSlot>> fint;  // 6 per element
   int maxlevel = 0;
+  int fv_surface_interp = 0;        // rows from FV neighbours: 0 piecewise linear, 1 parabolic
 };
 
 // builders (return "" on success, else an error message)
@@ -103,6 +104,10 @@ std::string set_fv_elems(Mesh &M, const std::vector<int> &fv);
 // put every element of the listed patches in FV mode on uniform points
 // (amr_uniform_p, amr_fv_p; src/main/amr/mesh.c:40-80) and rebuild
 std::string set_fv_patches(Mesh &M, const std::vector<int> &patches);
+
+// fv_surface_interp (0 linear, 1 parabolic) for faces taken from FV
+// neighbours; tables rebuilt
+std::string set_fv_surface_interp(Mesh &M, int mode);
 
 double mesh_dt(const Mesh &M, double dtfac);
 

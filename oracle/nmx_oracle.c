@@ -144,6 +144,68 @@ static double pw_linear_of_x(int k, double x, int np, const double *xp)
   return 0.;
 }
 
+/* piecewise parabolic basis function k on points xp (basis_pw_parab,
+   src/main/basis/interpolate.c:104-224): near node m, between the
+   midpoints to its neighbours, the parabola through m-1, m, m+1; the first
+   and last three points at the ends */
+static double parab3(double x, const double *xp, int a, int b, int c)
+{
+  return (x - xp[b]) * (x - xp[c]) / ((xp[a] - xp[b]) * (xp[a] - xp[c]));
+}
+
+static double pw_parab_of_x(int k, double x, int np, const double *xp)
+{
+  if (np < 3) return pw_linear_of_x(k, x, np, xp);
+  if (np == 3 || x < 0.5 * (xp[0] + xp[1])) {
+    if (k == 0) return parab3(x, xp, 0, 1, 2);
+    if (k == 1) return parab3(x, xp, 1, 0, 2);
+    if (k == 2) return parab3(x, xp, 2, 0, 1);
+    return 0.;
+  }
+  if (x >= 0.5 * (xp[np - 2] + xp[np - 1])) {
+    int n0 = np - 3, n1 = np - 2, n2 = np - 1, l = k - n0;
+    if (l == 0) return parab3(x, xp, n0, n1, n2);
+    if (l == 1) return parab3(x, xp, n1, n0, n2);
+    if (l == 2) return parab3(x, xp, n2, n0, n1);
+    return 0.;
+  }
+  int m, m1, m2;
+  double xml, xmr, xmll, xmrr;
+  if (k <= 0) {
+    m = 0;
+    xmr = 0.5 * (xp[m] + xp[m + 1]);
+    xml = xp[m] - (xmr - xp[m]);
+    xmll = xml - (xmr - xml);
+    xmrr = 0.5 * (xp[m + 1] + xp[m + 2]);
+    m1 = 1; m2 = 2;
+  } else if (k < np - 1) {
+    m = k;
+    xml = 0.5 * (xp[m] + xp[m - 1]);
+    xmr = 0.5 * (xp[m] + xp[m + 1]);
+    xmrr = (k < np - 2) ? 0.5 * (xp[m + 1] + xp[m + 2]) : xmr + (xmr - xml);
+    xmll = (k >= 2) ? 0.5 * (xp[m - 1] + xp[m - 2]) : xml - (xmr - xml);
+    m1 = k - 1; m2 = k + 1;
+  } else {
+    m = np - 1;
+    xml = 0.5 * (xp[m - 1] + xp[m]);
+    xmr = xp[m] + (xp[m] - xml);
+    xmll = 0.5 * (xp[m - 2] + xp[m - 1]);
+    xmrr = xmr + (xmr - xml);
+    m1 = m - 2; m2 = m - 1;
+  }
+  if (x >= xml && x < xmr) return parab3(x, xp, m, m1, m2);
+  if (x >= xmll && x < xml) return (m >= 2) ? parab3(x, xp, m, m - 2, m - 1) : 0.;
+  if (x >= xmr && x < xmrr) return (m + 2 < np) ? parab3(x, xp, m, m + 1, m + 2) : 0.;
+  return 0.;
+}
+
+double orc_pw_parab(int k, double x, int np, const double *xp) { return pw_parab_of_x(k, x, np, xp); }
+
+/* fv_surface_interp for faces taken from FV neighbours: 0 linear (the
+   default), 1 parabolic (surface.c:1182-1196) */
+static int g_fv_surface_interp = 0;
+void orc_set_fv_surface_interp(int mode) { g_fv_surface_interp = mode; }
+
 static OBasis *basis(int n)
 {
   OBasis *b = &g_basis[n];
@@ -1229,7 +1291,10 @@ static void build_ajsurf(const OMesh *M, const OElm *E, int f, const double *pla
         OBasis *bn = basis(nn_);
         double B1[ORC_MAXN], B2[ORC_MAXN];
         for (int q = 0; q < nn_; q++) {
-          if (N->use_fv) {
+          if (N->use_fv && g_fv_surface_interp == 1) {
+            B1[q] = pw_parab_of_x(q, Cb0, nn_, bn->Xu);
+            B2[q] = pw_parab_of_x(q, Cb1, nn_, bn->Xu);
+          } else if (N->use_fv) {
             B1[q] = pw_linear_of_x(q, Cb0, nn_, bn->Xu);
             B2[q] = pw_linear_of_x(q, Cb1, nn_, bn->Xu);
           } else {

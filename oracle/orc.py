@@ -68,6 +68,8 @@ def lib():
             "orc_limit_minmodB": (None, [vp, vp]),
             "orc_set_fv_rec": (None, [i]),
             "orc_set_fv_midpoint": (None, [i]),
+            "orc_set_fv_surface_interp": (None, [i]),
+            "orc_pw_parab": (d, [i, d, i, vp]),
             "orc_dnfn_extrap1": (None, [i, vp]),
             "orc_rec1d_p_WENOm3": (d, [vp, i]),
             "orc_rec1d_m_WENOm3": (d, [vp, i]),
@@ -274,6 +276,18 @@ def set_fv_midpoint(on):
     """coordinates_midpoint_data of the oracle's FV elements: False (the
     default, "no") or True ("yes")"""
     lib().orc_set_fv_midpoint(1 if on else 0)
+
+
+def pw_parab(k, x, xp):
+    """piecewise parabolic basis function k of the points xp at x"""
+    xp = np.ascontiguousarray(xp, dtype=np.float64)
+    return lib().orc_pw_parab(int(k), float(x), len(xp), _p(xp))
+
+
+def set_fv_surface_interp(mode):
+    """fv_surface_interp of the oracle's faces taken from FV neighbours:
+    "linear" (the default) or "parabolic" """
+    lib().orc_set_fv_surface_interp({"linear": 0, "parabolic": 1}[mode])
 
 
 def set_fv_rec(mode):

@@ -237,53 +237,110 @@ def lgl(n):
     return x
 
 
-def test_face_values_across_dg_fv_faces():
+def _pw_parab_cells(x, xp, g):
+    """the piecewise parabolic interpolant of g on the points xp, stated by
+    cells: below the first midpoint the parabola through points 0, 1, 2,
+    above the last one through the last three, elsewhere the parabola
+    through the nearest point and its two neighbours (numpy's polyfit)"""
+    n = len(xp)
+    if x < 0.5 * (xp[0] + xp[1]):
+        m = 1
+    elif x >= 0.5 * (xp[n - 2] + xp[n - 1]):
+        m = n - 2
+    else:
+        m = max(i for i in range(1, n - 1) if x >= 0.5 * (xp[i - 1] + xp[i]))
+    pts = xp[m - 1:m + 2]
+    return float(np.polyval(np.polyfit(pts, g(pts), 2), x))
+
+
+@pytest.mark.parametrize("mode,deg", [("linear", 2), ("parabolic", 2), ("parabolic", 4)])
+def test_face_values_across_dg_fv_faces(mode, deg):
     """set_ajsurf_vl on the shell with FV patches 0, 2, 4 (the reference's
     sw7FV-DG layout): a DG element takes its FV neighbour's face values by
-    piecewise linear interpolation from the uniform points (surface.c:
-    1182-1245, fv_surface_interp = linear), an FV element its DG
-    neighbour's by Lagrange interpolation from the LGL points.  The
-    neighbour's face plane is set to g(x1) + g(x2) of its own in-plane
-    point coordinates with g(x) = x^2, symmetric under the face
-    orientations, so the expected values at my face point (a, b) are
-    G(x_a) + G(x_b) of my own point coordinates: G = numpy's piecewise
-    linear interp of x^2 on the uniform points for a DG receiver, and x^2
-    itself (Lagrange is exact on it) for an FV receiver."""
+    piecewise linear or parabolic interpolation from the uniform points
+    (surface.c:1182-1245, fv_surface_interp = linear or parabolic), an FV
+    element its DG neighbour's by Lagrange interpolation from the LGL
+    points.  The neighbour's face plane is set to g(x1) + g(x2) of its own
+    in-plane point coordinates with g(x) = x^deg, even so symmetric under
+    the face orientations, so the expected values at my face point (a, b)
+    are G(x_a) + G(x_b) of my own point coordinates: G = numpy's piecewise
+    linear interp of g on the uniform points, or the parabola by cells
+    through the three nearest uniform points (x^2 itself: the parabolas
+    reproduce it), for a DG receiver, and g itself (Lagrange on 5 LGL points
+    is exact to degree 4) for an FV receiver."""
     n = 5
     om = OMesh("shell", 1, n, rin=2.0, rout=6.0, fv_patches=[0, 2, 4])
     xl = lgl(n)
     xu = np.linspace(-1.0, 1.0, n)
+
+    def g(x):
+        return x ** deg
+
+    if mode == "linear":
+        def G(x):
+            return np.interp(x, xu, g(xu))
+    else:
+        def G(x):
+            return np.array([_pw_parab_cells(t, xu, g) for t in np.ravel(x)]).reshape(np.shape(x))
+        if deg == 2:
+            assert np.abs(G(xl) - xl ** 2).max() < 1e-15
     checked = {0: 0, 1: 0}
-    for e in range(om.nelem):
-        info = om.elm_info(e)
-        fv_me = info[0] in (0, 2, 4)
-        for f in range(6):
-            nb, nbf = om.elm_fnb(e, f)
-            if len(nb) != 1:
-                continue
-            ninfo = om.elm_info(nb[0])
-            fv_nb = ninfo[0] in (0, 2, 4)
-            if fv_nb == fv_me:
-                continue
-            assert info[10 + f] == 3           # interpolated, never copied
-            # neighbour's nodes: g of its coordinates along its face's plane
-            od = nbf[0] // 2
-            od1, od2 = (1, 2) if od == 0 else ((0, 2) if od == 1 else (0, 1))
-            xn = xu if fv_nb else xl
-            u = np.zeros((5, om.nnodes))
-            I = np.indices((n, n, n))[::-1]                 # I[0] = i, I[1] = j, I[2] = k
-            blk = xn[I[od1]] ** 2 + xn[I[od2]] ** 2         # [k, j, i]
-            off = ninfo[3]
-            u[1, off:off + n ** 3] = blk.ravel()
-            out = np.zeros(n * n)
-            lib().orc_face_values(om.h, _p(np.ascontiguousarray(u.ravel())), e, f, 1, _p(out))
-            xm = xu if fv_me else xl
-            G = (lambda x: x ** 2) if fv_me else (lambda x: np.interp(x, xu, xu ** 2))
-            a = np.arange(n)
-            expect = G(xm[a][None, :]) + G(xm[a][:, None])    # [b, a]
-            assert np.abs(out.reshape(n, n) - expect).max() < 1e-13, (e, f, fv_me)
-            checked[int(fv_me)] += 1
+    orc.set_fv_surface_interp(mode)
+    try:
+        for e in range(om.nelem):
+            info = om.elm_info(e)
+            fv_me = info[0] in (0, 2, 4)
+            for f in range(6):
+                nb, nbf = om.elm_fnb(e, f)
+                if len(nb) != 1:
+                    continue
+                ninfo = om.elm_info(nb[0])
+                fv_nb = ninfo[0] in (0, 2, 4)
+                if fv_nb == fv_me:
+                    continue
+                assert info[10 + f] == 3           # interpolated, never copied
+                # neighbour's nodes: g of its coordinates along its face's plane
+                od = nbf[0] // 2
+                od1, od2 = (1, 2) if od == 0 else ((0, 2) if od == 1 else (0, 1))
+                xn = xu if fv_nb else xl
+                u = np.zeros((5, om.nnodes))
+                I = np.indices((n, n, n))[::-1]                 # I[0] = i, I[1] = j, I[2] = k
+                blk = g(xn[I[od1]]) + g(xn[I[od2]])             # [k, j, i]
+                off = ninfo[3]
+                u[1, off:off + n ** 3] = blk.ravel()
+                out = np.zeros(n * n)
+                lib().orc_face_values(om.h, _p(np.ascontiguousarray(u.ravel())), e, f, 1, _p(out))
+                xm = xu if fv_me else xl
+                Gm = g if fv_me else G
+                a = np.arange(n)
+                expect = Gm(xm[a][None, :]) + Gm(xm[a][:, None])    # [b, a]
+                assert np.abs(out.reshape(n, n) - expect).max() < 1e-13, (e, f, fv_me)
+                checked[int(fv_me)] += 1
+    finally:
+        orc.set_fv_surface_interp("linear")
     assert checked[0] > 0 and checked[1] > 0
+
+
+def test_pw_parab_basis_by_cells():
+    """basis_pw_parab (interpolate.c:104-224) against the cell statement of
+    the same interpolant: for every point count 3..9 and random x in
+    [-1, 1] (and the points and midpoints themselves) the weights of the
+    oracle's face rows applied to random data equal the parabola through the
+    three points nearest x; they sum to one and reproduce quadratics"""
+    rng = np.random.default_rng(7)
+    for n in range(3, 10):
+        xp = np.linspace(-1.0, 1.0, n)
+        mids = 0.5 * (xp[1:] + xp[:-1])
+        for x in np.concatenate([rng.uniform(-1.0, 1.0, 30), xp, mids]):
+            w = np.array([orc.pw_parab(k, x, xp) for k in range(n)])
+            assert abs(w.sum() - 1.0) < 1e-13
+            assert abs(w @ (0.3 - xp + 2.0 * xp ** 2) - (0.3 - x + 2.0 * x * x)) < 1e-13
+            vals = rng.standard_normal(n)
+            ref = _pw_parab_cells(x, xp, lambda t: np.interp(t, xp, vals))
+            assert abs(w @ vals - ref) < 1e-12, (n, x)
+    # fewer than 3 points: the piecewise linear basis
+    xp = np.array([-1.0, 1.0])
+    assert orc.pw_parab(0, 0.5, xp) == _pwl(0, 0.5, xp)
 
 
 def test_pw_linear_face_interpolation_from_fv_neighbours():
@@ -330,10 +387,11 @@ def _pwl(k, x, xp):
 
 def test_unbuilt_fv_options_are_refused():
     m = nm.Mesh({"amr_mesh_type": "BoxMesh"}, device=-1)
-    for k, v in [("fv_surface_interp", "parabolic"), ("fv_surface_interp", "WENO"),
+    for k, v in [("fv_surface_interp", "WENO"), ("fv_surface_interp", "linear Lagrange_if_dg"),
                  ("fv2dg_interp_use_extrap1", "yes"), ("fv_rec", "WENOm5_1"),
                  ("coordinates_midpoint_data", "sometimes")]:
         with pytest.raises(nm.NmxError):
             m.set_par(k, v)
     m.set_par("fv_surface_interp", "linear")
+    m.set_par("fv_surface_interp", "parabolic")
     m.set_par("coordinates_midpoint_data", "yes all")

@@ -173,6 +173,37 @@ def test_fv_buckets_rhs_matches_oracle(case):
     assert rel(r, r_o) <= TOL
 
 
+@pytest.mark.parametrize("levels", [0, 1])
+def test_mixed_parabolic_surface_interp_matches_oracle(levels):
+    """fv_surface_interp = parabolic: DG elements take their FV neighbours'
+    face values by the piecewise parabolic rows (basis_pw_parab); the RHS
+    and RK4 steps against the oracle's with the same mode; switching the
+    parameter on a built mesh rebuilds the face tables (back to linear)
+    and keeps the state"""
+    print("mixed parabolic", levels, flush=True)
+    m, om = make_mixed(5, "WENOm5_2", levels=levels, extra={"fv_surface_interp": "parabolic"})
+    u, _ = om.profile(K, 0.2)
+    u = u + 0.1 * np.random.default_rng(8).uniform(-1, 1, u.shape)
+    m.upload(u)
+    orc.set_fv_surface_interp("parabolic")
+    try:
+        r = nm.evolve_setrhs_mesh(m, t=0.2)
+        assert rel(r, om.rhs(u, 0.2, K, "upwind")) <= TOL
+        m.time = 0.0
+        v, t = om.evolve(u, m.dt, 2, 0.0, K, "RK4")
+        nm.evolve_myln(m, 2)
+        assert rel(m.download(), v) <= TOL
+    finally:
+        orc.set_fv_surface_interp("linear")
+    m.upload(u)
+    m.set_par("fv_surface_interp", "linear")
+    assert np.array_equal(m.download(), u)
+    r_lin = nm.evolve_setrhs_mesh(m, t=0.2)
+    assert rel(r_lin, om.rhs(u, 0.2, K, "upwind")) <= TOL
+    assert rel(r_lin, r) > 1e-8
+    orc.set_fv_midpoint(False)
+
+
 def test_mixed_dg_fv_steps_match_oracle():
     """RK4 on the mixed shell, every step against the oracle; a captured
     multi-step graph (order buckets on stream lanes) gives the same states
