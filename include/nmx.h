@@ -65,7 +65,9 @@ int nmx_synchronize(nmx_ctx *ctx);
 /* Parameters under nmesh's names (replaces AddPar/Par, nmesh_main.h:89-107):
  * amr_mesh_type (BoxMesh|Shell), amr_n0/1/2, amr_luni, amr_BoxMesh_dout,
  * amr_mesh_xc, amr_Shell_rin, amr_Shell_rout, amr_hrefine_sphere_radius,
- * amr_hrefine_sphere_levels, dt (auto|number), dtfac, evolve_method
+ * amr_hrefine_sphere_levels, dt (auto|number), dtfac, uniform_dtfac (dt on
+ * FV elements, nmesh_main.c:32), iterations, finaltime (nmx_evolve_mesh),
+ * evolve_method
  * (RK4|sspRK3|Euler), scalarwave1_k, scalarwave1_numflux (upwind|LLF),
  * scalarwave1_profile (sin), the FILTER bin's evolve_filter_all (yes|no),
  * evolve_filter_varlist, evolve_filter_alp, evolve_filter_s, evolve_filter_dn
@@ -163,6 +165,13 @@ int nmx_setrhs(nmx_ctx *ctx, int reg_out, int reg_in, double t);
 /* nsteps full time steps with evolve_method (evolve_myln, evolve.c:25-176;
  * RungeKutta.c:19-188).  Advances NMX_REG_U and the mesh time. */
 int nmx_evolve(nmx_ctx *ctx, int nsteps);
+/* evolve_mesh (main.c:583-666): full steps until the mesh time reaches the
+ * parameter finaltime, or iterations * dt when finaltime <= 0, within
+ * max(dt 1e-7, finaltime 1e-14); nothing when both are 0 (their defaults,
+ * nmesh_main.c:34-35).  Te starts at the mesh time as in evolve_mesh. */
+int nmx_evolve_mesh(nmx_ctx *ctx);
+/* full steps taken since the mesh was built (mesh->iteration, main.c:617) */
+int nmx_get_iteration(nmx_ctx *ctx, long *iteration);
 /* The FILTER bin's exponential filter, in place on register reg, with the
  * evolve_filter_* and basis_expfilter_* parameters (evolve_filter_evosys_mesh,
  * evolve.c:656-710; expfilter_var, filter.c:75-92).  nmx_evolve applies it

@@ -44,6 +44,8 @@ def _proto(lib):
         "nmx_set_dt": (i, [vp, d]),
         "nmx_get_time": (i, [vp, P(d)]),
         "nmx_set_time": (i, [vp, d]),
+        "nmx_evolve_mesh": (i, [vp]),
+        "nmx_get_iteration": (i, [vp, P(l)]),
         "nmx_state_upload": (i, [vp, i, vp]),
         "nmx_state_download": (i, [vp, i, vp]),
         "nmx_state_upload_elm": (i, [vp, i, i, i, vp, i]),

@@ -56,6 +56,7 @@ struct nmx_ctx {
   long nstate = 0;       // doubles per register
   double dt = 0, time = 0;
   long double Te = 0, te = 0;   // time = Te + te (main.c:588-624)
+  long iteration = 0;           // full steps taken (mesh->iteration, main.c:617)
   // device mirror
   double *dU = nullptr, *dA = nullptr, *dW0 = nullptr, *dW1 = nullptr, *dR = nullptr, *dP = nullptr;
   double *dXdx = nullptr, *dXbdX = nullptr, *dxyz = nullptr, *dD = nullptr, *dT = nullptr;
@@ -166,6 +167,9 @@ static void set_defaults(nmx_ctx *c) {
       {"fv2dg_interp_use_extrap1", "no"},
       {"dt", "auto"},
       {"dtfac", "0.25"},
+      {"iterations", "0"},
+      {"finaltime", "0"},
+      {"uniform_dtfac", "0.125"},
       {"evolve_method", "RK4"},
       {"scalarwave1_k", "0.8 -0.5 0.33166247903554"},
       {"scalarwave1_numflux", "upwind"},
@@ -357,8 +361,10 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
       if (r) { c->par[k] = prev; return r; }
     }
   }
+  if ((k == "dtfac" || k == "uniform_dtfac") && c->have_mesh && P(c, "dt") == "auto")
+    c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"), Pd(c, "uniform_dtfac"));
   if (k == "dt" && c->have_mesh) {
-    if (v == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"));
+    if (v == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"), Pd(c, "uniform_dtfac"));
     else c->dt = std::atof(v.c_str());
   }
   return NMX_OK;
@@ -800,7 +806,7 @@ extern "C" int nmx_mesh_build(nmx_ctx *c) {
   }
   c->have_mesh = true;
   c->time = 0; c->Te = 0; c->te = 0;
-  if (P(c, "dt") == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"));
+  if (P(c, "dt") == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"), Pd(c, "uniform_dtfac"));
   else c->dt = Pd(c, "dt");
   if (c->device < 0) return NMX_OK;                    // host-only context
   return upload_mesh(c);
@@ -933,7 +939,7 @@ static int adapt_state(nmx_ctx *c, const Mesh &Mold, const double *uold) {
 // If the new mirror cannot be made, the old mesh and state are restored.
 static int remirror_with_state(nmx_ctx *c, const Mesh &old) {
   const double dt_old = c->dt;
-  if (P(c, "dt") == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"));
+  if (P(c, "dt") == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"), Pd(c, "uniform_dtfac"));
   if (c->device < 0) return NMX_OK;
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -998,7 +1004,7 @@ extern "C" int nmx_mesh_set_elm_n(nmx_ctx *c, const int *n, int nelem) {
   std::vector<int> v(n, n + nelem);
   std::string err = set_elm_n(c->mesh, v);
   if (!err.empty()) return fail(NMX_EINVAL, err);
-  if (P(c, "dt") == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"));
+  if (P(c, "dt") == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"), Pd(c, "uniform_dtfac"));
   if (c->device < 0) return NMX_OK;
   return upload_mesh(c);
 }
@@ -1665,6 +1671,20 @@ extern "C" int nmx_autotune(nmx_ctx *c) {
   return NMX_OK;
 }
 
+// the time after one full step: te accumulates the steps and moves into Te
+// every 10^6 iterations (evolve_mesh, main.c:612-624)
+static void advance_time(long double &Te, long double &te, long &iteration, double &time, double dt) {
+  iteration++;
+  te += dt;
+  if (iteration % 1000000 == 0) {
+    Te += te;
+    te = 0;
+  }
+  time = (double)(Te + te);
+}
+
+static void advance_time(nmx_ctx *c, double dt) { advance_time(c->Te, c->te, c->iteration, c->time, dt); }
+
 extern "C" int nmx_evolve(nmx_ctx *c, int nsteps) {
   NEED_DEV(c);
   if (nsteps < 0) return fail(NMX_EINVAL, "nsteps < 0");
@@ -1708,16 +1728,12 @@ extern "C" int nmx_evolve(nmx_ctx *c, int nsteps) {
     while (nsteps - done >= G) {
       // stage times of the G steps (time = Te + te, main.c:588-624)
       HIPCHK(hipStreamSynchronize(c->stream));   // hT is reused
-      long double te = c->te;
       for (int s = 0; s < G; ++s) {
-        double t = (double)(c->Te + te);
-        stage_times(method, t, dt, c->hT + 4 * s);
-        te += dt;
+        stage_times(method, c->time, dt, c->hT + 4 * s);
+        advance_time(c, dt);
       }
       HIPCHK(hipMemcpyAsync(c->dT, c->hT, 4 * G * sizeof(double), hipMemcpyHostToDevice, c->stream));
       HIPCHK(hipGraphLaunch(c->gexec, c->stream));
-      c->te = te;
-      c->time = (double)(c->Te + c->te);
       done += G;
     }
   }
@@ -1728,8 +1744,7 @@ extern "C" int nmx_evolve(nmx_ctx *c, int nsteps) {
     int r = issue_step(c, method, dt, 0, parity);
     if (r) return r;
     parity = !parity;
-    c->te += dt;
-    c->time = (double)(c->Te + c->te);
+    advance_time(c, dt);
   }
   // keep register pointers fixed (captured graphs refer to them)
   if (parity) HIPCHK(hipMemcpyAsync(c->dU, c->dA, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream));
@@ -1738,6 +1753,39 @@ extern "C" int nmx_evolve(nmx_ctx *c, int nsteps) {
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
+  return NMX_OK;
+}
+
+// evolve_mesh (main.c:583-666): steps until the mesh time reaches finaltime
+// (or iterations * dt when finaltime <= 0) within max(dt 1e-7,
+// finaltime 1e-14), Te starting at the mesh time.  The number of steps is
+// counted on the host with the same time accumulation, then taken by
+// nmx_evolve (graphs included).
+extern "C" int nmx_evolve_mesh(nmx_ctx *c) {
+  NEED_DEV(c);
+  const int itmax = Pi(c, "iterations");
+  double timemax = Pd(c, "finaltime");
+  const double dt = c->dt;
+  const double ttol = std::max(dt * 1e-7, timemax * 1e-14);
+  if (timemax <= 0. && itmax > 0) timemax = itmax * dt;
+  if (timemax <= 0.) return NMX_OK;
+  if (!(dt > 0.)) return fail(NMX_EINVAL, "dt must be positive");
+  c->Te = c->time;
+  c->te = 0;
+  long double Te = c->Te, te = 0;
+  long it = c->iteration;
+  double t = c->time;
+  long k = 0;
+  while (t < timemax - ttol) {
+    advance_time(Te, te, it, t, dt);
+    if (++k > 2000000000L) return fail(NMX_EINVAL, "finaltime needs more than 2e9 steps");
+  }
+  return nmx_evolve(c, (int)k);
+}
+
+extern "C" int nmx_get_iteration(nmx_ctx *c, long *it) {
+  if (!c || !it) return fail(NMX_EINVAL, "NULL argument");
+  *it = c->iteration;
   return NMX_OK;
 }
 

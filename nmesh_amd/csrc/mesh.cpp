@@ -550,13 +550,14 @@ static double dist3(const double a[3], const double b[3]) {
 
 // smallest axial node distance at the 8 corners, starting from the space
 // diagonal / sqrt(3) (src/main/coordinates/utilities.c:1028-1124)
-static double find_hmin(const Mesh &M, const Elm &E) {
+static double find_hmin(const Mesh &M, const Elm &E, bool *pair) {
   const Patch &P = M.pat[E.p];
   double X0[3] = {E.bbox[0], E.bbox[2], E.bbox[4]}, X1[3] = {E.bbox[1], E.bbox[3], E.bbox[5]};
   double x0[3], x1[3];
   patch_xyz(P, X0, x0);
   patch_xyz(P, X1, x1);
   double h = dist3(x0, x1) / std::sqrt(3.);
+  *pair = false;
   int n = E.n;
   for (int c = 0; c < 8; ++c) {
     int I[3] = {(c & 1) ? n - 1 : 0, (c & 2) ? n - 1 : 0, (c & 4) ? n - 1 : 0};
@@ -570,7 +571,7 @@ static double find_hmin(const Mesh &M, const Elm &E) {
         double xb[3];
         node_x(M, E, J[0], J[1], J[2], xb);
         double dd = dist3(xa, xb);
-        if (dd < h) h = dd;
+        if (dd < h) { h = dd; *pair = true; }
       }
   }
   return h;
@@ -606,7 +607,7 @@ static void geometry(Mesh &M) {
             M.sqrtgdiag[a * nn + g] = dXbdX[a] * std::sqrt(gam);
           }
         }
-    E.hmin = find_hmin(M, E);
+    E.hmin = find_hmin(M, E, &E.hmin_pair);
   }
 }
 
@@ -961,9 +962,12 @@ std::string set_fv_surface_interp(Mesh &M, int mode) {
   return face_tables(M);
 }
 
-double mesh_dt(const Mesh &M, double dtfac) {
+// (hmin_is_in_uniform_direction and adapt_node_dt_and_mesh_dt,
+// src/main/coordinates/utilities.c:1218-1304: the pair at hmin differs in
+// one index, and an FV element is uniform along every direction)
+double mesh_dt(const Mesh &M, double dtfac, double uniform_dtfac) {
   double dt = 0.1 * 1.7976931348623157e308;
-  for (auto &E : M.e) dt = std::min(dt, dtfac * E.hmin);
+  for (auto &E : M.e) dt = std::min(dt, (E.fv && E.hmin_pair ? uniform_dtfac : dtfac) * E.hmin);
   return dt;
 }
 

@@ -41,6 +41,7 @@ struct Elm {
   int ajmode[6] = {0, 0, 0, 0, 0, 0};
   int orient[6] = {0, 0, 0, 0, 0, 0};
   double hmin = 0;
+  bool hmin_pair = false;   // hmin is a distance between two points (not the diagonal)
 };
 
 struct Basis {
@@ -109,7 +110,9 @@ std::string set_fv_patches(Mesh &M, const std::vector<int> &patches);
 // neighbours; tables rebuilt
 std::string set_fv_surface_interp(Mesh &M, int mode);
 
-double mesh_dt(const Mesh &M, double dtfac);
+// dt = min over elements of dtfac hmin, or uniform_dtfac hmin for FV
+// elements whose hmin lies between two (uniform) points
+double mesh_dt(const Mesh &M, double dtfac, double uniform_dtfac);
 
 // maps
 void patch_xyz(const Patch &P, const double X[3], double x[3]);

@@ -97,6 +97,13 @@ class Mesh:
     def time(self, v):
         check(lib().nmx_set_time(self._ctx, float(v)))
 
+    @property
+    def iteration(self):
+        """full steps taken since the mesh was built (mesh->iteration)"""
+        it = C.c_long()
+        check(lib().nmx_get_iteration(self._ctx, C.byref(it)))
+        return it.value
+
     def elm_info(self, e):
         a = np.zeros(22, dtype=np.int64)
         check(lib().nmx_mesh_elm_info(self._ctx, int(e), _p(a)))
@@ -261,6 +268,13 @@ def evolve_filter_evosys_mesh(mesh, reg=REG_U):
     (evolve_filter_evosys_mesh, evolve.c:656-710); nmx_evolve applies it after
     every step when evolve_filter_all / evolve_filter_varlist ask for it."""
     check(lib().nmx_filter(mesh.ctx, int(reg)))
+    return 0
+
+
+def evolve_mesh(mesh):
+    """evolve_mesh (main.c:583-666): steps until finaltime, or iterations *
+    dt when finaltime <= 0 (both parameters of the mesh)"""
+    check(lib().nmx_evolve_mesh(mesh.ctx))
     return 0
 
 

@@ -381,6 +381,7 @@ typedef struct {
   int ajmode[6];    /* 0 none, 1 alias, 2 oriented copy, 3 interpolation */
   int orient[6];    /* intrch*4 + rev1*2 + rev2 for ajmode 2 */
   double hmin;
+  int hmin_pair;    /* hmin is a distance between two points (not the diagonal) */
 } OElm;
 
 typedef struct {
@@ -696,7 +697,7 @@ static void set_ajmodes(OMesh *M)
 }
 
 /* smallest axial spacing at the 8 corners (src/main/coordinates/utilities.c:1028-1124) */
-static double elm_find_hmin(const OMesh *M, const OElm *E)
+static double elm_find_hmin(const OMesh *M, const OElm *E, int *pair)
 {
   const OPat *P = &M->pat[E->p];
   int n = E->n;
@@ -707,6 +708,7 @@ static double elm_find_hmin(const OMesh *M, const OElm *E)
   double d2 = 0.;
   for (int d = 0; d < 3; d++) d2 += (x1[d] - x0[d]) * (x1[d] - x0[d]);
   double hmin = sqrt(d2) / sqrt(3.);
+  *pair = 0;
   int cs[8][3] = {{0, 0, 0}, {n - 1, 0, 0}, {0, n - 1, 0}, {n - 1, n - 1, 0},
                   {0, 0, n - 1}, {n - 1, 0, n - 1}, {0, n - 1, n - 1}, {n - 1, n - 1, n - 1}};
   for (int c = 0; c < 8; c++) {
@@ -733,7 +735,7 @@ static double elm_find_hmin(const OMesh *M, const OElm *E)
           double dd = 0.;
           for (int d = 0; d < 3; d++) dd += (xb_[d] - xa[d]) * (xb_[d] - xa[d]);
           dd = sqrt(dd);
-          if (dd < hmin) hmin = dd;
+          if (dd < hmin) { hmin = dd; *pair = 1; }
         }
       }
     }
@@ -794,7 +796,7 @@ static void mesh_geometry(OMesh *M)
             M->sqrtgdiag[a * nn + g] = dXbdX[a] * sqrt(gam);
           }
         }
-    E->hmin = elm_find_hmin(M, E);
+    E->hmin = elm_find_hmin(M, E, &E->hmin_pair);
   }
 }
 
@@ -967,13 +969,16 @@ void orc_mesh_geom(const OMesh *M, double *dXdx, double *sqrtgdiag, double *det)
   if (det) memcpy(det, M->det, sizeof(double) * M->nnodes);
 }
 
-/* dt = dtfac * min_elements hmin (src/main/amr/storage.c:1305-1366,
-   src/main/coordinates/utilities.c:1264-1304) */
-double orc_mesh_dt(const OMesh *M, double dtfac)
+/* dt = min over elements of dtfac * hmin, or uniform_dtfac * hmin where
+   hmin lies between two points along a uniform (FV) direction
+   (src/main/amr/storage.c:1305-1366, src/main/coordinates/utilities.c:
+   1218-1304; the FV elements here are uniform along all three directions) */
+double orc_mesh_dt(const OMesh *M, double dtfac, double uniform_dtfac)
 {
   double dt = 0.1 * 1.7976931348623157e308;
   for (int e = 0; e < M->nelem; e++) {
-    double dtm = dtfac * M->e[e].hmin;
+    const OElm *E = &M->e[e];
+    double dtm = (E->use_fv && E->hmin_pair) ? uniform_dtfac * E->hmin : dtfac * E->hmin;
     if (dtm < dt) dt = dtm;
   }
   return dt;

@@ -44,7 +44,7 @@ def lib():
             "orc_mesh_pat_bound": (i, [vp, i, i]),
             "orc_mesh_coords": (None, [vp, vp, vp]),
             "orc_mesh_geom": (None, [vp, vp, vp, vp]),
-            "orc_mesh_dt": (d, [vp, d]),
+            "orc_mesh_dt": (d, [vp, d, d]),
             "orc_mesh_elm_hmin": (d, [vp, i]),
             "orc_set_profile": (None, [vp, vp, d, vp, vp]),
             "orc_rhs": (None, [vp, i, vp, vp, d, vp]),
@@ -174,8 +174,8 @@ class OMesh:
         lib().orc_mesh_geom(self.h, _p(g), _p(sg), _p(det))
         return g.reshape(9, -1), sg.reshape(3, -1), det
 
-    def dt(self, dtfac=0.25):
-        return lib().orc_mesh_dt(self.h, float(dtfac))
+    def dt(self, dtfac=0.25, uniform_dtfac=0.125):
+        return lib().orc_mesh_dt(self.h, float(dtfac), float(uniform_dtfac))
 
     def profile(self, k, t):
         k = np.ascontiguousarray(k, dtype=np.float64)

@@ -230,6 +230,32 @@ def test_host_tables_of_fv_meshes_match_the_oracle(kind, luni, n, pats):
         assert np.array_equal(m.elm_info(e), om.elm_info(e))
 
 
+@pytest.mark.parametrize("n", [4, 6, 9])
+def test_fv_dt_uses_uniform_dtfac(n):
+    """adapt_node_dt_and_mesh_dt (utilities.c:1264-1304): on FV elements the
+    smallest spacing is between two uniform points, so dt = uniform_dtfac *
+    hmin: on a Cartesian FV box hmin is the uniform spacing width / (n - 1)
+    (closed form), on a DG box dtfac * the first LGL spacing; the host's dt
+    equals the oracle's, and follows both parameters"""
+    pars = {"amr_mesh_type": "BoxMesh", "amr_luni": 1, "amr_n0": n, "amr_n1": n, "amr_n2": n,
+            "amr_BoxMesh_dout": 8.0}
+    width = 8.0   # luni 1: two elements of dout per direction
+    xl = lgl(n)
+    for fv in (False, True):
+        extra = {"amr_uniform_p": "0", "amr_fv_p": "0"} if fv else {}
+        m = nm.Mesh(dict(pars, **extra), device=-1)
+        nm.amr_setup_mesh(m)
+        om = OMesh("box", 1, n, dout=8.0, fv=fv)
+        h = width / (n - 1) if fv else width * (xl[1] - xl[0]) / 2
+        expect = (0.125 if fv else 0.25) * h
+        assert abs(om.dt() - expect) <= 1e-15 * expect
+        assert m.dt == om.dt()
+        m.set_par("uniform_dtfac", "0.1")
+        m.set_par("dtfac", "0.3")
+        assert m.dt == om.dt(0.3, 0.1)
+        assert abs(m.dt - (0.1 if fv else 0.3) * h) <= 1e-15 * h
+
+
 def lgl(n):
     x = np.zeros(n)
     w = np.zeros(n)
