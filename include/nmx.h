@@ -76,7 +76,9 @@ int nmx_synchronize(nmx_ctx *ctx);
  * amr_uniform_p, amr_fv_p, fv_rec (WENOm3_2|WENOm5_2|WENOmZ_2) and
  * coordinates_midpoint_data (no|yes|yes all, nmesh_coordinates.c:84),
  * fv_surface_interp (linear|parabolic, nmesh_dg.c:53; set on a built mesh
- * it rebuilds the face tables and keeps the state),
+ * it rebuilds the face tables and keeps the state), fv_divf_extrap_s1,
+ * fv_divf_extrap_s2 (a number or DBL_MAX), fv_divf_extrap_opt
+ * (nmesh_dg.c:41-46),
  * scalarwave1_limiter (none|MRS|minmodB; nmesh_scalarwave1.c:34) with
  * limiter_alpha, limiter_beta, limiter_scaleBound (nmesh_limiter.c:19-23):
  * the limiter on every later RK stage's input

@@ -164,6 +164,9 @@ static void set_defaults(nmx_ctx *c) {
       {"fv_divf_use_only_right_flux", "yes"},
       {"fv_WENOm3_opt_weightratio", "2"},
       {"fv_surface_interp", "linear"},
+      {"fv_divf_extrap_s1", "0"},
+      {"fv_divf_extrap_s2", "DBL_MAX"},
+      {"fv_divf_extrap_opt", "0"},
       {"fv2dg_interp_use_extrap1", "no"},
       {"dt", "auto"},
       {"dtfac", "0.25"},
@@ -198,8 +201,24 @@ static void set_defaults(nmx_ctx *c) {
 }
 
 static std::string P(nmx_ctx *c, const char *k) { return c->par[k]; }
-static double Pd(nmx_ctx *c, const char *k) { return std::atof(c->par[k].c_str()); }
+// numerical value, with DBL_MAX and -DBL_MAX as the reference's parameter
+// database translates them (translatevalue, parameters.c:406-428)
+static double Pd(nmx_ctx *c, const char *k) {
+  const std::string &v = c->par[k];
+  if (v == "DBL_MAX") return 1.7976931348623157e308;
+  if (v == "-DBL_MAX") return -1.7976931348623157e308;
+  return std::atof(v.c_str());
+}
 static int Pi(nmx_ctx *c, const char *k) { return std::atoi(c->par[k].c_str()); }
+
+// fv_divf_extrap_s1, _s2, _opt for dnfn_extrap1 (dg.c:393-395; the limit is
+// on when s1 > 0 or s2 < DBL_MAX, rec1d.c:560)
+static void fv_extrap_pars(nmx_ctx *c, double &s1, double &s2, int &opt, int &on) {
+  s1 = Pd(c, "fv_divf_extrap_s1");
+  s2 = Pd(c, "fv_divf_extrap_s2");
+  opt = Pi(c, "fv_divf_extrap_opt");
+  on = (s1 > 0. || s2 < 1.7976931348623157e308) ? 1 : 0;
+}
 
 extern "C" int nmx_ctx_create(nmx_ctx **out, int device) {
   if (!out) return fail(NMX_EINVAL, "ctx pointer is NULL");
@@ -1237,6 +1256,7 @@ static int launch_bucket(nmx_ctx *c, AmrArgs a, const nmx_ctx::Bucket &b, int di
     const std::string &fr = P(c, "fv_rec");
     const int rec = fr == "WENOm5_2" ? 5 : fr == "WENOmZ_2" ? 7 : 3;
     a.fv_mid = P(c, "coordinates_midpoint_data") != "no";
+    fv_extrap_pars(c, a.ex_s1, a.ex_s2, a.ex_opt, a.ex_on);
     err = launch_fvelm_amr(b.n, rec, c->geom == 0 ? 0 : 2, a, c->dwcall + b.woff, grid, st);
   } else {
     err = launch_dg_amr(b.n, c->geom, a, grid, div, st);
@@ -1310,6 +1330,7 @@ static int launch(nmx_ctx *c, StageArgs a) {
     const std::string &fr = P(c, "fv_rec");
     int rec = fr == "WENOm5_2" ? 5 : fr == "WENOmZ_2" ? 7 : 3;
     a.fv_mid = P(c, "coordinates_midpoint_data") != "no";
+    fv_extrap_pars(c, a.ex_s1, a.ex_s2, a.ex_opt, a.ex_on);
     hipError_t err = launch_fvelm_stage(c->np1, rec, c->geom == 0 ? 0 : 2, a, c->dwc, grid, c->stream);
     if (err != hipSuccess) return fail(NMX_EHIP, std::string("fv stage launch: ") + hipGetErrorString(err));
     return NMX_OK;

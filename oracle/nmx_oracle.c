@@ -17,6 +17,7 @@
  * Variable order is scalarwave1's vlu: (pi, cx, cy, cz, phi)
  * (src/tests/scalarwave1/scalarwave1.c:670-672).
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -1506,16 +1507,47 @@ static void sw_fnum(const OSW *sw, const double nrm[3], const double *ui, const 
   else sw_numflux_upwind(nrm, lam, ui, ua, fnum);
 }
 
-/* dnfn_extrap1 on both ends of one line (rec1d_LR_uface_to_uin_1_Carray
-   with forward = 0 and no s1/s2 limiting, rec1d.c:542-626): the end value
-   sits a quarter spacing inside the face and the next one a full spacing,
-   so the face value is df_end / 0.75 + (1 - 1/0.75) df_next */
+/* fv_divf_extrap_s1, _s2, _opt (fv.c:147-150; defaults 0, DBL_MAX, 0) */
+static double g_ex_s1 = 0., g_ex_s2 = DBL_MAX;
+static int g_ex_opt = 0;
+void orc_set_fv_divf_extrap(double s1, double s2, int opt)
+{
+  g_ex_s1 = s1; g_ex_s2 = s2; g_ex_opt = opt;
+}
+
+/* the face value at end i0 (0 or n-1) of one line (rec1d_compute_1s1_u
+   with forward = 0, rec1d.c:542-609): the end value sits a quarter spacing
+   inside the face and the next one a full spacing, so the face value is
+   df_end / 0.75 + (1 - 1/0.75) df_next; with s1 > 0 or s2 < DBL_MAX the
+   ratio s = |d1| / 0.75 / |d2| of the first two differences inward (and
+   opt on their signs) can switch to the end value itself */
+static double extrap_end(int n, const double *u, int i0)
+{
+  int sign = (i0 > 0) ? 1 : -1;
+  int i1 = i0 - sign;
+  double w0 = 1. / 0.75;
+  double w1 = 1. - w0;
+  if ((g_ex_s1 > 0. || g_ex_s2 < DBL_MAX) && n > 2) {
+    int i2 = i1 - sign;
+    double d2 = u[i2] - u[i1];
+    double d1 = u[i1] - u[i0];
+    double s = 1.3333333333 * fabs(d1) / (fabs(d2) + 1e-100 * 1.0);
+    if (s < g_ex_s1 || s >= g_ex_s2) w1 = 0.;
+    if (g_ex_opt == -1) w1 = 0.;
+    if (g_ex_opt == 1 && d1 * d2 <= 0.) w1 = 0.;
+    if (g_ex_opt == 2 && d1 * d2 >= 0.) w1 = 0.;
+    w0 = 1. - w1;
+  }
+  return w0 * u[i0] + w1 * u[i1];
+}
+
+/* dnfn_extrap1 on both ends of one line, the left end first
+   (rec1d_LR_uface_to_uin_1_Carray, rec1d.c:612-626; fv.c:558-573) */
 void orc_dnfn_extrap1(int n, double *df)
 {
   if (n <= 2) return;
-  double w0 = 1. / 0.75, w1 = 1. - w0;
-  df[0] = w0 * df[0] + w1 * df[1];
-  df[n - 1] = w0 * df[n - 1] + w1 * df[n - 2];
+  df[0] = extrap_end(n, df, 0);
+  df[n - 1] = extrap_end(n, df, n - 1);
 }
 
 /* r of one FV element: divf from fv_divf, r_{pi,c} = -divf, r_phi = pi */

@@ -69,6 +69,7 @@ def lib():
             "orc_set_fv_rec": (None, [i]),
             "orc_set_fv_midpoint": (None, [i]),
             "orc_set_fv_surface_interp": (None, [i]),
+            "orc_set_fv_divf_extrap": (None, [d, d, i]),
             "orc_pw_parab": (d, [i, d, i, vp]),
             "orc_dnfn_extrap1": (None, [i, vp]),
             "orc_rec1d_p_WENOm3": (d, [vp, i]),
@@ -282,6 +283,12 @@ def pw_parab(k, x, xp):
     """piecewise parabolic basis function k of the points xp at x"""
     xp = np.ascontiguousarray(xp, dtype=np.float64)
     return lib().orc_pw_parab(int(k), float(x), len(xp), _p(xp))
+
+
+def set_fv_divf_extrap(s1=0.0, s2=None, opt=0):
+    """fv_divf_extrap_s1, _s2 (None: DBL_MAX) and _opt of the oracle's
+    dnfn_extrap1"""
+    lib().orc_set_fv_divf_extrap(float(s1), 1.7976931348623157e308 if s2 is None else float(s2), int(opt))
 
 
 def set_fv_surface_interp(mode):

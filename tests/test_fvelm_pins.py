@@ -114,6 +114,49 @@ def test_dnfn_extrap1_is_linear_extrapolation_from_the_fv_difference_locations()
         assert abs(df[0] - a) < 1e-13 and abs(df[-1] - (a + b * L)) < 1e-13
 
 
+def _extrap(df, s1=0.0, s2=None, opt=0):
+    df = np.array(df, dtype=np.float64)   # a copy: the oracle works in place
+    orc.set_fv_divf_extrap(s1, s2, opt)
+    try:
+        lib().orc_dnfn_extrap1(len(df), _p(df))
+    finally:
+        orc.set_fv_divf_extrap()
+    return df
+
+
+def test_dnfn_extrap1_s1_s2_opt_limits():
+    """fv_divf_extrap_s1 / _s2 / _opt (rec1d_compute_1s1_u, rec1d.c:542-609):
+    on linear data at the FV difference locations the ratio s =
+    1.3333333333 |d1| / |d2| is 1 - 2.5e-11 (the reference's truncated 4/3),
+    so s1 = 1 already switches to the end value while s1 = 0.999 does not,
+    and s2 = 1 keeps the extrapolation while s2 = 0.9 does not; opt 1 / 2
+    switch off on opposite / equal signs of d1 and d2, opt -1 always, but
+    only while s1 > 0 or s2 < DBL_MAX turns the limit on; with 3 points the
+    right end sees the already extrapolated left end."""
+    xs = np.array([0.25, 1.0, 2.0, 3.0, 3.75])
+    lin = 0.5 + 2.0 * xs
+    exact = [0.5, 0.5 + 2.0 * 4.0]
+    for kw, extrap in [({}, True), ({"s1": 0.999}, True), ({"s1": 1.0}, False),
+                       ({"s2": 1.0}, True), ({"s2": 0.9}, False),
+                       ({"s1": 0.5, "opt": 1}, True), ({"s1": 0.5, "opt": 2}, False),
+                       ({"s1": 0.5, "opt": -1}, False), ({"opt": -1}, True)]:
+        out = _extrap(lin, **kw)
+        want = exact if extrap else [lin[0], lin[-1]]
+        assert abs(out[0] - want[0]) < 1e-13 and abs(out[-1] - want[1]) < 1e-13, kw
+        assert np.array_equal(out[1:-1], lin[1:-1])
+    # a zigzag: d1 and d2 of opposite signs at both ends
+    zz = np.array([0.0, 1.0, 0.5, 1.0, 0.0])
+    out = _extrap(zz, s1=1e-3, opt=1)
+    assert out[0] == 0.0 and out[-1] == 0.0
+    out = _extrap(zz, s1=1e-3, opt=2)
+    assert abs(out[0] - (-1.0 / 3.0)) < 1e-15 and abs(out[-1] - (-1.0 / 3.0)) < 1e-15
+    # 3 points, left first: [0, 1, 2] -> left extrapolates to -1/3; the
+    # right end then has d2 = -4/3, d1 = -1, s = 0.99999999997 < 1.2 and
+    # keeps 2 (with the unmodified left value s would be 1.333 and give 7/3)
+    out = _extrap([0.0, 1.0, 2.0], s1=1.2)
+    assert abs(out[0] + 1.0 / 3.0) < 1e-15 and out[1] == 1.0 and out[2] == 2.0
+
+
 @pytest.mark.parametrize("rec", ["WENOm3_2", "WENOm5_2", "WENOmZ_2"])
 @pytest.mark.parametrize("n", [5, 7])
 def test_fv_element_rhs_exact_for_linear_states(rec, n):
@@ -413,6 +456,9 @@ def _pwl(k, x, xp):
 
 def test_unbuilt_fv_options_are_refused():
     m = nm.Mesh({"amr_mesh_type": "BoxMesh"}, device=-1)
+    m.set_par("fv_divf_extrap_s1", "0.5")
+    m.set_par("fv_divf_extrap_s2", "DBL_MAX")
+    m.set_par("fv_divf_extrap_opt", "1")
     for k, v in [("fv_surface_interp", "WENO"), ("fv_surface_interp", "linear Lagrange_if_dg"),
                  ("fv2dg_interp_use_extrap1", "yes"), ("fv_rec", "WENOm5_1"),
                  ("coordinates_midpoint_data", "sometimes")]:

@@ -59,6 +59,56 @@ def test_fv_element_rhs_matches_oracle(rec, numflux, luni, n):
     assert rel(r, r_o) <= TOL
 
 
+@pytest.mark.parametrize("rec", ["WENOm3_2", "WENOmZ_2"])
+def test_fv_elements_of_3_points_match_oracle(rec):
+    """the smallest FV element (3 points per direction: one interior point,
+    both ends extrapolated from it), alone and next to DG elements"""
+    print("fvelm n=3", rec, flush=True)
+    m, om = make(2, 3, rec)
+    u, _ = om.profile(K, 0.2)
+    u = u + 0.1 * np.random.default_rng(3).uniform(-1, 1, u.shape)
+    m.upload(u)
+    assert rel(nm.evolve_setrhs_mesh(m, t=0.2), om.rhs(u, 0.2, K, "upwind")) <= TOL
+    m, om = make_mixed(3, rec, levels=1)
+    u, _ = om.profile(K, 0.2)
+    u = u + 0.1 * np.random.default_rng(4).uniform(-1, 1, u.shape)
+    m.upload(u)
+    r = nm.evolve_setrhs_mesh(m, t=0.2)
+    r_o = om.rhs(u, 0.2, K, "upwind")
+    orc.set_fv_midpoint(False)
+    assert rel(r, r_o) <= TOL
+
+
+@pytest.mark.parametrize("s1,s2,opt", [(0.5, None, 0), (0.0, 1.2, 1), (0.9, 3.0, 2), (1.0, None, -1)])
+@pytest.mark.parametrize("where", ["box", "mixed_refined"])
+def test_fv_divf_extrap_limits_match_oracle(s1, s2, opt, where):
+    """fv_divf_extrap_s1 / _s2 / _opt (the kernels' limited dnfn_extrap1,
+    fvelm_ex_kernels.hip) against the oracle's, on an FV box and on an
+    h-refined mixed shell; the limit changes the RHS"""
+    print("fv extrap limits", s1, s2, opt, where, flush=True)
+    extra = {"fv_divf_extrap_s1": s1, "fv_divf_extrap_s2": "DBL_MAX" if s2 is None else s2,
+             "fv_divf_extrap_opt": opt}
+    if where == "box":
+        m, om = make(1, 6, "WENOmZ_2")
+        for k, v in extra.items():
+            m.set_par(k, v)
+    else:
+        m, om = make_mixed(5, "WENOm5_2", levels=1, extra=extra)
+    u, _ = om.profile(K, 0.2)
+    u = u + 0.2 * np.random.default_rng(11).uniform(-1, 1, u.shape)
+    m.upload(u)
+    r = nm.evolve_setrhs_mesh(m, t=0.2)
+    r_plain = om.rhs(u, 0.2, K, "upwind")
+    orc.set_fv_divf_extrap(s1, s2, opt)
+    try:
+        r_o = om.rhs(u, 0.2, K, "upwind")
+    finally:
+        orc.set_fv_divf_extrap()
+        orc.set_fv_midpoint(False)
+    assert rel(r, r_o) <= TOL
+    assert rel(r_plain, r_o) > 1e-6
+
+
 @pytest.mark.parametrize("method", ["RK4", "sspRK3", "Euler"])
 @pytest.mark.parametrize("rec", ["WENOm3_2", "WENOm5_2", "WENOmZ_2"])
 def test_fv_element_steps_match_oracle(method, rec):
