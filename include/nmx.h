@@ -84,6 +84,13 @@ int nmx_synchronize(nmx_ctx *ctx);
  * the limiter on every later RK stage's input
  * and on the final u of each step (RungeKutta.c:45-67,161-173;
  * evolve.c:143-144),
+ * accepted with the reference's defaults only: amr_dir_active0/1/2 (yes),
+ * coordinates_surface_metric (sqrtgdiag), amr_Shell_nr (1), amr_Shell_r1
+ * (<= amr_Shell_rin); amr_nmax bounds amr_n0; amr_fv_if_uniform (yes: FV
+ * exactly in amr_uniform_p, mesh.c:86-91); accepted without effect on this
+ * path: dg_outerBC_flux_fac (3 numbers; its use in scalarwave1 is commented
+ * out, scalarwave1.c:202), evolve_redo_troubled (scalarwave1 has no TROUBLE
+ * callback), evolve_u_surfaces (one device: no surface exchange),
  * plus library knobs
  * prefixed nmx_ (nmx_path:
  * plain|batched|perelement, nmx_graph: yes|no, nmx_graph_steps, nmx_geometry:

@@ -158,6 +158,18 @@ static void set_defaults(nmx_ctx *c) {
       {"amr_hrefine_sphere_levels", "0"},
       {"amr_uniform_p", ""},
       {"amr_fv_p", ""},
+      {"amr_fv_if_uniform", "no"},
+      {"amr_nmax", "55"},
+      {"amr_Shell_nr", "1"},
+      {"amr_Shell_r1", "0"},
+      {"amr_Shell_r2", "0"},
+      {"amr_dir_active0", "yes"},
+      {"amr_dir_active1", "yes"},
+      {"amr_dir_active2", "yes"},
+      {"coordinates_surface_metric", "sqrtgdiag"},
+      {"dg_outerBC_flux_fac", "1 1 1"},
+      {"evolve_redo_troubled", "substep"},
+      {"evolve_u_surfaces", "no"},
       {"fv_rec", "WENOm3_2"},
       {"fv_flux", "fnum"},
       {"fv_divf_extrap", "dnfn_extrap1"},
@@ -346,6 +358,19 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
       (k == "fv_surface_interp" && v != "linear" && v != "parabolic") ||
       (k == "fv2dg_interp_use_extrap1" && v != "no"))
     return fail(NMX_EUNSUP, k + ": only nmesh's default is built");
+  if ((k == "amr_dir_active0" || k == "amr_dir_active1" || k == "amr_dir_active2") && v != "yes")
+    return fail(NMX_EUNSUP, k + ": only 3d meshes are built");
+  if (k == "coordinates_surface_metric" && v != "sqrtgdiag")
+    return fail(NMX_EUNSUP, "coordinates_surface_metric: only sqrtgdiag is built");
+  if (k == "amr_Shell_nr" && v != "1")
+    return fail(NMX_EUNSUP, "amr_Shell_nr: one radial shell of 6 patches is built");
+  if ((k == "amr_fv_if_uniform" || k == "evolve_u_surfaces") && v != "yes" && v != "no")
+    return fail(NMX_EINVAL, k + " must be yes or no");
+  if (k == "dg_outerBC_flux_fac") {
+    double f[3];
+    if (std::sscanf(v.c_str(), "%lg %lg %lg", &f[0], &f[1], &f[2]) != 3)
+      return fail(NMX_EINVAL, "dg_outerBC_flux_fac must contain 3 numbers");
+  }
   if (k == "scalarwave1_limiter" && v != "none" && v != "MRS" && v != "minmodB")
     return fail(NMX_EINVAL, "scalarwave1_limiter must be none, MRS or minmodB");
   if (k == "limiter_scaleBound" && v != "yes" && v != "no")
@@ -791,12 +816,15 @@ extern "C" int nmx_mesh_build(nmx_ctx *c) {
   if (n0 != n1 || n0 != n2) return fail(NMX_EUNSUP, "amr_n0, amr_n1, amr_n2 must be equal");
   int luni = Pi(c, "amr_luni");
   if (luni < 0 || luni > 8) return fail(NMX_EINVAL, "amr_luni out of range");
+  if (n0 > Pi(c, "amr_nmax")) return fail(NMX_EINVAL, "amr_n0 is above amr_nmax");
   std::string err;
   if (type.find("BoxMesh") != std::string::npos) {
     double xc[3] = {0, 0, 0};
     std::sscanf(P(c, "amr_mesh_xc").c_str(), "%lg %lg %lg", &xc[0], &xc[1], &xc[2]);
     err = build_box(c->mesh, luni, n0, Pd(c, "amr_BoxMesh_dout"), xc);
   } else if (type.find("Shell") != std::string::npos) {
+    if (Pd(c, "amr_Shell_r1") > Pd(c, "amr_Shell_rin"))   // setup_Shell_mesh, mesh.c:656-664
+      return fail(NMX_EUNSUP, "amr_Shell_r1 > amr_Shell_rin (two or three radial shells) is not built");
     err = build_shell(c->mesh, luni, n0, Pd(c, "amr_Shell_rin"), Pd(c, "amr_Shell_rout"),
                       (double)Pi(c, "amr_hrefine_sphere_radius"), Pi(c, "amr_hrefine_sphere_levels"));
   } else {
@@ -810,6 +838,7 @@ extern "C" int nmx_mesh_build(nmx_ctx *c) {
     std::istringstream su(P(c, "amr_uniform_p")), sf(P(c, "amr_fv_p"));
     for (int q; su >> q;) up.push_back(q);
     for (int q; sf >> q;) fp.push_back(q);
+    if (P(c, "amr_fv_if_uniform") == "yes") fp = up;   // use_fv iff P_UNIFORM (mesh.c:86-91)
     std::sort(up.begin(), up.end());
     std::sort(fp.begin(), fp.end());
     if (up != fp)

@@ -454,6 +454,27 @@ def _pwl(k, x, xp):
     return 0.0
 
 
+def test_amr_fv_if_uniform_and_nmax():
+    """amr_fv_if_uniform = yes puts FV exactly in the uniform patches
+    (amr_use_fv_if_P_UNIFORM, mesh.c:86-91), whatever amr_fv_p says;
+    amr_n0 above amr_nmax and two radial shells are refused"""
+    base = {"amr_mesh_type": "Shell", "amr_luni": 1, "amr_n0": 5, "amr_n1": 5, "amr_n2": 5,
+            "amr_Shell_rin": 2.0, "amr_Shell_rout": 6.0, "amr_uniform_p": "0 2 4"}
+    m = nm.Mesh(dict(base, amr_fv_if_uniform="yes"), device=-1)
+    nm.amr_setup_mesh(m)
+    m2 = nm.Mesh(dict(base, amr_fv_p="0 2 4"), device=-1)
+    nm.amr_setup_mesh(m2)
+    for e in range(m.nelem):
+        assert np.array_equal(m.elm_info(e), m2.elm_info(e))
+    assert m.dt == m2.dt
+    with pytest.raises(nm.NmxError):   # uniform patches without FV are not built
+        nm.amr_setup_mesh(nm.Mesh(dict(base), device=-1))
+    with pytest.raises(nm.NmxError):
+        nm.amr_setup_mesh(nm.Mesh(dict(base, amr_nmax=4), device=-1))
+    with pytest.raises(nm.NmxError):
+        nm.amr_setup_mesh(nm.Mesh(dict(base, amr_uniform_p="", amr_Shell_r1=3.0), device=-1))
+
+
 def test_unbuilt_fv_options_are_refused():
     m = nm.Mesh({"amr_mesh_type": "BoxMesh"}, device=-1)
     m.set_par("fv_divf_extrap_s1", "0.5")
@@ -464,6 +485,13 @@ def test_unbuilt_fv_options_are_refused():
                  ("coordinates_midpoint_data", "sometimes")]:
         with pytest.raises(nm.NmxError):
             m.set_par(k, v)
+    for k, v in [("amr_dir_active1", "no"), ("coordinates_surface_metric", "sqrtgdiagdet"),
+                 ("amr_Shell_nr", "2"), ("dg_outerBC_flux_fac", "1 0"), ("evolve_u_surfaces", "maybe")]:
+        with pytest.raises(nm.NmxError):
+            m.set_par(k, v)
+    for k, v in [("dg_outerBC_flux_fac", "1 0 0"), ("evolve_redo_troubled", "step"),
+                 ("evolve_u_surfaces", "yes"), ("amr_fv_if_uniform", "yes"), ("amr_nmax", "20")]:
+        m.set_par(k, v)
     m.set_par("fv_surface_interp", "linear")
     m.set_par("fv_surface_interp", "parabolic")
     m.set_par("coordinates_midpoint_data", "yes all")
