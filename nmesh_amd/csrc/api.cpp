@@ -176,6 +176,9 @@ static void set_defaults(nmx_ctx *c) {
       {"fv_divf_use_only_right_flux", "yes"},
       {"fv_WENOm3_opt_weightratio", "2"},
       {"fv_surface_interp", "linear"},
+      {"fv_rec_WENOm_s1", "0"},
+      {"fv_rec_WENOm_s2", "DBL_MAX"},
+      {"fv_rec_WENOm_opt", "0"},
       {"fv_divf_extrap_s1", "0"},
       {"fv_divf_extrap_s2", "DBL_MAX"},
       {"fv_divf_extrap_opt", "0"},
@@ -222,6 +225,23 @@ static double Pd(nmx_ctx *c, const char *k) {
   return std::atof(v.c_str());
 }
 static int Pi(nmx_ctx *c, const char *k) { return std::atoi(c->par[k].c_str()); }
+
+// fv_rec: the interior scheme (3 WENOm3, 5 WENO5, 7 WENO-Z) and the end
+// midpoints (dg.c:295-326, rec1d.c:1032-1186): recend 0 for the _2 modes, 1
+// for _1, 2 for WENOm3 / WENOm5 / WENOmZ with fv_rec_WENOm_s1/_s2/_opt
+// (dg.c:333-335)
+static int fv_rec_pars(nmx_ctx *c, int &recend, double &s1, double &s2, int &opt, int &on) {
+  const std::string &fr = P(c, "fv_rec");
+  const int rec = fr.compare(0, 6, "WENOm5") == 0 ? 5 : fr.compare(0, 6, "WENOmZ") == 0 ? 7 : 3;
+  const size_t u = fr.find('_');
+  recend = u == std::string::npos ? 2 : (fr.substr(u) == "_1" ? 1 : 0);
+  s1 = Pd(c, "fv_rec_WENOm_s1");
+  s2 = Pd(c, "fv_rec_WENOm_s2");
+  opt = (int)Pd(c, "fv_rec_WENOm_opt");
+  on = (recend == 2 && (s1 > 0. || s2 < 1.7976931348623157e308)) ? 1 : 0;
+  if (recend == 2 && !on) recend = 0;   // the average at both ends, as _2
+  return rec;
+}
 
 // fv_divf_extrap_s1, _s2, _opt for dnfn_extrap1 (dg.c:393-395; the limit is
 // on when s1 > 0 or s2 < DBL_MAX, rec1d.c:560)
@@ -351,8 +371,9 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
     return fail(NMX_EINVAL, "nmx_geometry must be analytic or stored");
   if (k == "nmx_geometry" && c->have_mesh)
     return fail(NMX_ESTATE, "set nmx_geometry before amr_setup_mesh");
-  if (k == "fv_rec" && v != "WENOm3_2" && v != "WENOm5_2" && v != "WENOmZ_2")
-    return fail(NMX_EUNSUP, "fv_rec: WENOm3_2, WENOm5_2 and WENOmZ_2 are built");
+  if (k == "fv_rec" && v != "WENOm3_2" && v != "WENOm5_2" && v != "WENOmZ_2" && v != "WENOm3_1" &&
+      v != "WENOm5_1" && v != "WENOmZ_1" && v != "WENOm3" && v != "WENOm5" && v != "WENOmZ")
+    return fail(NMX_EUNSUP, "fv_rec: WENOm3, WENOm5, WENOmZ and their _1 and _2 modes are built");
   if ((k == "fv_flux" && v != "fnum") || (k == "fv_divf_extrap" && v != "dnfn_extrap1") ||
       (k == "fv_divf_use_only_right_flux" && v != "yes") || (k == "fv_WENOm3_opt_weightratio" && v != "2") ||
       (k == "fv_surface_interp" && v != "linear" && v != "parabolic") ||
@@ -1282,8 +1303,7 @@ static int launch_bucket(nmx_ctx *c, AmrArgs a, const nmx_ctx::Bucket &b, int di
   int grid = a.xcd_map ? 8 * ((b.count + 7) / 8) : b.count;
   hipError_t err;
   if (b.fv) {
-    const std::string &fr = P(c, "fv_rec");
-    const int rec = fr == "WENOm5_2" ? 5 : fr == "WENOmZ_2" ? 7 : 3;
+    const int rec = fv_rec_pars(c, a.recend, a.rec_s1, a.rec_s2, a.rec_opt, a.rec_on);
     a.fv_mid = P(c, "coordinates_midpoint_data") != "no";
     fv_extrap_pars(c, a.ex_s1, a.ex_s2, a.ex_opt, a.ex_on);
     err = launch_fvelm_amr(b.n, rec, c->geom == 0 ? 0 : 2, a, c->dwcall + b.woff, grid, st);
@@ -1356,8 +1376,7 @@ static int launch(nmx_ctx *c, StageArgs a) {
   if (c->fvelm) {
     int nel = a.e1 - a.e0;
     int grid = a.xcd_map ? 8 * ((nel + 7) / 8) : nel;
-    const std::string &fr = P(c, "fv_rec");
-    int rec = fr == "WENOm5_2" ? 5 : fr == "WENOmZ_2" ? 7 : 3;
+    const int rec = fv_rec_pars(c, a.recend, a.rec_s1, a.rec_s2, a.rec_opt, a.rec_on);
     a.fv_mid = P(c, "coordinates_midpoint_data") != "no";
     fv_extrap_pars(c, a.ex_s1, a.ex_s2, a.ex_opt, a.ex_on);
     hipError_t err = launch_fvelm_stage(c->np1, rec, c->geom == 0 ? 0 : 2, a, c->dwc, grid, c->stream);

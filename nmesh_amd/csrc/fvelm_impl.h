@@ -193,10 +193,42 @@ __device__ __forceinline__ void fnum_n(int numflux, const double n[3], const dou
 // ooJ = det(dXb/dx) are constant.  GEOM 2 (CubedShell): the metric is
 // recomputed at each point from the element's box in patch coordinates.
 // fv_divf_extrap_s1, _s2, _opt of dnfn_extrap1 (fv.c:147-150, 558-573)
+// and the end-midpoint reconstruction of fv_rec (dg.c:295-326):
+// recend 0 for the _2 modes (average of the two end points), 1 for the _1
+// modes (the end point), 2 for WENOm3 / WENOm5 / WENOmZ (the average unless
+// fv_rec_WENOm_s1 / _s2 / _opt take the end point, rec1d.c:1096-1186, which
+// then also keeps that end's df from dnfn_extrap1, fv.c:45-79, 558-573)
 struct FvEx {
   double s1, s2;
   int opt, on;   // on: s1 > 0 or s2 < DBL_MAX
+  int recend;
+  double rs1, rs2;
+  int ropt, ron; // ron: fv_rec_WENOm_s1 > 0 or _s2 < DBL_MAX
 };
+
+// rec at an end midpoint from the end point u0 and the next two inward
+// (rec1d_p_WENOm3 / rec1d_m_WENOm3 with rec1d_u_in1_weightfac,
+// rec1d.c:1096-1186; the _1 and _2 modes, rec1d.c:1032-1090, 909-1030);
+// *kept: the end point itself was taken under the WENOm limit
+template <bool EX>
+__device__ __forceinline__ double rec_end(double u0, double u1, double u2, const FvEx &ex, bool *kept) {
+  *kept = false;
+  if (!EX || ex.recend == 0) return 0.5 * u0 + 0.5 * u1;
+  if (ex.recend == 1) return u0;
+  double w1fac = 1.;
+  if (ex.ron) {
+    const double d2 = u2 - u1, d1 = u1 - u0;
+    const double s = fabs(d1) / (fabs(d2) + 1e-100);
+    if (s < ex.rs1 || s >= ex.rs2) w1fac = 0.;
+    if (ex.ropt == -1) w1fac = 0.;
+    else if (ex.ropt == 1 && d1 * d2 <= 0.) w1fac = 0.;
+    else if (ex.ropt == 2 && d1 * d2 >= 0.) w1fac = 0.;
+  }
+  const double w1 = 0.5 * w1fac, w0 = 1. - w1;
+  const double r = w0 * u0 + w1 * u1;
+  *kept = ex.ron && r == u0;   // fv_stat_WENOm_1or2: the rec equals the end point
+  return r;
+}
 
 // The face value of df from the end value u0 and the next two inward, u1
 // and u2 (rec1d_compute_1s1_u with forward = 0, rec1d.c:542-609): u0 sits a
@@ -304,6 +336,7 @@ __device__ __forceinline__ void fv_line(const double *eu, const Face &face, int 
   // dm3, dm2, dm1: points NP-3 .. NP-1 for the right end (NP - 3 is put as
   // well when it is not one of the held points)
   double fR[4], fL[4], d0[4], d1[4], dm3[4], dm2[4], dm1[4];
+  bool keepL[4] = {false, false, false, false}, keepR[4] = {false, false, false, false};
   double oj0 = G.detb, oj1 = G.detb, ojm2 = G.detb, ojm1 = G.detb;
   double sojL = G.dXbdX[DIR] / G.detb;   // sqrtgdiag / ooJ left of i0
 #pragma unroll 1
@@ -351,11 +384,11 @@ __device__ __forceinline__ void fv_line(const double *eu, const Face &face, int 
         if (REC == 5 && im >= 2 && im <= NP - 3) ui[v] = w5_p(w[0], w[1], w[2], w[3], w[4]);
         else if (REC == 7 && im >= 2 && im <= NP - 3) ui[v] = wz_p(w[0], w[1], w[2], w[3], w[4]);
         else if (im >= 1) ui[v] = wm3_p(w[1], w[2], w[3]);
-        else ui[v] = 0.5 * w[2] + 0.5 * w[3];
+        else ui[v] = rec_end<EX>(w[2], w[3], w[4], ex, &keepL[v]);
         if (REC == 5 && im >= 1 && im <= NP - 4) ua[v] = w5_m(w[1], w[2], w[3], w[4], w[5]);
         else if (REC == 7 && im >= 1 && im <= NP - 4) ua[v] = wz_m(w[1], w[2], w[3], w[4], w[5]);
         else if (im <= NP - 3) ua[v] = wm3_m(w[2], w[3], w[4]);
-        else ua[v] = 0.5 * w[2] + 0.5 * w[3];
+        else ua[v] = rec_end<EX>(w[3], w[2], w[1], ex, &keepR[v]);
       }
     } else {
 #pragma unroll
@@ -378,7 +411,7 @@ __device__ __forceinline__ void fv_line(const double *eu, const Face &face, int 
       if (EX && NP >= 5 && i0 == NP - 3) dm3[l] = d;
       if (i0 == 2) {
         // the left end (rec1d_LR_uface_to_uin_1_Carray, left first)
-        d0[l] = fv_extrap1<EX>(d0[l], d1[l], d, ex);
+        if (!(EX && keepL[l])) d0[l] = fv_extrap1<EX>(d0[l], d1[l], d, ex);
         put(0, l, d0[l], oj0);
         if (NP >= 4) put(1, l, d1[l], oj1);
       }
@@ -398,7 +431,7 @@ __device__ __forceinline__ void fv_line(const double *eu, const Face &face, int 
   for (int l = 0; l < 4; ++l) {
     const double b1 = NP == 3 ? d1[l] : dm2[l];
     const double a2 = NP == 3 ? d0[l] : (NP == 4 ? d1[l] : (EX ? dm3[l] : 0.0));
-    put(NP - 1, l, fv_extrap1<EX>(dm1[l], b1, a2, ex), ojm1);
+    put(NP - 1, l, (EX && keepR[l]) ? dm1[l] : fv_extrap1<EX>(dm1[l], b1, a2, ex), ojm1);
     put(NP - 2, l, b1, ojm2);
   }
 }
@@ -512,7 +545,7 @@ __global__ __launch_bounds__(NT) void fvelm_stage_kernel(StageArgs a, const doub
     }
     auto face = [&](int f, int fa, int fb, double ua[4]) { face_value<NP>(a, e, f, fa, fb, ua); };
     fv_element<NP, REC, NT, GEOM, EX>(o, 5L * NP3 * e, face, a.numflux, a.fv_mid,
-                                  FvEx{a.ex_s1, a.ex_s2, a.ex_opt, a.ex_on}, s_wc, s_xb, G, acc);
+                                  FvEx{a.ex_s1, a.ex_s2, a.ex_opt, a.ex_on, a.recend, a.rec_s1, a.rec_s2, a.rec_opt, a.rec_on}, s_wc, s_xb, G, acc);
   }
 }
 
@@ -603,7 +636,7 @@ __global__ __launch_bounds__(NT) void fvelm_amr_kernel(AmrArgs a, const double *
       }
     };
     fv_element<NP, REC, NT, GEOM, EX>(o, 5 * noff, face, a.numflux, a.fv_mid,
-                                  FvEx{a.ex_s1, a.ex_s2, a.ex_opt, a.ex_on}, s_wc, s_xb, G, acc);
+                                  FvEx{a.ex_s1, a.ex_s2, a.ex_opt, a.ex_on, a.recend, a.rec_s1, a.rec_s2, a.rec_opt, a.rec_on}, s_wc, s_xb, G, acc);
   }
 }
 

@@ -38,6 +38,10 @@ struct StageArgs {
   // FV elements: fv_divf_extrap_s1, _s2, _opt (ex_on: s1 > 0 or s2 < DBL_MAX)
   double ex_s1 = 0, ex_s2 = 1.7976931348623157e308;
   int ex_opt = 0, ex_on = 0;
+  // fv_rec's end midpoints: recend 0 (_2 modes), 1 (_1), 2 (WENOm3/5/Z with
+  // fv_rec_WENOm_s1/_s2/_opt; rec_on: s1 > 0 or s2 < DBL_MAX)
+  int recend = 0, rec_opt = 0, rec_on = 0;
+  double rec_s1 = 0, rec_s2 = 1.7976931348623157e308;
 };
 
 // Arguments of one fused adjoint RHS + adjoint RK stage launch: y = L^T lam

@@ -1447,15 +1447,61 @@ double orc_rec1d_m_WENOm3(const double *u, int im)
 
 /* WENOm3_2 at midpoint im = 0..n-2 (rec1d.c:909-1030): WENOm3 inside,
    the average of the two points next to a face */
+/* the end midpoints by fv_rec (dg.c:295-326): 0 the _2 modes, 1 the _1
+   modes (the end point, rec1d.c:1032-1090), 2 WENOm3 / WENOm5 / WENOmZ
+   (rec1d_p_WENOm3 / rec1d_m_WENOm3 with rec1d_u_in1_weightfac,
+   rec1d.c:1096-1186) with fv_rec_WENOm_s1 / _s2 / _opt */
+static int g_rec_end = 0;
+static double g_rec_s1 = 0., g_rec_s2 = DBL_MAX;
+static int g_rec_opt = 0;
+void orc_set_fv_rec_end(int mode, double s1, double s2, int opt)
+{
+  g_rec_end = mode; g_rec_s1 = s1; g_rec_s2 = s2; g_rec_opt = opt;
+}
+static int rec_limit_on(void) { return g_rec_end == 2 && (g_rec_s1 > 0. || g_rec_s2 < DBL_MAX); }
+
+/* rec1d_u_in1_weightfac: 1 if u[i0in2] - u[i0in1] is like u[i0in1] - u[i0] */
+static double rec_weightfac(int n, const double *u, int i0)
+{
+  int sign = (i0 > 0) ? 1 : -1;
+  int i1 = i0 - sign;
+  double w1fac;
+  if (n <= 2) return 0.;
+  w1fac = 1.;
+  if (g_rec_s1 > 0. || g_rec_s2 < DBL_MAX) {
+    int i2 = i1 - sign;
+    double d2 = u[i2] - u[i1];
+    double d1 = u[i1] - u[i0];
+    double s = fabs(d1) / (fabs(d2) + 1e-100 * 1.0);
+    if (s < g_rec_s1 || s >= g_rec_s2) w1fac = 0.;
+    if (g_rec_opt == -1) w1fac = 0.;
+    if (g_rec_opt == 1 && d1 * d2 <= 0.) w1fac = 0.;
+    if (g_rec_opt == 2 && d1 * d2 >= 0.) w1fac = 0.;
+  }
+  return w1fac;
+}
+
 static double rec_p_WENOm3_2(int n, const double *u, int im)
 {
   if (im >= 1 && im <= n - 2) return orc_rec1d_p_WENOm3(u, im);
-  return 0.5 * u[0] + 0.5 * u[1];   /* im == 0 */
+  /* im == 0 */
+  if (g_rec_end == 1) return u[0];
+  if (g_rec_end == 2) {
+    double w1 = 0.5 * rec_weightfac(n, u, 0), w0 = 1. - w1;
+    return w0 * u[0] + w1 * u[1];
+  }
+  return 0.5 * u[0] + 0.5 * u[1];
 }
 static double rec_m_WENOm3_2(int n, const double *u, int im)
 {
   if (im >= 0 && im <= n - 3) return orc_rec1d_m_WENOm3(u, im);
-  return 0.5 * u[n - 2] + 0.5 * u[n - 1];   /* im == n-2 */
+  /* im == n-2 */
+  if (g_rec_end == 1) return u[n - 1];
+  if (g_rec_end == 2) {
+    double w1 = 0.5 * rec_weightfac(n, u, n - 1), w0 = 1. - w1;
+    return w0 * u[n - 1] + w1 * u[n - 2];
+  }
+  return 0.5 * u[n - 2] + 0.5 * u[n - 1];
 }
 /* WENOm5_2 (rec1d.c:1354-1370): WENO5 inside, WENOm3_2 near the faces */
 static double rec_p_WENOm5_2(int n, const double *u, int im)
@@ -1495,6 +1541,14 @@ static double fv_rec_m(int n, const double *u, int im)
 {
   return g_fv_rec == 5 ? rec_m_WENOm5_2(n, u, im)
                        : g_fv_rec == 7 ? rec_m_WENOmZ_2(n, u, im) : rec_m_WENOm3_2(n, u, im);
+}
+
+/* the reconstructions at the end midpoints: out[0] = rec_p at midpoint 0,
+   out[1] = rec_m at midpoint n-2 */
+void orc_fv_rec_ends(int n, const double *u, double *out)
+{
+  out[0] = fv_rec_p(n, u, 0);
+  out[1] = fv_rec_m(n, u, n - 2);
 }
 
 /* scalarwave1 numflux for normal nrm between ui and ua */
@@ -1542,13 +1596,15 @@ static double extrap_end(int n, const double *u, int i0)
 }
 
 /* dnfn_extrap1 on both ends of one line, the left end first
-   (rec1d_LR_uface_to_uin_1_Carray, rec1d.c:612-626; fv.c:558-573) */
-void orc_dnfn_extrap1(int n, double *df)
+   (rec1d_LR_uface_to_uin_1_Carray, rec1d.c:612-626; fv.c:558-573), an end
+   skipped where the reconstruction kept the end point (FV_REC_NO_*_EXTRAP1) */
+static void dnfn_extrap1_sides(int n, double *df, int left, int right)
 {
   if (n <= 2) return;
-  df[0] = extrap_end(n, df, 0);
-  df[n - 1] = extrap_end(n, df, n - 1);
+  if (left) df[0] = extrap_end(n, df, 0);
+  if (right) df[n - 1] = extrap_end(n, df, n - 1);
 }
+void orc_dnfn_extrap1(int n, double *df) { dnfn_extrap1_sides(n, df, 1, 1); }
 
 /* r of one FV element: divf from fv_divf, r_{pi,c} = -divf, r_phi = pi */
 /* unit normal of face 2*dir+side at grid point g: +-row dir of dXdx over
@@ -1651,6 +1707,7 @@ static void fv_elm_rhs(const OMesh *M, const OSW *sw, const OElm *E, double t, c
           for (int v = 0; v < NV; v++) qc[v][i0] = u[v * nn + gl[i0]];
         }
         double fnumR[5] = {0, 0, 0, 0, 0}, fnumL[5];
+        int noL[4] = {0, 0, 0, 0}, noR[4] = {0, 0, 0, 0};
         const int JK = a + n * b;
         /* metric factor sqrtgdiag / det(dXb/dx) and unit normal at each
            midpoint i0+1/2 (fv.c:347-401, node_normal_from_DGinfo
@@ -1698,6 +1755,11 @@ static void fv_elm_rhs(const OMesh *M, const OSW *sw, const OElm *E, double t, c
             for (int v = 0; v < NV; v++) {
               ui[v] = fv_rec_p(n, qc[v], i0);
               ua[v] = fv_rec_m(n, qc[v], i0);
+              /* fv_stat_WENOm_1or2 (fv.c:45-79): the end point kept */
+              if (rec_limit_on() && v < 4) {
+                if (i0 == 0) { if (ui[v] == qc[v][0]) noL[v] = 1; }
+                else if (i0 == n - 2) { if (ua[v] == qc[v][n - 1]) noR[v] = 1; }
+              }
             }
             sw_fnum(sw, nR, ui, ua, fnumR);
           } else {
@@ -1707,7 +1769,7 @@ static void fv_elm_rhs(const OMesh *M, const OSW *sw, const OElm *E, double t, c
           }
           for (int l = 0; l < 4; l++) df[l][i0] = JR * fnumR[l] + JL * fnumL[l];
         }
-        for (int l = 0; l < 4; l++) orc_dnfn_extrap1(n, df[l]);
+        for (int l = 0; l < 4; l++) dnfn_extrap1_sides(n, df[l], !noL[l], !noR[l]);
         for (int i0 = 0; i0 < n; i0++)
           for (int l = 0; l < 4; l++) divf[l][gl[i0] - o] += df[l][i0] * M->det[gl[i0]];
       }

@@ -70,6 +70,8 @@ def lib():
             "orc_set_fv_midpoint": (None, [i]),
             "orc_set_fv_surface_interp": (None, [i]),
             "orc_set_fv_divf_extrap": (None, [d, d, i]),
+            "orc_set_fv_rec_end": (None, [i, d, d, i]),
+            "orc_fv_rec_ends": (None, [i, vp, vp]),
             "orc_pw_parab": (d, [i, d, i, vp]),
             "orc_dnfn_extrap1": (None, [i, vp]),
             "orc_rec1d_p_WENOm3": (d, [vp, i]),
@@ -297,9 +299,14 @@ def set_fv_surface_interp(mode):
     lib().orc_set_fv_surface_interp({"linear": 0, "parabolic": 1}[mode])
 
 
-def set_fv_rec(mode):
-    """reconstruction of the oracle's FV elements: "WENOm5_2" or "WENOm3_2" """
-    lib().orc_set_fv_rec({"WENOm5_2": 5, "WENOm3_2": 3, "WENOmZ_2": 7}[mode])
+def set_fv_rec(mode, s1=0.0, s2=None, opt=0):
+    """reconstruction of the oracle's FV elements (fv_rec): WENOm3, WENOm5,
+    WENOmZ and their _1 and _2 modes, with fv_rec_WENOm_s1 / _s2 (None:
+    DBL_MAX) / _opt for the modes without suffix"""
+    base = mode.split("_")[0]
+    lib().orc_set_fv_rec({"WENOm5": 5, "WENOm3": 3, "WENOmZ": 7}[base])
+    end = {"_2": 0, "_1": 1, "": 2}[mode[len(base):]]
+    lib().orc_set_fv_rec_end(end, float(s1), 1.7976931348623157e308 if s2 is None else float(s2), int(opt))
 
 
 # ---------------------------------------------------------------------------

@@ -157,6 +157,63 @@ def test_dnfn_extrap1_s1_s2_opt_limits():
     assert abs(out[0] + 1.0 / 3.0) < 1e-15 and out[1] == 1.0 and out[2] == 2.0
 
 
+def _ends(u, mode, s1=0.0, s2=None, opt=0):
+    u = np.array(u, dtype=np.float64)
+    out = np.zeros(2)
+    orc.set_fv_rec(mode, s1, s2, opt)
+    try:
+        lib().orc_fv_rec_ends(len(u), _p(u), _p(out))
+    finally:
+        orc.set_fv_rec("WENOm3_2")
+    return out
+
+
+def test_fv_rec_end_midpoints_by_mode():
+    """fv_rec's end midpoints (rec1d.c:1032-1186): the _2 modes average the
+    two end points, the _1 modes take the end point, WENOm3/5/Z average
+    unless fv_rec_WENOm_s1/_s2/_opt take the end point; s = |d1| / |d2| has
+    no factor here, so on linear data s = 1 exactly: s1 = 1 keeps the
+    average, s1 = 1 + 1e-9 takes the end point, s2 = 1 takes it,
+    s2 = 1 + 1e-9 does not; opt 1 takes it on a zigzag"""
+    lin = 0.5 + 2.0 * np.arange(6.0)
+    avg = [0.5 * lin[0] + 0.5 * lin[1], 0.5 * lin[-2] + 0.5 * lin[-1]]
+    end = [lin[0], lin[-1]]
+    for base in ("WENOm3", "WENOm5", "WENOmZ"):
+        assert np.array_equal(_ends(lin, base + "_2"), avg)
+        assert np.array_equal(_ends(lin, base + "_1"), end)
+        assert np.array_equal(_ends(lin, base), avg)
+        for kw, want in [({"s1": 1.0}, avg), ({"s1": 1.0 + 1e-9}, end), ({"s2": 1.0}, end),
+                         ({"s2": 1.0 + 1e-9}, avg), ({"s1": 0.5, "opt": 2}, end),
+                         ({"opt": -1}, avg), ({"s1": 0.5, "opt": -1}, end)]:
+            assert np.array_equal(_ends(lin, base, **kw), want), (base, kw)
+    zz = np.array([0.0, 1.0, 0.5, 0.7, 1.0, 0.0])
+    assert np.array_equal(_ends(zz, "WENOm3", s1=1e-3, opt=1), [0.0, 0.0])
+
+
+def test_fv_rec_end_point_kept_skips_the_extrapolation():
+    """where WENOm keeps the end point (fv_stat_WENOm_1or2, fv.c:45-79),
+    dnfn_extrap1 leaves that end's df alone (fv.c:558-573): with
+    s1 = 1e300 every end keeps its point, so the RHS equals the one of the
+    _1 mode (the same end points, extrapolated) at all nodes off the element
+    faces, and differs on them"""
+    n = 6
+    om = OMesh("box", 1, n, dout=2.0, fv=True)
+    u, _ = om.profile(K, 0.1)
+    u = u + 0.1 * np.random.default_rng(9).uniform(-1, 1, u.shape)
+    try:
+        orc.set_fv_rec("WENOm5", s1=1e300)
+        r_kept = om.rhs(u, 0.1, K, "upwind")
+        orc.set_fv_rec("WENOm5_1")
+        r_one = om.rhs(u, 0.1, K, "upwind")
+    finally:
+        orc.set_fv_rec("WENOm3_2")
+    I = np.indices((n, n, n)).reshape(3, -1)
+    inner = np.all((I > 0) & (I < n - 1), axis=0)
+    mask = np.tile(np.tile(inner, om.nelem), 5)
+    assert np.array_equal(r_kept[mask], r_one[mask])
+    assert np.abs(r_kept[~mask] - r_one[~mask]).max() > 1e-6
+
+
 @pytest.mark.parametrize("rec", ["WENOm3_2", "WENOm5_2", "WENOmZ_2"])
 @pytest.mark.parametrize("n", [5, 7])
 def test_fv_element_rhs_exact_for_linear_states(rec, n):
@@ -480,8 +537,11 @@ def test_unbuilt_fv_options_are_refused():
     m.set_par("fv_divf_extrap_s1", "0.5")
     m.set_par("fv_divf_extrap_s2", "DBL_MAX")
     m.set_par("fv_divf_extrap_opt", "1")
+    for mode in ("WENOm3", "WENOm5_1", "WENOmZ"):
+        m.set_par("fv_rec", mode)
+    m.set_par("fv_rec_WENOm_s1", "0.5")
     for k, v in [("fv_surface_interp", "WENO"), ("fv_surface_interp", "linear Lagrange_if_dg"),
-                 ("fv2dg_interp_use_extrap1", "yes"), ("fv_rec", "WENOm5_1"),
+                 ("fv2dg_interp_use_extrap1", "yes"), ("fv_rec", "WENOmT_2"),
                  ("coordinates_midpoint_data", "sometimes")]:
         with pytest.raises(nm.NmxError):
             m.set_par(k, v)

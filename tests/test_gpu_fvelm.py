@@ -109,6 +109,38 @@ def test_fv_divf_extrap_limits_match_oracle(s1, s2, opt, where):
     assert rel(r_plain, r_o) > 1e-6
 
 
+@pytest.mark.parametrize("rec,s1,s2,opt", [("WENOm3_1", 0.0, None, 0), ("WENOm5_1", 0.0, None, 0),
+                                           ("WENOm5", 0.5, None, 0), ("WENOmZ", 0.3, 3.0, 1),
+                                           ("WENOm3", 1e300, None, 0)])
+@pytest.mark.parametrize("where", ["box", "mixed_refined"])
+def test_fv_rec_end_modes_match_oracle(rec, s1, s2, opt, where):
+    """fv_rec's _1 modes and WENOm3/5/Z with fv_rec_WENOm_s1/_s2/_opt (the
+    end midpoints, and dnfn_extrap1 skipped where the end point was kept)
+    against the oracle; plus an RK4 step on the box"""
+    print("fv rec ends", rec, s1, s2, opt, where, flush=True)
+    extra = {"fv_rec_WENOm_s1": s1, "fv_rec_WENOm_s2": "DBL_MAX" if s2 is None else s2,
+             "fv_rec_WENOm_opt": opt}
+    if where == "box":
+        m, om = make(1, 6, rec, extra=extra)
+    else:
+        m, om = make_mixed(5, rec, levels=1, extra=extra)
+    orc.set_fv_rec(rec, s1, s2, opt)
+    try:
+        u, _ = om.profile(K, 0.2)
+        u = u + 0.2 * np.random.default_rng(12).uniform(-1, 1, u.shape)
+        m.upload(u)
+        r = nm.evolve_setrhs_mesh(m, t=0.2)
+        assert rel(r, om.rhs(u, 0.2, K, "upwind")) <= TOL
+        if where == "box":
+            m.time = 0.2
+            v, _ = om.evolve(u, m.dt, 1, 0.2, K, "RK4")
+            nm.evolve_myln(m, 1)
+            assert rel(m.download(), v) <= TOL
+    finally:
+        orc.set_fv_rec("WENOm3_2")
+        orc.set_fv_midpoint(False)
+
+
 @pytest.mark.parametrize("method", ["RK4", "sspRK3", "Euler"])
 @pytest.mark.parametrize("rec", ["WENOm3_2", "WENOm5_2", "WENOmZ_2"])
 def test_fv_element_steps_match_oracle(method, rec):
