@@ -232,7 +232,9 @@ static int Pi(nmx_ctx *c, const char *k) { return std::atoi(c->par[k].c_str()); 
 // (dg.c:333-335)
 static int fv_rec_pars(nmx_ctx *c, int &recend, double &s1, double &s2, int &opt, int &on) {
   const std::string &fr = P(c, "fv_rec");
-  const int rec = fr.compare(0, 6, "WENOm5") == 0 ? 5 : fr.compare(0, 6, "WENOmZ") == 0 ? 7 : 3;
+  const int rec = fr.compare(0, 6, "WENOm5") == 0   ? 5
+                  : fr.compare(0, 6, "WENOmZ") == 0 ? 7
+                  : fr.compare(0, 6, "WENOmT") == 0 ? 9 : 3;
   const size_t u = fr.find('_');
   recend = u == std::string::npos ? 2 : (fr.substr(u) == "_1" ? 1 : 0);
   s1 = Pd(c, "fv_rec_WENOm_s1");
@@ -372,8 +374,9 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
   if (k == "nmx_geometry" && c->have_mesh)
     return fail(NMX_ESTATE, "set nmx_geometry before amr_setup_mesh");
   if (k == "fv_rec" && v != "WENOm3_2" && v != "WENOm5_2" && v != "WENOmZ_2" && v != "WENOm3_1" &&
-      v != "WENOm5_1" && v != "WENOmZ_1" && v != "WENOm3" && v != "WENOm5" && v != "WENOmZ")
-    return fail(NMX_EUNSUP, "fv_rec: WENOm3, WENOm5, WENOmZ and their _1 and _2 modes are built");
+      v != "WENOm5_1" && v != "WENOmZ_1" && v != "WENOm3" && v != "WENOm5" && v != "WENOmZ" &&
+      v != "WENOmT_2")
+    return fail(NMX_EUNSUP, "fv_rec: WENOm3, WENOm5, WENOmZ (and _1, _2) and WENOmT_2 are built");
   if ((k == "fv_flux" && v != "fnum") || (k == "fv_divf_extrap" && v != "dnfn_extrap1") ||
       (k == "fv_divf_use_only_right_flux" && v != "yes") || (k == "fv_WENOm3_opt_weightratio" && v != "2") ||
       (k == "fv_surface_interp" && v != "linear" && v != "parabolic") ||

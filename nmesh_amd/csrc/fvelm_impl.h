@@ -87,6 +87,38 @@ __device__ __forceinline__ double w5_m(double a, double b, double c, double d, d
   return (p0 * q0 + p1 * q1 + p2 * q2) / (p0 + p1 + p2);
 }
 
+// WENOT (rec1d_p/m_WENOT with uniform weights, rec1d.c:1412-1578): the
+// weights d_k (1 + |b0 - b2|) / (b_k + 1e-40) normalize to d_k / g_k over
+// the sum, i.e. WENO5 with the smoothness indicators to the first power
+__device__ __forceinline__ double wt_p(double a, double b, double c, double d, double e) {
+  const double s01 = a - 2. * b + c, s02 = a - 4. * b + 3. * c;
+  const double s11 = b - 2. * c + d, s12 = b - d;
+  const double s21 = c - 2. * d + e, s22 = 3. * c - 4. * d + e;
+  const double g0 = (13. / 12.) * s01 * s01 + 0.25 * s02 * s02 + kWeno5EpsE;
+  const double g1 = (13. / 12.) * s11 * s11 + 0.25 * s12 * s12 + kWeno5EpsE;
+  const double g2 = (13. / 12.) * s21 * s21 + 0.25 * s22 * s22 + kWeno5EpsE;
+  const double p0 = g1 * g2, p1 = 6. * (g0 * g2), p2 = 3. * (g0 * g1);
+  const double o6 = 1. / 6.;
+  const double q0 = (2. * o6) * a + (-7. * o6) * b + (11. * o6) * c;
+  const double q1 = (-o6) * b + (5. * o6) * c + (2. * o6) * d;
+  const double q2 = (2. * o6) * c + (5. * o6) * d + (-o6) * e;
+  return (p0 * q0 + p1 * q1 + p2 * q2) / (p0 + p1 + p2);
+}
+__device__ __forceinline__ double wt_m(double a, double b, double c, double d, double e) {
+  const double s21 = c - 2. * d + e, s22 = 3. * c - 4. * d + e;
+  const double s11 = b - 2. * c + d, s12 = b - d;
+  const double s01 = a - 2. * b + c, s02 = a - 4. * b + 3. * c;
+  const double g2 = (13. / 12.) * s21 * s21 + 0.25 * s22 * s22 + kWeno5EpsE;
+  const double g1 = (13. / 12.) * s11 * s11 + 0.25 * s12 * s12 + kWeno5EpsE;
+  const double g0 = (13. / 12.) * s01 * s01 + 0.25 * s02 * s02 + kWeno5EpsE;
+  const double p2 = g0 * g1, p1 = 6. * (g0 * g2), p0 = 3. * (g1 * g2);
+  const double o6 = 1. / 6.;
+  const double q2 = (11. * o6) * c + (-7. * o6) * d + (2. * o6) * e;
+  const double q1 = (2. * o6) * b + (5. * o6) * c + (-o6) * d;
+  const double q0 = (-o6) * a + (5. * o6) * b + (2. * o6) * c;
+  return (p0 * q0 + p1 * q1 + p2 * q2) / (p0 + p1 + p2);
+}
+
 // WENO-Z (rec1d_p/m_WENOZ with uniform weights, rec1d.c:1627-1781): weights
 // d_k (1 + tau5 / (beta_k + 1e-40)), tau5 = |beta0 - beta2|
 __device__ __forceinline__ double wz_p(double a, double b, double c, double d, double e) {
@@ -383,10 +415,12 @@ __device__ __forceinline__ void fv_line(const double *eu, const Face &face, int 
         const double *w = q[v];
         if (REC == 5 && im >= 2 && im <= NP - 3) ui[v] = w5_p(w[0], w[1], w[2], w[3], w[4]);
         else if (REC == 7 && im >= 2 && im <= NP - 3) ui[v] = wz_p(w[0], w[1], w[2], w[3], w[4]);
+        else if (REC == 9 && im >= 2 && im <= NP - 3) ui[v] = wt_p(w[0], w[1], w[2], w[3], w[4]);
         else if (im >= 1) ui[v] = wm3_p(w[1], w[2], w[3]);
         else ui[v] = rec_end<EX>(w[2], w[3], w[4], ex, &keepL[v]);
         if (REC == 5 && im >= 1 && im <= NP - 4) ua[v] = w5_m(w[1], w[2], w[3], w[4], w[5]);
         else if (REC == 7 && im >= 1 && im <= NP - 4) ua[v] = wz_m(w[1], w[2], w[3], w[4], w[5]);
+        else if (REC == 9 && im >= 1 && im <= NP - 4) ua[v] = wt_m(w[1], w[2], w[3], w[4], w[5]);
         else if (im <= NP - 3) ua[v] = wm3_m(w[2], w[3], w[4]);
         else ua[v] = rec_end<EX>(w[3], w[2], w[1], ex, &keepR[v]);
       }
@@ -668,6 +702,7 @@ static hipError_t launch_fvelm_stage_v(int np, int rec, int geom, const StageArg
   case N:                                                                           \
     return rec == 5   ? launch_fvelm_t<N, 5, EX>(geom, a, wc, grid, s)              \
            : rec == 7 ? launch_fvelm_t<N, 7, EX>(geom, a, wc, grid, s)              \
+           : rec == 9 ? launch_fvelm_t<N, 9, EX>(geom, a, wc, grid, s)              \
                       : launch_fvelm_t<N, 3, EX>(geom, a, wc, grid, s);
   switch (np) {
     NMX_CASE(3)
@@ -692,6 +727,7 @@ static hipError_t launch_fvelm_amr_v(int np, int rec, int geom, const AmrArgs &a
   case N:                                                                           \
     return rec == 5   ? launch_fvelm_amr_t<N, 5, EX>(geom, a, wc, grid, s)          \
            : rec == 7 ? launch_fvelm_amr_t<N, 7, EX>(geom, a, wc, grid, s)          \
+           : rec == 9 ? launch_fvelm_amr_t<N, 9, EX>(geom, a, wc, grid, s)          \
                       : launch_fvelm_amr_t<N, 3, EX>(geom, a, wc, grid, s);
   switch (np) {
     NMX_CASE(3)

@@ -67,8 +67,8 @@ struct AdjArgs {
 hipError_t launch_dg_stage(int np, int geom, const StageArgs &a, int grid, int nt, hipStream_t s);
 int dg_stage_threads(int np, int div);   // div: nodes per thread 1, 2 or 4
 hipError_t launch_dg_adjoint(int np, int geom, const AdjArgs &a, int grid, hipStream_t s);
-// scalarwave1 on FV elements (fv_divf); rec 5 = WENOm5_2, 7 = WENOmZ_2,
-// 3 = WENOm3_2;
+// scalarwave1 on FV elements (fv_divf); rec: the interior scheme, 5 WENO5,
+// 7 WENO-Z, 9 WENOT, 3 WENOm3 (the end midpoints by StageArgs::recend);
 // wc[np]: distances between the midpoints around each uniform point
 // geom: 0 Cartesian, 2 CubedShell (metric recomputed at nodes and midpoints
 // from egeo, edom and the uniform points in xb)

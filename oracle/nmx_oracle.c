@@ -1503,6 +1503,69 @@ static double rec_m_WENOm3_2(int n, const double *u, int im)
   }
   return 0.5 * u[n - 2] + 0.5 * u[n - 1];
 }
+/* WENOT (rec1d_p_WENOT / rec1d_m_WENOT with the uniform weights,
+   rec1d.c:1412-1578): WENO5's stencils and ideal weights (1, 6, 3)/10 with
+   the non-normalized weights d_k (1 + |b0 - b2|) / (b_k + eps), i.e. the
+   smoothness indicators to the first power */
+static double wenot(double a, double b, double c, double d, double e, const double qw[3][3],
+                    const double optw[3], int mirror)
+{
+  double b01, b02, b11, b12, b21, b22;
+  if (!mirror) {
+    b01 = a - 2. * b + c; b02 = a - 4. * b + 3. * c;
+    b11 = b - 2. * c + d; b12 = b - d;
+    b21 = c - 2. * d + e; b22 = 3. * c - 4. * d + e;
+  } else {   /* rec1d_m_WENOT: b2 from the three right points */
+    b21 = c - 2. * d + e; b22 = 3. * c - 4. * d + e;
+    b11 = b - 2. * c + d; b12 = b - d;
+    b01 = a - 2. * b + c; b02 = a - 4. * b + 3. * c;
+  }
+  double b0 = (1.083333333333333333333333) * b01 * b01 + 0.25 * b02 * b02;
+  double b1 = (1.083333333333333333333333) * b11 * b11 + 0.25 * b12 * b12;
+  double b2 = (1.083333333333333333333333) * b21 * b21 + 0.25 * b22 * b22;
+  double o_p_db02 = 1. + fabs(b0 - b2);
+  double w0 = optw[0] * o_p_db02 / (b0 + 1e-40);
+  double w1 = optw[1] * o_p_db02 / (b1 + 1e-40);
+  double w2 = optw[2] * o_p_db02 / (b2 + 1e-40);
+  double inv = 1. / (w0 + w1 + w2);
+  double u0, u1, u2;
+  if (!mirror) {
+    u0 = qw[0][0] * a + qw[0][1] * b + qw[0][2] * c;
+    u1 = qw[1][0] * b + qw[1][1] * c + qw[1][2] * d;
+    u2 = qw[2][0] * c + qw[2][1] * d + qw[2][2] * e;
+  } else {
+    u0 = qw[0][0] * a + qw[0][1] * b + qw[0][2] * c;
+    u1 = qw[1][0] * b + qw[1][1] * c + qw[1][2] * d;
+    u2 = qw[2][0] * c + qw[2][1] * d + qw[2][2] * e;
+  }
+  return (w0 * inv) * u0 + (w1 * inv) * u1 + (w2 * inv) * u2;
+}
+static double orc_rec1d_p_WENOT(const double *u, int im)
+{
+  const double o6 = 1. / 6.;
+  const double qw[3][3] = {{2. * o6, -7. * o6, 11. * o6}, {-o6, 5. * o6, 2. * o6}, {2. * o6, 5. * o6, -o6}};
+  const double optw[3] = {1., 6., 3.};
+  return wenot(u[im - 2], u[im - 1], u[im], u[im + 1], u[im + 2], qw, optw, 0);
+}
+static double orc_rec1d_m_WENOT(const double *u, int im)
+{
+  /* points im-1 .. im+3; stencil 2 on the right end, ideal weights mirrored */
+  const double o6 = 1. / 6.;
+  const double qw[3][3] = {{-o6, 5. * o6, 2. * o6}, {2. * o6, 5. * o6, -o6}, {11. * o6, -7. * o6, 2. * o6}};
+  const double optw[3] = {3., 6., 1.};
+  return wenot(u[im - 1], u[im], u[im + 1], u[im + 2], u[im + 3], qw, optw, 1);
+}
+/* WENOmT_2 (rec1d.c:1579-1594): WENOT inside, WENOm3_2 near the faces */
+static double rec_p_WENOmT_2(int n, const double *u, int im)
+{
+  if (im >= 2 && im <= n - 3) return orc_rec1d_p_WENOT(u, im);
+  return rec_p_WENOm3_2(n, u, im);
+}
+static double rec_m_WENOmT_2(int n, const double *u, int im)
+{
+  if (im >= 1 && im <= n - 4) return orc_rec1d_m_WENOT(u, im);
+  return rec_m_WENOm3_2(n, u, im);
+}
 /* WENOm5_2 (rec1d.c:1354-1370): WENO5 inside, WENOm3_2 near the faces */
 static double rec_p_WENOm5_2(int n, const double *u, int im)
 {
@@ -1534,17 +1597,24 @@ void orc_set_fv_rec(int mode) { g_fv_rec = mode; }
 
 static double fv_rec_p(int n, const double *u, int im)
 {
-  return g_fv_rec == 5 ? rec_p_WENOm5_2(n, u, im)
-                       : g_fv_rec == 7 ? rec_p_WENOmZ_2(n, u, im) : rec_p_WENOm3_2(n, u, im);
+  return g_fv_rec == 5   ? rec_p_WENOm5_2(n, u, im)
+         : g_fv_rec == 7 ? rec_p_WENOmZ_2(n, u, im)
+         : g_fv_rec == 9 ? rec_p_WENOmT_2(n, u, im) : rec_p_WENOm3_2(n, u, im);
 }
 static double fv_rec_m(int n, const double *u, int im)
 {
-  return g_fv_rec == 5 ? rec_m_WENOm5_2(n, u, im)
-                       : g_fv_rec == 7 ? rec_m_WENOmZ_2(n, u, im) : rec_m_WENOm3_2(n, u, im);
+  return g_fv_rec == 5   ? rec_m_WENOm5_2(n, u, im)
+         : g_fv_rec == 7 ? rec_m_WENOmZ_2(n, u, im)
+         : g_fv_rec == 9 ? rec_m_WENOmT_2(n, u, im) : rec_m_WENOm3_2(n, u, im);
 }
 
 /* the reconstructions at the end midpoints: out[0] = rec_p at midpoint 0,
    out[1] = rec_m at midpoint n-2 */
+void orc_fv_rec_pm(int n, const double *u, int im, double *out)
+{
+  out[0] = fv_rec_p(n, u, im);
+  out[1] = fv_rec_m(n, u, im);
+}
 void orc_fv_rec_ends(int n, const double *u, double *out)
 {
   out[0] = fv_rec_p(n, u, 0);

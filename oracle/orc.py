@@ -72,6 +72,7 @@ def lib():
             "orc_set_fv_divf_extrap": (None, [d, d, i]),
             "orc_set_fv_rec_end": (None, [i, d, d, i]),
             "orc_fv_rec_ends": (None, [i, vp, vp]),
+            "orc_fv_rec_pm": (None, [i, vp, i, vp]),
             "orc_pw_parab": (d, [i, d, i, vp]),
             "orc_dnfn_extrap1": (None, [i, vp]),
             "orc_rec1d_p_WENOm3": (d, [vp, i]),
@@ -304,7 +305,7 @@ def set_fv_rec(mode, s1=0.0, s2=None, opt=0):
     WENOmZ and their _1 and _2 modes, with fv_rec_WENOm_s1 / _s2 (None:
     DBL_MAX) / _opt for the modes without suffix"""
     base = mode.split("_")[0]
-    lib().orc_set_fv_rec({"WENOm5": 5, "WENOm3": 3, "WENOmZ": 7}[base])
+    lib().orc_set_fv_rec({"WENOm5": 5, "WENOm3": 3, "WENOmZ": 7, "WENOmT": 9}[base])
     end = {"_2": 0, "_1": 1, "": 2}[mode[len(base):]]
     lib().orc_set_fv_rec_end(end, float(s1), 1.7976931348623157e308 if s2 is None else float(s2), int(opt))
 

@@ -190,6 +190,53 @@ def test_fv_rec_end_midpoints_by_mode():
     assert np.array_equal(_ends(zz, "WENOm3", s1=1e-3, opt=1), [0.0, 0.0])
 
 
+def _pm(u, im, mode):
+    u = np.array(u, dtype=np.float64)
+    out = np.zeros(2)
+    orc.set_fv_rec(mode)
+    try:
+        lib().orc_fv_rec_pm(len(u), _p(u), im, _p(out))
+    finally:
+        orc.set_fv_rec("WENOm3_2")
+    return out
+
+
+def test_wenot_closed_form_and_symmetry():
+    """WENOmT_2's interior (rec1d_p/m_WENOT, rec1d.c:1412-1578): on the
+    zigzag 0, 1, 0, 1, 0 the smoothness indicators are 25/3, 13/3, 25/3 and
+    the stencils give -7/6, 1/6, 5/6, so the weights d_k / b_k with
+    d = (1, 6, 3) give 0.2096 (WENO5's squared indicators give 0.1922, and
+    the test tells them apart); linear point values are reproduced at the
+    midpoints (every stencil is exact on them); rec_m is rec_p of the
+    reversed line"""
+    from fractions import Fraction as Fr
+    b = [Fr(25, 3), Fr(13, 3), Fr(25, 3)]
+    q = [Fr(-7, 6), Fr(1, 6), Fr(5, 6)]
+    d = [1, 6, 3]
+    a = [d[k] / b[k] for k in range(3)]
+    want = float(sum(a[k] * q[k] for k in range(3)) / sum(a))
+    u = [0.0, 1.0, 0.0, 1.0, 0.0, 1.0, 0.0]
+    got = _pm(u, 2, "WENOmT_2")[0]
+    assert abs(got - want) < 1e-14
+    a5 = [d[k] / b[k] ** 2 for k in range(3)]
+    w5 = float(sum(a5[k] * q[k] for k in range(3)) / sum(a5))
+    assert abs(_pm(u, 2, "WENOm5_2")[0] - w5) < 1e-14 and abs(want - w5) > 1e-2
+    rng = np.random.default_rng(13)
+    for _ in range(20):
+        x = np.arange(8.0)
+        c = rng.standard_normal(2)
+        lin = c[0] + c[1] * x
+        for im in range(2, 5):
+            p, m = _pm(lin, im, "WENOmT_2")
+            exact = c[0] + c[1] * (im + 0.5)
+            assert abs(p - exact) < 1e-12 * (1 + abs(exact)) and abs(m - exact) < 1e-12 * (1 + abs(exact))
+        v = rng.standard_normal(8)
+        for im in range(1, 5):
+            m = _pm(v, im, "WENOmT_2")[1]
+            p = _pm(v[::-1].copy(), 8 - 2 - im, "WENOmT_2")[0]
+            assert abs(m - p) < 1e-14 * (1 + abs(p))
+
+
 def test_fv_rec_end_point_kept_skips_the_extrapolation():
     """where WENOm keeps the end point (fv_stat_WENOm_1or2, fv.c:45-79),
     dnfn_extrap1 leaves that end's df alone (fv.c:558-573): with
@@ -541,7 +588,7 @@ def test_unbuilt_fv_options_are_refused():
         m.set_par("fv_rec", mode)
     m.set_par("fv_rec_WENOm_s1", "0.5")
     for k, v in [("fv_surface_interp", "WENO"), ("fv_surface_interp", "linear Lagrange_if_dg"),
-                 ("fv2dg_interp_use_extrap1", "yes"), ("fv_rec", "WENOmT_2"),
+                 ("fv2dg_interp_use_extrap1", "yes"), ("fv_rec", "WENO3_2"),
                  ("coordinates_midpoint_data", "sometimes")]:
         with pytest.raises(nm.NmxError):
             m.set_par(k, v)
