@@ -73,8 +73,8 @@ int nmx_synchronize(nmx_ctx *ctx);
  * evolve_filter_varlist, evolve_filter_alp, evolve_filter_s, evolve_filter_dn
  * (nmesh_evolve.c:35-39), basis_expfilter_JacobianPower (1|0|-1) and
  * basis_filter_fv (yes|no) (nmesh_basis.c:22-24), the FV elements'
- * amr_uniform_p, amr_fv_p, fv_rec (WENOm3|WENOm5|WENOmZ, each also _1 and
- * _2, and WENOmT_2) with fv_rec_WENOm_s1, _s2, _opt (nmesh_dg.c:35-40), and
+ * amr_uniform_p, amr_fv_p, fv_rec (1|WENOm3|WENOm5|WENOmZ, each WENOm also
+ * _1 and _2, and WENOmT_2) with fv_rec_WENOm_s1, _s2, _opt (nmesh_dg.c:35-40), and
  * coordinates_midpoint_data (no|yes|yes all, nmesh_coordinates.c:84),
  * fv_surface_interp (linear|parabolic, nmesh_dg.c:53; set on a built mesh
  * it rebuilds the face tables and keeps the state), fv_divf_extrap_s1,

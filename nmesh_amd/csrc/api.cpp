@@ -237,6 +237,7 @@ static int fv_rec_pars(nmx_ctx *c, int &recend, double &s1, double &s2, int &opt
                   : fr.compare(0, 6, "WENOmT") == 0 ? 9 : 3;
   const size_t u = fr.find('_');
   recend = u == std::string::npos ? 2 : (fr.substr(u) == "_1" ? 1 : 0);
+  if (fr == "1") recend = 0;
   s1 = Pd(c, "fv_rec_WENOm_s1");
   s2 = Pd(c, "fv_rec_WENOm_s2");
   opt = (int)Pd(c, "fv_rec_WENOm_opt");
@@ -375,8 +376,8 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
     return fail(NMX_ESTATE, "set nmx_geometry before amr_setup_mesh");
   if (k == "fv_rec" && v != "WENOm3_2" && v != "WENOm5_2" && v != "WENOmZ_2" && v != "WENOm3_1" &&
       v != "WENOm5_1" && v != "WENOmZ_1" && v != "WENOm3" && v != "WENOm5" && v != "WENOmZ" &&
-      v != "WENOmT_2")
-    return fail(NMX_EUNSUP, "fv_rec: WENOm3, WENOm5, WENOmZ (and _1, _2) and WENOmT_2 are built");
+      v != "WENOmT_2" && v != "1")
+    return fail(NMX_EUNSUP, "fv_rec: 1, WENOm3, WENOm5, WENOmZ (and _1, _2) and WENOmT_2 are built");
   if ((k == "fv_flux" && v != "fnum") || (k == "fv_divf_extrap" && v != "dnfn_extrap1") ||
       (k == "fv_divf_use_only_right_flux" && v != "yes") || (k == "fv_WENOm3_opt_weightratio" && v != "2") ||
       (k == "fv_surface_interp" && v != "linear" && v != "parabolic") ||
@@ -1307,6 +1308,7 @@ static int launch_bucket(nmx_ctx *c, AmrArgs a, const nmx_ctx::Bucket &b, int di
   hipError_t err;
   if (b.fv) {
     const int rec = fv_rec_pars(c, a.recend, a.rec_s1, a.rec_s2, a.rec_opt, a.rec_on);
+    a.rec_first = P(c, "fv_rec") == "1";
     a.fv_mid = P(c, "coordinates_midpoint_data") != "no";
     fv_extrap_pars(c, a.ex_s1, a.ex_s2, a.ex_opt, a.ex_on);
     err = launch_fvelm_amr(b.n, rec, c->geom == 0 ? 0 : 2, a, c->dwcall + b.woff, grid, st);
@@ -1380,6 +1382,7 @@ static int launch(nmx_ctx *c, StageArgs a) {
     int nel = a.e1 - a.e0;
     int grid = a.xcd_map ? 8 * ((nel + 7) / 8) : nel;
     const int rec = fv_rec_pars(c, a.recend, a.rec_s1, a.rec_s2, a.rec_opt, a.rec_on);
+    a.rec_first = P(c, "fv_rec") == "1";
     a.fv_mid = P(c, "coordinates_midpoint_data") != "no";
     fv_extrap_pars(c, a.ex_s1, a.ex_s2, a.ex_opt, a.ex_on);
     hipError_t err = launch_fvelm_stage(c->np1, rec, c->geom == 0 ? 0 : 2, a, c->dwc, grid, c->stream);

@@ -236,6 +236,7 @@ struct FvEx {
   int recend;
   double rs1, rs2;
   int ropt, ron; // ron: fv_rec_WENOm_s1 > 0 or _s2 < DBL_MAX
+  int first;     // fv_rec = 1: the point left / right of each midpoint (rec1d.c:16-27)
 };
 
 // rec at an end midpoint from the end point u0 and the next two inward
@@ -413,6 +414,11 @@ __device__ __forceinline__ void fv_line(const double *eu, const Face &face, int 
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const double *w = q[v];
+        if (EX && ex.first) {
+          ui[v] = w[2];
+          ua[v] = w[3];
+          continue;
+        }
         if (REC == 5 && im >= 2 && im <= NP - 3) ui[v] = w5_p(w[0], w[1], w[2], w[3], w[4]);
         else if (REC == 7 && im >= 2 && im <= NP - 3) ui[v] = wz_p(w[0], w[1], w[2], w[3], w[4]);
         else if (REC == 9 && im >= 2 && im <= NP - 3) ui[v] = wt_p(w[0], w[1], w[2], w[3], w[4]);
@@ -579,7 +585,8 @@ __global__ __launch_bounds__(NT) void fvelm_stage_kernel(StageArgs a, const doub
     }
     auto face = [&](int f, int fa, int fb, double ua[4]) { face_value<NP>(a, e, f, fa, fb, ua); };
     fv_element<NP, REC, NT, GEOM, EX>(o, 5L * NP3 * e, face, a.numflux, a.fv_mid,
-                                  FvEx{a.ex_s1, a.ex_s2, a.ex_opt, a.ex_on, a.recend, a.rec_s1, a.rec_s2, a.rec_opt, a.rec_on}, s_wc, s_xb, G, acc);
+                                  FvEx{a.ex_s1, a.ex_s2, a.ex_opt, a.ex_on, a.recend, a.rec_s1, a.rec_s2, a.rec_opt, a.rec_on,
+                                       a.rec_first}, s_wc, s_xb, G, acc);
   }
 }
 
@@ -670,7 +677,8 @@ __global__ __launch_bounds__(NT) void fvelm_amr_kernel(AmrArgs a, const double *
       }
     };
     fv_element<NP, REC, NT, GEOM, EX>(o, 5 * noff, face, a.numflux, a.fv_mid,
-                                  FvEx{a.ex_s1, a.ex_s2, a.ex_opt, a.ex_on, a.recend, a.rec_s1, a.rec_s2, a.rec_opt, a.rec_on}, s_wc, s_xb, G, acc);
+                                  FvEx{a.ex_s1, a.ex_s2, a.ex_opt, a.ex_on, a.recend, a.rec_s1, a.rec_s2, a.rec_opt, a.rec_on,
+                                       a.rec_first}, s_wc, s_xb, G, acc);
   }
 }
 

@@ -41,6 +41,7 @@ struct StageArgs {
   // fv_rec's end midpoints: recend 0 (_2 modes), 1 (_1), 2 (WENOm3/5/Z with
   // fv_rec_WENOm_s1/_s2/_opt; rec_on: s1 > 0 or s2 < DBL_MAX)
   int recend = 0, rec_opt = 0, rec_on = 0;
+  int rec_first = 0;               // fv_rec = 1 (first order)
   double rec_s1 = 0, rec_s2 = 1.7976931348623157e308;
 };
 

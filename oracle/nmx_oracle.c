@@ -1597,12 +1597,14 @@ void orc_set_fv_rec(int mode) { g_fv_rec = mode; }
 
 static double fv_rec_p(int n, const double *u, int im)
 {
+  if (g_fv_rec == 1) return u[im];   /* rec1d_p_1, rec1d.c:16-19 */
   return g_fv_rec == 5   ? rec_p_WENOm5_2(n, u, im)
          : g_fv_rec == 7 ? rec_p_WENOmZ_2(n, u, im)
          : g_fv_rec == 9 ? rec_p_WENOmT_2(n, u, im) : rec_p_WENOm3_2(n, u, im);
 }
 static double fv_rec_m(int n, const double *u, int im)
 {
+  if (g_fv_rec == 1) return u[im + 1];   /* rec1d_m_1, rec1d.c:24-27 */
   return g_fv_rec == 5   ? rec_m_WENOm5_2(n, u, im)
          : g_fv_rec == 7 ? rec_m_WENOmZ_2(n, u, im)
          : g_fv_rec == 9 ? rec_m_WENOmT_2(n, u, im) : rec_m_WENOm3_2(n, u, im);

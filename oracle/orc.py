@@ -305,8 +305,8 @@ def set_fv_rec(mode, s1=0.0, s2=None, opt=0):
     WENOmZ and their _1 and _2 modes, with fv_rec_WENOm_s1 / _s2 (None:
     DBL_MAX) / _opt for the modes without suffix"""
     base = mode.split("_")[0]
-    lib().orc_set_fv_rec({"WENOm5": 5, "WENOm3": 3, "WENOmZ": 7, "WENOmT": 9}[base])
-    end = {"_2": 0, "_1": 1, "": 2}[mode[len(base):]]
+    lib().orc_set_fv_rec({"WENOm5": 5, "WENOm3": 3, "WENOmZ": 7, "WENOmT": 9, "1": 1}[base])
+    end = {"_2": 0, "_1": 1, "": 2}[mode[len(base):]] if base != "1" else 0
     lib().orc_set_fv_rec_end(end, float(s1), 1.7976931348623157e308 if s2 is None else float(s2), int(opt))
 
 

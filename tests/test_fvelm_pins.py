@@ -237,6 +237,14 @@ def test_wenot_closed_form_and_symmetry():
             assert abs(m - p) < 1e-14 * (1 + abs(p))
 
 
+def test_fv_rec_1_takes_the_neighbouring_points():
+    """fv_rec = 1 (rec1d_p_1 / rec1d_m_1, rec1d.c:16-27): the values left
+    and right of each midpoint are the grid points themselves"""
+    v = np.random.default_rng(14).standard_normal(7)
+    for im in range(6):
+        assert np.array_equal(_pm(v, im, "1"), [v[im], v[im + 1]])
+
+
 def test_fv_rec_end_point_kept_skips_the_extrapolation():
     """where WENOm keeps the end point (fv_stat_WENOm_1or2, fv.c:45-79),
     dnfn_extrap1 leaves that end's df alone (fv.c:558-573): with
@@ -584,7 +592,7 @@ def test_unbuilt_fv_options_are_refused():
     m.set_par("fv_divf_extrap_s1", "0.5")
     m.set_par("fv_divf_extrap_s2", "DBL_MAX")
     m.set_par("fv_divf_extrap_opt", "1")
-    for mode in ("WENOm3", "WENOm5_1", "WENOmZ"):
+    for mode in ("WENOm3", "WENOm5_1", "WENOmZ", "1"):
         m.set_par("fv_rec", mode)
     m.set_par("fv_rec_WENOm_s1", "0.5")
     for k, v in [("fv_surface_interp", "WENO"), ("fv_surface_interp", "linear Lagrange_if_dg"),

@@ -111,7 +111,8 @@ def test_fv_divf_extrap_limits_match_oracle(s1, s2, opt, where):
 
 @pytest.mark.parametrize("rec,s1,s2,opt", [("WENOm3_1", 0.0, None, 0), ("WENOm5_1", 0.0, None, 0),
                                            ("WENOm5", 0.5, None, 0), ("WENOmZ", 0.3, 3.0, 1),
-                                           ("WENOm3", 1e300, None, 0), ("WENOmT_2", 0.0, None, 0)])
+                                           ("WENOm3", 1e300, None, 0), ("WENOmT_2", 0.0, None, 0),
+                                           ("1", 0.0, None, 0)])
 @pytest.mark.parametrize("where", ["box", "mixed_refined"])
 def test_fv_rec_end_modes_match_oracle(rec, s1, s2, opt, where):
     """fv_rec's _1 modes and WENOm3/5/Z with fv_rec_WENOm_s1/_s2/_opt (the
