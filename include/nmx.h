@@ -76,7 +76,8 @@ int nmx_synchronize(nmx_ctx *ctx);
  * amr_uniform_p, amr_fv_p, fv_rec (1|WENOm3|WENOm5|WENOmZ, each WENOm also
  * _1 and _2, and WENOmT_2) with fv_rec_WENOm_s1, _s2, _opt (nmesh_dg.c:35-40), and
  * coordinates_midpoint_data (no|yes|yes all, nmesh_coordinates.c:84),
- * fv_surface_interp (linear|parabolic, nmesh_dg.c:53; set on a built mesh
+ * fv_surface_interp (linear|parabolic, optionally with the word
+ * Lagrange_if_dg, nmesh_dg.c:53; set on a built mesh
  * it rebuilds the face tables and keeps the state), fv_divf_extrap_s1,
  * fv_divf_extrap_s2 (a number or DBL_MAX), fv_divf_extrap_opt
  * (nmesh_dg.c:41-46),

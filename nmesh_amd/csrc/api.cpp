@@ -358,6 +358,24 @@ extern "C" int nmx_synchronize(nmx_ctx *c) {
 
 static int remirror_with_state(nmx_ctx *c, const Mesh &old);
 
+// fv_surface_interp: "linear" or "parabolic", optionally with the word
+// "Lagrange_if_dg" (dg.c:369-379); false for anything else
+static bool surface_interp_mode(const std::string &v, int *mode, bool *lag) {
+  std::istringstream in(v);
+  int m = -1;
+  bool l = false;
+  for (std::string w; in >> w;) {
+    if (w == "linear" && m < 0) m = 0;
+    else if (w == "parabolic" && m < 0) m = 1;
+    else if (w == "Lagrange_if_dg" && !l) l = true;
+    else return false;
+  }
+  if (m < 0) return false;
+  if (mode) *mode = m;
+  if (lag) *lag = l;
+  return true;
+}
+
 extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
   if (!c || !name || !value) return fail(NMX_EINVAL, "NULL argument");
   std::string k(name), v(value);
@@ -380,7 +398,7 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
     return fail(NMX_EUNSUP, "fv_rec: 1, WENOm3, WENOm5, WENOmZ (and _1, _2) and WENOmT_2 are built");
   if ((k == "fv_flux" && v != "fnum") || (k == "fv_divf_extrap" && v != "dnfn_extrap1") ||
       (k == "fv_divf_use_only_right_flux" && v != "yes") || (k == "fv_WENOm3_opt_weightratio" && v != "2") ||
-      (k == "fv_surface_interp" && v != "linear" && v != "parabolic") ||
+      (k == "fv_surface_interp" && !surface_interp_mode(v, nullptr, nullptr)) ||
       (k == "fv2dg_interp_use_extrap1" && v != "no"))
     return fail(NMX_EUNSUP, k + ": only nmesh's default is built");
   if ((k == "amr_dir_active0" || k == "amr_dir_active1" || k == "amr_dir_active2") && v != "yes")
@@ -421,10 +439,12 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
   c->agraph_steps = 0;
   if (k == "fv_surface_interp" && c->have_mesh) {
     // new face tables for faces taken from FV neighbours; the state stays
-    const int mode = v == "parabolic" ? 1 : 0;
-    if (mode != c->mesh.fv_surface_interp) {
+    int mode = 0;
+    bool lag = false;
+    surface_interp_mode(v, &mode, &lag);
+    if (mode != c->mesh.fv_surface_interp || lag != c->mesh.fv_lagrange_if_dg) {
       Mesh old = c->mesh;
-      std::string err = set_fv_surface_interp(c->mesh, mode);
+      std::string err = set_fv_surface_interp(c->mesh, mode, lag);
       if (!err.empty()) { c->mesh = old; c->par[k] = prev; return fail(NMX_EINVAL, err); }
       int r = remirror_with_state(c, old);
       if (r) { c->par[k] = prev; return r; }
@@ -856,7 +876,7 @@ extern "C" int nmx_mesh_build(nmx_ctx *c) {
     return fail(NMX_EUNSUP, "amr_mesh_type must be BoxMesh or Shell");
   }
   if (!err.empty()) return fail(NMX_EINVAL, err);
-  c->mesh.fv_surface_interp = P(c, "fv_surface_interp") == "parabolic" ? 1 : 0;
+  surface_interp_mode(P(c, "fv_surface_interp"), &c->mesh.fv_surface_interp, &c->mesh.fv_lagrange_if_dg);
   {
     // FV elements: amr_fv_p patches on uniform points (amr_uniform_p)
     std::vector<int> up, fp;

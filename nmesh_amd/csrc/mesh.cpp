@@ -87,14 +87,21 @@ static void make_basis(int n, Basis &b) {
 static void make_uniform(int n, Basis &b) {
   b.n = n;
   int N = n - 1;
-  if (N == 0) { b.x[0] = 0; return; }
+  if (N == 0) { b.x[0] = 0; b.wb[0] = 1; return; }
   b.x[0] = -1; b.x[N] = 1;
-  if (N == 1) return;
   double h = 2. / N;
-  for (int j = 1; j <= N / 2; ++j) {
+  for (int j = 1; j <= N / 2 && N > 1; ++j) {
     double y = (j == N / 2.0) ? 0. : -1. + j * h;
     b.x[N - j] = -y;
     b.x[j] = y;
+  }
+  // barycentric weights 1 / prod_{m != q} (x_q - x_m) for Lagrange rows on
+  // these points (fv_surface_interp = ... Lagrange_if_dg)
+  for (int q = 0; q < n; ++q) {
+    double p = 1.0;
+    for (int m = 0; m < n; ++m)
+      if (m != q) p *= b.x[q] - b.x[m];
+    b.wb[q] = 1.0 / p;
   }
 }
 
@@ -725,6 +732,9 @@ static std::string face_tables(Mesh &M) {
         const Basis &bn = N.fv ? uniform_basis(N.n) : lgl_basis(N.n);
         auto row = [&](int q, double x) {
           if (!N.fv) return lagrange(bn, q, x);
+          // Lagrange_if_dg: full Lagrange on the uniform points into a DG
+          // element (surface.c:1198-1238; its 12-point stencil covers n <= 10)
+          if (M.fv_lagrange_if_dg && !E.fv) return lagrange(bn, q, x);
           return M.fv_surface_interp == 1 ? pw_parab(bn, q, x) : pw_linear(bn, q, x);
         };
         // neighbour in-plane Xb coordinates of all my face points
@@ -956,9 +966,10 @@ std::string set_fv_patches(Mesh &M, const std::vector<int> &patches) {
   return face_tables(M);
 }
 
-std::string set_fv_surface_interp(Mesh &M, int mode) {
+std::string set_fv_surface_interp(Mesh &M, int mode, bool lagrange_if_dg) {
   if (mode != 0 && mode != 1) return "fv_surface_interp must be linear or parabolic";
   M.fv_surface_interp = mode;
+  M.fv_lagrange_if_dg = lagrange_if_dg;
   return face_tables(M);
 }
 

@@ -86,6 +86,7 @@ struct Mesh {
   std::vector<std::vector<InterpSlot>> fint;  // 6 per element
   int maxlevel = 0;
   int fv_surface_interp = 0;        // rows from FV neighbours: 0 piecewise linear, 1 parabolic
+  bool fv_lagrange_if_dg = false;   // ... but Lagrange rows into DG elements
 };
 
 // builders (return "" on success, else an error message)
@@ -108,7 +109,7 @@ std::string set_fv_patches(Mesh &M, const std::vector<int> &patches);
 
 // fv_surface_interp (0 linear, 1 parabolic) for faces taken from FV
 // neighbours; tables rebuilt
-std::string set_fv_surface_interp(Mesh &M, int mode);
+std::string set_fv_surface_interp(Mesh &M, int mode, bool lagrange_if_dg);
 
 // dt = min over elements of dtfac hmin, or uniform_dtfac hmin for FV
 // elements whose hmin lies between two (uniform) points

@@ -204,8 +204,17 @@ double orc_pw_parab(int k, double x, int np, const double *xp) { return pw_parab
 
 /* fv_surface_interp for faces taken from FV neighbours: 0 linear (the
    default), 1 parabolic (surface.c:1182-1196) */
-static int g_fv_surface_interp = 0;
-void orc_set_fv_surface_interp(int mode) { g_fv_surface_interp = mode; }
+static int g_fv_surface_interp = 0, g_fv_lagrange_if_dg = 0;
+void orc_set_fv_surface_interp(int mode) { g_fv_surface_interp = mode & 1; g_fv_lagrange_if_dg = (mode >> 1) & 1; }
+
+/* Lagrange basis function q of the points xp at x, as a product */
+static double lagrange_on_points(int q, double x, int n, const double *xp)
+{
+  double r = 1.;
+  for (int m = 0; m < n; m++)
+    if (m != q) r *= (x - xp[m]) / (xp[q] - xp[m]);
+  return r;
+}
 
 static OBasis *basis(int n)
 {
@@ -1297,7 +1306,12 @@ static void build_ajsurf(const OMesh *M, const OElm *E, int f, const double *pla
         OBasis *bn = basis(nn_);
         double B1[ORC_MAXN], B2[ORC_MAXN];
         for (int q = 0; q < nn_; q++) {
-          if (N->use_fv && g_fv_surface_interp == 1) {
+          if (N->use_fv && g_fv_lagrange_if_dg && !E->use_fv) {
+            /* Lagrange on the uniform points into a DG element
+               (surface.c:1198-1238, LagrangeUNIorder 12 >= n) */
+            B1[q] = lagrange_on_points(q, Cb0, nn_, bn->Xu);
+            B2[q] = lagrange_on_points(q, Cb1, nn_, bn->Xu);
+          } else if (N->use_fv && g_fv_surface_interp == 1) {
             B1[q] = pw_parab_of_x(q, Cb0, nn_, bn->Xu);
             B2[q] = pw_parab_of_x(q, Cb1, nn_, bn->Xu);
           } else if (N->use_fv) {

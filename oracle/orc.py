@@ -297,7 +297,9 @@ def set_fv_divf_extrap(s1=0.0, s2=None, opt=0):
 def set_fv_surface_interp(mode):
     """fv_surface_interp of the oracle's faces taken from FV neighbours:
     "linear" (the default) or "parabolic" """
-    lib().orc_set_fv_surface_interp({"linear": 0, "parabolic": 1}[mode])
+    words = mode.split()
+    lib().orc_set_fv_surface_interp({"linear": 0, "parabolic": 1}[words[0]] +
+                                    (2 if "Lagrange_if_dg" in words[1:] else 0))
 
 
 def set_fv_rec(mode, s1=0.0, s2=None, opt=0):

@@ -434,7 +434,8 @@ def _pw_parab_cells(x, xp, g):
     return float(np.polyval(np.polyfit(pts, g(pts), 2), x))
 
 
-@pytest.mark.parametrize("mode,deg", [("linear", 2), ("parabolic", 2), ("parabolic", 4)])
+@pytest.mark.parametrize("mode,deg", [("linear", 2), ("parabolic", 2), ("parabolic", 4),
+                                      ("linear Lagrange_if_dg", 4)])
 def test_face_values_across_dg_fv_faces(mode, deg):
     """set_ajsurf_vl on the shell with FV patches 0, 2, 4 (the reference's
     sw7FV-DG layout): a DG element takes its FV neighbour's face values by
@@ -448,7 +449,8 @@ def test_face_values_across_dg_fv_faces(mode, deg):
     linear interp of g on the uniform points, or the parabola by cells
     through the three nearest uniform points (x^2 itself: the parabolas
     reproduce it), for a DG receiver, and g itself (Lagrange on 5 LGL points
-    is exact to degree 4) for an FV receiver."""
+    is exact to degree 4) for an FV receiver.  With Lagrange_if_dg the DG
+    receiver takes Lagrange rows on the uniform points: x^4 itself."""
     n = 5
     om = OMesh("shell", 1, n, rin=2.0, rout=6.0, fv_patches=[0, 2, 4])
     xl = lgl(n)
@@ -457,7 +459,9 @@ def test_face_values_across_dg_fv_faces(mode, deg):
     def g(x):
         return x ** deg
 
-    if mode == "linear":
+    if mode == "linear Lagrange_if_dg":
+        G = g   # Lagrange on the 5 uniform points is exact to degree 4
+    elif mode == "linear":
         def G(x):
             return np.interp(x, xu, g(xu))
     else:
@@ -595,7 +599,8 @@ def test_unbuilt_fv_options_are_refused():
     for mode in ("WENOm3", "WENOm5_1", "WENOmZ", "1"):
         m.set_par("fv_rec", mode)
     m.set_par("fv_rec_WENOm_s1", "0.5")
-    for k, v in [("fv_surface_interp", "WENO"), ("fv_surface_interp", "linear Lagrange_if_dg"),
+    m.set_par("fv_surface_interp", "parabolic Lagrange_if_dg")
+    for k, v in [("fv_surface_interp", "WENO"), ("fv_surface_interp", "Lagrange_if_dg"),
                  ("fv2dg_interp_use_extrap1", "yes"), ("fv_rec", "WENO3_2"),
                  ("coordinates_midpoint_data", "sometimes")]:
         with pytest.raises(nm.NmxError):

@@ -256,6 +256,24 @@ def test_fv_buckets_rhs_matches_oracle(case):
     assert rel(r, r_o) <= TOL
 
 
+@pytest.mark.parametrize("mode", ["linear Lagrange_if_dg", "parabolic Lagrange_if_dg"])
+def test_mixed_lagrange_if_dg_matches_oracle(mode):
+    """fv_surface_interp with Lagrange_if_dg: DG elements take their FV
+    neighbours' faces by Lagrange rows on the uniform points, FV elements
+    keep the piecewise rows, on an h-refined mixed shell"""
+    print("mixed", mode, flush=True)
+    m, om = make_mixed(5, "WENOm5_2", levels=1, extra={"fv_surface_interp": mode})
+    u, _ = om.profile(K, 0.2)
+    u = u + 0.1 * np.random.default_rng(15).uniform(-1, 1, u.shape)
+    m.upload(u)
+    orc.set_fv_surface_interp(mode)
+    try:
+        assert rel(nm.evolve_setrhs_mesh(m, t=0.2), om.rhs(u, 0.2, K, "upwind")) <= TOL
+    finally:
+        orc.set_fv_surface_interp("linear")
+        orc.set_fv_midpoint(False)
+
+
 @pytest.mark.parametrize("levels", [0, 1])
 def test_mixed_parabolic_surface_interp_matches_oracle(levels):
     """fv_surface_interp = parabolic: DG elements take their FV neighbours'
