@@ -1647,6 +1647,14 @@ static void sw_fnum(const OSW *sw, const double nrm[3], const double *ui, const 
   else sw_numflux_upwind(nrm, lam, ui, ua, fnum);
 }
 
+/* the numerical flux at one face point, all 5 rows (scalarwave1_fluxes_pt,
+   scalarwave1.c:139-209): numflux 0 upwind, 1 LLF */
+void orc_numflux(int numflux, const double *nrm, const double *ui, const double *ua, double *fnum)
+{
+  OSW sw = {numflux, {0., 0., 0.}};
+  sw_fnum(&sw, nrm, ui, ua, fnum);
+}
+
 /* fv_divf_extrap_s1, _s2, _opt (fv.c:147-150; defaults 0, DBL_MAX, 0) */
 static double g_ex_s1 = 0., g_ex_s2 = DBL_MAX;
 static int g_ex_opt = 0;

@@ -48,6 +48,7 @@ def lib():
             "orc_mesh_elm_hmin": (d, [vp, i]),
             "orc_set_profile": (None, [vp, vp, d, vp, vp]),
             "orc_rhs": (None, [vp, i, vp, vp, d, vp]),
+            "orc_numflux": (None, [i, vp, vp, vp, vp]),
             "orc_evolve": (d, [vp, i, vp, i, d, i, d, vp]),
             "orc_filter": (None, [vp, vp, i, d, d, i, i]),
             "orc_num_threads": (i, []),
