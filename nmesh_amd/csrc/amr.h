@@ -37,9 +37,69 @@ struct SlotRec {
                         // from one neighbour, 3 from several
   int fs1[6], fs2[6];   // kind 1: neighbour node = base + a*fs1 + b*fs2
   int dom;              // CubedShell patch direction and sign
-  int pad[9];
+  int n;                // points per direction
+  int jfirst, jcount;   // batched DG path: the element's interpolation jobs
+  int pad[6];
 };
 static_assert(sizeof(SlotRec) == 256, "SlotRec is 256 bytes");
+
+// One (interpolated face, neighbour) job of the batched DG path, done by the
+// receiving element's workgroup in its load phase.  An element's jobs are
+// grouped in rounds; a round holds whole faces, its lines (4 m per job, one
+// thread each) fit the workgroup and its first-pass results and rows fit the
+// stage buffer of the own fields, which is free until those are written.
+struct RJob {
+  long nbase;           // the neighbour's first state double + the plane's base node
+  int t1, t2;           // neighbour plane node = base + q1*t1 + q2*t2
+  int pk;               // m | f << 4 | sw << 7 | a0 << 8 | a1 << 12 | b0 << 16 | b1 << 20
+                        // | (first job of its face) << 24
+  int offA, offB;       // rows (m doubles each) for my a and my b points in Btab
+  int line0;            // first of this job's 4 m lines in its round
+  int toff, r1off, r2off;   // first-pass results, pass-1 and pass-2 rows in the buffer
+  int nround;           // jobs in the round that starts here (0 inside a round)
+};
+static_assert(sizeof(RJob) == 48, "RJob is 48 bytes");
+constexpr int kBatchLines = 256;      // lines per round
+constexpr int kBatchRows = 1024;      // row values (both passes) per round
+constexpr int kBatchScratch = 4000;   // the own-field buffer at n = 10, in doubles
+constexpr int kBatchMaxN = 10;
+// doubles a round of an element of order n may use: the own-field buffer,
+// at least enough for one face of 4 neighbours of up to 10 points
+__host__ __device__ constexpr int batch_scratch(int n) {
+  return 4 * n * n * n > 240 * n ? (4 * n * n * n < kBatchScratch ? 4 * n * n * n : kBatchScratch) : 240 * n;
+}
+
+// One launch per RK stage for every DG element of a batched mesh: items are
+// elements in 8 lists (one per XCD, contiguous in element order, by
+// decreasing n inside); workgroup q of XCD x takes items lo[x] + q, + g8, ...
+struct BatchArgs {
+  const double *win = nullptr, *U = nullptr, *Ain = nullptr;
+  double *Aout = nullptr, *Wout = nullptr, *R = nullptr;
+  const SlotRec *rec = nullptr;
+  const RJob *jobs = nullptr;
+  const double *Btab = nullptr;
+  const double *dXdx = nullptr;   // element blocks of 9*np
+  const double *xyz = nullptr;    // element blocks of 3*np
+  const double *Dall = nullptr;   // D of order n at batch_doff(n)
+  const double *xball = nullptr;  // LGL nodes of order n at batch_xoff(n)
+  const double *wend_inv = nullptr;   // [n]: 1 / (end weight of the LGL quadrature)
+  const int *xlo = nullptr;       // [9]: items of XCD x in [xlo[x], xlo[x + 1])
+  const double *tdev = nullptr;
+  int tidx = 0;
+  double cA = 0, cW = 0;
+  double k[3] = {0, 0, 0};
+  double om = 0;
+  int wmode = 0, numflux = 0, profile = 0;
+};
+// offsets of order n (2 <= n <= 10) in the concatenated D and node tables
+__host__ __device__ constexpr int batch_doff(int n) { return (n - 1) * n * (2 * n - 1) / 6 - 1; }
+__host__ __device__ constexpr int batch_xoff(int n) { return n * (n - 1) / 2 - 1; }
+// grid: a multiple of 8 (XCD lists)
+hipError_t launch_dg_batch(int geom, const BatchArgs &a, int grid, hipStream_t s);
+// one order: items [a.xlo[0], a.xlo[1]); tile 1 or 2: nodes per thread, 3
+// or 4: 1 or 2 nodes per thread with the epilogue's operands requested at
+// each element's start; grid a multiple of 8
+hipError_t launch_dg_bucket(int np, int geom, const BatchArgs &a, int grid, int tile, hipStream_t s);
 
 struct AmrArgs {
   const double *win = nullptr, *U = nullptr, *Ain = nullptr;

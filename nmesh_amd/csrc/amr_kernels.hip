@@ -18,15 +18,6 @@
 
 namespace nmx {
 
-// Tables that no kernel writes are read through the constant address space:
-// wave-uniform reads from them become scalar loads issued together, instead
-// of a chain of vector loads each waiting for the last.
-#define NMX_CONST __attribute__((address_space(4)))
-template <class T>
-__device__ __forceinline__ const NMX_CONST T *as_const(const T *p) {
-  return (const NMX_CONST T *)p;
-}
-
 // ---------------------------------------------------------------------------
 // face interpolation, one wave per (face, neighbour) job:
 //   pass 1: T[v][x1][q2] = sum_q1 P[v](q1, q2) R1[x1][q1]; a lane owns one

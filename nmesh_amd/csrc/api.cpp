@@ -76,12 +76,28 @@ struct nmx_ctx {
   int filt_jpow = 0;
   bool filt_on = false;
   double *dBtab = nullptr, *dDall = nullptr, *dxball = nullptr, *daj = nullptr;
+  // persistent batched DG path: every DG element as an item in 8 XCD lists,
+  // the items' interpolation jobs, D and nodes of every order
+  SlotRec *dbrec = nullptr;
+  RJob *drjobs = nullptr;
+  double *dbD = nullptr, *dbxb = nullptr, *dbW = nullptr;
+  int *dbxlo = nullptr;
+  // the order-per-launch shape: records by decreasing n, each order's range,
+  // element count and tuned tile (nodes per thread, 0: the nmx_block parameter)
+  SlotRec *dbrec2 = nullptr;
+  int *dbnr = nullptr;
+  std::vector<int> bcount, bdiv;
+  int ncu = 0;
+  int bgrid = 0;          // workgroups of the persistent launch
+  bool anydg = false;     // the batched mesh has DG elements
+  bool batch_fixed = false;   // nmx_batch set by the caller (the autotuner leaves it)
   // div: nodes per thread of this bucket's tile (0: the nmx_block parameter)
   // f1first/f1count: the face interpolation jobs of its elements in djobs1
   struct Bucket {
     int n; int count; SlotRec *drec; long doff; long xoff; int div;
     int f1first, f1count;
-    float ms;   // tuned stage time alone (0: not tuned)
+    float ms;   // tuned stage time alone (0: not tuned) of the shape in use
+    float ms_legacy, ms_order;   // ... with the face kernel, and in the order's own launch
     int fv;     // 1: FV elements on uniform points (fv_divf)
     long woff;  // FV: the bucket's midpoint distances in dwcall
   };
@@ -212,6 +228,7 @@ static void set_defaults(nmx_ctx *c) {
       {"nmx_block", "half"},
       {"nmx_autotune", "yes"},
       {"nmx_concurrent", "yes"},
+      {"nmx_batch", "persistent"},
   };
 }
 
@@ -292,7 +309,9 @@ static void free_mesh_dev(nmx_ctx *c) {
   if (c->dedom) { (void)hipFree(c->dedom); c->dedom = nullptr; }
   {
     void **ptrs[] = {(void **)&c->djobs1, (void **)&c->dmdesc, (void **)&c->dFilt, (void **)&c->dfrec,
-                     (void **)&c->dBtab, (void **)&c->dDall, (void **)&c->dxball, (void **)&c->daj};
+                     (void **)&c->dBtab, (void **)&c->dDall, (void **)&c->dxball, (void **)&c->daj,
+                     (void **)&c->dbrec, (void **)&c->drjobs, (void **)&c->dbD, (void **)&c->dbxb,
+                     (void **)&c->dbW, (void **)&c->dbxlo, (void **)&c->dbrec2, (void **)&c->dbnr};
     for (auto q : ptrs)
       if (*q) { (void)hipFree(*q); *q = nullptr; }
     for (auto &b : c->buckets)
@@ -386,6 +405,8 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
     return fail(NMX_EINVAL, "scalarwave1_numflux must be upwind or LLF");
   if (k == "scalarwave1_profile" && v != "sin")
     return fail(NMX_EUNSUP, "only scalarwave1_profile = sin is supported");
+  if (k == "nmx_batch" && v != "persistent" && v != "buckets" && v != "legacy")
+    return fail(NMX_EINVAL, "nmx_batch must be persistent, buckets or legacy");
   if (k == "nmx_path" && v != "plain" && v != "perelement" && v != "batched")
     return fail(NMX_EINVAL, "nmx_path must be plain, perelement or batched");
   if (k == "nmx_geometry" && v != "analytic" && v != "stored")
@@ -429,6 +450,7 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
   const std::string prev = c->par[k];
   c->par[k] = v;
   if (k == "nmx_block" || k == "nmx_xcd") c->tuned = true;   // explicit choice wins
+  if (k == "nmx_batch") c->batch_fixed = true;
   if (k == "nmx_block")
     for (auto &b : c->buckets) b.div = 0;
   if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
@@ -493,6 +515,7 @@ static void slot_geometry(const Mesh &M, int e, SlotRec &R) {
   R.s0 = M.pat[E.p].s0;
   R.s1 = M.pat[E.p].s1;
   R.dom = M.pat[E.p].dom;
+  R.n = E.n;
 }
 
 // Launch order and lanes of the concurrent buckets, largest first.  Running
@@ -547,6 +570,144 @@ static std::vector<double> fv_cell_widths(int n) {
   return wc;
 }
 
+// Tables of the persistent batched DG launch.  Items: the DG elements in 8
+// lists, one per XCD, each a contiguous range of the element order holding
+// about an eighth of the work, with decreasing n inside (stable), so a
+// workgroup walks its elements order by order.  Each item's interpolation
+// jobs are split into rounds of whole faces whose lines fit the workgroup and
+// whose rows and first-pass results fit the own-field buffer.
+static int upload_batch_tables(nmx_ctx *c, const std::vector<int> &fd,
+                               std::vector<std::vector<RJob>> &erj) {
+  const Mesh &M = c->mesh;
+  const size_t ne = M.e.size();
+  std::vector<int> dg;
+  for (size_t i = 0; i < ne; ++i)
+    if (!M.e[i].fv) dg.push_back((int)i);
+  c->anydg = !dg.empty();
+  if (dg.empty()) return NMX_OK;
+  auto cost = [&](int e) {
+    const double n = M.e[e].n;
+    return n * n * n + 0.25 * (double)erj[e].size() * n * n * n / 2.;
+  };
+  double total = 0;
+  for (int e : dg) total += cost(e);
+  std::vector<int> items, xlo(9, 0);
+  items.reserve(dg.size());
+  size_t k = 0;
+  double acc = 0;
+  for (int x = 0; x < 8; ++x) {
+    xlo[x] = (int)items.size();
+    const size_t k0 = k;
+    while (k < dg.size() && (x == 7 || acc + 0.5 * cost(dg[k]) <= total * (x + 1) / 8.)) acc += cost(dg[k++]);
+    std::vector<int> part(dg.begin() + k0, dg.begin() + k);
+    std::stable_sort(part.begin(), part.end(), [&](int a, int b) { return M.e[a].n > M.e[b].n; });
+    items.insert(items.end(), part.begin(), part.end());
+  }
+  xlo[8] = (int)items.size();
+  std::vector<SlotRec> recs(items.size());
+  std::vector<RJob> jobs;
+  for (size_t it = 0; it < items.size(); ++it) {
+    const int e = items[it];
+    const int n = M.e[e].n;
+    if (n < 2 || n > kBatchMaxN) return fail(NMX_EUNSUP, "batched DG elements need 2 to 10 points per direction");
+    SlotRec &R = recs[it];
+    slot_geometry(M, e, R);
+    for (int f = 0; f < 6; ++f) {
+      const int *d = &fd[48 * e + 8 * f];
+      R.fkind[f] = d[0];
+      if (d[0] == 1) {
+        R.foff[f] = 5 * M.e[d[1]].off + d[2];
+        R.fs1[f] = d[3];
+        R.fs2[f] = d[4];
+      }
+    }
+    R.jfirst = (int)jobs.size();
+    R.jcount = (int)erj[e].size();
+    // rounds of whole faces
+    std::vector<RJob> &J = erj[e];
+    const int cap = batch_scratch(n);
+    size_t j = 0;
+    while (j < J.size()) {
+      const size_t r0 = j;
+      int lines = 0, scr = 0, rows = 0;
+      while (j < J.size()) {
+        size_t je = j + 1;
+        while (je < J.size() && !((J[je].pk >> 24) & 1)) ++je;
+        int fl = 0, fs = 0, fr = 0;
+        for (size_t q = j; q < je; ++q) {
+          const int m = J[q].pk & 15, sw = (J[q].pk >> 7) & 1;
+          const int a0 = (J[q].pk >> 8) & 15, a1 = (J[q].pk >> 12) & 15;
+          const int b0 = (J[q].pk >> 16) & 15, b1 = (J[q].pk >> 20) & 15;
+          const int nx1 = (sw ? b1 - b0 : a1 - a0) + 1, nx2 = (sw ? a1 - a0 : b1 - b0) + 1;
+          fl += 4 * m;
+          fs += 4 * nx1 * m + nx1 * m + nx2 * m;
+          fr += nx1 * m + nx2 * m;
+        }
+        if (fl > kBatchLines || fs > cap || fr > kBatchRows)
+          return fail(NMX_EUNSUP, "face too large for one round");
+        if (j > r0 && (lines + fl > kBatchLines || scr + fs > cap || rows + fr > kBatchRows)) break;
+        rows += fr;
+        for (size_t q = j; q < je; ++q) {
+          const int m = J[q].pk & 15, sw = (J[q].pk >> 7) & 1;
+          const int a0 = (J[q].pk >> 8) & 15, a1 = (J[q].pk >> 12) & 15;
+          const int b0 = (J[q].pk >> 16) & 15, b1 = (J[q].pk >> 20) & 15;
+          const int nx1 = (sw ? b1 - b0 : a1 - a0) + 1, nx2 = (sw ? a1 - a0 : b1 - b0) + 1;
+          if (a1 >= n || b1 >= n) return fail(NMX_EINVAL, "interpolation coverage out of range");
+          J[q].line0 = lines;
+          J[q].toff = scr;
+          J[q].r1off = scr + 4 * nx1 * m;
+          J[q].r2off = J[q].r1off + nx1 * m;
+          J[q].nround = 0;
+          lines += 4 * m;
+          scr += 4 * nx1 * m + nx1 * m + nx2 * m;
+        }
+        j = je;
+      }
+      J[r0].nround = (int)(j - r0);
+    }
+    jobs.insert(jobs.end(), J.begin(), J.end());
+  }
+  if (jobs.empty()) jobs.resize(1);
+  std::vector<double> D, xb, wi(kBatchMaxN + 1, 0.0);
+  for (int n = 2; n <= kBatchMaxN; ++n) {
+    const Basis &b = lgl_basis(n);
+    if ((int)D.size() != batch_doff(n) || (int)xb.size() != batch_xoff(n))
+      return fail(NMX_EINVAL, "batched D table layout");
+    D.insert(D.end(), b.D, b.D + n * n);
+    xb.insert(xb.end(), b.x, b.x + n);
+    wi[n] = 1.0 / b.w[0];
+  }
+  // the order-per-launch shape: the same records by decreasing n, element
+  // order inside; order n holds [nrange[2n], nrange[2n + 1])
+  std::vector<int> perm(items.size());
+  for (size_t i = 0; i < perm.size(); ++i) perm[i] = (int)i;
+  std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) {
+    if (recs[x].n != recs[y].n) return recs[x].n > recs[y].n;
+    return items[x] < items[y];
+  });
+  std::vector<SlotRec> recs2(perm.size());
+  std::vector<int> nrange(2 * (kBatchMaxN + 1), 0);
+  for (size_t i = 0; i < perm.size(); ++i) recs2[i] = recs[perm[i]];
+  for (size_t i = 0; i < recs2.size(); ++i) {
+    const int n = recs2[i].n;
+    if (i == 0 || recs2[i - 1].n != n) nrange[2 * n] = (int)i;
+    nrange[2 * n + 1] = (int)i + 1;
+  }
+  c->bcount.assign(kBatchMaxN + 1, 0);
+  for (int n = 2; n <= kBatchMaxN; ++n) c->bcount[n] = nrange[2 * n + 1] - nrange[2 * n];
+  c->bdiv.assign(kBatchMaxN + 1, 0);
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, c->device));
+  c->ncu = prop.multiProcessorCount;
+  c->bgrid = 8 * ((3 * prop.multiProcessorCount + 7) / 8);
+  int r;
+  if ((r = dev_upload(&c->dbrec, recs)) || (r = dev_upload(&c->drjobs, jobs)) || (r = dev_upload(&c->dbD, D)) ||
+      (r = dev_upload(&c->dbxb, xb)) || (r = dev_upload(&c->dbW, wi)) || (r = dev_upload(&c->dbxlo, xlo)) ||
+      (r = dev_upload(&c->dbrec2, recs2)) || (r = dev_upload(&c->dbnr, nrange)))
+    return r;
+  return NMX_OK;
+}
+
 // Tables of the batched path: face descriptors, face interpolation jobs
 // (single-neighbour faces and faces with several neighbours in separate
 // lists, each grouped by the receiving element's order bucket) with
@@ -558,8 +719,10 @@ static int upload_amr_tables(nmx_ctx *c) {
   std::vector<long> soff(ne);
   for (size_t i = 0; i < ne; ++i) soff[i] = 5 * M.e[i].off;
   std::vector<int> fd(48 * ne, 0);
-  // jobs of each element's interpolated faces, one per (face, neighbour)
+  // jobs of each element's interpolated faces, one per (face, neighbour),
+  // in the face kernel's form (ej1) and the persistent DG kernel's (erj)
   std::vector<std::vector<int>> ej1(ne);
+  std::vector<std::vector<RJob>> erj(ne);
   std::vector<int> mdesc;
   std::vector<double> btab;
   std::map<std::vector<double>, int> bdedup;
@@ -635,6 +798,14 @@ static int upload_amr_tables(nmx_ctx *c) {
         const long so = soff[q.nb];
         std::memcpy(rec + 14, &so, sizeof(long));
         ej1[i].insert(ej1[i].end(), rec, rec + kJobInts);
+        RJob rj;
+        std::memset(&rj, 0, sizeof rj);
+        rj.nbase = so + base;
+        rj.t1 = q.t1; rj.t2 = q.t2;
+        rj.pk = m | (f << 4) | (q.swap << 7) | (q.a0 << 8) | (q.a1 << 12) | (q.b0 << 16) | (q.b1 << 20) |
+                ((si == 0 ? 1 : 0) << 24);
+        rj.offA = oa; rj.offB = ob;
+        erj[i].push_back(rj);
         if (ns > 1) {
           const int md[5] = {(int)region, q.a0, q.a1, q.b0, q.b1};
           mdesc.insert(mdesc.end(), md, md + 5);
@@ -664,7 +835,7 @@ static int upload_amr_tables(nmx_ctx *c) {
     b.count = (int)list.size();
     b.drec = nullptr;
     b.div = fv ? 1 : 0;
-    b.ms = 0.f;
+    b.ms = b.ms_legacy = b.ms_order = 0.f;
     b.fv = fv;
     b.woff = 0;
     if (fv) {
@@ -708,7 +879,8 @@ static int upload_amr_tables(nmx_ctx *c) {
     return r;
   assign_lanes(c);
   if (jobs1.empty()) jobs1.assign(kJobInts, 0);
-  return dev_upload(&c->djobs1, jobs1);
+  if ((r = dev_upload(&c->djobs1, jobs1))) return r;
+  return upload_batch_tables(c, fd, erj);
 }
 
 // host -> device mirror of the mesh (element-block layout)
@@ -1339,9 +1511,91 @@ static int launch_bucket(nmx_ctx *c, AmrArgs a, const nmx_ctx::Bucket &b, int di
   return NMX_OK;
 }
 
+static BatchArgs batch_args(nmx_ctx *c, const StageArgs &sa) {
+  BatchArgs b;
+  b.win = sa.win; b.U = sa.U; b.Ain = sa.Ain; b.Aout = sa.Aout; b.Wout = sa.Wout; b.R = sa.R;
+  b.rec = c->dbrec; b.jobs = c->drjobs; b.Btab = c->dBtab;
+  b.dXdx = c->dXdx; b.xyz = c->dxyz; b.Dall = c->dbD; b.xball = c->dbxb;
+  b.tdev = sa.tdev; b.tidx = sa.tidx;
+  b.wend_inv = c->dbW; b.xlo = c->dbxlo;
+  b.cA = sa.cA; b.cW = sa.cW;
+  for (int d = 0; d < 3; ++d) b.k[d] = sa.k[d];
+  b.om = sa.om; b.wmode = sa.wmode; b.numflux = sa.numflux; b.profile = sa.profile;
+  return b;
+}
+
+// the persistent launch of every DG element of a batched mesh (one per stage)
+static int launch_batch(nmx_ctx *c, const StageArgs &sa) {
+  hipError_t err = launch_dg_batch(c->geom, batch_args(c, sa), c->bgrid, c->stream);
+  if (err != hipSuccess) return fail(NMX_EHIP, std::string("batch launch: ") + hipGetErrorString(err));
+  return NMX_OK;
+}
+
+// the DG elements of order n in one launch of the order's own kernel, with
+// tile (launch_dg_bucket)
+static int launch_order(nmx_ctx *c, const StageArgs &sa, int n, int div, hipStream_t st) {
+  BatchArgs b = batch_args(c, sa);
+  b.rec = c->dbrec2;
+  b.xlo = c->dbnr + 2 * n;
+  const int count = c->bcount[n];
+  const int grid = 8 * ((std::min(count, 8 * c->ncu) + 7) / 8);
+  hipError_t err = launch_dg_bucket(n, c->geom, b, grid, div, st);
+  if (err != hipSuccess) return fail(NMX_EHIP, std::string("bucket launch: ") + hipGetErrorString(err));
+  return NMX_OK;
+}
+
 static int launch_amr(nmx_ctx *c, const StageArgs &sa) {
   const bool perelm = (P(c, "nmx_path") == "perelement");
   AmrArgs a = amr_args(c, sa);
+  if (!perelm && P(c, "nmx_batch") == "buckets") {
+    // one launch per order, side by side on the stream lanes (FV buckets
+    // with their face kernel)
+    std::string blk = P(c, "nmx_block");
+    const int div0 = blk == "half" ? 2 : blk == "quarter" ? 4 : 1;
+    const bool fork = c->buckets.size() > 1 && P(c, "nmx_concurrent") == "yes";
+    auto lane_stream = [c](int lane) { return lane == 0 ? c->stream : c->helper[lane - 1]; };
+    if (fork) {
+      HIPCHK(hipEventRecord(c->evfork, c->stream));
+      for (int l = 1; l < nmx_ctx::kLanes; ++l) HIPCHK(hipStreamWaitEvent(lane_stream(l), c->evfork, 0));
+    }
+    for (size_t k = 0; k < c->buckets.size(); ++k) {
+      const int bi = c->border[k];
+      const auto &b = c->buckets[bi];
+      hipStream_t st = fork ? lane_stream(c->blane[bi]) : c->stream;
+      if (b.fv) {
+        FaceArgs fa;
+        fa.win = sa.win; fa.Btab = c->dBtab; fa.aj = c->daj;
+        fa.jobs = c->djobs1 + (long)kJobInts * b.f1first; fa.nfaces = b.f1count;
+        hipError_t err = launch_face_interp(fa, st);
+        if (err != hipSuccess) return fail(NMX_EHIP, std::string("face launch: ") + hipGetErrorString(err));
+        if (int r = launch_bucket(c, a, b, 1, sa.xcd_map, st)) return r;
+      } else {
+        const int tile = c->bdiv[b.n] ? c->bdiv[b.n] : (div0 >= 2 ? 2 : 1);
+        if (int r = launch_order(c, sa, b.n, tile, st)) return r;
+      }
+    }
+    if (fork)
+      for (int l = 1; l < nmx_ctx::kLanes; ++l) {
+        HIPCHK(hipEventRecord(c->evjoin[l - 1], lane_stream(l)));
+        HIPCHK(hipStreamWaitEvent(c->stream, c->evjoin[l - 1], 0));
+      }
+    return NMX_OK;
+  }
+  if (!perelm && P(c, "nmx_batch") == "persistent") {
+    // FV buckets (faces from the face kernel), then every DG element at once
+    std::string blk = P(c, "nmx_block");
+    for (size_t k = 0; k < c->buckets.size(); ++k) {
+      const auto &b = c->buckets[c->border[k]];
+      if (!b.fv) continue;
+      FaceArgs fa;
+      fa.win = sa.win; fa.Btab = c->dBtab; fa.aj = c->daj;
+      fa.jobs = c->djobs1 + (long)kJobInts * b.f1first; fa.nfaces = b.f1count;
+      hipError_t err = launch_face_interp(fa, c->stream);
+      if (err != hipSuccess) return fail(NMX_EHIP, std::string("face launch: ") + hipGetErrorString(err));
+      if (int r = launch_bucket(c, a, b, 1, sa.xcd_map, c->stream)) return r;
+    }
+    return c->anydg ? launch_batch(c, sa) : NMX_OK;
+  }
   std::string blk = P(c, "nmx_block");
   const int div0 = blk == "half" ? 2 : blk == "quarter" ? 4 : 1;
   // interpolated faces of one bucket's elements, just before its stage kernel
@@ -1731,28 +1985,71 @@ extern "C" int nmx_autotune(nmx_ctx *c) {
     sa.wmode = 1; sa.cA = c->dt / 3.; sa.cW = c->dt / 2.; sa.tidx = 0;
     HIPCHK(hipMemcpyAsync(c->dW0, c->dU, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->dA, c->dU, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream));
-    if ((r = launch_amr(c, sa))) return r;   // fills the face buffer
+    {   // the face buffer of every bucket, for the face-kernel shape
+      const std::string shape = P(c, "nmx_batch");
+      c->par["nmx_batch"] = "legacy";
+      r = launch_amr(c, sa);
+      c->par["nmx_batch"] = shape;
+      if (r) return r;
+    }
     const int xcd = best_x == "yes" ? 1 : 0;
     AmrArgs a = amr_args(c, sa);
     for (auto &b : c->buckets) {
-      int best_div = 0;
-      float bms = 1e30f;
-      const std::vector<int> divs = b.fv ? std::vector<int>{1} : std::vector<int>{1, 2, 4};
-      for (int div : divs) {
-        if ((r = launch_bucket(c, a, b, div, xcd, c->stream))) return r;   // warm-up
+      // each order's tile, timed alone: with the face kernel's buffer
+      // (FV buckets and the legacy shape) and in the order's own launch of
+      // the receiver-gathering kernel (DG buckets)
+      for (int variant = 0; variant < (b.fv ? 1 : 2); ++variant) {
+        int best_div = 0;
+        float bms = 1e30f;
+        const std::vector<int> divs = b.fv ? std::vector<int>{1}
+                                      : variant == 0 ? std::vector<int>{1, 2, 4} : std::vector<int>{1, 2, 3, 4};
+        auto one = [&](int div) {
+          return variant == 0 ? launch_bucket(c, a, b, div, xcd, c->stream)
+                              : launch_order(c, sa, b.n, div, c->stream);
+        };
+        for (int div : divs) {
+          if ((r = one(div))) return r;   // warm-up
+          HIPCHK(hipEventRecord(t0, c->stream));
+          for (int s = 0; s < 5 && !r; ++s) r = one(div);
+          if (r) return r;
+          HIPCHK(hipEventRecord(t1, c->stream));
+          HIPCHK(hipEventSynchronize(t1));
+          float ms = 0;
+          HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+          if (ms < bms) { bms = ms; best_div = div; }
+        }
+        if (variant == 0) { b.div = best_div; b.ms_legacy = bms; }
+        else { c->bdiv[b.n] = best_div; b.ms_order = bms; }
+      }
+      if (b.fv) b.ms_order = b.ms_legacy;
+    }
+    auto lanes_for = [c](const std::string &shape) {
+      for (auto &b : c->buckets) b.ms = shape == "legacy" ? b.ms_legacy : b.ms_order;
+      assign_lanes(c);
+    };
+    lanes_for(P(c, "nmx_batch"));
+    // launch shape: one persistent launch per stage, one launch per order on
+    // the stream lanes, or per order with the face kernel first, whichever
+    // takes the steps fastest (unless set)
+    if (!c->batch_fixed && c->anydg) {
+      std::string best_shape = P(c, "nmx_batch");
+      float bt = 1e30f;
+      for (const char *shape : {"persistent", "buckets", "legacy"}) {
+        c->par["nmx_batch"] = shape;
+        lanes_for(shape);
+        if ((r = issue_step(c, method, c->dt, 0, false))) return r;   // warm-up
         HIPCHK(hipEventRecord(t0, c->stream));
-        for (int s = 0; s < 5 && !r; ++s) r = launch_bucket(c, a, b, div, xcd, c->stream);
+        for (int s = 0; s < 4 && !r; ++s) r = issue_step(c, method, c->dt, 0, (s & 1) != 0);
         if (r) return r;
         HIPCHK(hipEventRecord(t1, c->stream));
         HIPCHK(hipEventSynchronize(t1));
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, t0, t1));
-        if (ms < bms) { bms = ms; best_div = div; }
+        if (ms < bt) { bt = ms; best_shape = shape; }
       }
-      b.div = best_div;
-      b.ms = bms;
+      c->par["nmx_batch"] = best_shape;
+      lanes_for(best_shape);
     }
-    assign_lanes(c);
   }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);

@@ -6,6 +6,15 @@
 
 namespace nmx {
 
+// Tables that no kernel writes are read through the constant address space:
+// wave-uniform reads from them become scalar loads issued together, instead
+// of a chain of vector loads each waiting for the last.
+#define NMX_CONST __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const NMX_CONST T *as_const(const T *p) {
+  return (const NMX_CONST T *)p;
+}
+
 // upwind: F = fnum - f_inner for the 4 flux-carrying fields.  With A = f.n
 // Jacobian [[0,-n^T],[-n,0]], A+/- = (A +- |A|)/2 gives
 // F_pi = (dpi + n.dc)/2 and F_c = n F_pi with d = inner - adjacent
@@ -95,12 +104,15 @@ struct ElmGeo {
 // RHS of scalarwave1 at one node and the RK stage combination.
 // s_u[v][np3]: own pi,cx,cy,cz; s_f[v][f][np2]: adjacent values on face f;
 // gbase: element's first state double; node: ijk.
+// Upre/Apre: the epilogue's u_p and accumulator base at this node, loaded
+// earlier by the caller (nullptr: loaded here)
 template <int NP, int GEOM>
 __device__ __forceinline__ void node_update(const NodeOut &o, const ElmGeo &g,
                                             const double (*s_u)[NP * NP * NP],
                                             const double (*s_f)[6][NP * NP],
                                             const double *s_D, const double *s_xb, long gbase,
-                                            int node) {
+                                            int node, const double *Upre = nullptr,
+                                            const double *Apre = nullptr) {
   constexpr int NP2 = NP * NP;
   constexpr int NP3 = NP * NP * NP;
   constexpr bool CURV = (GEOM != 0);
@@ -203,12 +215,12 @@ __device__ __forceinline__ void node_update(const NodeOut &o, const ElmGeo &g,
   double Ubuf[5], Abuf[5];
   if (o.wmode) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) Ubuf[v] = o.U[gi + v * NP3];
+    for (int v = 0; v < 5; ++v) Ubuf[v] = Upre ? Upre[v] : o.U[gi + v * NP3];
   }
   if (o.Aout) {
     const double *bas = o.Ain ? o.Ain : o.win;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) Abuf[v] = bas[gi + v * NP3];
+    for (int v = 0; v < 5; ++v) Abuf[v] = Apre ? Apre[v] : bas[gi + v * NP3];
   }
   if (o.R) {
 #pragma unroll

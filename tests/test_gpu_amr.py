@@ -19,13 +19,15 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-def make(kind, luni, n, nvec=None, path="batched", radius=10, levels=0, k=K, numflux="upwind"):
+def make(kind, luni, n, nvec=None, path="batched", radius=10, levels=0, k=K, numflux="upwind", shape=None):
     pars = {"amr_mesh_type": "BoxMesh" if kind == "box" else "Shell", "amr_luni": luni,
             "amr_n0": n, "amr_n1": n, "amr_n2": n, "amr_BoxMesh_dout": 8.0,
             "amr_Shell_rin": 2.0, "amr_Shell_rout": 20.0,
             "amr_hrefine_sphere_radius": radius, "amr_hrefine_sphere_levels": levels,
             "scalarwave1_k": "%.17g %.17g %.17g" % tuple(k),
             "scalarwave1_numflux": numflux, "nmx_path": path}
+    if shape is not None:
+        pars["nmx_batch"] = shape
     m = nm.Mesh(pars, device=0)
     nm.amr_setup_mesh(m)
     om = OMesh(kind, luni, n, dout=8.0, rin=2.0, rout=20.0, radius=radius, levels=levels)
@@ -40,34 +42,40 @@ def state(om, seed, amp=0.1):
     return u + amp * np.random.default_rng(seed).uniform(-1, 1, u.shape)
 
 
+SHAPES = ["persistent", "buckets", "legacy"]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("kind,luni,n", [("box", 2, 5), ("shell", 1, 6)])
-def test_batched_kernel_on_uniform_mesh(kind, luni, n):
-    print("batched uniform", kind, flush=True)
-    m, om = make(kind, luni, n)
+def test_batched_kernel_on_uniform_mesh(kind, luni, n, shape):
+    print("batched uniform", kind, shape, flush=True)
+    m, om = make(kind, luni, n, shape=shape)
     u = state(om, 1)
     m.upload(u)
     assert rel(nm.evolve_setrhs_mesh(m, t=0.1), om.rhs(u, 0.1, K)) <= TOL
 
 
+@pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("numflux", ["upwind", "LLF"])
-def test_mixed_order_shell_rhs(numflux):
-    print("mixed order", numflux, flush=True)
+def test_mixed_order_shell_rhs(numflux, shape):
+    print("mixed order", numflux, shape, flush=True)
     rng = np.random.default_rng(12)
     nvec = rng.integers(4, 11, 48).astype(np.int32)     # N in 3..9
-    m, om = make("shell", 1, 5, nvec=nvec, numflux=numflux)
+    m, om = make("shell", 1, 5, nvec=nvec, numflux=numflux, shape=shape)
     u = state(om, 2)
     m.upload(u)
     assert rel(nm.evolve_setrhs_mesh(m, t=0.1), om.rhs(u, 0.1, K, numflux)) <= TOL
 
 
-def test_refined_mixed_order_shell_rk4_steps():
+@pytest.mark.parametrize("shape", SHAPES)
+def test_refined_mixed_order_shell_rk4_steps(shape):
     """h-refined shell (nested spheres: coarse faces see 4 fine neighbours)
     with random orders; every RK4 step checked."""
-    print("amr mixed steps", flush=True)
+    print("amr mixed steps", shape, flush=True)
     base = OMesh("shell", 1, 4, rin=2.0, rout=20.0, radius=7, levels=1)
     rng = np.random.default_rng(13)
     nvec = rng.integers(4, 11, base.nelem).astype(np.int32)
-    m, om = make("shell", 1, 4, nvec=nvec, radius=7, levels=1)
+    m, om = make("shell", 1, 4, nvec=nvec, radius=7, levels=1, shape=shape)
     info = np.array([m.elm_info(e) for e in range(m.nelem)])
     assert (info[:, 1] == 2).any() and (info[:, 1] == 1).any()
     u = state(om, 3, amp=0.02)
@@ -82,11 +90,12 @@ def test_refined_mixed_order_shell_rk4_steps():
         assert rel(m.download(), u) <= TOL, "step %d" % s
 
 
-def test_perelement_and_batched_launches_agree_bitwise():
-    print("perelement vs batched", flush=True)
+@pytest.mark.parametrize("shape", SHAPES)
+def test_perelement_and_batched_launches_agree_bitwise(shape):
+    print("perelement vs batched", shape, flush=True)
     rng = np.random.default_rng(14)
     nvec = rng.integers(3, 9, 48).astype(np.int32)
-    m1, om = make("shell", 1, 5, nvec=nvec, path="batched")
+    m1, om = make("shell", 1, 5, nvec=nvec, path="batched", shape=shape)
     m2, _ = make("shell", 1, 5, nvec=nvec, path="perelement")
     u = state(om, 4)
     m1.upload(u)
@@ -118,16 +127,17 @@ def test_refined_mixed_order_shell_other_methods(method):
         assert rel(m.download(), u) <= TOL, "step %d" % s
 
 
-def test_graph_steps_match_single_steps_and_rhs_after_evolve():
+@pytest.mark.parametrize("shape", SHAPES)
+def test_graph_steps_match_single_steps_and_rhs_after_evolve(shape):
     """Captured multi-step graphs against one step per call (bitwise), and an
     RHS of the evolved state (its faces recomputed from HBM) against the
     oracle."""
-    print("amr graph vs single", flush=True)
+    print("amr graph vs single", shape, flush=True)
     base = OMesh("shell", 1, 4, rin=2.0, rout=20.0, radius=7, levels=1)
     rng = np.random.default_rng(16)
     nvec = rng.integers(4, 11, base.nelem).astype(np.int32)
-    m1, om = make("shell", 1, 4, nvec=nvec, radius=7, levels=1)
-    m2, _ = make("shell", 1, 4, nvec=nvec, radius=7, levels=1)
+    m1, om = make("shell", 1, 4, nvec=nvec, radius=7, levels=1, shape=shape)
+    m2, _ = make("shell", 1, 4, nvec=nvec, radius=7, levels=1, shape=shape)
     m1.set_par("nmx_graph_steps", 4)
     u = state(om, 6, amp=0.02)
     m1.upload(u)
