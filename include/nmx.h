@@ -210,6 +210,15 @@ int nmx_filter(nmx_ctx *ctx, int reg);
 int nmx_checkpoint_save_vars(nmx_ctx *ctx, int reg, const char *path);
 int nmx_checkpoint_load_vars(nmx_ctx *ctx, int reg, const char *path);
 int nmx_vars_crc(nmx_ctx *ctx, int reg, uint64_t *crc);
+/* nmx_vars_crc_noalloc: no counterpart in the reference.  It is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same result as
+ * nmx_vars_crc, but it calls no allocator: the CRC tables, the per-element
+ * contributions and the result word live in one static device array per GPU
+ * (256 KiB + 45 KiB, set up by nmx_ctx_create), so meshes of at most 32,768
+ * elements (else NMX_EUNSUP).  The array keeps the last call's contents;
+ * calls that overlap on one GPU race on it. */
+int nmx_vars_crc_noalloc(nmx_ctx *ctx, int reg, uint64_t *crc);
 uint64_t nmx_crc64(uint64_t crc, const void *buf, size_t nbytes);
 uint64_t nmx_crc64_combine(uint64_t crc1, uint64_t crc2, uint64_t nbytes2);
 

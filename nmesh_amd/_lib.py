@@ -87,6 +87,7 @@ def _proto(lib):
         "nmx_checkpoint_save_vars": (i, [vp, i, C.c_char_p]),
         "nmx_checkpoint_load_vars": (i, [vp, i, C.c_char_p]),
         "nmx_vars_crc": (i, [vp, i, P(C.c_uint64)]),
+        "nmx_vars_crc_noalloc": (i, [vp, i, P(C.c_uint64)]),
         "nmx_crc64": (C.c_uint64, [C.c_uint64, vp, C.c_size_t]),
         "nmx_crc64_combine": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64]),
     }

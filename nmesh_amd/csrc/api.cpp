@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include <map>
 #include <sstream>
 #include <string>
@@ -272,6 +273,25 @@ static void fv_extrap_pars(nmx_ctx *c, double &s1, double &s2, int &opt, int &on
   on = (s1 > 0. || s2 < 1.7976931348623157e308) ? 1 : 0;
 }
 
+// the static scratch of nmx_vars_crc_noalloc on this device (current):
+// its address recorded and the CRC tables copied in, once per device
+static int noalloc_setup(int device) {
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
+  if (crc_scratch(device)) return NMX_OK;
+  std::vector<uint64_t> tab, vtab, klane, aw, ypow;
+  crc_device_tables(tab, vtab, klane, aw, ypow);
+  std::vector<uint64_t> t(kCrcScrPart, 0);
+  std::copy(tab.begin(), tab.end(), t.begin() + kCrcScrTab);
+  std::copy(vtab.begin(), vtab.end(), t.begin() + kCrcScrVtab);
+  std::copy(klane.begin(), klane.end(), t.begin() + kCrcScrKlane);
+  std::copy(aw.begin(), aw.end(), t.begin() + kCrcScrAw);
+  std::copy(ypow.begin(), ypow.end(), t.begin() + kCrcScrYpow);
+  hipError_t e = crc_scratch_setup(device, t.data(), (long)t.size());
+  if (e != hipSuccess) return fail(NMX_EHIP, std::string("static CRC scratch: ") + hipGetErrorString(e));
+  return NMX_OK;
+}
+
 extern "C" int nmx_ctx_create(nmx_ctx **out, int device) {
   if (!out) return fail(NMX_EINVAL, "ctx pointer is NULL");
   if (device == -1) {   // host-only context: mesh and tables, no device mirror
@@ -297,6 +317,7 @@ extern "C" int nmx_ctx_create(nmx_ctx **out, int device) {
     HIPCHK(hipStreamCreateWithFlags(&c->helper[i], hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&c->evjoin[i], hipEventDisableTiming));
   }
+  if (int r = noalloc_setup(device)) return r;
   *out = c;
   return NMX_OK;
 }
@@ -1814,6 +1835,31 @@ extern "C" int nmx_vars_crc(nmx_ctx *c, int reg, uint64_t *crc) {
   if (err != hipSuccess) return fail(NMX_EHIP, std::string("crc launch: ") + hipGetErrorString(err));
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(crc, c->dcrcout, sizeof(uint64_t), hipMemcpyDeviceToHost));
+  return NMX_OK;
+}
+
+// The same CRC as nmx_vars_crc with no device allocation: the tables,
+// element contributions and result live in the static scratch set up by
+// nmx_ctx_create.  Calls that overlap on one GPU race on that scratch.
+extern "C" int nmx_vars_crc_noalloc(nmx_ctx *c, int reg, uint64_t *crc) {
+  NEED_DEV(c);
+  double *u = reg_ptr(c, reg);
+  if (!u || !crc) return fail(NMX_EINVAL, "bad register or NULL result");
+  if (c->mesh.nnodes >= (1L << 32)) return fail(NMX_EUNSUP, "the device CRC takes fewer than 2^32 nodes");
+  if ((long)c->mesh.e.size() > kCrcNoAllocMaxElems)
+    return fail(NMX_EUNSUP, "nmx_vars_crc_noalloc takes at most " + std::to_string(kCrcNoAllocMaxElems) +
+                                " elements (its static scratch); use nmx_vars_crc");
+  HIPCHK(hipSetDevice(c->device));
+  uint64_t *p = crc_scratch(c->device);
+  if (!p) return fail(NMX_ESTATE, "the static CRC scratch of this device is not set up");
+  CrcArgs a;
+  a.u = u; a.eoff = c->deoff; a.en = c->den; a.nelem = (int)c->mesh.e.size(); a.nnodes = c->mesh.nnodes;
+  a.tab = p + kCrcScrTab; a.vtab = p + kCrcScrVtab; a.klane = p + kCrcScrKlane; a.aw = p + kCrcScrAw;
+  a.ypow = p + kCrcScrYpow; a.part = p + kCrcScrPart;
+  hipError_t err = launch_vars_crc(a, p + kCrcScrOut, c->stream);
+  if (err != hipSuccess) return fail(NMX_EHIP, std::string("crc launch: ") + hipGetErrorString(err));
+  HIPCHK(hipMemcpyAsync(crc, p + kCrcScrOut, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return NMX_OK;
 }
 

@@ -123,6 +123,34 @@ __global__ __launch_bounds__(1024) void crc_xor_kernel(const uint64_t *part, int
   }
 }
 
+// ---------------------------------------------------------------------------
+// The scratch of nmx_vars_crc_noalloc: the CRC tables, the per-element
+// contributions and the result word in one static device array, so that
+// entry allocates nothing.  Each device holds its own copy for the life of
+// the process; its address is taken once per device with that device
+// current and kept here.  In host code `scratch` names a host-side stand-in,
+// never the device memory: only the recorded addresses are used.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) uint64_t scratch[kCrcScratchWords];
+constexpr int kMaxDevices = 64;
+static uint64_t *g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t crc_scratch_setup(int device, const uint64_t *tables, long nwords) {
+  if (device < 0 || device >= kMaxDevices || nwords != kCrcScrPart) return hipErrorInvalidValue;
+  if (g_scratch_ptr[device]) return hipSuccess;
+  uint64_t *ptr = nullptr;
+  hipError_t err = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+  if (err != hipSuccess) return err;
+  err = hipMemcpy(ptr, tables, nwords * sizeof(uint64_t), hipMemcpyHostToDevice);
+  if (err != hipSuccess) return err;
+  g_scratch_ptr[device] = ptr;
+  return hipSuccess;
+}
+
+uint64_t *crc_scratch(int device) {
+  return (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+}
+
 hipError_t launch_vars_crc(const CrcArgs &a, uint64_t *out, hipStream_t s) {
   if (a.nelem <= 0) return hipErrorInvalidValue;
   int grid = (a.nelem + kCrcThreads / 64 - 1) / (kCrcThreads / 64);

@@ -93,6 +93,20 @@ struct CrcArgs {
 // out[0] = CRC (device memory)
 hipError_t launch_vars_crc(const CrcArgs &a, uint64_t *out, hipStream_t s);
 
+// The static scratch of nmx_vars_crc_noalloc, in 64-bit words: the tables
+// of CrcArgs (tab, vtab, klane, aw, ypow) at fixed offsets, the result word,
+// then one contribution per element, for at most kCrcNoAllocMaxElems
+// elements (256 KiB plus 45 KiB per device).
+constexpr long kCrcNoAllocMaxElems = 1L << 15;
+constexpr long kCrcScrTab = 0, kCrcScrVtab = 2048, kCrcScrKlane = 4096, kCrcScrAw = 4096 + 576,
+               kCrcScrYpow = 4096 + 576 + 16, kCrcScrOut = kCrcScrYpow + 1024, kCrcScrPart = kCrcScrOut + 32;
+constexpr long kCrcScratchWords = kCrcScrPart + kCrcNoAllocMaxElems;
+// once per device, with the device current: record the array's device
+// address and copy the tables (the first kCrcScrPart words) into it
+hipError_t crc_scratch_setup(int device, const uint64_t *tables, long nwords);
+// the array's device address on this device (nullptr before the set-up)
+uint64_t *crc_scratch(int device);
+
 // One source of a new element's state after mesh adaptation: the element's
 // points in [lo, hi] (per direction) take A2 (x) A1 (x) A0 applied to the
 // source element (rows of ns doubles at a0, a1, a2 in atab).  a0 < 0: copy
