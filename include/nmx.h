@@ -97,7 +97,9 @@ int nmx_synchronize(nmx_ctx *ctx);
  * prefixed nmx_ (nmx_path:
  * plain|batched|perelement, nmx_graph: yes|no, nmx_graph_steps, nmx_geometry:
  * stored|analytic, nmx_block: full|half|quarter, nmx_xcd: yes|no,
- * nmx_autotune: yes|no). */
+ * nmx_autotune: yes|no, nmx_batch: legacy|buckets|persistent (the batched
+ * path's launch shape; the autotuner times the three unless it is set),
+ * nmx_stream_hints: 0..7 (streaming cache hints of the stage epilogue)). */
 int nmx_set_param(nmx_ctx *ctx, const char *name, const char *value);
 int nmx_get_param(nmx_ctx *ctx, const char *name, char *buf, int buflen);
 
@@ -105,6 +107,26 @@ int nmx_get_param(nmx_ctx *ctx, const char *name, char *buf, int buflen);
  * parameters (replaces amr_setup_mesh, mesh.c:181-319, and coordinates_init,
  * coordinates.c:133-443).  Sets mesh time 0 and dt per the dt parameter. */
 int nmx_mesh_build(nmx_ctx *ctx);
+/* Take the mesh nmesh already holds (the patch and element tables behind
+ * amr_setup_mesh, mesh.c:181-319; tElm, nmesh_amr.h:87-102), instead of
+ * building one:
+ *   npat patches: pat_type 0 box, 1 CubedShell; pat_geo[12 p ..]: dom, s0,
+ *     s1 (CubedSphere radii), bbox[6], xc[3];
+ *   nelem elements in eid order: elm_pat, elm_level, elm_loc[24 e + k] the
+ *     octant digit of level k+1 (nmesh's eloc->loc, '0'+ijk or ijk),
+ *     elm_n[3 e + d] points per direction (equal), elm_bbox[6 e ..];
+ *   per face: nfnb[6 e + f] neighbours (node->nfnb), fnb[48 e + 8 f + k] their
+ *     element index (node->fnb in eid order), nbface[...] the face of the
+ *     neighbour touching f (amr_get_nbface, connections.c:1114-1157).
+ * The library rebuilds the forest from the locations and requires every
+ * table it derives to equal the caller's: the boxes bit for bit, neighbour
+ * counts, neighbours in nmesh's order and their faces
+ * (amr_elm_nbinfo_to_elm_fnb, connections.c:2882; locate_facenb_in_fnbs,
+ * connections.c:1090).  A difference is NMX_EINVAL naming the first element
+ * and face that differ; on success the mesh is taken (as nmx_mesh_build). */
+int nmx_mesh_import(nmx_ctx *ctx, int npat, const int *pat_type, const double *pat_geo, int nelem,
+                    const int *elm_pat, const int *elm_level, const unsigned char *elm_loc, const int *elm_n,
+                    const double *elm_bbox, const int *nfnb, const int *fnb, const int *nbface);
 /* Set points per direction of every element (n[nelem]) at set-up and rebuild
  * tables; the registers start from zero (element orders chosen before the
  * initial data, as mesh.c:181-319 does with amr_n). */
@@ -149,7 +171,9 @@ int nmx_mesh_elm_fnb(nmx_ctx *ctx, int elm, int f, int *nb, int *nbface, int *nn
 int nmx_mesh_coords(nmx_ctx *ctx, double *x);
 /* dXdx[(3a+b)*nnodes+node], sqrtgdiag[a*nnodes+node] (coordinates.c:133-224,728-805) */
 int nmx_mesh_geometry(nmx_ctx *ctx, double *dXdx, double *sqrtgdiag);
-/* time step from the dt/dtfac parameters (storage.c:1305-1386) and time */
+/* time step from the dt/dtfac parameters (storage.c:1305-1386) and time;
+ * nmx_set_dt sets the parameter dt to the value, so it stays through mesh
+ * changes until dt is set to auto again */
 int nmx_get_dt(nmx_ctx *ctx, double *dt);
 int nmx_set_dt(nmx_ctx *ctx, double dt);
 int nmx_get_time(nmx_ctx *ctx, double *t);
@@ -162,6 +186,14 @@ int nmx_state_download(nmx_ctx *ctx, int reg, double *u);
 /* one variable of one element, as Vard(node, vi) would hold it */
 int nmx_state_upload_elm(nmx_ctx *ctx, int reg, int elm, int var, const double *d, int np);
 int nmx_state_download_elm(nmx_ctx *ctx, int reg, int elm, int var, double *d, int np);
+/* every variable of every element in one call: ptrs[5*elm + var] is the
+ * caller's array of that variable of that element (Vard(node, vi),
+ * nmesh_amr_loops.h:181; np doubles in ijk order), var in scalarwave1's vlu
+ * order pi, cx, cy, cz, phi (scalarwave1.c:670-672); nvar must be 5.  The
+ * arrays are gathered into pinned host staging and moved in one copy (and
+ * the reverse), which replaces nelem*5 single transfers. */
+int nmx_state_upload_ptrs(nmx_ctx *ctx, int reg, const double *const *ptrs, int nelem, int nvar);
+int nmx_state_download_ptrs(nmx_ctx *ctx, int reg, double *const *ptrs, int nelem, int nvar);
 /* set u to scalarwave1's profile at time t (scalarwave1_set_profile, scalarwave1.c:574-642) */
 int nmx_state_set_profile(nmx_ctx *ctx, int reg, double t);
 
@@ -179,7 +211,10 @@ int nmx_evolve(nmx_ctx *ctx, int nsteps);
 /* evolve_mesh (main.c:583-666): full steps until the mesh time reaches the
  * parameter finaltime, or iterations * dt when finaltime <= 0, within
  * max(dt 1e-7, finaltime 1e-14); nothing when both are 0 (their defaults,
- * nmesh_main.c:34-35).  Te starts at the mesh time as in evolve_mesh. */
+ * nmesh_main.c:34-35).  Te starts at the mesh time as in evolve_mesh.  The
+ * step count is fixed at the call from the current dt, finaltime and
+ * iterations (the reference re-reads them after each step): callers that
+ * change dt or those parameters during a run call this again in chunks. */
 int nmx_evolve_mesh(nmx_ctx *ctx);
 /* full steps taken since the mesh was built (mesh->iteration, main.c:617) */
 int nmx_get_iteration(nmx_ctx *ctx, long *iteration);

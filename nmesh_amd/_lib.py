@@ -29,6 +29,7 @@ def _proto(lib):
         "nmx_set_param": (i, [vp, C.c_char_p, C.c_char_p]),
         "nmx_get_param": (i, [vp, C.c_char_p, C.c_char_p, i]),
         "nmx_mesh_build": (i, [vp]),
+        "nmx_mesh_import": (i, [vp, i, vp, vp, i, vp, vp, vp, vp, vp, vp, vp, vp]),
         "nmx_mesh_set_elm_n": (i, [vp, vp, i]),
         "nmx_mesh_hadapt": (i, [vp, vp, i]),
         "nmx_mesh_padapt": (i, [vp, vp, i]),
@@ -50,6 +51,8 @@ def _proto(lib):
         "nmx_state_download": (i, [vp, i, vp]),
         "nmx_state_upload_elm": (i, [vp, i, i, i, vp, i]),
         "nmx_state_download_elm": (i, [vp, i, i, i, vp, i]),
+        "nmx_state_upload_ptrs": (i, [vp, i, vp, i, i]),
+        "nmx_state_download_ptrs": (i, [vp, i, vp, i, i]),
         "nmx_state_set_profile": (i, [vp, i, d]),
         "nmx_setrhs": (i, [vp, i, i, d]),
         "nmx_autotune": (i, [vp]),

@@ -90,6 +90,7 @@ struct BatchArgs {
   double k[3] = {0, 0, 0};
   double om = 0;
   int wmode = 0, numflux = 0, profile = 0;
+  int nt = 0;                     // streaming hints of the epilogue (NodeOut::nt)
 };
 // offsets of order n (2 <= n <= 10) in the concatenated D and node tables
 __host__ __device__ constexpr int batch_doff(int n) { return (n - 1) * n * (2 * n - 1) / 6 - 1; }
@@ -118,6 +119,7 @@ struct AmrArgs {
   double k[3] = {0, 0, 0};
   double om = 0;
   int wmode = 0, numflux = 0, profile = 0, xcd_map = 0;
+  int nt = 0;                     // streaming hints of the epilogue (NodeOut::nt)
   int fv_mid = 0;                 // FV buckets: metric and normal at the midpoints
   // FV buckets: fv_divf_extrap_s1, _s2, _opt (ex_on: s1 > 0 or s2 < DBL_MAX)
   double ex_s1 = 0, ex_s2 = 1.7976931348623157e308;

@@ -111,6 +111,8 @@ struct nmx_ctx {
   double graph_dt = 0;
   int graph_method = -1;
   double *hT = nullptr;     // pinned stage-time staging
+  double *hstage = nullptr; // pinned state staging (element blocks) for host transfers
+  long hstage_n = 0;
   int tcap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // helper streams for concurrent order buckets (fork/join inside graphs)
@@ -229,7 +231,8 @@ static void set_defaults(nmx_ctx *c) {
       {"nmx_block", "half"},
       {"nmx_autotune", "yes"},
       {"nmx_concurrent", "yes"},
-      {"nmx_batch", "persistent"},
+      {"nmx_batch", "legacy"},
+      {"nmx_stream_hints", "0"},
   };
 }
 
@@ -371,6 +374,7 @@ extern "C" int nmx_ctx_destroy(nmx_ctx *c) {
   (void)hipStreamSynchronize(c->stream);
   free_mesh_dev(c);
   if (c->hT) (void)hipHostFree(c->hT);
+  if (c->hstage) (void)hipHostFree(c->hstage);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->evfork) (void)hipEventDestroy(c->evfork);
@@ -1046,6 +1050,70 @@ static int upload_mesh(nmx_ctx *c) {
   return NMX_OK;
 }
 
+// the tail of a mesh set-up: time, dt and the device mirror
+static int mesh_ready(nmx_ctx *c) {
+  c->have_mesh = true;
+  c->time = 0; c->Te = 0; c->te = 0; c->iteration = 0;
+  if (P(c, "dt") == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"), Pd(c, "uniform_dtfac"));
+  else c->dt = Pd(c, "dt");
+  if (c->device < 0) return NMX_OK;                    // host-only context
+  return upload_mesh(c);
+}
+
+extern "C" int nmx_mesh_import(nmx_ctx *c, int npat, const int *pat_type, const double *pat_geo, int nelem,
+                               const int *elm_pat, const int *elm_level, const unsigned char *elm_loc,
+                               const int *elm_n, const double *elm_bbox, const int *nfnb, const int *fnb,
+                               const int *nbface) {
+  if (!c || !pat_type || !pat_geo || !elm_pat || !elm_level || !elm_loc || !elm_n || !elm_bbox || !nfnb || !fnb ||
+      !nbface)
+    return fail(NMX_EINVAL, "NULL argument");
+  if (npat < 1 || nelem < 1) return fail(NMX_EINVAL, "need patches and elements");
+  if (c->device >= 0) HIPCHK(hipSetDevice(c->device));
+  std::vector<ImportPatch> pat(npat);
+  for (int p = 0; p < npat; ++p) {
+    const double *g = pat_geo + 12 * p;
+    pat[p].type = pat_type[p];
+    pat[p].dom = (int)g[0];
+    pat[p].s0 = g[1];
+    pat[p].s1 = g[2];
+    for (int f = 0; f < 6; ++f) pat[p].bbox[f] = g[3 + f];
+    for (int d = 0; d < 3; ++d) pat[p].xc[d] = g[9 + d];
+  }
+  std::vector<ImportElm> elm(nelem);
+  for (int e = 0; e < nelem; ++e) {
+    ImportElm &q = elm[e];
+    q.p = elm_pat[e];
+    q.l = elm_level[e];
+    const int *n3 = elm_n + 3 * e;
+    if (n3[0] != n3[1] || n3[0] != n3[2]) return fail(NMX_EUNSUP, "elements need the same n in all directions");
+    q.n = n3[0];
+    if (q.n > Pi(c, "amr_nmax")) return fail(NMX_EINVAL, "element n is above amr_nmax");
+    if (q.l < 0 || q.l >= 24) return fail(NMX_EINVAL, "element level out of range");
+    for (int k = 0; k < q.l; ++k) {
+      const unsigned char ch = elm_loc[24 * e + k];
+      q.path[k] = (unsigned char)(ch >= '0' ? ch - '0' : ch);   // nmesh's '0' + octant, or the octant
+    }
+    for (int f = 0; f < 6; ++f) q.bbox[f] = elm_bbox[6 * e + f];
+    for (int f = 0; f < 6; ++f) {
+      q.nfnb[f] = nfnb[6 * e + f];
+      if (q.nfnb[f] < 0 || q.nfnb[f] > kMaxNb) return fail(NMX_EUNSUP, "more than 8 neighbours on a face");
+      for (int k = 0; k < q.nfnb[f]; ++k) {
+        q.fnb[f][k] = fnb[48 * e + 8 * f + k];
+        q.nbf[f][k] = nbface[48 * e + 8 * f + k];
+      }
+    }
+  }
+  Mesh M;
+  std::string err = import_mesh(M, pat, elm);
+  if (!err.empty()) return fail(NMX_EINVAL, "nmx_mesh_import: " + err);
+  for (auto &Pt : M.pat)
+    if (Pt.type == PAT_CUBEDSHELL && (Pt.xc[0] != 0 || Pt.xc[1] != 0 || Pt.xc[2] != 0))
+      return fail(NMX_EUNSUP, "cubed-sphere patches need their centre at the origin");
+  surface_interp_mode(P(c, "fv_surface_interp"), &M.fv_surface_interp, &M.fv_lagrange_if_dg);
+  c->mesh = std::move(M);
+  return mesh_ready(c);
+}
+
 extern "C" int nmx_mesh_build(nmx_ctx *c) {
   if (!c) return fail(NMX_EINVAL, "ctx is NULL");
   if (c->device >= 0) HIPCHK(hipSetDevice(c->device));
@@ -1090,12 +1158,7 @@ extern "C" int nmx_mesh_build(nmx_ctx *c) {
           return fail(NMX_EUNSUP, "FV elements need the shell centre at the origin");
     }
   }
-  c->have_mesh = true;
-  c->time = 0; c->Te = 0; c->te = 0;
-  if (P(c, "dt") == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"), Pd(c, "uniform_dtfac"));
-  else c->dt = Pd(c, "dt");
-  if (c->device < 0) return NMX_OK;                    // host-only context
-  return upload_mesh(c);
+  return mesh_ready(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -1366,7 +1429,17 @@ extern "C" int nmx_mesh_geometry(nmx_ctx *c, double *dXdx, double *sg) {
 }
 
 extern "C" int nmx_get_dt(nmx_ctx *c, double *dt) { NEED_MESH(c); *dt = c->dt; return NMX_OK; }
-extern "C" int nmx_set_dt(nmx_ctx *c, double dt) { NEED_MESH(c); c->dt = dt; return NMX_OK; }
+// an explicit dt: recorded as the parameter, so a later mesh change or a
+// change of dtfac keeps it (the auto rule applies only while dt = auto)
+extern "C" int nmx_set_dt(nmx_ctx *c, double dt) {
+  NEED_MESH(c);
+  if (!(dt > 0.)) return fail(NMX_EINVAL, "dt must be positive");
+  char buf[32];
+  std::snprintf(buf, sizeof buf, "%.17g", dt);
+  c->par["dt"] = buf;
+  c->dt = dt;
+  return NMX_OK;
+}
 extern "C" int nmx_get_time(nmx_ctx *c, double *t) { NEED_MESH(c); *t = c->time; return NMX_OK; }
 extern "C" int nmx_set_time(nmx_ctx *c, double t) {
   NEED_MESH(c);
@@ -1400,25 +1473,80 @@ static void blocks_to_flat(const Mesh &M, const double *b, double *u) {
   }
 }
 
+// pinned host staging of one register in element blocks: one copy moves
+// the whole state
+static int host_stage(nmx_ctx *c, double **h) {
+  if (c->hstage_n < c->nstate) {
+    if (c->hstage) (void)hipHostFree(c->hstage);
+    c->hstage = nullptr;
+    c->hstage_n = 0;
+    HIPCHK(hipHostMalloc((void **)&c->hstage, c->nstate * sizeof(double), hipHostMallocDefault));
+    c->hstage_n = c->nstate;
+  }
+  *h = c->hstage;
+  return NMX_OK;
+}
+
 extern "C" int nmx_state_upload(nmx_ctx *c, int reg, const double *u) {
   NEED_DEV(c);
-  double *d = reg_ptr(c, reg);
-  if (!d) return fail(NMX_EINVAL, "bad register");
-  std::vector<double> b(c->nstate);
-  flat_to_blocks(c->mesh, u, b.data());
-  HIPCHK(hipMemcpyAsync(d, b.data(), c->nstate * 8, hipMemcpyHostToDevice, c->stream));
+  double *d = reg_ptr(c, reg), *b = nullptr;
+  if (!d || !u) return fail(NMX_EINVAL, "bad register or NULL state");
+  if (int r = host_stage(c, &b)) return r;
+  HIPCHK(hipStreamSynchronize(c->stream));   // the staging buffer is free
+  flat_to_blocks(c->mesh, u, b);
+  HIPCHK(hipMemcpyAsync(d, b, c->nstate * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return NMX_OK;
 }
 
 extern "C" int nmx_state_download(nmx_ctx *c, int reg, double *u) {
   NEED_DEV(c);
-  double *d = reg_ptr(c, reg);
-  if (!d) return fail(NMX_EINVAL, "bad register");
-  std::vector<double> b(c->nstate);
-  HIPCHK(hipMemcpyAsync(b.data(), d, c->nstate * 8, hipMemcpyDeviceToHost, c->stream));
+  double *d = reg_ptr(c, reg), *b = nullptr;
+  if (!d || !u) return fail(NMX_EINVAL, "bad register or NULL state");
+  if (int r = host_stage(c, &b)) return r;
+  HIPCHK(hipMemcpyAsync(b, d, c->nstate * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  blocks_to_flat(c->mesh, b.data(), u);
+  blocks_to_flat(c->mesh, b, u);
+  return NMX_OK;
+}
+
+// every variable of every element through per-element pointers, as nmesh
+// holds them (Vard(node, vi), nmesh_amr_loops.h:181): gathered into the
+// pinned staging in element-block order, then one copy (and the reverse)
+extern "C" int nmx_state_upload_ptrs(nmx_ctx *c, int reg, const double *const *ptrs, int nelem, int nvar) {
+  NEED_DEV(c);
+  double *d = reg_ptr(c, reg), *b = nullptr;
+  if (!d || !ptrs) return fail(NMX_EINVAL, "bad register or NULL pointer table");
+  if (nelem != (int)c->mesh.e.size() || nvar != 5) return fail(NMX_EINVAL, "need nelem elements of 5 variables");
+  for (long i = 0; i < 5L * nelem; ++i)
+    if (!ptrs[i]) return fail(NMX_EINVAL, "NULL variable pointer at element " + std::to_string(i / 5));
+  if (int r = host_stage(c, &b)) return r;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int e = 0; e < nelem; ++e) {
+    const Elm &E = c->mesh.e[e];
+    const long np = (long)E.n * E.n * E.n;
+    for (int v = 0; v < 5; ++v) std::memcpy(b + 5 * E.off + v * np, ptrs[5L * e + v], np * 8);
+  }
+  HIPCHK(hipMemcpyAsync(d, b, c->nstate * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return NMX_OK;
+}
+
+extern "C" int nmx_state_download_ptrs(nmx_ctx *c, int reg, double *const *ptrs, int nelem, int nvar) {
+  NEED_DEV(c);
+  double *d = reg_ptr(c, reg), *b = nullptr;
+  if (!d || !ptrs) return fail(NMX_EINVAL, "bad register or NULL pointer table");
+  if (nelem != (int)c->mesh.e.size() || nvar != 5) return fail(NMX_EINVAL, "need nelem elements of 5 variables");
+  for (long i = 0; i < 5L * nelem; ++i)
+    if (!ptrs[i]) return fail(NMX_EINVAL, "NULL variable pointer at element " + std::to_string(i / 5));
+  if (int r = host_stage(c, &b)) return r;
+  HIPCHK(hipMemcpyAsync(b, d, c->nstate * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int e = 0; e < nelem; ++e) {
+    const Elm &E = c->mesh.e[e];
+    const long np = (long)E.n * E.n * E.n;
+    for (int v = 0; v < 5; ++v) std::memcpy(ptrs[5L * e + v], b + 5 * E.off + v * np, np * 8);
+  }
   return NMX_OK;
 }
 
@@ -1491,6 +1619,7 @@ static StageArgs base_args(nmx_ctx *c) {
   a.om = std::sqrt(a.k[0] * a.k[0] + a.k[1] * a.k[1] + a.k[2] * a.k[2]);
   a.profile = (a.k[0] != 0 || a.k[1] != 0 || a.k[2] != 0);
   a.numflux = (P(c, "scalarwave1_numflux") == "LLF") ? 1 : 0;
+  a.nt = Pi(c, "nmx_stream_hints") & 7;
   a.e0 = 0;
   a.e1 = (int)c->mesh.e.size();
   return a;
@@ -1504,7 +1633,7 @@ static AmrArgs amr_args(nmx_ctx *c, const StageArgs &sa) {
   a.dXdx = c->dXdx; a.xyz = c->dxyz; a.tdev = sa.tdev; a.tidx = sa.tidx;
   a.cA = sa.cA; a.cW = sa.cW;
   for (int d = 0; d < 3; ++d) a.k[d] = sa.k[d];
-  a.om = sa.om; a.wmode = sa.wmode; a.numflux = sa.numflux; a.profile = sa.profile;
+  a.om = sa.om; a.wmode = sa.wmode; a.numflux = sa.numflux; a.profile = sa.profile; a.nt = sa.nt;
   return a;
 }
 
@@ -1541,7 +1670,7 @@ static BatchArgs batch_args(nmx_ctx *c, const StageArgs &sa) {
   b.wend_inv = c->dbW; b.xlo = c->dbxlo;
   b.cA = sa.cA; b.cW = sa.cW;
   for (int d = 0; d < 3; ++d) b.k[d] = sa.k[d];
-  b.om = sa.om; b.wmode = sa.wmode; b.numflux = sa.numflux; b.profile = sa.profile;
+  b.om = sa.om; b.wmode = sa.wmode; b.numflux = sa.numflux; b.profile = sa.profile; b.nt = sa.nt;
   return b;
 }
 
@@ -2157,8 +2286,11 @@ extern "C" int nmx_evolve(nmx_ctx *c, int nsteps) {
       for (int s = 0; s < G && !r; ++s) r = issue_step(c, method, dt, 4 * s, (s & 1) != 0);
       hipGraph_t g = nullptr;
       hipError_t ec = hipStreamEndCapture(c->stream, &g);
-      if (r) return r;
-      if (ec != hipSuccess) return fail(NMX_EHIP, std::string("capture: ") + hipGetErrorString(ec));
+      if (r || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (r) return r;
+        return fail(NMX_EHIP, std::string("capture: ") + hipGetErrorString(ec));
+      }
       c->graph = g;
       HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
       c->graph_steps = G; c->graph_dt = dt; c->graph_method = method;
@@ -2349,8 +2481,11 @@ extern "C" int nmx_adjoint_evolve(nmx_ctx *c, int nsteps) {
       for (int s = 0; s < G && !r; ++s) r = issue_adj_step(c, method, dt, (s & 1) != 0);
       hipGraph_t g = nullptr;
       hipError_t ec = hipStreamEndCapture(c->stream, &g);
-      if (r) return r;
-      if (ec != hipSuccess) return fail(NMX_EHIP, std::string("capture: ") + hipGetErrorString(ec));
+      if (r || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (r) return r;
+        return fail(NMX_EHIP, std::string("capture: ") + hipGetErrorString(ec));
+      }
       c->agraph = g;
       HIPCHK(hipGraphInstantiate(&c->agexec, c->agraph, nullptr, nullptr, 0));
       c->agraph_steps = G; c->agraph_dt = dt; c->agraph_method = method;

@@ -280,7 +280,7 @@ __device__ __forceinline__ int elem_loop(const NMX_CONST BatchArgs &a, int it, i
 
     NodeOut o;
     o.win = a.win; o.U = a.U; o.Ain = a.Ain; o.Aout = a.Aout; o.Wout = a.Wout; o.R = a.R;
-    o.cA = a.cA; o.cW = a.cW; o.wend_inv = wend_inv; o.wmode = a.wmode; o.numflux = a.numflux;
+    o.cA = a.cA; o.cW = a.cW; o.wend_inv = wend_inv; o.wmode = a.wmode; o.numflux = a.numflux; o.nt = a.nt;
     ElmGeo g;
     g.dXbdX[0] = R.dXbdX[0]; g.dXbdX[1] = R.dXbdX[1]; g.dXbdX[2] = R.dXbdX[2];
     g.dXdx = a.dXdx + 9 * noff;

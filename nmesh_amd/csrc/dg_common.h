@@ -90,7 +90,19 @@ struct NodeOut {
   double *Aout, *Wout, *R;
   double cA, cW, wend_inv;
   int wmode, numflux;
+  // streaming hints for the registers each node touches once per stage:
+  // bit 0 loads of u_p and the accumulator base, bit 1 stores of the
+  // accumulator and the RHS, bit 2 stores of the next stage input
+  int nt = 0;
 };
+
+__device__ __forceinline__ double ld_stream(const double *p, bool nt) {
+  return nt ? __builtin_nontemporal_load(p) : *p;
+}
+__device__ __forceinline__ void st_stream(double *p, double v, bool nt) {
+  if (nt) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
 
 // Geometry of one element.
 struct ElmGeo {
@@ -212,32 +224,33 @@ __device__ __forceinline__ void node_update(const NodeOut &o, const ElmGeo &g,
   // r = -(div f + lift), r_phi = pi (scalarwave1.c:242-321) and the stage
   // combination (RungeKutta.c:19-188, variables.c:932-1041)
   const double r[5] = {rpi, rc[0], rc[1], rc[2], s_u[0][node]};
+  const bool ntl = o.nt & 1, nta = o.nt & 2, ntw = o.nt & 4;
   double Ubuf[5], Abuf[5];
   if (o.wmode) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) Ubuf[v] = Upre ? Upre[v] : o.U[gi + v * NP3];
+    for (int v = 0; v < 5; ++v) Ubuf[v] = Upre ? Upre[v] : ld_stream(o.U + gi + v * NP3, ntl);
   }
   if (o.Aout) {
     const double *bas = o.Ain ? o.Ain : o.win;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) Abuf[v] = Apre ? Apre[v] : bas[gi + v * NP3];
+    for (int v = 0; v < 5; ++v) Abuf[v] = Apre ? Apre[v] : ld_stream(bas + gi + v * NP3, ntl);
   }
   if (o.R) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) o.R[gi + v * NP3] = r[v];
+    for (int v = 0; v < 5; ++v) st_stream(o.R + gi + v * NP3, r[v], nta);
   }
   if (o.Aout) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) o.Aout[gi + v * NP3] = Abuf[v] + o.cA * r[v];
+    for (int v = 0; v < 5; ++v) st_stream(o.Aout + gi + v * NP3, Abuf[v] + o.cA * r[v], nta);
   }
   if (o.wmode == 1) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) o.Wout[gi + v * NP3] = Ubuf[v] + o.cW * r[v];
+    for (int v = 0; v < 5; ++v) st_stream(o.Wout + gi + v * NP3, Ubuf[v] + o.cW * r[v], ntw);
   } else if (o.wmode == 2) {
     const double wv[5] = {s_u[0][node], s_u[1][node], s_u[2][node], s_u[3][node],
                           o.win[gi + 4 * NP3]};
 #pragma unroll
-    for (int v = 0; v < 5; ++v) o.Wout[gi + v * NP3] = 0.75 * Ubuf[v] + 0.25 * (wv[v] + o.cW * r[v]);
+    for (int v = 0; v < 5; ++v) st_stream(o.Wout + gi + v * NP3, 0.75 * Ubuf[v] + 0.25 * (wv[v] + o.cW * r[v]), ntw);
   }
 }
 

@@ -113,7 +113,7 @@ __global__ __launch_bounds__(NT, NMX_DG_WAVES) void dg_stage_kernel(StageArgs a)
     // ---- phase 2: compute from LDS and registers, node by node ----
     NodeOut o;
     o.win = a.win; o.U = a.U; o.Ain = a.Ain; o.Aout = a.Aout; o.Wout = a.Wout; o.R = a.R;
-    o.cA = a.cA; o.cW = a.cW; o.wend_inv = a.wend_inv; o.wmode = a.wmode; o.numflux = a.numflux;
+    o.cA = a.cA; o.cW = a.cW; o.wend_inv = a.wend_inv; o.wmode = a.wmode; o.numflux = a.numflux; o.nt = a.nt;
     ElmGeo g;
     g.dXbdX[0] = dXbdX[0]; g.dXbdX[1] = dXbdX[1]; g.dXbdX[2] = dXbdX[2];
     g.dXdx = a.dXdx + 9L * NP3 * e;

@@ -315,8 +315,11 @@ extern "C" int nmx_fv_evolve(nmx_fv *f, int nsteps) {
       for (int s = 0; s < G && !r; ++s) r = fv_issue_step(f, method, dt, (s & 1) != 0);
       hipGraph_t g = nullptr;
       hipError_t ec = hipStreamEndCapture(f->stream, &g);
-      if (r) return r;
-      if (ec != hipSuccess) return fail(NMX_EHIP, std::string("capture: ") + hipGetErrorString(ec));
+      if (r || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (r) return r;
+        return fail(NMX_EHIP, std::string("capture: ") + hipGetErrorString(ec));
+      }
       f->graph = g;
       HIPCHK(hipGraphInstantiate(&f->gexec, f->graph, nullptr, nullptr, 0));
       f->graph_steps = G; f->graph_dt = dt; f->graph_method = method;

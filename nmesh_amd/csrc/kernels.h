@@ -34,6 +34,7 @@ struct StageArgs {
   double om = 0;
   int e0 = 0, e1 = 0;
   int wmode = 0, numflux = 0, profile = 0;
+  int nt = 0;                    // streaming hints of the epilogue (NodeOut::nt)
   int fv_mid = 0;                // FV elements: metric and normal at the midpoints
   // FV elements: fv_divf_extrap_s1, _s2, _opt (ex_on: s1 > 0 or s2 < DBL_MAX)
   double ex_s1 = 0, ex_s2 = 1.7976931348623157e308;

@@ -885,6 +885,82 @@ std::string build_shell(Mesh &M, int luni, int n, double rin, double rout, doubl
   return finish(M);
 }
 
+std::string import_mesh(Mesh &M, const std::vector<ImportPatch> &pat, const std::vector<ImportElm> &elm) {
+  Mesh N;
+  for (const ImportPatch &q : pat) {
+    Patch P;
+    if (q.type != PAT_BOX && q.type != PAT_CUBEDSHELL) return "patch type must be box (0) or CubedShell (1)";
+    P.type = q.type;
+    P.dom = q.dom;
+    P.s0 = q.s0;
+    P.s1 = q.s1;
+    for (int f = 0; f < 6; ++f) P.bbox[f] = q.bbox[f];
+    for (int d = 0; d < 3; ++d) P.xc[d] = q.xc[d];
+    if (P.type == PAT_CUBEDSHELL) {
+      if (q.dom < 0 || q.dom > 5) return "cubed-sphere dom must be 0..5";
+      if (!(q.s0 > 0 && q.s1 > q.s0)) return "cubed-sphere radii need 0 < s0 < s1";
+      const double bb[6] = {0., 1., -1., 1., -1., 1.};
+      for (int f = 0; f < 6; ++f)
+        if (q.bbox[f] != bb[f]) return "cubed-sphere patch box must be lam [0,1], A, B [-1,1]";
+    } else {
+      for (int d = 0; d < 3; ++d)
+        if (!(q.bbox[2 * d + 1] > q.bbox[2 * d])) return "empty patch box";
+    }
+    N.pat.push_back(P);
+  }
+  for (size_t i = 0; i < elm.size(); ++i) {
+    const ImportElm &q = elm[i];
+    if (q.p < 0 || q.p >= (int)N.pat.size()) return "element " + std::to_string(i) + ": no such patch";
+    if (q.l < 0 || q.l >= (int)sizeof(q.path)) return "element " + std::to_string(i) + ": level out of range";
+    if (q.n < 2 || q.n > kMaxN) return "element " + std::to_string(i) + ": n out of range";
+    Elm E;
+    E.p = q.p; E.l = q.l; E.n = q.n;
+    for (int d = 0; d < 3; ++d) E.ic[d] = 0;
+    for (int k = 0; k < q.l; ++k) {
+      if (q.path[k] > 7) return "element " + std::to_string(i) + ": octant digit above 7";
+      E.path[k] = q.path[k];
+      for (int d = 0; d < 3; ++d) E.ic[d] = 2 * E.ic[d] + ((q.path[k] >> d) & 1);
+    }
+    elm_bbox(N.pat[E.p], E);
+    for (int f = 0; f < 6; ++f)
+      if (E.bbox[f] != q.bbox[f]) return "element " + std::to_string(i) + ": box differs from the location's";
+    N.e.push_back(E);
+  }
+  // a forest in nmesh's order: patches in order, depth-first octants inside,
+  // each patch covered exactly once
+  for (size_t i = 1; i < N.e.size(); ++i) {
+    const Elm &A = N.e[i - 1], &B = N.e[i];
+    bool ok = A.p < B.p;
+    if (A.p == B.p) {
+      // B follows A in depth-first order: at the first level where they
+      // differ, B's octant is the larger
+      int k = 0;
+      while (k < A.l && k < B.l && A.path[k] == B.path[k]) ++k;
+      ok = k < A.l && k < B.l && B.path[k] > A.path[k];
+    }
+    if (!ok) return "elements " + std::to_string(i - 1) + " and " + std::to_string(i) + " are not in depth-first order";
+  }
+  // volumes in units of the deepest possible octant (8^-23 of a patch)
+  std::vector<unsigned __int128> vol(N.pat.size(), 0);
+  for (const Elm &E : N.e) vol[E.p] += (unsigned __int128)1 << (3 * (23 - E.l));
+  for (size_t p = 0; p < vol.size(); ++p)
+    if (vol[p] != ((unsigned __int128)1 << 69)) return "the elements of patch " + std::to_string(p) + " do not cover it";
+  std::string err = finish(N);
+  if (!err.empty()) return err;
+  for (size_t i = 0; i < N.e.size(); ++i)
+    for (int f = 0; f < 6; ++f) {
+      const Elm &E = N.e[i];
+      const ImportElm &q = elm[i];
+      bool same = E.nfnb[f] == q.nfnb[f];
+      for (int k = 0; same && k < E.nfnb[f]; ++k) same = E.fnb[f][k] == q.fnb[f][k] && E.nbf[f][k] == q.nbf[f][k];
+      if (!same)
+        return "element " + std::to_string(i) + " face " + std::to_string(f) +
+               ": neighbour list differs from the one built here";
+    }
+  M = std::move(N);
+  return "";
+}
+
 std::string set_elm_n(Mesh &M, const std::vector<int> &n) {
   if (n.size() != M.e.size()) return "need one n per element";
   for (size_t i = 0; i < n.size(); ++i) {

@@ -111,6 +111,26 @@ std::string set_fv_patches(Mesh &M, const std::vector<int> &patches);
 // neighbours; tables rebuilt
 std::string set_fv_surface_interp(Mesh &M, int mode, bool lagrange_if_dg);
 
+// A mesh handed over from nmesh's own tables (nmesh_amr.h:87-102): patches
+// (type, cubed-sphere dom, radii s0 s1, box, centre), elements in eid order
+// (patch, level, child octant per level, n), and the caller's boxes and face
+// neighbour lists.  The forest is rebuilt from the element locations and
+// every table the library derives (element boxes bit for bit, neighbour
+// counts, neighbours in nmesh's order, their faces) must equal the caller's;
+// the first difference is returned as the error.
+struct ImportPatch {
+  int type = PAT_BOX, dom = 0;
+  double s0 = 0, s1 = 0, bbox[6] = {0, 0, 0, 0, 0, 0}, xc[3] = {0, 0, 0};
+};
+struct ImportElm {
+  int p = 0, l = 0, n = 2;
+  uint8_t path[24] = {0};
+  double bbox[6] = {0, 0, 0, 0, 0, 0};
+  int nfnb[6] = {0, 0, 0, 0, 0, 0};
+  int fnb[6][kMaxNb], nbf[6][kMaxNb];
+};
+std::string import_mesh(Mesh &M, const std::vector<ImportPatch> &pat, const std::vector<ImportElm> &elm);
+
 // dt = min over elements of dtfac hmin, or uniform_dtfac hmin for FV
 // elements whose hmin lies between two (uniform) points
 double mesh_dt(const Mesh &M, double dtfac, double uniform_dtfac);

@@ -176,6 +176,33 @@ class Mesh:
         check(lib().nmx_state_download_elm(self._ctx, int(reg), int(e), int(vi), _p(d), np_))
         return d
 
+    def set_Vard(self, e, vi, d, reg=REG_U):
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        check(lib().nmx_state_upload_elm(self._ctx, int(reg), int(e), int(vi), _p(d), int(d.size)))
+
+    def upload_vars(self, arrays, reg=REG_U):
+        """every variable of every element from per-element arrays
+        (arrays[e][v], as nmesh's Vard holds them) in one transfer"""
+        ptrs = (C.c_void_p * (NVARS * self.nelem))()
+        keep = []
+        for e in range(self.nelem):
+            for v in range(NVARS):
+                a = np.ascontiguousarray(arrays[e][v], dtype=np.float64)
+                keep.append(a)
+                ptrs[NVARS * e + v] = a.ctypes.data
+        check(lib().nmx_state_upload_ptrs(self._ctx, int(reg), ptrs, self.nelem, NVARS))
+
+    def download_vars(self, arrays, reg=REG_U):
+        """the reverse of upload_vars, into the given float64 arrays"""
+        ptrs = (C.c_void_p * (NVARS * self.nelem))()
+        for e in range(self.nelem):
+            for v in range(NVARS):
+                a = arrays[e][v]
+                if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+                    raise ValueError("download_vars needs contiguous float64 arrays")
+                ptrs[NVARS * e + v] = a.ctypes.data
+        check(lib().nmx_state_download_ptrs(self._ctx, int(reg), ptrs, self.nelem, NVARS))
+
     # device-resident state (flat [var][node] at a device pointer), ordered
     # on the context's stream
     def set_stream(self, stream_handle):
@@ -216,6 +243,43 @@ class Mesh:
 def amr_setup_mesh(mesh):
     """Build patches, elements, tables, geometry and dt (mesh.c:181-319)."""
     check(lib().nmx_mesh_build(mesh.ctx))
+    return 0
+
+
+def amr_import_mesh(mesh, patches, elements):
+    """Take a mesh nmesh already holds instead of building one
+    (nmx_mesh_import).  patches: dicts with type (0 box, 1 CubedShell), dom,
+    s0, s1, bbox (6), xc (3); elements in eid order: dicts with p, loc (the
+    octant digits, a string like nmesh's eloc->loc), n, bbox (6), fnb (6 lists
+    of neighbour indices) and nbface (6 lists of their faces)."""
+    npat, ne = len(patches), len(elements)
+    ptype = np.array([q["type"] for q in patches], np.int32)
+    pgeo = np.zeros((npat, 12))
+    for i, q in enumerate(patches):
+        pgeo[i, 0] = q.get("dom", 0)
+        pgeo[i, 1] = q.get("s0", 0.0)
+        pgeo[i, 2] = q.get("s1", 0.0)
+        pgeo[i, 3:9] = q["bbox"]
+        pgeo[i, 9:12] = q.get("xc", (0.0, 0.0, 0.0))
+    epat = np.array([q["p"] for q in elements], np.int32)
+    elev = np.array([len(q["loc"]) for q in elements], np.int32)
+    eloc = np.zeros((ne, 24), np.uint8)
+    en = np.zeros((ne, 3), np.int32)
+    ebb = np.zeros((ne, 6))
+    nfnb = np.zeros((ne, 6), np.int32)
+    fnb = np.zeros((ne, 6, 8), np.int32)
+    nbf = np.zeros((ne, 6, 8), np.int32)
+    for e, q in enumerate(elements):
+        for k, ch in enumerate(q["loc"]):
+            eloc[e, k] = ord(ch)
+        en[e] = q["n"]
+        ebb[e] = q["bbox"]
+        for f in range(6):
+            nfnb[e, f] = len(q["fnb"][f])
+            fnb[e, f, :nfnb[e, f]] = q["fnb"][f]
+            nbf[e, f, :nfnb[e, f]] = q["nbface"][f]
+    check(lib().nmx_mesh_import(mesh.ctx, npat, _p(ptype), _p(pgeo), ne, _p(epat), _p(elev), _p(eloc),
+                                _p(en), _p(ebb), _p(nfnb), _p(fnb), _p(nbf)))
     return 0
 
 

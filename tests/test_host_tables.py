@@ -69,3 +69,23 @@ def test_mixed_order_tables_match():
         assert np.array_equal(p.elm_info(e), o.elm_info(e))
     _, x = o.coords()
     assert np.array_equal(p.coords(), x)
+
+
+def test_explicit_dt_survives_mesh_changes():
+    """nmx_set_dt records an explicit dt: a p-change, an h-refinement or a
+    new dtfac keep it; dt = auto brings back the rule (storage.c:1305-1386)."""
+    p = product("shell", 1, 4)
+    auto = p.dt
+    p.dt = 1.25e-3
+    p.padapt(np.full(p.nelem, 5, np.int32))
+    assert p.dt == 1.25e-3
+    flags = np.zeros(p.nelem, np.int32)
+    flags[3] = 1
+    p.hadapt(flags)
+    p.set_par("dtfac", 0.1)
+    assert p.dt == 1.25e-3 and p.par("dt") != "auto"
+    p.set_par("dt", "auto")
+    o = OMesh("shell", 1, 4)
+    o, _ = o.adapted(np.zeros(5 * o.nnodes), n=np.full(o.nelem, 5, np.int32))
+    o, _ = o.adapted(np.zeros(5 * o.nnodes), hflag=flags)
+    assert p.dt == o.dt(dtfac=0.1) and p.dt != auto

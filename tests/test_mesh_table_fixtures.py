@@ -80,3 +80,60 @@ def test_fixture_covers_the_cases():
     assert max(cross) == 4
     assert any(m == 2 for e in FIX["shell_luni1"]["elms"] for m in e["ajmode"])
     assert max(len(e["faces"][f][0]) for e in FIX["box_two_levels"]["elms"] for f in range(6)) >= 4
+
+
+def fixture_import(case, n=N, edit=None):
+    """nmx_mesh_import of a fixture mesh into a host-only context"""
+    if case["kind"] == "box":
+        d = case["dout"]
+        pats = [{"type": 0, "bbox": [-d, d, -d, d, -d, d]}]
+    else:
+        pats = [{"type": 1, "dom": p, "s0": case["rin"], "s1": case["rout"],
+                 "bbox": [0.0, 1.0, -1.0, 1.0, -1.0, 1.0]} for p in range(6)]
+    elms = [{"p": e["p"], "loc": e["loc"], "n": n, "bbox": list(e["bbox"]),
+             "fnb": [list(f[0]) for f in e["faces"]], "nbface": [list(f[1]) for f in e["faces"]]}
+            for e in case["elms"]]
+    if edit:
+        edit(elms)
+    m = nm.Mesh(device=-1)
+    nm.amr_import_mesh(m, pats, elms)
+    return m
+
+
+@pytest.mark.parametrize("name", sorted(FIX))
+def test_import_of_nmesh_tables_equals_build(name):
+    """nmx_mesh_import takes nmesh's tables (here the fixtures) and the
+    result equals the built mesh: boxes, neighbours, modes, coordinates, dt"""
+    case = FIX[name]
+    m = fixture_import(case)
+    check(m, case)
+    b = product_mesh(case)
+    assert m.nelem == b.nelem and m.nnodes == b.nnodes
+    for e in range(m.nelem):
+        assert np.array_equal(m.elm_info(e), b.elm_info(e))
+    assert np.array_equal(m.coords(), b.coords())
+    assert m.dt == b.dt
+
+
+def test_import_refuses_tables_that_differ():
+    case = FIX["shell_luni1_refined"]
+    big = [i for i, e in enumerate(case["elms"]) if any(len(f[0]) > 1 for f in e["faces"])][0]
+    f = [k for k in range(6) if len(case["elms"][big]["faces"][k][0]) > 1][0]
+
+    def swap(elms):        # two neighbours of one face in the other order
+        elms[big]["fnb"][f] = elms[big]["fnb"][f][::-1]
+        elms[big]["nbface"][f] = elms[big]["nbface"][f][::-1]
+
+    def face(elms):        # a wrong neighbour face
+        elms[0]["nbface"][1] = [(elms[0]["nbface"][1][0] + 1) % 6]
+
+    def box(elms):         # a box one ulp off
+        elms[5]["bbox"][1] = np.nextafter(elms[5]["bbox"][1], 9.0)
+
+    def order(elms):       # two elements swapped in the list
+        elms[2], elms[3] = elms[3], elms[2]
+
+    for edit, words in ((swap, "neighbour list differs"), (face, "neighbour list differs"),
+                        (box, "box differs"), (order, "depth-first")):
+        with pytest.raises(nm.NmxError, match=words):
+            fixture_import(case, edit=edit)
