@@ -152,10 +152,13 @@ int nmx_mesh_padapt(nmx_ctx *ctx, const int *n, int nelem);
  * (INTERP_UNIFORM_TO_n_LGL, UNI_to_nLGLt with basis_BackInterpMatrix =
  * PseudoInv; gridpoints.c:158-290).  The mechanism of
  * evolve_switch_troubled_nodes_mesh / evolve_switch_nontroubled_nodes_mesh
- * (src/main/evolve/trouble.c:186-330); which elements switch is the
- * caller's choice (scalarwave1 registers no TROUBLE function).  Meshes
- * mixing both kinds run on the batched path. */
+ * (src/main/evolve/trouble.c:186-330), called by hand here; with
+ * nmx_trouble set, nmx_evolve switches by the trouble scores itself.
+ * Meshes mixing both kinds run on the batched path. */
 int nmx_mesh_switch_fv(nmx_ctx *ctx, const int *to_fv, int nelem);
+/* fv[e] = 1 for an FV element on uniform points, 0 for DG on LGL points
+ * (node->dat->use_fv, refine_set_use_fv_true_iff_P_UNIFORM, trouble.c:246) */
+int nmx_mesh_fv_flags(nmx_ctx *ctx, int *fv, int nelem);
 
 int nmx_mesh_nelem(nmx_ctx *ctx, int *nelem);
 int nmx_mesh_nnodes(nmx_ctx *ctx, long *nnodes);
@@ -202,6 +205,29 @@ int nmx_state_set_profile(nmx_ctx *ctx, int reg, double t);
  * before the first nmx_evolve when nmx_autotune = yes.  No counterpart in the
  * reference (its element loop is an OpenMP for, nmesh_amr_loops.h:61-63). */
 int nmx_autotune(nmx_ctx *ctx);
+
+/* Troubled elements (trouble.c:27-400; RungeKutta.c:19-344; evolve.c:
+ * 25-176).  With nmx_trouble = RDMP or hook, nmx_evolve scores every element
+ * after each substep (evolve_redo_troubled = substep, the default) or after
+ * the step (= step): a TROUBLE value 1 / 0 / -1 per element moves its
+ * trouble score (evolve_set_trouble_score); when the mesh's score is
+ * positive the troubled elements become FV elements on uniform points
+ * (same n) and redo their substep (evolve_trouble_redo_u_step_mesh), or the
+ * whole step is redone from u_p (evolve_prepare_do_over_mesh); after the
+ * step, elements whose score reached -11 per substep go back to DG
+ * (evolve_switch_nontroubled_nodes_mesh).  RDMP is the library's indicator:
+ * a value of u outside the u_p extremes of the element and its face
+ * neighbours widened by max(nmx_trouble_delta0, nmx_trouble_eps (M - m))
+ * makes a DG element troubled, an FV element in bounds is good.
+ * nmx_trouble_fn is a project's TROUBLE function on the device
+ * (evosys->f[TROUBLE]): for every element it writes tr[e] in {1, 0, -1}
+ * from u (after the substep) and u_p (the step's start), registers in
+ * element blocks (nmx_state_device_ptr), ordered on stream (a hipStream_t);
+ * it returns 0 on success.  nmx_get_trouble_scores reads the scores
+ * (node->dat->info->trbl_score) and counts of redone steps and switches. */
+typedef int (*nmx_trouble_fn)(void *user, const double *u, const double *u_p, int *tr, int nelem, void *stream);
+int nmx_set_trouble_hook(nmx_ctx *ctx, nmx_trouble_fn fn, void *user);
+int nmx_get_trouble_scores(nmx_ctx *ctx, int *score, int nelem, long *redos, long *switches);
 
 /* r = RHS(u, t) on the device (evolve_setrhs_mesh, evolve.c:182-232) */
 int nmx_setrhs(nmx_ctx *ctx, int reg_out, int reg_in, double t);

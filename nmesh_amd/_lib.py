@@ -34,6 +34,7 @@ def _proto(lib):
         "nmx_mesh_hadapt": (i, [vp, vp, i]),
         "nmx_mesh_padapt": (i, [vp, vp, i]),
         "nmx_mesh_switch_fv": (i, [vp, vp, i]),
+        "nmx_mesh_fv_flags": (i, [vp, vp, i]),
         "nmx_mesh_nelem": (i, [vp, P(i)]),
         "nmx_mesh_nnodes": (i, [vp, P(l)]),
         "nmx_mesh_elm_info": (i, [vp, i, vp]),
@@ -56,6 +57,8 @@ def _proto(lib):
         "nmx_state_set_profile": (i, [vp, i, d]),
         "nmx_setrhs": (i, [vp, i, i, d]),
         "nmx_autotune": (i, [vp]),
+        "nmx_set_trouble_hook": (i, [vp, vp, vp]),
+        "nmx_get_trouble_scores": (i, [vp, vp, i, P(l), P(l)]),
         "nmx_evolve": (i, [vp, i]),
         "nmx_filter": (i, [vp, i]),
         "nmx_state_device_ptr": (i, [vp, i, P(vp), P(l)]),

@@ -91,6 +91,14 @@ struct nmx_ctx {
   int ncu = 0;
   int bgrid = 0;          // workgroups of the persistent launch
   bool anydg = false;     // the batched mesh has DG elements
+  // troubled elements: TROUBLE values, scores, the mesh score (device),
+  // the scores kept across a re-mirror, the per-element FV flags, a
+  // caller's TROUBLE hook
+  int *dtr = nullptr, *dscore = nullptr, *dtrout = nullptr, *defv = nullptr;
+  std::vector<int> hscore;
+  nmx_trouble_fn trouble_hook = nullptr;
+  void *trouble_user = nullptr;
+  long trouble_redos = 0, trouble_switches = 0;
   bool batch_fixed = false;   // nmx_batch set by the caller (the autotuner leaves it)
   // div: nodes per thread of this bucket's tile (0: the nmx_block parameter)
   // f1first/f1count: the face interpolation jobs of its elements in djobs1
@@ -233,6 +241,10 @@ static void set_defaults(nmx_ctx *c) {
       {"nmx_concurrent", "yes"},
       {"nmx_batch", "legacy"},
       {"nmx_stream_hints", "0"},
+      {"nmx_trouble", "none"},
+      {"nmx_trouble_delta0", "1e-4"},
+      {"nmx_trouble_eps", "1e-3"},
+      {"evolve_redo_troubled", "substep"},
   };
 }
 
@@ -335,7 +347,8 @@ static void free_mesh_dev(nmx_ctx *c) {
     void **ptrs[] = {(void **)&c->djobs1, (void **)&c->dmdesc, (void **)&c->dFilt, (void **)&c->dfrec,
                      (void **)&c->dBtab, (void **)&c->dDall, (void **)&c->dxball, (void **)&c->daj,
                      (void **)&c->dbrec, (void **)&c->drjobs, (void **)&c->dbD, (void **)&c->dbxb,
-                     (void **)&c->dbW, (void **)&c->dbxlo, (void **)&c->dbrec2, (void **)&c->dbnr};
+                     (void **)&c->dbW, (void **)&c->dbxlo, (void **)&c->dbrec2, (void **)&c->dbnr,
+                     (void **)&c->dtr, (void **)&c->dscore, (void **)&c->dtrout, (void **)&c->defv};
     for (auto q : ptrs)
       if (*q) { (void)hipFree(*q); *q = nullptr; }
     for (auto &b : c->buckets)
@@ -430,6 +443,10 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
     return fail(NMX_EINVAL, "scalarwave1_numflux must be upwind or LLF");
   if (k == "scalarwave1_profile" && v != "sin")
     return fail(NMX_EUNSUP, "only scalarwave1_profile = sin is supported");
+  if (k == "nmx_trouble" && v != "none" && v != "RDMP" && v != "hook")
+    return fail(NMX_EINVAL, "nmx_trouble must be none, RDMP or hook");
+  if (k == "evolve_redo_troubled" && v != "substep" && v != "step")
+    return fail(NMX_EINVAL, "evolve_redo_troubled must be step or substep");
   if (k == "nmx_batch" && v != "persistent" && v != "buckets" && v != "legacy")
     return fail(NMX_EINVAL, "nmx_batch must be persistent, buckets or legacy");
   if (k == "nmx_path" && v != "plain" && v != "perelement" && v != "batched")
@@ -1189,7 +1206,7 @@ static void lagrange_rows(int ns, const std::vector<double> &x, std::vector<doub
 }
 
 // jobs moving the state of Mold (register uold) onto c->mesh, then the launch
-static int adapt_state(nmx_ctx *c, const Mesh &Mold, const double *uold) {
+static int adapt_state(nmx_ctx *c, const Mesh &Mold, const double *uold, double *unew = nullptr) {
   const Mesh &M = c->mesh;
   std::map<std::string, int> where;
   for (size_t e = 0; e < Mold.e.size(); ++e) where[elm_name(Mold.e[e])] = (int)e;
@@ -1275,7 +1292,7 @@ static int adapt_state(nmx_ctx *c, const Mesh &Mold, const double *uold) {
     return r;
   }
   AdaptArgs a;
-  a.uold = uold; a.unew = c->dU; a.jobs = djobs; a.jfirst = djf; a.atab = dat;
+  a.uold = uold; a.unew = unew ? unew : c->dU; a.jobs = djobs; a.jfirst = djf; a.atab = dat;
   a.nelem = (int)M.e.size();
   hipError_t err = launch_adapt(a, c->stream);
   if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
@@ -1393,6 +1410,14 @@ extern "C" int nmx_mesh_elm_info(nmx_ctx *c, int e, long *info) {
     info[10 + f] = E.ajmode[f];
     info[16 + f] = E.orient[f];
   }
+  return NMX_OK;
+}
+
+extern "C" int nmx_mesh_fv_flags(nmx_ctx *c, int *fv, int nelem) {
+  NEED_MESH(c);
+  if (!fv) return fail(NMX_EINVAL, "NULL argument");
+  if (nelem != (int)c->mesh.e.size()) return fail(NMX_EINVAL, "need one flag per element");
+  for (int e = 0; e < nelem; ++e) fv[e] = c->mesh.e[e].fv != 0;
   return NMX_OK;
 }
 
@@ -2023,6 +2048,8 @@ extern "C" int nmx_checkpoint_load_vars(nmx_ctx *c, int reg, const char *path) {
 // ---------------------------------------------------------------------------
 
 // limiter on or off from the parameters; its tables on first use
+static int nbr_tables(nmx_ctx *c);
+
 static int limiter_prepare(nmx_ctx *c) {
   const std::string &lim = P(c, "scalarwave1_limiter");
   c->lim_on = lim == "MRS" || lim == "minmodB";
@@ -2032,6 +2059,12 @@ static int limiter_prepare(nmx_ctx *c) {
   c->lim_alpha = Pd(c, "limiter_alpha");
   c->lim_beta = Pd(c, "limiter_beta");
   c->lim_scale = P(c, "limiter_scaleBound") == "yes";
+  return nbr_tables(c);
+}
+
+// face-neighbour lists, h^1.5, quadrature and Legendre tables, element data
+// (the limiters' and the trouble indicator's), made once per mirror
+static int nbr_tables(nmx_ctx *c) {
   if (c->dnbr) return NMX_OK;
   const Mesh &M = c->mesh;
   const size_t ne = M.e.size();
@@ -2058,6 +2091,319 @@ static int limiter_prepare(nmx_ctx *c) {
       (r = dev_upload(&c->dwqall, wq)) || (r = dev_upload(&c->damat, am)) || (r = dev_upload(&c->dsmat, sm)) ||
       (r = dev_alloc(&c->dldat, 20 * ne)))
     return r;
+  return NMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// troubled elements: scores, switches and redone substeps (trouble.c,
+// RungeKutta.c:19-188, evolve.c:25-176)
+// ---------------------------------------------------------------------------
+
+static bool trouble_on(nmx_ctx *c) { return P(c, "nmx_trouble") != "none"; }
+
+// device arrays of the trouble machinery for this mirror; the scores come
+// back from the host copy kept across re-mirrors
+static int trouble_prepare(nmx_ctx *c) {
+  if (int r = nbr_tables(c)) return r;
+  const size_t ne = c->mesh.e.size();
+  if (c->hscore.size() != ne) c->hscore.assign(ne, 0);
+  if (c->dtr) return NMX_OK;
+  std::vector<int> fv(ne);
+  for (size_t e = 0; e < ne; ++e) fv[e] = c->mesh.e[e].fv != 0;
+  int r;
+  if ((r = dev_alloc(&c->dtr, ne)) || (r = dev_upload(&c->dscore, c->hscore)) || (r = dev_alloc(&c->dtrout, 1)) ||
+      (r = dev_upload(&c->defv, fv)))
+    return r;
+  return NMX_OK;
+}
+
+// evolve_set_trouble_score_mesh (trouble.c:112-127) on state u against u_p
+// (the TROUBLE values from the library's indicator or the caller's hook),
+// or evolve_read_trouble_score_mesh when update is false: the mesh's score
+static int trouble_score(nmx_ctx *c, const double *u, const double *up, bool update, int *score) {
+  if (int r = trouble_prepare(c)) return r;
+  const int ne = (int)c->mesh.e.size();
+  if (update) {
+    if (c->trouble_hook) {
+      if (P(c, "nmx_trouble") != "hook") return fail(NMX_EINVAL, "a TROUBLE hook is set: nmx_trouble must be hook");
+      if (int r = c->trouble_hook(c->trouble_user, u, up, c->dtr, ne, (void *)c->stream))
+        return fail(NMX_EINVAL, "the TROUBLE hook returned " + std::to_string(r));
+    } else {
+      if (P(c, "nmx_trouble") != "RDMP") return fail(NMX_ESTATE, "nmx_trouble = hook needs nmx_set_trouble_hook");
+      LimArgs la;
+      la.u = const_cast<double *>(up); la.eoff = c->deoff; la.en = c->den; la.wq = c->dwqall;
+      la.ldat = c->dldat; la.nelem = ne;
+      hipError_t err = launch_lim_data(la, c->stream);
+      if (err != hipSuccess) return fail(NMX_EHIP, std::string("trouble data: ") + hipGetErrorString(err));
+      TroubleArgs ta;
+      ta.u = u; ta.ldat = c->dldat; ta.eoff = c->deoff; ta.en = c->den; ta.efv = c->defv; ta.nbr = c->dnbr;
+      ta.tr = c->dtr; ta.nelem = ne;
+      ta.delta0 = Pd(c, "nmx_trouble_delta0");
+      ta.eps = Pd(c, "nmx_trouble_eps");
+      err = launch_rdmp(ta, c->stream);
+      if (err != hipSuccess) return fail(NMX_EHIP, std::string("trouble indicator: ") + hipGetErrorString(err));
+    }
+  }
+  hipError_t err = launch_trouble_score(update ? c->dtr : nullptr, c->dscore, ne, c->dtrout, c->stream);
+  if (err != hipSuccess) return fail(NMX_EHIP, std::string("trouble score: ") + hipGetErrorString(err));
+  HIPCHK(hipMemcpyAsync(score, c->dtrout, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hscore.data(), c->dscore, ne * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return NMX_OK;
+}
+
+// registers a switch carries, besides U (which the re-mirror adapts)
+enum { CARRY_A = 1, CARRY_W0 = 2, CARRY_W1 = 4 };
+
+// elements to FV on uniform points (to_fv 1) or back to DG on LGL points,
+// at the same n (PARENT_n_P_UNIFORM / PARENT_n_P_LGL; trouble.c:186-330,
+// RungeKutta.c:239-312): the tables are rebuilt, U and the carried registers
+// interpolated, the scores kept
+static int trouble_switch(nmx_ctx *c, const std::vector<int> &to_fv, int carry) {
+  const Mesh old = c->mesh;
+  std::string err = set_fv_elems(c->mesh, to_fv);
+  if (!err.empty()) { c->mesh = old; return fail(NMX_EINVAL, err); }
+  struct Held { int flag; double *tmp; };
+  std::vector<Held> held;
+  for (int flag : {CARRY_A, CARRY_W0, CARRY_W1}) {
+    if (!(carry & flag)) continue;
+    double *src = flag == CARRY_A ? c->dA : flag == CARRY_W0 ? c->dW0 : c->dW1;
+    double *tmp = nullptr;
+    if (int r = dev_alloc(&tmp, c->nstate)) return r;
+    HIPCHK(hipMemcpyAsync(tmp, src, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream));
+    held.push_back({flag, tmp});
+  }
+  const std::vector<int> keep = c->hscore;
+  int r = remirror_with_state(c, old);
+  for (const Held &h : held) {
+    double *dst = h.flag == CARRY_A ? c->dA : h.flag == CARRY_W0 ? c->dW0 : c->dW1;
+    if (!r) r = adapt_state(c, old, h.tmp, dst);
+    (void)hipFree(h.tmp);
+  }
+  if (r) return r;
+  // the step's stage times (the mirror's copy is new)
+  HIPCHK(hipMemcpyAsync(c->dT, c->hT, 4 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  c->hscore = keep;
+  c->tuned = true;                // mid-run: keep the launch shapes chosen so far
+  c->trouble_switches++;
+  return trouble_prepare(c);
+}
+
+static int trouble_fix(nmx_ctx *c, double *y, const double *x, const double *x2, const double *r, double sfac,
+                       int mode, const int *mask = nullptr) {
+  TroubleFix f;
+  f.y = y; f.x = x; f.x2 = x2; f.r = r; f.s = sfac; f.mode = mode;
+  f.score = mask ? mask : c->dscore; f.eoff = c->deoff; f.en = c->den; f.nelem = (int)c->mesh.e.size();
+  hipError_t err = launch_trouble_fix(f, c->stream);
+  if (err != hipSuccess) return fail(NMX_EHIP, std::string("trouble fix: ") + hipGetErrorString(err));
+  return NMX_OK;
+}
+
+static int launch(nmx_ctx *c, StageArgs a);
+static void stage_times(int method, double t, double dt, double *ts);
+
+// one stage of evolve_RK4_mesh / evolve_sspRK3_mesh / evolve_Euler_mesh
+// (RungeKutta.c:19-188) with u_p in U, u in A, r kept in R
+struct TStage { int win, wout, wmode; double cA, cW; };   // win/wout: 0 U, 1 W0, 2 W1, -1 none
+static void trouble_stages(int method, double dt, std::vector<TStage> &st) {
+  if (method == 0)
+    st = {{0, 1, 1, dt / 6., dt / 2.}, {1, 2, 1, dt / 3., dt / 2.}, {2, 1, 1, dt / 3., dt}, {1, -1, 0, dt / 6., 0.}};
+  else if (method == 1)
+    st = {{0, 1, 1, dt / 6., dt}, {1, 2, 2, dt / 6., dt}, {2, -1, 0, dt * 2. / 3., 0.}};
+  else
+    st = {{0, -1, 0, dt, 0.}};
+}
+
+static double *treg(nmx_ctx *c, int which) { return which == 0 ? c->dU : which == 1 ? c->dW0 : which == 2 ? c->dW1 : nullptr; }
+
+// classes of the redone elements: evolve_trouble_redo_u_step_mesh redoes
+// one element at a time, its neighbours as they were before any switch.  A
+// face that is a copy face once both sides are FV (same n, level and
+// points) hands over the same values either way (the polynomial at the
+// uniform face points), so only two redone elements that share another
+// face, at least one of them newly switched, go into different classes.
+// Greedy colouring in element order; the class of element e in cls[e] (-1:
+// not redone), the count returned.
+static int redo_classes(const Mesh &M0, const Mesh &Mall, const std::vector<int> &score, std::vector<int> &cls) {
+  const size_t ne = M0.e.size();
+  cls.assign(ne, -1);
+  int ncls = 0;
+  for (size_t e = 0; e < ne; ++e) {
+    if (score[e] <= 0) continue;
+    uint64_t used = 0;
+    for (int f = 0; f < 6; ++f)
+      for (int ni = 0; ni < M0.e[e].nfnb[f]; ++ni) {
+        const int nb = M0.e[e].fnb[f][ni];
+        if (cls[nb] < 0 || (M0.e[e].fv && M0.e[nb].fv)) continue;
+        const int nf = M0.e[e].nbf[f][ni];
+        const bool copy = (Mall.e[e].ajmode[f] == 1 || Mall.e[e].ajmode[f] == 2) &&
+                          (Mall.e[nb].ajmode[nf] == 1 || Mall.e[nb].ajmode[nf] == 2);
+        if (!copy) used |= 1ull << cls[nb];
+      }
+    int k = 0;
+    while (used >> k & 1) ++k;
+    if (k >= 64) return -1;
+    cls[e] = k;
+    ncls = std::max(ncls, k + 1);
+  }
+  return ncls;
+}
+
+// a new device mirror of c->mesh in the middle of a step: tables, stage
+// times, scores; the launch shapes chosen so far kept
+static int mirror_again(nmx_ctx *c) {
+  if (int r = upload_mesh(c)) return r;
+  HIPCHK(hipMemcpyAsync(c->dT, c->hT, 4 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  c->tuned = true;
+  return trouble_prepare(c);
+}
+
+// the redo of stage S (evolve_trouble_redo_u_step_mesh, RungeKutta.c:
+// 223-330) after u -= r rfac: the troubled elements become FV (u, u_p and w
+// interpolated), then R of each redone element is the RHS of w with that
+// element switched and its neighbours as before.  With one class that is
+// the RHS on the switched mesh; with more, each class is switched on its
+// own in turn and its R kept.
+static int redo_rhs(nmx_ctx *c, const TStage &S, int s) {
+  const Mesh M0 = c->mesh;
+  const size_t ne = M0.e.size();
+  const std::vector<int> score = c->hscore;
+  std::vector<int> fv(ne);
+  for (size_t e = 0; e < ne; ++e) fv[e] = M0.e[e].fv || score[e] > 0;
+  Mesh Mall = M0;
+  std::string err = set_fv_elems(Mall, fv);
+  if (!err.empty()) return fail(NMX_EINVAL, err);
+  std::vector<int> cls;
+  const int ncls = redo_classes(M0, Mall, score, cls);
+  if (ncls < 0) return fail(NMX_EUNSUP, "more than 64 classes of redone elements");
+  const long ns = c->nstate;
+  double *win0 = nullptr;
+  if (ncls > 1) {
+    if (int r = dev_alloc(&win0, ns)) return r;
+    HIPCHK(hipMemcpyAsync(win0, treg(c, S.win), ns * 8, hipMemcpyDeviceToDevice, c->stream));
+  }
+  const int carry = CARRY_A | (S.win == 1 || S.wout == 1 ? CARRY_W0 : 0) | (S.win == 2 || S.wout == 2 ? CARRY_W1 : 0);
+  if (int r = trouble_switch(c, fv, carry)) { (void)hipFree(win0); return r; }
+  if (ncls <= 1) {
+    StageArgs b = base_args(c);
+    b.win = treg(c, S.win); b.R = c->dR; b.tidx = s;
+    return launch(c, b);
+  }
+  // registers of the switched mesh held across the class mirrors
+  double *held[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // U A W0 W1 R
+  int *dmask = nullptr;
+  int r = NMX_OK;
+  for (int q = 0; q < 5 && !r; ++q) r = dev_alloc(&held[q], ns);
+  if (!r) {
+    double *src[4] = {c->dU, c->dA, c->dW0, c->dW1};
+    for (int q = 0; q < 4; ++q)
+      if (hipMemcpyAsync(held[q], src[q], ns * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) r = fail(NMX_EHIP, "redo: hold registers");
+    if (hipMemsetAsync(held[4], 0, ns * 8, c->stream) != hipSuccess) r = fail(NMX_EHIP, "redo: clear R");
+  }
+  const double *wall = S.win == 0 ? held[0] : S.win == 1 ? held[2] : held[3];
+  for (int k = 0; k < ncls && !r; ++k) {
+    std::vector<int> fk(ne), mk(ne);
+    for (size_t e = 0; e < ne; ++e) { mk[e] = cls[e] == k; fk[e] = M0.e[e].fv || mk[e]; }
+    c->mesh = M0;
+    err = set_fv_elems(c->mesh, fk);
+    if (!err.empty()) { r = fail(NMX_EINVAL, err); break; }
+    if ((r = mirror_again(c))) break;
+    (void)hipFree(dmask); dmask = nullptr;
+    if ((r = dev_upload(&dmask, mk))) break;
+    // w: as before the switch, this class's elements interpolated
+    if (hipMemcpyAsync(c->dW0, win0, ns * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) { r = fail(NMX_EHIP, "redo: w"); break; }
+    if ((r = trouble_fix(c, c->dW0, wall, nullptr, nullptr, 0., 1, dmask))) break;
+    StageArgs b = base_args(c);
+    b.win = c->dW0; b.R = c->dR; b.tidx = s;
+    if ((r = launch(c, b))) break;
+    r = trouble_fix(c, held[4], c->dR, nullptr, nullptr, 0., 1, dmask);
+  }
+  if (!r) {
+    c->mesh = Mall;
+    r = mirror_again(c);
+  }
+  if (!r) {
+    double *dst[5] = {c->dU, c->dA, c->dW0, c->dW1, c->dR};
+    for (int q = 0; q < 5; ++q)
+      if (hipMemcpyAsync(dst[q], held[q], ns * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) r = fail(NMX_EHIP, "redo: restore registers");
+    if (hipStreamSynchronize(c->stream) != hipSuccess) r = fail(NMX_EHIP, "redo: sync");
+  }
+  for (double *h : held) (void)hipFree(h);
+  (void)hipFree(dmask);
+  (void)hipFree(win0);
+  return r;
+}
+
+// Evolve_mesh with evolve_redo_troubled = substep when redo: after each
+// stage the scores of u; if the mesh's score is positive, each troubled
+// element takes its substep back, becomes FV (u, u_p and w interpolated),
+// and redoes it with the RHS of w (evolve_trouble_redo_u_step_mesh,
+// RungeKutta.c:223-330).  The step's times are ts.
+static int trouble_evolve_mesh(nmx_ctx *c, int method, double dt, bool redo, int *nredo) {
+  std::vector<TStage> st;
+  trouble_stages(method, dt, st);
+  for (size_t s = 0; s < st.size(); ++s) {
+    const TStage &S = st[s];
+    StageArgs a = base_args(c);
+    a.win = treg(c, S.win); a.U = c->dU; a.Ain = s ? c->dA : nullptr; a.Aout = c->dA;
+    a.Wout = treg(c, S.wout); a.wmode = S.wmode; a.cA = S.cA; a.cW = S.cW; a.R = c->dR; a.tidx = (int)s;
+    if (int r = launch(c, a)) return r;
+    if (!redo) continue;
+    int score = 0;
+    if (int r = trouble_score(c, c->dA, c->dU, true, &score)) return r;
+    if (score <= 0) continue;
+    ++*nredo;
+    c->trouble_redos++;
+    // u -= r rfac on the troubled elements, they become FV and get their r
+    // again, then u += r rfac and the next w
+    if (int r = trouble_fix(c, c->dA, c->dA, nullptr, c->dR, -S.cA, 0)) return r;
+    if (int r = redo_rhs(c, S, (int)s)) return r;
+    if (int r = trouble_fix(c, c->dA, c->dA, nullptr, c->dR, S.cA, 0)) return r;
+    if (S.wout >= 0 && S.wmode == 1)
+      if (int r = trouble_fix(c, treg(c, S.wout), c->dU, nullptr, c->dR, S.cW, 0)) return r;
+    if (S.wout >= 0 && S.wmode == 2)
+      if (int r = trouble_fix(c, treg(c, S.wout), c->dU, treg(c, S.win), c->dR, S.cW, 2)) return r;
+  }
+  return NMX_OK;
+}
+
+// evolve_myln (evolve.c:25-176) with troubled elements: the step, its
+// scores (after each substep, or after the step and then the step redone
+// from u_p on the switched mesh), then the elements whose score is at most
+// -notroubles back to DG.  The state ends in U.
+static int trouble_step(nmx_ctx *c, int method, double dt) {
+  const bool substep = P(c, "evolve_redo_troubled") == "substep";
+  const int nsub = method == 0 ? 4 : method == 1 ? 3 : 1;
+  const int notroubles = 11 * (substep ? nsub : 1);   // NOTROUBLES (evolve.h:60), evolve_notroubles
+  int nredo = 0, score = 0;
+  if (int r = trouble_evolve_mesh(c, method, dt, substep, &nredo)) return r;
+  if (!substep) {
+    if (int r = trouble_score(c, c->dA, c->dU, true, &score)) return r;
+    if (score > 0) {
+      // evolve_prepare_do_over_mesh: back to u_p, troubled elements to FV,
+      // the step again
+      std::vector<int> fv(c->mesh.e.size());
+      for (size_t e = 0; e < fv.size(); ++e) fv[e] = c->mesh.e[e].fv || c->hscore[e] > 0;
+      if (int r = trouble_switch(c, fv, 0)) return r;
+      c->trouble_redos++;
+      if (int r = trouble_evolve_mesh(c, method, dt, false, &nredo)) return r;
+    }
+  } else {
+    if (int r = trouble_score(c, c->dA, c->dU, false, &score)) return r;
+  }
+  HIPCHK(hipMemcpyAsync(c->dU, c->dA, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream));
+  if (score > 0 || score <= -notroubles) {
+    // evolve_switch_nontroubled_nodes_mesh: scores <= -notroubles back to DG
+    std::vector<int> fv(c->mesh.e.size());
+    bool any = false;
+    for (size_t e = 0; e < fv.size(); ++e) {
+      const bool back = c->mesh.e[e].fv && c->hscore[e] <= -notroubles;
+      fv[e] = c->mesh.e[e].fv && !back;
+      any = any || back;
+    }
+    if (any)
+      if (int r = trouble_switch(c, fv, 0)) return r;
+  }
   return NMX_OK;
 }
 
@@ -2117,6 +2463,25 @@ static void stage_times(int method, double t, double dt, double *ts) {
 // Autotuner: time the stage kernel's tile size (nodes per thread) and the
 // XCD-aware element mapping on this mesh with a few RK steps, keep the
 // fastest.  The state and time are restored afterwards.
+extern "C" int nmx_set_trouble_hook(nmx_ctx *c, nmx_trouble_fn fn, void *user) {
+  if (!c) return fail(NMX_EINVAL, "ctx is NULL");
+  c->trouble_hook = fn;
+  c->trouble_user = user;
+  return NMX_OK;
+}
+
+extern "C" int nmx_get_trouble_scores(nmx_ctx *c, int *score, int nelem, long *redos, long *switches) {
+  if (!c) return fail(NMX_EINVAL, "ctx is NULL");
+  if (!c->have_mesh) return fail(NMX_ESTATE, "no mesh");
+  if (score) {
+    if (nelem != (int)c->mesh.e.size()) return fail(NMX_EINVAL, "need one score per element");
+    for (int e = 0; e < nelem; ++e) score[e] = (size_t)e < c->hscore.size() ? c->hscore[e] : 0;
+  }
+  if (redos) *redos = c->trouble_redos;
+  if (switches) *switches = c->trouble_switches;
+  return NMX_OK;
+}
+
 extern "C" int nmx_autotune(nmx_ctx *c) {
   NEED_DEV(c);
   HIPCHK(hipSetDevice(c->device));
@@ -2269,6 +2634,26 @@ extern "C" int nmx_evolve(nmx_ctx *c, int nsteps) {
   }
   const int method = method_id(c);
   const double dt = c->dt;
+  if (trouble_on(c)) {
+    // troubled elements: scores read on the host after each substep, so one
+    // step at a time, no graphs
+    if (c->lim_on || c->filt_on) return fail(NMX_EUNSUP, "troubled elements with a limiter or filter are not built");
+    if (P(c, "nmx_trouble") == "hook" && !c->trouble_hook) return fail(NMX_ESTATE, "nmx_trouble = hook needs nmx_set_trouble_hook");
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (int k = 0; k < nsteps; ++k) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      stage_times(method, c->time, dt, c->hT);
+      HIPCHK(hipMemcpyAsync(c->dT, c->hT, 4 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      if (int r = trouble_step(c, method, dt)) return r;
+      advance_time(c, dt);
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    return NMX_OK;
+  }
   const bool use_graph = (P(c, "nmx_graph") == "yes") && P(c, "nmx_path") != "perelement";
   int G = Pi(c, "nmx_graph_steps");
   if (G < 2) G = 2;

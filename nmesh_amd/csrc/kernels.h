@@ -146,6 +146,38 @@ struct LimArgs {
   int nelem = 0;
 };
 hipError_t launch_limiter(const LimArgs &a, hipStream_t s);
+hipError_t launch_lim_data(const LimArgs &a, hipStream_t s);
+
+// Troubled elements (evolve_set_trouble_score, trouble.c:27-113): the
+// library's RDMP indicator writes each element's TROUBLE value tr (1, 0,
+// -1) from u and the element data of u_p (ldat as the MRS data pass
+// leaves it, [e][5][4]: average, min, max); the score kernel moves the
+// scores and writes the mesh's score; the fix kernel redoes the stage
+// epilogue on elements whose score is positive.
+struct TroubleArgs {
+  const double *u = nullptr;
+  const double *ldat = nullptr;
+  const long *eoff = nullptr;
+  const int *en = nullptr;
+  const int *efv = nullptr;        // [e]: 1 FV element
+  const int *nbr = nullptr;        // [e][6][4] face neighbours (-1 none)
+  int *tr = nullptr;
+  int nelem = 0;
+  double delta0 = 1e-4, eps = 1e-3;
+};
+hipError_t launch_rdmp(const TroubleArgs &a, hipStream_t s);
+hipError_t launch_trouble_score(const int *tr, int *score, int n, int *out, hipStream_t s);
+struct TroubleFix {
+  double *y = nullptr;
+  const double *x = nullptr, *x2 = nullptr, *r = nullptr;
+  double s = 0;
+  int mode = 0;                    // 0: y = x + s r; 1: y = x; 2: y = 0.75 x + 0.25 (x2 + s r)
+  const int *score = nullptr;
+  const long *eoff = nullptr;
+  const int *en = nullptr;
+  int nelem = 0;
+};
+hipError_t launch_trouble_fix(const TroubleFix &f, hipStream_t s);
 
 // flat [v][node] <-> element blocks [e][v][np_e] on the device
 // (to_blocks: 1 flat -> blocks, 0 blocks -> flat)

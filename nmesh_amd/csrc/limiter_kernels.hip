@@ -227,6 +227,15 @@ __global__ __launch_bounds__(kLimThreads) void mm_apply_kernel(LimArgs a) {
   }
 }
 
+// the MRS data pass alone (limdata_MRS): average, minimum, maximum per
+// element and variable into a.ldat
+hipError_t launch_lim_data(const LimArgs &a, hipStream_t s) {
+  if (a.nelem <= 0) return hipSuccess;
+  const int grid = a.nelem < 65536 ? a.nelem : 65536;
+  hipLaunchKernelGGL(lim_data_kernel, dim3(grid), dim3(kLimThreads), 0, s, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_limiter(const LimArgs &a, hipStream_t s) {
   if (a.nelem <= 0) return hipSuccess;
   const int grid = a.nelem < 65536 ? a.nelem : 65536;

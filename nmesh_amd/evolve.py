@@ -151,6 +151,34 @@ class Mesh:
         f = np.ascontiguousarray(to_fv, dtype=np.int32)
         check(lib().nmx_mesh_switch_fv(self._ctx, _p(f), int(f.size)))
 
+    def fv_flags(self):
+        """1 for FV elements on uniform points, 0 for DG (node->dat->use_fv)"""
+        f = np.zeros(self.nelem, dtype=np.int32)
+        check(lib().nmx_mesh_fv_flags(self._ctx, _p(f), int(f.size)))
+        return f
+
+    def set_trouble_hook(self, fn):
+        """Install a TROUBLE function (evosys->f[TROUBLE]): fn(u, u_p, tr,
+        nelem, stream) with device pointers (element blocks) and a
+        hipStream_t, returning 0; it must write tr[e] in {1, 0, -1} for
+        every element in stream order.  None removes it.  The callback
+        object is kept alive with the mesh."""
+        if fn is None:
+            self._trouble_cb = None
+            check(lib().nmx_set_trouble_hook(self._ctx, None, None))
+            return
+        proto = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p)
+        self._trouble_cb = proto(lambda user, u, up, tr, n, stream: int(fn(u, up, tr, n, stream)))
+        check(lib().nmx_set_trouble_hook(self._ctx, C.cast(self._trouble_cb, C.c_void_p), None))
+
+    def trouble_scores(self):
+        """(scores per element, redone substeps or steps, switches)
+        (node->dat->info->trbl_score, trouble.c:27-113)"""
+        s = np.zeros(self.nelem, dtype=np.int32)
+        redos, switches = C.c_long(), C.c_long()
+        check(lib().nmx_get_trouble_scores(self._ctx, _p(s), int(s.size), C.byref(redos), C.byref(switches)))
+        return s, redos.value, switches.value
+
     def padapt(self, n):
         """new points per direction per element, the state interpolated
         (update_node_n_pt_typ, storage.c:858-989)"""

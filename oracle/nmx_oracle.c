@@ -18,6 +18,7 @@
  * (src/tests/scalarwave1/scalarwave1.c:670-672).
  */
 #include <float.h>
+#include <limits.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -2636,4 +2637,84 @@ int orc_adapt_state(const OMesh *Mold, const double *uold, const OMesh *Mnew, do
     free(sum); free(cnt); free(tmp);
   }
   return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Troubled elements (src/main/evolve/trouble.c)                       */
+/* ------------------------------------------------------------------ */
+
+/* evolve_set_trouble_score (trouble.c:27-110) on every element from its
+   TROUBLE value tr[e] (1 trouble, 0 keep, -1 all good): 1 starts the score
+   at 1 or raises it, 0 sets it to 0, -1 starts it at -1 or lowers it,
+   clamped to +-2^30.  tr == NULL reads the scores only.  Returns the
+   mesh's score (evolve_read_trouble_score_mesh, trouble.c:129-161): the
+   maximum when it is positive, else the minimum. */
+int orc_trouble_score(int nelem, const int *tr, int *score)
+{
+  const int max_trouble = 1073741824;
+  int mx = INT_MIN, mn = INT_MAX;
+  for (int e = 0; e < nelem; e++) {
+    int s = score[e];
+    if (tr) {
+      if (tr[e] > 0) s = (s <= 0) ? 1 : s + 1;
+      else if (tr[e] == 0) s = 0;
+      else s = (s >= 0) ? -1 : s - 1;
+      if (s < -max_trouble) s = -max_trouble;
+      if (s > max_trouble) s = max_trouble;
+      score[e] = s;
+    }
+    if (s > mx) mx = s;
+    if (s < mn) mn = s;
+  }
+  return mx > 0 ? mx : mn;
+}
+
+/* The library's RDMP indicator on the data evolve_collect_u_p_data_mesh
+   gathers (trouble.c:355-400: limdata_MRS of u_p, the minimum and maximum
+   over the element's nodes): m and M are the extremes of u_p over the
+   element and its face neighbours, d = max(delta0, eps (M - m)) per
+   variable; a value of u outside [m - d, M + d] (or NaN) puts the element
+   out of bounds.  tr = 1 for a DG element out of bounds, -1 for an FV
+   element in bounds, 0 otherwise. */
+void orc_trouble_rdmp(const OMesh *M, const double *u, const double *up, double delta0, double eps, int *tr)
+{
+  const long nn = M->nnodes;
+  double *mm = malloc(sizeof(double) * 2 * NV * M->nelem);
+  for (int e = 0; e < M->nelem; e++) {
+    const OElm *E = &M->e[e];
+    const int np = elm_np(E);
+    for (int v = 0; v < NV; v++) {
+      const double *q = up + v * nn + E->off;
+      double lo = q[0], hi = q[0];
+      for (int i = 1; i < np; i++) { if (q[i] < lo) lo = q[i]; if (q[i] > hi) hi = q[i]; }
+      mm[2 * (NV * e + v)] = lo;
+      mm[2 * (NV * e + v) + 1] = hi;
+    }
+  }
+  for (int e = 0; e < M->nelem; e++) {
+    const OElm *E = &M->e[e];
+    const int np = elm_np(E);
+    int out = 0;
+    for (int v = 0; v < NV && !out; v++) {
+      double m = mm[2 * (NV * e + v)], Mx = mm[2 * (NV * e + v) + 1];
+      for (int f = 0; f < 6; f++)
+        for (int ni = 0; ni < E->nfnb[f]; ni++) {
+          int nb = E->fnb[f][ni];
+          m = fmin(m, mm[2 * (NV * nb + v)]);
+          Mx = fmax(Mx, mm[2 * (NV * nb + v) + 1]);
+        }
+      double d = fmax(delta0, eps * (Mx - m));
+      const double *q = u + v * nn + E->off;
+      for (int i = 0; i < np; i++)
+        if (!(q[i] >= m - d && q[i] <= Mx + d)) { out = 1; break; }
+    }
+    tr[e] = E->use_fv ? (out ? 0 : -1) : (out ? 1 : 0);
+  }
+  free(mm);
+}
+
+/* the FV flags of all elements (node->dat->use_fv) */
+void orc_mesh_fv_flags(const OMesh *M, int *fv)
+{
+  for (int e = 0; e < M->nelem; e++) fv[e] = M->e[e].use_fv;
 }

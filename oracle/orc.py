@@ -91,6 +91,9 @@ def lib():
             "orc_fv_rhs_prim": (None, [i, d, vp, vp]),
             "orc_fv_rhs": (i, [i, d, vp, vp, vp, d]),
             "orc_fv_evolve": (i, [i, d, d, i, vp, vp, d]),
+            "orc_trouble_score": (i, [i, vp, vp]),
+            "orc_trouble_rdmp": (None, [vp, vp, vp, d, d, vp]),
+            "orc_mesh_fv_flags": (None, [vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -223,6 +226,20 @@ class OMesh:
         if r != 0:
             raise ValueError("orc_adapt_state failed (%d)" % r)
         return new, out
+
+    def fv_flags(self):
+        f = np.zeros(self.nelem, dtype=np.int32)
+        lib().orc_mesh_fv_flags(self.h, _p(f))
+        return f
+
+    def trouble_rdmp(self, u, up, delta0=1e-4, eps=1e-3):
+        """TROUBLE values of the library's RDMP indicator for state u
+        against u_p (1 DG out of bounds, -1 FV in bounds, 0 otherwise)"""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        up = np.ascontiguousarray(up, dtype=np.float64)
+        tr = np.zeros(self.nelem, dtype=np.int32)
+        lib().orc_trouble_rdmp(self.h, _p(u), _p(up), float(delta0), float(eps), _p(tr))
+        return tr
 
     def limit_MRS(self, u, alpha):
         """one application of the MRS limiter to state u (limdata_MRS then
@@ -409,3 +426,12 @@ class OFV:
         P = np.ascontiguousarray(P_guess, dtype=np.float64).ravel().copy()
         bad = lib().orc_fv_evolve(self.N, self.gam, float(dt), int(nsteps), _ptr(u), _ptr(P), self.tol)
         return u, P, bad
+
+
+def trouble_score(score, tr=None):
+    """evolve_set_trouble_score on every element (tr None: read only);
+    updates score in place, returns the mesh's score"""
+    assert score.dtype == np.int32 and score.flags.c_contiguous
+    if tr is not None:
+        tr = np.ascontiguousarray(tr, dtype=np.int32)
+    return int(lib().orc_trouble_score(len(score), None if tr is None else _p(tr), _p(score)))
