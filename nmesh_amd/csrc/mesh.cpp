@@ -155,28 +155,11 @@ void legendre_analysis_synthesis(int n, double *A, double *S) {
 }
 
 void filter_matrix(int n, double alp, double s, int dn, double *F) {
-  const Basis &b = lgl_basis(n);
-  const int N = n - 1;
   std::vector<double> A(n * n), S(n * n), f(n);
+  legendre_analysis_synthesis(n, A.data(), S.data());
   double Nf = n - 1 - dn;
   if (Nf <= 0.) Nf = 1.;
-  for (int i = 0; i < n; ++i) {
-    f[i] = std::exp(-alp * std::pow(i / Nf, s));
-    double ooci = 0.5 * (2 * i + 1);
-    if (i == N && N != 0) ooci = 0.5 * i;
-    const double oosci = std::sqrt(ooci);
-    for (int j = 0; j < n; ++j) {
-      const double x = b.x[j];
-      double P0 = 1.0, P1 = x, Pk = (i == 0) ? 1.0 : x;
-      for (int k = 2; k <= i; ++k) {
-        Pk = (x * (2 * k - 1) * P1 - (k - 1) * P0) / k;
-        P0 = P1;
-        P1 = Pk;
-      }
-      S[j * n + i] = Pk * oosci;
-      A[i * n + j] = b.w[j] * S[j * n + i];
-    }
-  }
+  for (int i = 0; i < n; ++i) f[i] = std::exp(-alp * std::pow(i / Nf, s));
   for (int j = 0; j < n; ++j)
     for (int l = 0; l < n; ++l) {
       double sum = 0.0;
