@@ -1,17 +1,21 @@
 #!/usr/bin/env python3
-"""Benchmark of the DG RHS + RK4 hot path on one MI355X.
+"""Benchmark of the DG RHS + RK stage hot path on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): scalar wave
-(scalarwave1) on a cubed-sphere shell r in [2, 20], six patches of 8x8x8
-elements (3072 elements), N=7 (8 LGL points per direction, 1,572,864 nodes),
-FP64, upwind flux, outgoing boundaries (scalarwave1_k = 0 0 0), RK4 at
-dtfac 0.25.  Initial data: a Gaussian shell psi = exp(-(r-8)^2) (1 + sum_m
-a_m Y_2m) with a_m ~ U[-0.1, 0.1] from numpy's PCG64 seeded with 2026, the
-constraint-satisfying Phi_i = d_i psi and Pi = 0.
+Headline workload (the mixed-order AMR configuration the north-star target
+is stated on): scalar wave (scalarwave1) on a cubed-sphere shell r in [2, 20]
+refined to level 3 and, inside nested spheres of radius 6 and 3, to levels 4
+and 5 (19,200 elements); per-element n drawn uniformly from {4..10} points
+per direction (N in {3..9}, PCG64 seed 2026; 8,213,159 nodes); FP64, upwind
+flux, outgoing boundaries, RK4 at dtfac 0.25, batched order-bucket launches
+with hipGraph-captured steps and the autotuned launch shape.  Initial data:
+a Gaussian shell psi = exp(-(r-8)^2) (1 + sum_m a_m Y_2m) with a_m ~
+U[-0.1, 0.1] (PCG64 seed 2026), Phi_i = d_i psi, Pi = 0.
 
-One "step" is one full RK4 step (4 fused RHS+stage launches).  The unit of
-work is a node-update: one RHS evaluation at one node for all 5 fields.
-Prints one JSON line.
+One "step" is one full RK4 step (4 stages).  The unit of work is a
+node-update: one RHS evaluation at one node for all 5 fields, fused with its
+stage update.  Secondary legs: the plain per-order kernel on the uniform
+N=7 shell (6x8^3 elements), per-element launches, the adjoint, FV, filter
+and CRC.  Prints one JSON line.
 """
 import argparse
 import json
@@ -32,6 +36,15 @@ BYTES_PER_NU_CURV = 272.0
 # the least the batched path moves per node-update: 20 state doubles on
 # average over the 4 stages (dXdx recomputed)
 MIN_BYTES_PER_NU = 160.0
+
+
+def flops_per_nu(n):
+    """FP64 operations of one DG node-update of scalarwave1 at n points per
+    direction: 12 derivatives along the element's reference directions (of
+    pi and of the three c_i) at 2n each, and 36 for their combination with
+    the metric into grad pi and div c and the RHS.  Face terms, the metric's
+    own recomputation and the stage combination are not counted."""
+    return 24.0 * n + 36.0
 
 
 def shell_initial_data(x, seed=2026):
@@ -120,6 +133,69 @@ def c4_mesh(nm, path):
     rng = np.random.default_rng(2026)
     m.set_elm_n(rng.integers(4, 11, m.nelem).astype(np.int32))
     return m
+
+
+def c4_headline(nm, steps, warmup, have_torch):
+    """The headline: the C4 mesh on the batched path, W untimed steps, then
+    exactly K timed steps bracketed by device synchronisations."""
+    import torch
+    m = c4_mesh(nm, "batched")
+    x = m.coords()
+    m.upload(shell_initial_data(x))
+    nm.evolve_myln(m, 1)                   # autotune (launch shape, tiles) and graph capture
+    if warmup:
+        nm.evolve_myln(m, warmup)
+    m.synchronize()
+    if have_torch:
+        torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nm.evolve_myln(m, steps)
+    m.synchronize()
+    if have_torch:
+        torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    dev_s = m.last_evolve_ms() * 1e-3      # HIP events on the library's stream around the steps
+    info = np.array([m.elm_info(e) for e in range(m.nelem)])
+    n = info[:, 2].astype(np.float64)
+    nn = m.nnodes
+    stages = 4 * steps
+    stage_s = dev_s / stages
+    flops_stage = float((n ** 3 * flops_per_nu(n)).sum())
+    buckets = {int(k): int((info[:, 2] == k).sum()) for k in np.unique(info[:, 2])}
+    res = {"m": m, "wall": wall, "dev_s": dev_s, "nodes": nn, "elements": m.nelem,
+           "stage_us": stage_s * 1e6, "flops_per_stage": flops_stage,
+           "achieved": BYTES_PER_NU_CURV * nn / stage_s / 1e9,
+           "achieved_min": MIN_BYTES_PER_NU * nn / stage_s / 1e9,
+           "tflops": flops_stage / stage_s / 1e12, "buckets": buckets,
+           "shape": m.par("nmx_batch"), "dt": m.dt}
+    return res
+
+
+def c4_cpu_baseline(budget_s=12.0):
+    """The oracle (OpenMP C restatement of the reference algorithm) on the
+    same C4 mesh, orders and operator: as many RK4 steps as fit the budget
+    (at least one), timed on the host cores."""
+    from oracle.orc import OMesh, lib
+    om = OMesh("shell", 3, 5, rin=2.0, rout=20.0, radius=3, levels=2)
+    om.set_n(np.random.default_rng(2026).integers(4, 11, om.nelem).astype(np.int32))
+    X, x = om.coords()
+    u = shell_initial_data(x)
+    dt = om.dt()
+    k = [0.0, 0.0, 0.0]
+    t0 = time.perf_counter()
+    u, t = om.evolve(u, dt, 1, 0.0, k, "RK4")
+    one = time.perf_counter() - t0
+    steps, secs = 1, one
+    nmore = max(0, min(4, int((budget_s - one) / max(one, 1e-3))))
+    if nmore:
+        t0 = time.perf_counter()
+        u, t = om.evolve(u, dt, nmore, t, k, "RK4")
+        steps, secs = nmore, time.perf_counter() - t0
+    rate = steps * 4 * om.nnodes / secs
+    return {"value": rate, "unit": "node-updates/s", "cores": int(lib().orc_num_threads()),
+            "kind": "port",
+            "sample": "%d RK4 step(s) of the same 19,200-element mixed-order AMR shell (%d "
+                      "node-updates) with the oracle's OpenMP C restatement" % (steps, steps * 4 * om.nnodes)}
 
 
 def batched_leg(nm, steps, plain_steps):
@@ -409,10 +485,18 @@ def adjoint_leg(nm, steps=256, every=16, reps=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=200, help="timed RK4 steps of the headline (C4 batched)")
+    ap.add_argument("--warmup", type=int, default=20, help="untimed RK4 steps before them")
+    ap.add_argument("--shape-steps", type=int, default=40,
+                    help="RK4 steps timed for each other batched launch shape (0: skip)")
+    ap.add_argument("--perelement-steps", type=int, default=1,
+                    help="RK4 steps of the per-element launch leg (0: skip)")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain uniform-shell leg")
+    ap.add_argument("--plain-steps", type=int, default=400)
+    ap.add_argument("--plain-warmup", type=int, default=40)
+    ap.add_argument("--plain-n", type=int, default=8, help="points per direction of the plain leg")
+    ap.add_argument("--only-plain", action="store_true", help="run only the plain uniform-shell leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-batched", action="store_true", help="skip the AMR batched-vs-per-element leg")
     ap.add_argument("--amr-steps", type=int, default=40, help="RK4 steps of the batched AMR leg")
     ap.add_argument("--only-amr", action="store_true", help="run only the batched AMR leg (profiling)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-cost leg")
@@ -463,6 +547,9 @@ def main():
         out = batched_leg(nm, args.amr_steps, 0) if args.amr_steps else {}
         print(json.dumps(out), flush=True)
         return
+    if args.only_plain:
+        print(json.dumps(plain_leg(nm, args, have_torch)), flush=True)
+        return
     if args.only_adjoint:
         print(json.dumps(adjoint_leg(nm)), flush=True)
         return
@@ -478,7 +565,100 @@ def main():
     if args.only_fv:
         print(json.dumps(fv_leg(args.fv_steps, cpu=not args.no_cpu)), flush=True)
         return
-    m = nm.Mesh(c2_params(), device=0)
+    # the headline: C4 on the batched path
+    h = c4_headline(nm, args.steps, args.warmup, have_torch)
+    mh = h.pop("m")
+    shapes = {h["shape"]: 1e3 * h["wall"] / args.steps}
+    if args.shape_steps:
+        for shape in ("persistent", "buckets", "legacy"):
+            if shape in shapes:
+                continue
+            mh.set_par("nmx_batch", shape)
+            nm.evolve_myln(mh, 1)
+            mh.synchronize()
+            t0 = time.perf_counter()
+            nm.evolve_myln(mh, args.shape_steps)
+            mh.synchronize()
+            shapes[shape] = 1e3 * (time.perf_counter() - t0) / args.shape_steps
+    del mh
+    nn = h["nodes"]
+    value = 4.0 * args.steps * nn / h["wall"]
+    traffic = load_traffic("traffic_dg_batched_c4.json")
+    out = {
+        "metric": "DG RHS node-updates/s (FP64)",
+        "value": value,
+        "unit": "node-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * h["wall"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "scalarwave1 DG, mixed-order AMR cubed-sphere shell r in [2,20], "
+                               "levels 3-5 (19,200 elements), n uniform in {4..10} points per "
+                               "direction (%d nodes), upwind, outgoing, RK4, batched order "
+                               "buckets, hipGraph-captured steps" % nn,
+                   "elements": h["elements"], "nodes": nn, "dt": h["dt"], "path": "batched",
+                   "launch_shape": h["shape"], "order_buckets": h["buckets"]},
+        "roofline": {"bound": "hbm", "achieved": h["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": h["achieved"] / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_source": ("profile figure: profiles/%s, rocprofv3 --pmc FETCH_SIZE and "
+                                        "WRITE_SIZE summed over one stage's launches, calibrated"
+                                        % traffic["summary"]) if traffic else None,
+                     "kernel": "one RK4 stage of the batched path (%s launch shape: every order "
+                               "bucket's launches of the stage)" % h["shape"],
+                     "avg_launch_us": h["stage_us"],
+                     "bytes_per_node_update": BYTES_PER_NU_CURV,
+                     "min_bytes_per_node_update": MIN_BYTES_PER_NU,
+                     "achieved_min": h["achieved_min"], "frac_min": h["achieved_min"] / HBM_PEAK_GBS,
+                     "fp64": {"flops_per_node_update": "24 n + 36", "flops_per_stage": h["flops_per_stage"],
+                              "achieved_TFLOPs": h["tflops"], "peak_TFLOPs": FP64_PEAK_TFLOPS,
+                              "frac": h["tflops"] / FP64_PEAK_TFLOPS}},
+        "batched_shapes_ms_per_step": shapes,
+    }
+    if args.perelement_steps:
+        mp = c4_mesh(nm, "perelement")
+        mp.upload(shell_initial_data(mp.coords()))
+        nm.evolve_myln(mp, 1)
+        mp.synchronize()
+        t0 = time.perf_counter()
+        nm.evolve_myln(mp, args.perelement_steps)
+        mp.synchronize()
+        wall_pe = time.perf_counter() - t0
+        del mp
+        pe = 4.0 * args.perelement_steps * nn / wall_pe
+        out["perelement"] = {"value": pe, "unit": "node-updates/s", "steps": args.perelement_steps,
+                             "ms_per_step": 1e3 * wall_pe / args.perelement_steps,
+                             "speedup_batched_vs_perelement": value / pe}
+    if not args.no_plain:
+        out["plain_shell"] = plain_leg(nm, args, have_torch)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(nm)
+    if not args.no_fv:
+        out["fv_sr_euler"] = fv_leg(args.fv_steps, cpu=not args.no_cpu)
+    if not args.no_filter:
+        out["filter"] = filter_leg(nm)
+    if not args.no_fvelm:
+        out["fv_elements"] = fvelm_leg(nm)
+    if not args.no_crc:
+        out["checkpoint_crc"] = crc_leg(nm)
+    if not args.no_cpu:
+        out["cpu_baseline"] = c4_cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+def plain_leg(nm, args, have_torch):
+    """The plain per-order kernel on the uniform N=7 shell (6x8^3 elements,
+    n = args.plain_n points per direction), graph-captured RK4 steps."""
+    import torch
+    p = c2_params()
+    if args.plain_n != 8:
+        p.update({"amr_n0": args.plain_n, "amr_n1": args.plain_n, "amr_n2": args.plain_n})
+    m = nm.Mesh(p, device=0)
     m.set_par("nmx_path", args.path)
     m.set_par("nmx_graph", args.graph)
     m.set_par("nmx_geometry", args.geometry)
@@ -492,27 +672,24 @@ def main():
     nn = m.nnodes
     if args.block == "auto" and args.xcd == "auto":
         nm.autotune(m)            # times tile size / XCD mapping on this mesh
-    if args.warmup:
-        nm.evolve_myln(m, args.warmup)
+    if args.plain_warmup:
+        nm.evolve_myln(m, args.plain_warmup)
     m.synchronize()
     if have_torch:
         torch.cuda.synchronize()
     t0 = time.perf_counter()
-    nm.evolve_myln(m, args.steps)
+    nm.evolve_myln(m, args.plain_steps)
     m.synchronize()
     if have_torch:
         torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     dev_ms = m.last_evolve_ms()    # HIP events on the library's stream around the steps
-    launches = 4 * args.steps
+    launches = 4 * args.plain_steps
     avg_launch_s = dev_ms * 1e-3 / launches
-    node_updates = 4.0 * args.steps * nn
-    value = node_updates / wall
     # algorithmic bytes per node-update of a fused RHS + RK4 stage: read w,
     # u, u_p and write u, w (25 doubles) plus 9 doubles of dXdx per node on
     # curvilinear elements, face-neighbour re-reads counted 0: 272 B
-    bytes_per_nu = BYTES_PER_NU_CURV
-    achieved = bytes_per_nu * nn / avg_launch_s / 1e9
+    achieved = BYTES_PER_NU_CURV * nn / avg_launch_s / 1e9
     # the least this kernel must move: stage 1 reads u and writes u_acc, w
     # (15 doubles), stages 2-3 read w, u_acc, u_p and write u_acc, w (25),
     # stage 4 reads w, u_acc and writes u_acc (15); dXdx recomputed (0) or
@@ -520,51 +697,29 @@ def main():
     g = 9.0 if args.geometry == "stored" else 0.0
     min_bytes_per_nu = 8.0 * ((15 + 25 + 25 + 15) / 4.0 + g)
     achieved_min = min_bytes_per_nu * nn / avg_launch_s / 1e9
-    traffic = load_traffic("traffic_dg_stage_shell_n8.json")
-    out = {
-        "metric": "DG RHS node-updates/s (FP64)",
-        "value": value,
-        "unit": "node-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "scalarwave1 DG, cubed-sphere shell r in [2,20], 6x8^3 elements, "
-                               "N=7 (1,572,864 nodes), upwind, outgoing, RK4, fused RHS+stage",
-                   "elements": m.nelem, "nodes": nn, "dt": m.dt, "path": args.path},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "kernel": "dg_stage_kernel<8,%d,%d>" % (2 if args.geometry == "analytic" else 1,
-                                                             {"full": 512, "half": 256, "quarter": 128}[m.par("nmx_block")]),
-                     "tile": m.par("nmx_block"), "xcd_map": m.par("nmx_xcd"),
-                     "avg_launch_us": avg_launch_s * 1e6,
-                     "bytes_per_node_update": bytes_per_nu,
-                     "min_bytes_per_node_update": min_bytes_per_nu,
-                     "achieved_min": achieved_min, "frac_min": achieved_min / HBM_PEAK_GBS},
-    }
+    tflops = flops_per_nu(args.plain_n) * nn / avg_launch_s / 1e12
+    traffic = load_traffic("traffic_dg_stage_shell_n8.json") if args.plain_n == 8 else None
+    out = {"workload": "scalarwave1 DG, cubed-sphere shell r in [2,20], 6x8^3 elements, "
+                       "n=%d (%d nodes), upwind, outgoing, RK4, fused RHS+stage" % (args.plain_n, nn),
+           "value": 4.0 * args.plain_steps * nn / wall, "unit": "node-updates/s",
+           "steps": args.plain_steps, "warmup": args.plain_warmup,
+           "ms_per_step": 1e3 * wall / args.plain_steps, "elements": m.nelem, "nodes": nn,
+           "dt": m.dt, "path": args.path,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": traffic["bytes_per_launch"] if traffic else None,
+                        "traffic_source": "profile figure: profiles/dg_stage_kernel_shell_n8_rk4.md" if traffic else None,
+                        "kernel": "dg_stage_kernel, n=%d, %s tile, %s geometry" % (args.plain_n, m.par("nmx_block"),
+                                                                                    args.geometry),
+                        "tile": m.par("nmx_block"), "xcd_map": m.par("nmx_xcd"),
+                        "avg_launch_us": avg_launch_s * 1e6,
+                        "bytes_per_node_update": BYTES_PER_NU_CURV,
+                        "min_bytes_per_node_update": min_bytes_per_nu,
+                        "achieved_min": achieved_min, "frac_min": achieved_min / HBM_PEAK_GBS,
+                        "fp64": {"flops_per_node_update": flops_per_nu(args.plain_n),
+                                 "achieved_TFLOPs": tflops, "frac": tflops / FP64_PEAK_TFLOPS}}}
     del m
-    if not args.no_batched:
-        out["batched_amr"] = batched_leg(nm, args.amr_steps, 1)
-    if not args.no_adjoint:
-        out["adjoint"] = adjoint_leg(nm)
-    if not args.no_fv:
-        out["fv_sr_euler"] = fv_leg(args.fv_steps, cpu=not args.no_cpu)
-    if not args.no_filter:
-        out["filter"] = filter_leg(nm)
-    if not args.no_fvelm:
-        out["fv_elements"] = fvelm_leg(nm)
-    if not args.no_crc:
-        out["checkpoint_crc"] = crc_leg(nm)
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(x)
-    print(json.dumps(out), flush=True)
-
+    return out
 
 if __name__ == "__main__":
     main()
