@@ -292,7 +292,10 @@ int nmx_state_device_ptr(nmx_ctx *ctx, int reg, void **dptr, long *len);
  * evolve_sspRK3_mesh / evolve_Euler_mesh, RungeKutta.c:19-188).  The
  * scalarwave1 RHS is linear in u, so its Jacobian L does not depend on the
  * state or the time; the boundary profile enters only as a u-independent
- * source.  Conforming meshes of one element order only (NMX_EUNSUP otherwise).
+ * source.  DG meshes of every kind: conforming meshes of one order, and
+ * h-refined mixed-order meshes on the batched path, where the transpose of
+ * the interpolated and averaged faces (surface.c:1081-1337) is applied.
+ * Meshes with FV elements, the limiter and the filter: NMX_EUNSUP.
  *
  * reg_out = L^T reg_in */
 int nmx_adjoint_setrhs(nmx_ctx *ctx, int reg_out, int reg_in);

@@ -49,28 +49,6 @@ __device__ __forceinline__ void elm_metric(const AdjArgs &a, int e, int node, do
   }
 }
 
-// outward normal and lift scale of face f of element e at a node with metric M
-template <int GEOM>
-__device__ __forceinline__ void face_normal(const double M[3][3], const double *dXbdX, int f,
-                                            double wend_inv, double nrm[3], double &scale) {
-  const int dir = f >> 1;
-  const double sig = (f & 1) ? 1.0 : -1.0;
-  if (GEOM == 0) {
-    nrm[0] = (dir == 0) ? sig : 0.0;
-    nrm[1] = (dir == 1) ? sig : 0.0;
-    nrm[2] = (dir == 2) ? sig : 0.0;
-    scale = dXbdX[dir] * wend_inv;
-  } else {
-    const double r0 = (dir == 0) ? M[0][0] : (dir == 1) ? M[1][0] : M[2][0];
-    const double r1 = (dir == 0) ? M[0][1] : (dir == 1) ? M[1][1] : M[2][1];
-    const double r2 = (dir == 0) ? M[0][2] : (dir == 1) ? M[1][2] : M[2][2];
-    const double s = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
-    const double smag = sig * s;
-    nrm[0] = r0 / smag; nrm[1] = r1 / smag; nrm[2] = r2 / smag;
-    scale = dXbdX[dir] * s * wend_inv;
-  }
-}
-
 template <int NP, int GEOM, int NT>
 __global__ __launch_bounds__(NT, 2) void dg_adjoint_kernel(AdjArgs a) {
   constexpr int NP2 = NP * NP;

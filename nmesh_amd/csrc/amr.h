@@ -151,6 +151,77 @@ struct FilterArgs {
   int jpow = 0;                   // filter u J^jpow, J = 1/det(dXb/dx)
 };
 
+// ---------------------------------------------------------------------------
+// Discrete adjoint of the batched path.  With G = s_f A(n_f) lam at every
+// face point of every element (A the symmetric flux Jacobian, s_f the lift
+// scale), the transpose of the face gathers hands each element, at the
+// points of its face planes, the G of the points that read them:
+//   copy faces: the partner's G at the same point;
+//   interpolated faces: per (receiving face, neighbour) job the transposed
+//   tensor interpolation T = R1^T (w G) R2 on the neighbour's plane, w the
+//   averaging weight (1 / count where several neighbours hold a point),
+//   summed over the jobs that read the plane in a fixed order.
+// ---------------------------------------------------------------------------
+
+// one face of one element, as the sender of adjoint face terms
+struct AdjFace {
+  int kind;             // 0 outer boundary, 1 copy, 2 interpolated (one or several neighbours)
+  int nb, g;            // kind 1: the partner element and its face
+  int base, s1, s2;     // kind 1: partner node = base + a*s1 + b*s2 (element-local)
+  int jfirst, jcount;   // kind 2: the jobs that read this face's plane, in AdjAmrArgs::sjobs
+};
+static_assert(sizeof(AdjFace) == 32, "AdjFace is 32 bytes");
+
+// one (receiving face, neighbour) interpolation job, transposed
+struct AdjJob {
+  int e, g;             // receiving element and face
+  int n, m;             // points per direction of the receiver and of the neighbour
+  int sw, a0, a1, b0, b1;   // the forward job's axis swap and covered points
+  int offA, offB;       // rows for the receiver's a and b points in Btab (m doubles each)
+  int md;               // the receiving face's mdesc entry (several neighbours), -1 for one
+  int toff;             // this job's transposed plane [4][m*m] in Tbuf
+  // neighbour face coordinates (a, b) -> plane indices: q1 = A1 (a - A0) +
+  // B1 (b - B0), q2 = A2 (a - A0) + B2 (b - B0)
+  int A0, B0, A1, A2, B1, B2;
+  int pad;
+};
+static_assert(sizeof(AdjJob) == 80, "AdjJob is 80 bytes");
+
+struct AdjAmrArgs {
+  const double *lam = nullptr;
+  const double *Abase = nullptr, *Wbase = nullptr;
+  double *Aout = nullptr, *Wout = nullptr, *R = nullptr;
+  double cA = 0, cU = 0, cW = 0;
+  const int *list = nullptr;      // this launch's elements
+  int nlist = 0;
+  const long *eoff = nullptr;     // [e]: first node
+  const int *en = nullptr;        // [e]: points per direction
+  const double *dXbdX = nullptr;  // [e][3]
+  const double *egeo = nullptr;   // [e][8]: box in patch coordinates, s0, s1
+  const int *edom = nullptr;      // [e]
+  const double *dXdx = nullptr;   // element blocks of 9*np (stored geometry)
+  const double *Dall = nullptr;   // D of order n at batch_doff(n)
+  const double *xball = nullptr;  // LGL nodes of order n at batch_xoff(n)
+  const double *wend_inv = nullptr;   // [n]
+  const AdjFace *af = nullptr;    // [e][6]
+  const long *goff = nullptr;     // [e]: the element's G planes [6][4][n*n] in Gbuf
+  double *G = nullptr;
+  const AdjJob *jobs = nullptr;
+  int njobs = 0;
+  const int *sjobs = nullptr;     // job indices, grouped by the plane they read
+  const int *mdesc = nullptr;
+  const double *Btab = nullptr;
+  double *T = nullptr;
+  int numflux = 0;
+};
+
+// G at every face point of the listed elements (any orders)
+hipError_t launch_adj_gflux(int geom, const AdjAmrArgs &a, hipStream_t s);
+// the transposed interpolation jobs into T
+hipError_t launch_adj_jobs(const AdjAmrArgs &a, hipStream_t s);
+// the transposed RHS and adjoint stage combination of one order bucket
+hipError_t launch_dg_adjoint_amr(int np, int geom, const AdjAmrArgs &a, int grid, hipStream_t s);
+
 // div: nodes per thread (1, 2 or 4)
 hipError_t launch_dg_amr(int np, int geom, const AmrArgs &a, int grid, int div, hipStream_t s);
 hipError_t launch_face_interp(const FaceArgs &a, hipStream_t s);

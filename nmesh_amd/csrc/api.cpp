@@ -137,6 +137,20 @@ struct nmx_ctx {
   // path), its own graph cache, and the checkpoint pool
   double *dL = nullptr;
   int *dnbf = nullptr;
+  // adjoint of the batched path: the interpolation jobs as the forward
+  // tables made them (host), then the device tables, G and T buffers and
+  // one element list per order (plus one of all elements)
+  std::vector<AdjJob> hadjj;
+  std::vector<int> hadjsend;   // per job: sender element, sender face
+  AdjFace *dadjf = nullptr;
+  AdjJob *dadjj = nullptr;
+  int *dadjs = nullptr, *dadjl = nullptr;
+  long *dadjg = nullptr;
+  double *dadjG = nullptr, *dadjT = nullptr;
+  std::vector<std::pair<int, int>> adjlists;   // (n, first) in dadjl; all elements first
+  std::vector<int> adjcount;
+  int adjnjobs = 0;
+  bool adj_ready = false;
   // FV elements (fv_divf) on uniform points: every element (fvelm, plain
   // path) or some (anyfv; FV order buckets of the batched path)
   bool fvelm = false, anyfv = false;
@@ -369,9 +383,14 @@ static void free_mesh_dev(nmx_ctx *c) {
                      (void **)&c->dcrcaw, (void **)&c->dcrcypow,
                      (void **)&c->dcrcpart, (void **)&c->dcrcout, (void **)&c->dnbr,
                      (void **)&c->dlimh, (void **)&c->dwqall, (void **)&c->dldat, (void **)&c->dlimhw,
-                     (void **)&c->damat, (void **)&c->dsmat};
+                     (void **)&c->damat, (void **)&c->dsmat, (void **)&c->dadjf, (void **)&c->dadjj,
+                     (void **)&c->dadjs, (void **)&c->dadjl, (void **)&c->dadjg, (void **)&c->dadjG,
+                     (void **)&c->dadjT};
     for (auto q : ptrs)
       if (*q) { (void)hipFree(*q); *q = nullptr; }
+    c->adj_ready = false;
+    c->adjlists.clear();
+    c->adjcount.clear();
   }
   if (c->agexec) { (void)hipGraphExecDestroy(c->agexec); c->agexec = nullptr; }
   if (c->agraph) { (void)hipGraphDestroy(c->agraph); c->agraph = nullptr; }
@@ -777,6 +796,8 @@ static int upload_amr_tables(nmx_ctx *c) {
     bdedup[rows] = o;
     return o;
   };
+  c->hadjj.clear();
+  c->hadjsend.clear();
   for (size_t i = 0; i < ne; ++i) {
     const Elm &E = M.e[i];
     const int n = E.n;
@@ -851,6 +872,42 @@ static int upload_amr_tables(nmx_ctx *c) {
         if (ns > 1) {
           const int md[5] = {(int)region, q.a0, q.a1, q.b0, q.b1};
           mdesc.insert(mdesc.end(), md, md + 5);
+        }
+        {
+          // the job as the adjoint transposes it: the neighbour's face that
+          // holds the plane, and its face coordinates as a map of (q1, q2)
+          AdjJob aj;
+          std::memset(&aj, 0, sizeof aj);
+          aj.e = (int)i; aj.g = f; aj.n = n; aj.m = m; aj.sw = q.swap;
+          aj.a0 = q.a0; aj.a1 = q.a1; aj.b0 = q.b0; aj.b1 = q.b1; aj.offA = oa; aj.offB = ob;
+          aj.md = ns > 1 ? d[2] : -1;
+          int ijk[3][3];
+          const long pts[3] = {base, base + q.t1, base + q.t2};
+          for (int c3 = 0; c3 < 3; ++c3) {
+            ijk[c3][0] = (int)(pts[c3] % m); ijk[c3][1] = (int)((pts[c3] / m) % m); ijk[c3][2] = (int)(pts[c3] / ((long)m * m));
+          }
+          int nd = -1;
+          for (int dd = 0; dd < 3; ++dd)
+            if (ijk[1][dd] == ijk[0][dd] && ijk[2][dd] == ijk[0][dd]) nd = dd;
+          if (nd < 0 || (ijk[0][nd] != 0 && ijk[0][nd] != m - 1))
+            return fail(NMX_EINVAL, "interpolation plane is not a face");
+          const int sf = 2 * nd + (ijk[0][nd] ? 1 : 0);
+          auto fab = [&](const int *I, int &fa, int &fb) {
+            fa = (nd == 0) ? I[1] : I[0];
+            fb = (nd == 2) ? I[1] : I[2];
+          };
+          int fa0, fb0, fa1, fb1, fa2, fb2;
+          fab(ijk[0], fa0, fb0); fab(ijk[1], fa1, fb1); fab(ijk[2], fa2, fb2);
+          aj.A0 = fa0; aj.B0 = fb0;
+          aj.A1 = fa1 - fa0; aj.B1 = fb1 - fb0;   // d(a, b) / d q1
+          aj.A2 = fa2 - fa0; aj.B2 = fb2 - fb0;   // d(a, b) / d q2
+          // a signed permutation: its inverse is its transpose
+          if (std::abs(aj.A1) + std::abs(aj.B1) != 1 || std::abs(aj.A2) + std::abs(aj.B2) != 1 ||
+              aj.A1 * aj.A2 + aj.B1 * aj.B2 != 0)
+            return fail(NMX_EINVAL, "interpolation plane map is not a signed permutation");
+          c->hadjj.push_back(aj);
+          c->hadjsend.push_back(q.nb);
+          c->hadjsend.push_back(sf);
         }
       }
       ajlen += 4L * n * n * ns;
@@ -2775,11 +2832,112 @@ static AdjArgs adj_base_args(nmx_ctx *c) {
   return a;
 }
 
+// the adjoint tables of the batched path (made once per mirror, at the
+// first adjoint call): per element face its kind, copy partner or the jobs
+// that read its plane; per job its transposed plane in T; G planes per
+// element; element lists per order
+static int adj_amr_prepare(nmx_ctx *c) {
+  if (c->adj_ready) return NMX_OK;
+  const Mesh &M = c->mesh;
+  const int ne = (int)M.e.size();
+  std::vector<long> goff(ne);
+  long glen = 0;
+  for (int e = 0; e < ne; ++e) { goff[e] = glen; glen += 24L * M.e[e].n * M.e[e].n; }
+  std::vector<AdjFace> af(6L * ne);
+  for (int e = 0; e < ne; ++e) {
+    const Elm &E = M.e[e];
+    for (int f = 0; f < 6; ++f) {
+      AdjFace &F = af[6L * e + f];
+      std::memset(&F, 0, sizeof F);
+      if (E.ajmode[f] == 0) continue;
+      if (E.ajmode[f] == 1 || E.ajmode[f] == 2) {
+        const FaceCopy &fc = M.fcopy[6L * e + f];
+        const int nb = E.fnb[f][0];
+        const Elm &N = M.e[nb];
+        F.kind = 1; F.nb = nb; F.g = E.nbf[f][0];
+        F.base = (int)(fc.nbase - N.off); F.s1 = fc.s1; F.s2 = fc.s2;
+        if (N.n != E.n || F.g < 0 || F.g > 5) return fail(NMX_EINVAL, "adjoint: copy face between different orders");
+        const int n = E.n;
+        for (int cb = 0; cb < 2; ++cb)
+          for (int ca = 0; ca < 2; ++ca) {
+            const long p = F.base + (long)ca * (n - 1) * F.s1 + (long)cb * (n - 1) * F.s2;
+            if (p < 0 || p >= (long)n * n * n) return fail(NMX_EINVAL, "adjoint: copy table out of range");
+          }
+        // the partner's point lies on its face g
+        const long p0 = F.base;
+        const int I0[3] = {(int)(p0 % n), (int)((p0 / n) % n), (int)(p0 / ((long)n * n))};
+        if (I0[F.g >> 1] != (F.g & 1) * (n - 1)) return fail(NMX_EINVAL, "adjoint: copy partner face");
+        continue;
+      }
+      F.kind = 2;
+    }
+  }
+  const int nj = (int)c->hadjj.size();
+  std::vector<AdjJob> jobs = c->hadjj;
+  std::vector<std::vector<int>> byplane(6L * ne);
+  long tlen = 0;
+  for (int j = 0; j < nj; ++j) {
+    AdjJob &J = jobs[j];
+    if (tlen > INT32_MAX - 4L * J.m * J.m) return fail(NMX_EUNSUP, "adjoint: transposed planes too large");
+    J.toff = (int)tlen;
+    tlen += 4L * J.m * J.m;
+    const int se = c->hadjsend[2 * j], sf = c->hadjsend[2 * j + 1];
+    if (se < 0 || se >= ne || M.e[se].n != J.m || af[6L * se + sf].kind != 2)
+      return fail(NMX_EINVAL, "adjoint: a job reads a plane that is not an interpolated face");
+    if (J.e < 0 || J.e >= ne || M.e[J.e].n != J.n || J.g < 0 || J.g > 5)
+      return fail(NMX_EINVAL, "adjoint: job receiver out of range");
+    for (int cb = 0; cb < 2; ++cb)
+      for (int ca = 0; ca < 2; ++ca) {
+        const int fa = ca * (J.m - 1), fb = cb * (J.m - 1);
+        const int q1 = J.A1 * (fa - J.A0) + J.B1 * (fb - J.B0), q2 = J.A2 * (fa - J.A0) + J.B2 * (fb - J.B0);
+        if (q1 < 0 || q1 >= J.m || q2 < 0 || q2 >= J.m) return fail(NMX_EINVAL, "adjoint: plane map out of range");
+      }
+    byplane[6L * se + sf].push_back(j);
+  }
+  std::vector<int> sj;
+  for (long k = 0; k < 6L * ne; ++k) {
+    af[k].jfirst = (int)sj.size();
+    af[k].jcount = (int)byplane[k].size();
+    sj.insert(sj.end(), byplane[k].begin(), byplane[k].end());
+  }
+  // lists: all elements, then one per order
+  std::vector<int> lists;
+  c->adjlists.clear();
+  c->adjcount.clear();
+  c->adjlists.push_back({0, 0});
+  for (int e = 0; e < ne; ++e) lists.push_back(e);
+  c->adjcount.push_back(ne);
+  for (int n = 2; n <= kBatchMaxN; ++n) {
+    const int first = (int)lists.size();
+    for (int e = 0; e < ne; ++e)
+      if (M.e[e].n == n) lists.push_back(e);
+    if ((int)lists.size() > first) {
+      c->adjlists.push_back({n, first});
+      c->adjcount.push_back((int)lists.size() - first);
+    }
+  }
+  if (jobs.empty()) jobs.resize(1);
+  if (sj.empty()) sj.assign(1, 0);
+  int r;
+  if ((r = dev_upload(&c->dadjf, af)) || (r = dev_upload(&c->dadjj, jobs)) || (r = dev_upload(&c->dadjs, sj)) ||
+      (r = dev_upload(&c->dadjl, lists)) || (r = dev_upload(&c->dadjg, goff)) ||
+      (r = dev_alloc(&c->dadjG, glen)) || (r = dev_alloc(&c->dadjT, tlen > 0 ? tlen : 1)))
+    return r;
+  c->adjnjobs = nj;
+  c->adj_ready = true;
+  return NMX_OK;
+}
+
 #define NEED_ADJ(c)                                                                      \
   do {                                                                                   \
     NEED_DEV(c);                                                                         \
-    if ((c)->amr || !(c)->dnbf || (c)->fvelm)                                            \
-      return fail(NMX_EUNSUP, "the adjoint needs a conforming DG mesh of one element order"); \
+    if ((c)->fvelm || (c)->anyfv)                                                        \
+      return fail(NMX_EUNSUP, "the adjoint of FV elements is not built");               \
+    if ((c)->amr ? !(c)->anydg : !(c)->dnbf)                                             \
+      return fail(NMX_EUNSUP, "the adjoint needs a DG mesh");                           \
+    if ((c)->amr) {                                                                      \
+      if (int r_ = adj_amr_prepare(c)) return r_;                                        \
+    }                                                                                    \
     bool filt_ = false;                                                                  \
     if (int r_ = filter_prepare((c), filt_)) return r_;                                  \
     if (filt_) return fail(NMX_EUNSUP, "the adjoint of the exponential filter is not built"); \
@@ -2787,7 +2945,33 @@ static AdjArgs adj_base_args(nmx_ctx *c) {
       return fail(NMX_EUNSUP, "the adjoint of the limiter is not built");                \
   } while (0)
 
+// the batched path's adjoint stage: G, the transposed jobs, then one
+// launch per order, all on the context's stream
+static int launch_adj_amr(nmx_ctx *c, const AdjArgs &s) {
+  AdjAmrArgs a;
+  a.lam = s.lam; a.Abase = s.Abase; a.Wbase = s.Wbase; a.Aout = s.Aout; a.Wout = s.Wout; a.R = s.R;
+  a.cA = s.cA; a.cU = s.cU; a.cW = s.cW;
+  a.eoff = c->deoff; a.en = c->den; a.dXbdX = c->dXbdX; a.egeo = c->degeo; a.edom = c->dedom;
+  a.dXdx = c->dXdx; a.Dall = c->dbD; a.xball = c->dbxb; a.wend_inv = c->dbW;
+  a.af = c->dadjf; a.goff = c->dadjg; a.G = c->dadjG; a.jobs = c->dadjj; a.njobs = c->adjnjobs;
+  a.sjobs = c->dadjs; a.mdesc = c->dmdesc; a.Btab = c->dBtab; a.T = c->dadjT;
+  a.numflux = s.numflux;
+  a.list = c->dadjl;
+  a.nlist = c->adjcount[0];
+  hipError_t err = launch_adj_gflux(c->geom, a, c->stream);
+  if (err == hipSuccess) err = launch_adj_jobs(a, c->stream);
+  for (size_t b = 1; b < c->adjlists.size() && err == hipSuccess; ++b) {
+    a.list = c->dadjl + c->adjlists[b].second;
+    a.nlist = c->adjcount[b];
+    const int grid = std::min(a.nlist, 8 * c->ncu);
+    err = launch_dg_adjoint_amr(c->adjlists[b].first, c->geom, a, grid, c->stream);
+  }
+  if (err != hipSuccess) return fail(NMX_EHIP, std::string("adjoint launch: ") + hipGetErrorString(err));
+  return NMX_OK;
+}
+
 static int launch_adj(nmx_ctx *c, const AdjArgs &a) {
+  if (c->amr) return launch_adj_amr(c, a);
   int nel = a.e1 - a.e0;
   int grid = a.xcd_map ? 8 * ((nel + 7) / 8) : nel;
   hipError_t err = launch_dg_adjoint(c->np1, c->geom, a, grid, c->stream);

@@ -214,9 +214,9 @@ def test_trajectory_loss_gradient():
     assert np.abs(host(g) - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
-def test_adjoint_refuses_nonconforming_meshes():
+def test_adjoint_refuses_fv_elements():
     pars = {"amr_mesh_type": "Shell", "amr_luni": 1, "amr_n0": 4, "amr_n1": 4, "amr_n2": 4,
-            "amr_Shell_rin": 2.0, "amr_Shell_rout": 20.0, "nmx_path": "batched"}
+            "amr_Shell_rin": 2.0, "amr_Shell_rout": 20.0, "amr_uniform_p": "0", "amr_fv_p": "0"}
     m = nm.Mesh(pars, device=0)
     nm.amr_setup_mesh(m)
     with pytest.raises(nm.NmxError):
