@@ -121,6 +121,10 @@ def cpu_baseline(x_unused, budget_s=12.0):
                       "with the oracle's OpenMP C restatement" % (steps, steps * 4 * om.nnodes)}
 
 
+# extra library parameters of the C4 meshes (--amr-par name=value)
+C4_EXTRA = {}
+
+
 def c4_mesh(nm, path):
     """Shell r in [2,20] refined to level 3, then in nested spheres of radius
     6 and 3 (levels 4 and 5): 19,200 elements; per-element N drawn uniformly
@@ -128,6 +132,7 @@ def c4_mesh(nm, path):
     p = c2_params()
     p.update({"amr_n0": 5, "amr_n1": 5, "amr_n2": 5, "amr_hrefine_sphere_radius": 3,
               "amr_hrefine_sphere_levels": 2, "nmx_path": path})
+    p.update(C4_EXTRA)
     m = nm.Mesh(p, device=0)
     nm.amr_setup_mesh(m)
     rng = np.random.default_rng(2026)
@@ -519,7 +524,12 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="instead of the workload, run 10 device copies of MIB MiB with "
                          "torch (counter calibration for rocprofv3 --pmc) and exit")
+    ap.add_argument("--amr-par", action="append", default=[], metavar="NAME=VALUE",
+                    help="a library parameter of the C4 meshes (repeatable)")
     args = ap.parse_args()
+    for kv in args.amr_par:
+        k, v = kv.split("=", 1)
+        C4_EXTRA[k] = v
     if args.copy_calib:
         import torch
         n = args.copy_calib * 1024 * 1024 // 8
