@@ -121,8 +121,10 @@ def cpu_baseline(x_unused, budget_s=12.0):
                       "with the oracle's OpenMP C restatement" % (steps, steps * 4 * om.nnodes)}
 
 
-# extra library parameters of the C4 meshes (--amr-par name=value)
+# extra library parameters of the C4 meshes (--amr-par name=value) and of
+# the FV grid (--fv-par name=value)
 C4_EXTRA = {}
+FV_EXTRA = {}
 
 
 def c4_mesh(nm, path):
@@ -271,7 +273,9 @@ def fv_leg(steps=50, warmup=8, N=256, cpu=True):
     blobs, per-cell velocities U[-0.3, 0.3], p = 1; sspRK3 at dt = 0.25."""
     from nmesh_amd.fv import FvGrid, blob_initial_data
     P = blob_initial_data(N)
-    g = FvGrid({"fv_n": N, "evolve_method": "sspRK3"}, device=0)
+    pars = {"fv_n": N, "evolve_method": "sspRK3"}
+    pars.update(FV_EXTRA)
+    g = FvGrid(pars, device=0)
     g.setup()
     g.upload_prim(P)
     g.evolve(warmup)                 # includes the graph capture
@@ -526,10 +530,15 @@ def main():
                          "torch (counter calibration for rocprofv3 --pmc) and exit")
     ap.add_argument("--amr-par", action="append", default=[], metavar="NAME=VALUE",
                     help="a library parameter of the C4 meshes (repeatable)")
+    ap.add_argument("--fv-par", action="append", default=[], metavar="NAME=VALUE",
+                    help="a library parameter of the FV grid (repeatable)")
     args = ap.parse_args()
     for kv in args.amr_par:
         k, v = kv.split("=", 1)
         C4_EXTRA[k] = v
+    for kv in args.fv_par:
+        k, v = kv.split("=", 1)
+        FV_EXTRA[k] = v
     if args.copy_calib:
         import torch
         n = args.copy_calib * 1024 * 1024 // 8
