@@ -156,6 +156,14 @@ int nmx_mesh_padapt(nmx_ctx *ctx, const int *n, int nelem);
  * nmx_trouble set, nmx_evolve switches by the trouble scores itself.
  * Meshes mixing both kinds run on the batched path. */
 int nmx_mesh_switch_fv(nmx_ctx *ctx, const int *to_fv, int nelem);
+/* The mesh's dt computed on the device from the mirror's node coordinates
+ * (update_mesh_myelms_elm_eid_dt, storage.c:1305-1386, with find_hmin,
+ * utilities.c:1028-1128): per element the smallest corner-node distance
+ * along each direction (from the space diagonal / sqrt 3), times
+ * uniform_dtfac on FV elements whose minimum is such a pair and dtfac
+ * otherwise, minimised with wavefront shuffles.  A query: the mesh's dt
+ * (nmx_get_dt, set up on the host by the same rule) is not changed. */
+int nmx_mesh_dt_device(nmx_ctx *ctx, double *dt);
 /* fv[e] = 1 for an FV element on uniform points, 0 for DG on LGL points
  * (node->dat->use_fv, refine_set_use_fv_true_iff_P_UNIFORM, trouble.c:246) */
 int nmx_mesh_fv_flags(nmx_ctx *ctx, int *fv, int nelem);
@@ -358,6 +366,13 @@ int nmx_fv_upload(nmx_fv *fv, int which, const double *u);
 int nmx_fv_download(nmx_fv *fv, int which, double *u);
 /* NMX_FV_RHS = RHS(conserved state), with c2p first */
 int nmx_fv_setrhs(nmx_fv *fv);
+/* the CFL reduction: c2p of the conserved state, then the largest
+ * characteristic speed |lambda+-| (the SR-Euler eigenvalues of the LLF
+ * flux) over every cell and direction, reduced on the device with
+ * wavefront shuffles; a caller's CFL dt is fv_dtfac dx / *amax.  No
+ * counterpart in the reference (its SR projects are absent); dt itself stays
+ * the dt parameter's. */
+int nmx_fv_max_speed(nmx_fv *fv, double *amax);
 int nmx_fv_evolve(nmx_fv *fv, int nsteps);
 int nmx_fv_last_evolve_ms(nmx_fv *fv, double *ms);
 /* number of blocks whose c2p Newton solve failed since the last call */

@@ -35,6 +35,7 @@ def _proto(lib):
         "nmx_mesh_padapt": (i, [vp, vp, i]),
         "nmx_mesh_switch_fv": (i, [vp, vp, i]),
         "nmx_mesh_fv_flags": (i, [vp, vp, i]),
+        "nmx_mesh_dt_device": (i, [vp, P(d)]),
         "nmx_mesh_nelem": (i, [vp, P(i)]),
         "nmx_mesh_nnodes": (i, [vp, P(l)]),
         "nmx_mesh_elm_info": (i, [vp, i, vp]),
@@ -86,6 +87,7 @@ def _proto(lib):
         "nmx_fv_upload": (i, [vp, i, vp]),
         "nmx_fv_download": (i, [vp, i, vp]),
         "nmx_fv_setrhs": (i, [vp]),
+        "nmx_fv_max_speed": (i, [vp, P(d)]),
         "nmx_fv_evolve": (i, [vp, i]),
         "nmx_fv_last_evolve_ms": (i, [vp, P(d)]),
         "nmx_fv_status": (i, [vp, P(l)]),

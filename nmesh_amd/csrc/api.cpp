@@ -1470,6 +1470,33 @@ extern "C" int nmx_mesh_elm_info(nmx_ctx *c, int e, long *info) {
   return NMX_OK;
 }
 
+extern "C" int nmx_mesh_dt_device(nmx_ctx *c, double *dt) {
+  NEED_DEV(c);
+  if (!dt) return fail(NMX_EINVAL, "NULL argument");
+  const int ne = (int)c->mesh.e.size();
+  std::vector<int> fv(ne);
+  for (int e = 0; e < ne; ++e) fv[e] = c->mesh.e[e].fv != 0;
+  const int grid = dt_grid(ne);
+  int *dfv = nullptr;
+  double *dpart = nullptr;
+  int r;
+  if ((r = dev_upload(&dfv, fv)) || (r = dev_alloc(&dpart, grid + 1))) {
+    (void)hipFree(dfv);
+    return r;
+  }
+  DtArgs a;
+  a.xyz = c->dxyz; a.eoff = c->deoff; a.en = c->den; a.efv = dfv;
+  a.dtfac = Pd(c, "dtfac"); a.uniform_dtfac = Pd(c, "uniform_dtfac");
+  a.part = dpart; a.out = dpart + grid; a.nelem = ne;
+  hipError_t err = launch_dt(a, grid, c->stream);
+  if (err == hipSuccess) err = hipMemcpyAsync(dt, dpart + grid, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+  (void)hipFree(dfv);
+  (void)hipFree(dpart);
+  if (err != hipSuccess) return fail(NMX_EHIP, std::string("dt reduction: ") + hipGetErrorString(err));
+  return NMX_OK;
+}
+
 extern "C" int nmx_mesh_fv_flags(nmx_ctx *c, int *fv, int nelem) {
   NEED_MESH(c);
   if (!fv) return fail(NMX_EINVAL, "NULL argument");

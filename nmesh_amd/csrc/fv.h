@@ -40,6 +40,10 @@ struct FvStageArgs {
 };
 
 hipError_t launch_fv_c2p(const FvC2pArgs &a, hipStream_t s);
+// the largest characteristic speed |lambda+-| over all cells and directions
+// of primitives P: per-workgroup maxima in part[grid], the grid's in out[0]
+int fv_speed_grid(const FvGrid &g);
+hipError_t launch_fv_max_speed(const FvGrid &g, const double *P, double *part, double *out, hipStream_t s);
 hipError_t launch_fv_p2c(const FvGrid &g, const double *P, double *U, hipStream_t s);
 hipError_t launch_fv_stage(const FvStageArgs &a, hipStream_t s);
 // flat [v][cell] (cell = i + N (j + N k), N = B nbx) <-> block layout

@@ -250,6 +250,28 @@ static int fv_launch(nmx_fv *f, FvStageArgs a) {
   return NMX_OK;
 }
 
+extern "C" int nmx_fv_max_speed(nmx_fv *f, double *amax) {
+  NEED_GRID(f);
+  if (!amax) return fail(NMX_EINVAL, "NULL argument");
+  FvC2pArgs c;
+  c.g = f->g;
+  c.U = f->dU;
+  c.P = f->dP;
+  c.bad = f->dbad;
+  c.tol = std::atof(P(f, "fv_c2p_tol").c_str());
+  hipError_t e = launch_fv_c2p(c, f->stream);
+  if (e != hipSuccess) return fail(NMX_EHIP, std::string("c2p: ") + hipGetErrorString(e));
+  const int grid = fv_speed_grid(f->g);
+  double *part = nullptr;
+  if (int r = dalloc(&part, (size_t)grid + 1)) return r;
+  e = launch_fv_max_speed(f->g, f->dP, part, part + grid, f->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(amax, part + grid, sizeof(double), hipMemcpyDeviceToHost, f->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(f->stream);
+  (void)hipFree(part);
+  if (e != hipSuccess) return fail(NMX_EHIP, std::string("speed reduction: ") + hipGetErrorString(e));
+  return NMX_OK;
+}
+
 extern "C" int nmx_fv_setrhs(nmx_fv *f) {
   NEED_GRID(f);
   FvStageArgs a;

@@ -403,6 +403,82 @@ __global__ __launch_bounds__(64, 2) void fv_stage_kernel(FvStageArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// CFL: the largest |lambda+-| over cells and directions (the eigenvalues of
+// sr_flux), reduced with lane shuffles, then per workgroup and over the grid
+// ---------------------------------------------------------------------------
+constexpr int kSpeedThreads = 256;
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+__device__ __forceinline__ double sr_speed(double gam, double g1, const double P[5], int d) {
+  const double rho = P[0], p = P[4];
+  const double v2 = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
+  const double ph = p / rho;
+  const double h = 1. + g1 * ph;
+  const double cs2 = gam * ph / h;
+  const double cs = sqrt(cs2);
+  const double vd = P[1 + d];
+  const double oden = 1. / (1. - v2 * cs2);
+  const double root = cs * sqrt((1. - v2) * (1. - v2 * cs2 - vd * vd * (1. - cs2)));
+  const double lm = (vd * (1. - cs2) - root) * oden;
+  const double lp = (vd * (1. - cs2) + root) * oden;
+  return fmax(fabs(lm), fabs(lp));
+}
+
+__global__ __launch_bounds__(kSpeedThreads) void fv_speed_kernel(FvGrid g, const double *P, double *part) {
+  __shared__ double s_max[kSpeedThreads / 64];
+  const long ncell = (long)g.nbx * g.nbx * g.nbx * kFvB3;
+  const double gam = g.gam, g1 = gam / (gam - 1.);
+  double m = 0.0;
+  for (long c = (long)blockIdx.x * kSpeedThreads + threadIdx.x; c < ncell; c += (long)gridDim.x * kSpeedThreads) {
+    const long b = c / kFvB3, q = c % kFvB3;
+    const double *pc = P + b * 5 * kFvB3 + q;
+    const double Pc[5] = {pc[0], pc[kFvB3], pc[2 * kFvB3], pc[3 * kFvB3], pc[4 * kFvB3]};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) m = fmax(m, sr_speed(gam, g1, Pc, d));
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = s_max[0];
+    for (int w = 1; w < kSpeedThreads / 64; ++w) r = fmax(r, s_max[w]);
+    part[blockIdx.x] = r;
+  }
+}
+
+__global__ __launch_bounds__(kSpeedThreads) void fv_speed_max_kernel(const double *part, int n, double *out) {
+  __shared__ double s_max[kSpeedThreads / 64];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += kSpeedThreads) m = fmax(m, part[i]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = s_max[0];
+    for (int w = 1; w < kSpeedThreads / 64; ++w) r = fmax(r, s_max[w]);
+    out[0] = r;
+  }
+}
+
+int fv_speed_grid(const FvGrid &g) {
+  const long ncell = (long)g.nbx * g.nbx * g.nbx * kFvB3;
+  const long gr = (ncell + kSpeedThreads - 1) / kSpeedThreads;
+  return gr < 4096 ? (int)gr : 4096;
+}
+
+hipError_t launch_fv_max_speed(const FvGrid &g, const double *P, double *part, double *out, hipStream_t s) {
+  const int grid = fv_speed_grid(g);
+  hipLaunchKernelGGL(fv_speed_kernel, dim3(grid), dim3(kSpeedThreads), 0, s, g, P, part);
+  hipLaunchKernelGGL(fv_speed_max_kernel, dim3(1), dim3(kSpeedThreads), 0, s, part, grid, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_fv_stage(const FvStageArgs &a, hipStream_t s) {
   const int ntiles = a.g.nbx * a.g.nbx * a.g.nbx * 8;
   const int grid = a.xcd_map ? 8 * ((ntiles + 7) / 8) : ntiles;

@@ -65,6 +65,20 @@ struct AdjArgs {
   int numflux = 0, xcd_map = 0, e0 = 0, e1 = 0;
 };
 
+// the mesh's dt from the device's node coordinates (update_elm_eid_dt with
+// find_hmin): per-workgroup minima in part[grid], the mesh's in out[0]
+struct DtArgs {
+  const double *xyz = nullptr;   // element blocks [e][3][np]
+  const long *eoff = nullptr;
+  const int *en = nullptr;
+  const int *efv = nullptr;      // [e]: 1 FV element
+  double dtfac = 0, uniform_dtfac = 0;
+  double *part = nullptr, *out = nullptr;
+  int nelem = 0;
+};
+int dt_grid(int nelem);
+hipError_t launch_dt(const DtArgs &a, int grid, hipStream_t s);
+
 // geom: 0 Cartesian, 1 stored dXdx, 2 CubedShell recomputed; nt from dg_stage_threads
 hipError_t launch_dg_stage(int np, int geom, const StageArgs &a, int grid, int nt, hipStream_t s);
 int dg_stage_threads(int np, int div);   // div: nodes per thread 1, 2 or 4

@@ -151,6 +151,12 @@ class Mesh:
         f = np.ascontiguousarray(to_fv, dtype=np.int32)
         check(lib().nmx_mesh_switch_fv(self._ctx, _p(f), int(f.size)))
 
+    def dt_device(self):
+        """the mesh's dt from the device's node coordinates (a query)"""
+        d = C.c_double()
+        check(lib().nmx_mesh_dt_device(self._ctx, C.byref(d)))
+        return d.value
+
     def fv_flags(self):
         """1 for FV elements on uniform points, 0 for DG (node->dat->use_fv)"""
         f = np.zeros(self.nelem, dtype=np.int32)

@@ -98,6 +98,13 @@ class FvGrid:
         check(lib().nmx_fv_setrhs(self._h))
         return self.download(RHS)
 
+    def max_speed(self):
+        """largest characteristic speed over cells and directions (the CFL
+        reduction, c2p first)"""
+        a = C.c_double()
+        check(lib().nmx_fv_max_speed(self._h, C.byref(a)))
+        return a.value
+
     def evolve(self, nsteps):
         check(lib().nmx_fv_evolve(self._h, int(nsteps)))
 
