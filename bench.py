@@ -348,6 +348,38 @@ def fvelm_leg(nm, steps=20):
            "achieved_GBps": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
            "kernel": "fvelm_stage_kernel<8,5,*>"}
     del m
+    out["shell"] = fvelm_shell_leg(nm, steps)
+    return out
+
+
+def fvelm_shell_leg(nm, steps=20):
+    """The same on cubed-sphere elements: the shell r in [2,20] at level 3
+    (6x8^3 elements), all FV, n = 8 uniform points per direction (1,572,864
+    cells), WENOm5_2 with coordinates_midpoint_data = yes (the metric at the
+    midpoints, as nmesh's curved-mesh FV parfiles set it), upwind, RK4."""
+    p = {"amr_mesh_type": "Shell", "amr_Shell_rin": 2.0, "amr_Shell_rout": 20.0, "amr_luni": 3,
+         "amr_n0": 8, "amr_n1": 8, "amr_n2": 8, "amr_uniform_p": "0 1 2 3 4 5",
+         "amr_fv_p": "0 1 2 3 4 5", "fv_rec": "WENOm5_2", "coordinates_midpoint_data": "yes",
+         "scalarwave1_k": "0 0 0", "evolve_method": "RK4"}
+    m = nm.Mesh(p, device=0)
+    nm.amr_setup_mesh(m)
+    m.upload(shell_initial_data(m.coords()))
+    nm.evolve_myln(m, 2)
+    m.synchronize()
+    t0 = time.perf_counter()
+    nm.evolve_myln(m, steps)
+    m.synchronize()
+    wall = time.perf_counter() - t0
+    dev_s = m.last_evolve_ms() * 1e-3
+    nu = 4.0 * steps * m.nnodes
+    gbs = MIN_BYTES_PER_NU * nu / dev_s / 1e9
+    out = {"workload": "scalarwave1 on FV elements of the cubed-sphere shell (fv_divf, WENOm5_2, "
+                       "midpoint metric, upwind), 6x8^3 elements of 8^3 uniform points (%d cells), RK4" % m.nnodes,
+           "value": nu / wall, "unit": "cell-updates/s", "steps": steps,
+           "ms_per_step": 1e3 * wall / steps, "device_us_per_stage": dev_s / (4 * steps) * 1e6,
+           "bytes_per_cell_update": MIN_BYTES_PER_NU, "achieved_GBps": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
+           "kernel": "fvelm_stage_kernel<8,5,*,2,*> (shell)"}
+    del m
     return out
 
 
