@@ -166,13 +166,16 @@ __global__ __launch_bounds__(256) void fv_c2p_kernel(FvC2pArgs a) {
     double p = a.P[base + 4 * kFvB3];
     if (!(p > pmin)) p = pmin + 1e-10 * E;
     int it;
+    // (the reciprocals and 1/sqrt by rcp_nr / rsqrt_nr: 1/W = (1 - v^2) W)
     for (it = 1; it <= kC2pMaxIt; ++it) {
       const double Ep = E + p;
-      const double v2 = S2 / (Ep * Ep);
-      const double W = 1. / sqrt(1. - v2);
-      const double f = (gam - 1.) * (E - S2 / Ep - D / W) - p;
-      const double df = (gam - 1.) * v2 * (1. - D * W / Ep) - 1.;
-      double pn = p - f / df;
+      const double iEp = rcp_nr(Ep);
+      const double v2 = S2 * iEp * iEp;
+      const double W = rsqrt_nr(1. - v2);
+      const double iW = (1. - v2) * W;
+      const double f = (gam - 1.) * (E - S2 * iEp - D * iW) - p;
+      const double df = (gam - 1.) * v2 * (1. - D * W * iEp) - 1.;
+      double pn = p - f * rcp_nr(df);
       if (pn <= pmin) pn = 0.5 * (p + pmin);
       const double dp = fabs(pn - p);
       p = pn;
@@ -180,12 +183,13 @@ __global__ __launch_bounds__(256) void fv_c2p_kernel(FvC2pArgs a) {
     }
     if (it > kC2pMaxIt) a.bad[b] = 1;
     const double Ep = E + p;
-    const double v2 = S2 / (Ep * Ep);
-    const double W = 1. / sqrt(1. - v2);
-    a.P[base] = D / W;
-    a.P[base + kFvB3] = Sx / Ep;
-    a.P[base + 2 * kFvB3] = Sy / Ep;
-    a.P[base + 3 * kFvB3] = Sz / Ep;
+    const double iEp = rcp_nr(Ep);
+    const double v2 = S2 * iEp * iEp;
+    const double iW = (1. - v2) * rsqrt_nr(1. - v2);
+    a.P[base] = D * iW;
+    a.P[base + kFvB3] = Sx * iEp;
+    a.P[base + 2 * kFvB3] = Sy * iEp;
+    a.P[base + 3 * kFvB3] = Sz * iEp;
     a.P[base + 4 * kFvB3] = p;
   }
 }
