@@ -2253,8 +2253,15 @@ static int trouble_switch(nmx_ctx *c, const std::vector<int> &to_fv, int carry) 
     if (!(carry & flag)) continue;
     double *src = flag == CARRY_A ? c->dA : flag == CARRY_W0 ? c->dW0 : c->dW1;
     double *tmp = nullptr;
-    if (int r = dev_alloc(&tmp, c->nstate)) return r;
-    HIPCHK(hipMemcpyAsync(tmp, src, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream));
+    int r = dev_alloc(&tmp, c->nstate);
+    if (!r && hipMemcpyAsync(tmp, src, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+      r = fail(NMX_EHIP, "trouble switch: hold a register");
+    if (r) {
+      (void)hipFree(tmp);
+      for (const Held &h : held) (void)hipFree(h.tmp);
+      c->mesh = old;
+      return r;
+    }
     held.push_back({flag, tmp});
   }
   const std::vector<int> keep = c->hscore;
@@ -2402,16 +2409,19 @@ static int redo_rhs(nmx_ctx *c, const TStage &S, int s) {
     if ((r = launch(c, b))) break;
     r = trouble_fix(c, held[4], c->dR, nullptr, nullptr, 0., 1, dmask);
   }
-  if (!r) {
-    c->mesh = Mall;
-    r = mirror_again(c);
-  }
-  if (!r) {
+  // back to the switched mesh with its registers, also after a failure
+  // (the first error is the one returned)
+  const bool held_ok = held[0] && held[1] && held[2] && held[3] && held[4];
+  c->mesh = Mall;
+  int r2 = mirror_again(c);
+  if (!r2 && held_ok) {
     double *dst[5] = {c->dU, c->dA, c->dW0, c->dW1, c->dR};
     for (int q = 0; q < 5; ++q)
-      if (hipMemcpyAsync(dst[q], held[q], ns * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) r = fail(NMX_EHIP, "redo: restore registers");
-    if (hipStreamSynchronize(c->stream) != hipSuccess) r = fail(NMX_EHIP, "redo: sync");
+      if (hipMemcpyAsync(dst[q], held[q], ns * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+        r2 = fail(NMX_EHIP, "redo: restore registers");
+    if (hipStreamSynchronize(c->stream) != hipSuccess) r2 = fail(NMX_EHIP, "redo: sync");
   }
+  if (!r) r = r2;
   for (double *h : held) (void)hipFree(h);
   (void)hipFree(dmask);
   (void)hipFree(win0);
