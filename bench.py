@@ -517,9 +517,25 @@ def adjoint_leg(nm, steps=256, every=16, reps=3):
     out["shell_forward_ms_per_step"] = f2 / 32
     out["shell_adjoint_ms_per_step"] = b2 / 32
     out["shell_adjoint_vs_forward"] = b2 / f2
+    del m
+    # and on the C4 mixed-order AMR shell (the batched path's adjoint)
+    m = c4_mesh(nm, "batched")
+    uc = shell_initial_data(m.coords())
+    m.upload(uc)
+    lam = torch.from_numpy(rng.standard_normal(uc.shape)).cuda()
+    nm.evolve_myln(m, 8)
+    adj.adjoint_evolve(m, lam, 8)
+    nm.evolve_myln(m, 16)
+    f3 = m.last_evolve_ms()
+    adj.adjoint_evolve(m, lam, 16)
+    b3 = m.last_adjoint_ms()
+    out["c4_forward_ms_per_step"] = f3 / 16
+    out["c4_adjoint_ms_per_step"] = b3 / 16
+    out["c4_adjoint_vs_forward"] = b3 / f3
+    del m
     out["workload"] = ("C5: config-1 box (64 elements, N=4, 8000 nodes), upwind, zero exterior, "
                        "%d RK4 steps, checkpoints every %d steps; plus 32 RK4 steps of the "
-                       "3072-element N=7 shell" % (steps, every))
+                       "3072-element N=7 shell and 16 of the C4 mixed-order AMR shell" % (steps, every))
     return out
 
 
