@@ -91,8 +91,10 @@ int nmx_synchronize(nmx_ctx *ctx);
  * (<= amr_Shell_rin); amr_nmax bounds amr_n0; amr_fv_if_uniform (yes: FV
  * exactly in amr_uniform_p, mesh.c:86-91); accepted without effect on this
  * path: dg_outerBC_flux_fac (3 numbers; its use in scalarwave1 is commented
- * out, scalarwave1.c:202), evolve_redo_troubled (scalarwave1 has no TROUBLE
- * callback), evolve_u_surfaces (one device: no surface exchange),
+ * out, scalarwave1.c:202), evolve_u_surfaces (one device: no surface
+ * exchange); the troubled-element machinery below: nmx_trouble
+ * (none|RDMP|hook), evolve_redo_troubled (substep|step), nmx_trouble_delta0,
+ * nmx_trouble_eps,
  * plus library knobs
  * prefixed nmx_ (nmx_path:
  * plain|batched|perelement, nmx_graph: yes|no, nmx_graph_steps, nmx_geometry:

@@ -46,34 +46,31 @@ __device__ __forceinline__ void plane_wave(const double k[3], double om, double 
 }
 
 // dX/dx of a CubedShell (constant radii s0, s1, centre 0) at patch point X
-// (lam, A, B) (coordtrans_CubedSphere.c:259-509 with dsigma = 0).  Written
-// with compile-time component indices only: runtime-indexed private arrays
-// would live in scratch memory.
+// (lam, A, B) (coordtrans_CubedSphere.c:259-509 with dsigma = 0).  With
+// s = sqrt(1 + A^2 + B^2), the radius-like coordinate r = (pm/s)(s0 + (s1 -
+// s0) lam) along the patch axis dir and x_dir = r, x_ia = A r, x_ib = B r,
+// the reference's rows reduce to
+//   dlam/dx = pm / ((s1 - s0) s) (1, A, B),
+//   dA/dx   = (pm s / R) (-A, 1, 0),   dB/dx = (pm s / R) (-B, 0, 1),
+// R = s0 + (s1 - s0) lam, components ordered (dir, ia, ib): two divisions
+// and one square root per point instead of nine divisions.  Written with
+// compile-time component indices only: runtime-indexed private arrays would
+// live in scratch memory.
 __device__ __forceinline__ void shell_dXdx(int dom, double s0, double s1, const double X[3],
                                            double M[3][3]) {
   const double lam = X[0], A = X[1], B = X[2];
   const double pm = (dom & 1) ? 1.0 : -1.0;
   const double s = sqrt(1.0 + A * A + B * B);
-  const double a0 = pm * s0 / s, a1 = pm * s1 / s;
-  const double r = (a1 - a0) * lam + a0;
+  const double c0 = pm / ((s1 - s0) * s);
+  const double c1 = pm * s / (s0 + (s1 - s0) * lam);
   const int dir = dom >> 1;
   const int ia = (dir == 1) ? 0 : 1;      // axis scaled by A
-  const double ra = A * r, rb = B * r;
-  double rv[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) rv[d] = (d == dir) ? r : ((d == ia) ? ra : rb);
-  const double rc2 = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
-  const double ooda = 1.0 / (a1 - a0);
-  const double oor = 1.0 / r, rr2 = r * r;
-  const double mA = -ra / rr2, mB = -rb / rr2;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const bool isr = (d == dir), isa = (d == ia);
-    const double md = isr ? (oor - r / rc2) : (-rv[d] / rc2);
-    const double da0 = a0 * md, da1 = a1 * md;
-    M[0][d] = ((isr ? 1.0 : 0.0) - da0 - (da1 - da0) * lam) * ooda;
-    M[1][d] = isr ? mA : (isa ? oor : 0.0);
-    M[2][d] = isr ? mB : ((!isa) ? oor : 0.0);
+    M[0][d] = isr ? c0 : (isa ? A * c0 : B * c0);
+    M[1][d] = isr ? -A * c1 : (isa ? c1 : 0.0);
+    M[2][d] = isr ? -B * c1 : (isa ? 0.0 : c1);
   }
 }
 
@@ -123,8 +120,8 @@ __device__ __forceinline__ void face_normal(const double M[3][3], const double *
     const double r1 = (dir == 0) ? M[0][1] : (dir == 1) ? M[1][1] : M[2][1];
     const double r2 = (dir == 0) ? M[0][2] : (dir == 1) ? M[1][2] : M[2][2];
     const double s = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
-    const double smag = sig * s;
-    nrm[0] = r0 / smag; nrm[1] = r1 / smag; nrm[2] = r2 / smag;
+    const double ism = 1.0 / (sig * s);
+    nrm[0] = r0 * ism; nrm[1] = r1 * ism; nrm[2] = r2 * ism;
     scale = dXbdX[dir] * s * wend_inv;
   }
 }
@@ -223,8 +220,8 @@ __device__ __forceinline__ void node_update(const NodeOut &o, const ElmGeo &g,
       if (CURV) {
         double s2 = M[dir][0] * M[dir][0] + M[dir][1] * M[dir][1] + M[dir][2] * M[dir][2];
         double s = sqrt(s2);
-        double smag = sig * s;
-        nrm[0] = M[dir][0] / smag; nrm[1] = M[dir][1] / smag; nrm[2] = M[dir][2] / smag;
+        const double ism = 1.0 / (sig * s);
+        nrm[0] = M[dir][0] * ism; nrm[1] = M[dir][1] * ism; nrm[2] = M[dir][2] * ism;
         scale = g.dXbdX[dir] * s * o.wend_inv;
       } else {
         nrm[0] = nrm[1] = nrm[2] = 0.0;
