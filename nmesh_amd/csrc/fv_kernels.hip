@@ -21,6 +21,7 @@
 // Living Rev. Relativ. 6 (2003) 7); the reference has no SR-Euler system.
 #include <hip/hip_runtime.h>
 
+#include "fastmath.h"
 #include "fv.h"
 
 namespace nmx {
@@ -50,25 +51,10 @@ __device__ __forceinline__ double beta_r2(double x, double y, double z) {
   return (13. / 12.) * s1 * s1 + 0.25 * s2 * s2;
 }
 
-// Nonlinear combination with one division: the weights g_k / (beta_k + eps)^2
-// normalised equal  g_k prod_{j != k} (beta_j + eps)^2  normalised (for the
-// data here every product stays between 1e-240 and 1e+60).
-// 1 / b from the hardware's reciprocal estimate and two Newton steps (each
-// doubles the correct bits; within about an ulp for normal b, and the
-// operands here stay between 1e-240 and 1e+60), and 1 / sqrt(x) likewise:
-// fewer instructions than an IEEE division (scaling, two Newton steps, a
-// residual correction, fix-up) on the kernel's VALU-bound path
-__device__ __forceinline__ double rcp_nr(double b) {
-  double r = __builtin_amdgcn_rcp(b);
-  r = fma(fma(-b, r, 1.0), r, r);
-  return fma(fma(-b, r, 1.0), r, r);
-}
-__device__ __forceinline__ double rsqrt_nr(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
-  return fma(0.5 * y, fma(-x * y, y, 1.0), y);
-}
-
+// Nonlinear combination with one reciprocal: the weights g_k / (beta_k +
+// eps)^2 normalised equal  g_k prod_{j != k} (beta_j + eps)^2  normalised
+// (for the data here every product stays between 1e-240 and 1e+60, so the
+// sum is a normal number for rcp_nr, fastmath.h).
 __device__ __forceinline__ double weno5_combine(double g0, double g1, double g2, double b0, double b1,
                                                 double b2, double q0, double q1, double q2) {
   const double e0 = b0 + kWeno5Eps, e1 = b1 + kWeno5Eps, e2 = b2 + kWeno5Eps;

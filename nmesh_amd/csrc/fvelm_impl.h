@@ -26,6 +26,7 @@
 
 #include "amr.h"
 #include "dg_common.h"
+#include "fastmath.h"
 #include "kernels.h"
 
 namespace nmx {
@@ -36,22 +37,24 @@ constexpr double kWeno3Eps = 1e-6;    // WENO3_epsilon (rec1d.c:35), u_scale = 1
 constexpr double kWeno5EpsE = 1e-40;  // WENO5_epsilon (rec1d.c:1193)
 
 // The reconstructions below normalise their nonlinear weights with one
-// division: a weight d_k / b_k^p over their sum equals (d_k prod_{j!=k}
-// b_j^p) over the sum of those products.  Only rounding differs from the
-// reference's per-weight divisions.
+// reciprocal: a weight d_k / b_k^p over their sum equals (d_k prod_{j!=k}
+// b_j^p) over the sum of those products, whose reciprocal is rcp_nr's
+// (fastmath.h; the smoothness indicators are at least the epsilon, 1e-6 or
+// 1e-40, so the products stay normal numbers).  Only rounding differs from
+// the reference's per-weight divisions.
 
 // WENOm3 (rec1d_p_WENO3 with the uniform WENOm3 weights)
 __device__ __forceinline__ double wm3_p(double um1, double u0, double up1) {
   const double d1 = u0 - um1, d2 = up1 - u0;
   const double b1 = d1 * d1 + kWeno3Eps, b2 = d2 * d2 + kWeno3Eps;
   const double p1 = b2 * b2, p2 = 2. * (b1 * b1);   // weights 1/b1^2, 2/b2^2
-  return (p1 * (-0.5 * um1 + 1.5 * u0) + p2 * (0.5 * u0 + 0.5 * up1)) / (p1 + p2);
+  return (p1 * (-0.5 * um1 + 1.5 * u0) + p2 * (0.5 * u0 + 0.5 * up1)) * rcp_nr(p1 + p2);
 }
 __device__ __forceinline__ double wm3_m(double u0, double up1, double up2) {
   const double d1 = up1 - u0, d2 = up2 - up1;
   const double b1 = d1 * d1 + kWeno3Eps, b2 = d2 * d2 + kWeno3Eps;
   const double p1 = 2. * (b2 * b2), p2 = b1 * b1;   // weights 2/b1^2, 1/b2^2
-  return (p1 * (0.5 * u0 + 0.5 * up1) + p2 * (1.5 * up1 - 0.5 * up2)) / (p1 + p2);
+  return (p1 * (0.5 * u0 + 0.5 * up1) + p2 * (1.5 * up1 - 0.5 * up2)) * rcp_nr(p1 + p2);
 }
 
 // WENO5 (rec1d_p/m_WENO5 with uniform weights): ideal weights (1, 6, 3)
@@ -69,7 +72,7 @@ __device__ __forceinline__ double w5_p(double a, double b, double c, double d, d
   const double q0 = (2. * o6) * a + (-7. * o6) * b + (11. * o6) * c;
   const double q1 = (-o6) * b + (5. * o6) * c + (2. * o6) * d;
   const double q2 = (2. * o6) * c + (5. * o6) * d + (-o6) * e;
-  return (p0 * q0 + p1 * q1 + p2 * q2) / (p0 + p1 + p2);
+  return (p0 * q0 + p1 * q1 + p2 * q2) * rcp_nr(p0 + p1 + p2);
 }
 __device__ __forceinline__ double w5_m(double a, double b, double c, double d, double e) {
   const double s21 = c - 2. * d + e, s22 = 3. * c - 4. * d + e;
@@ -84,7 +87,7 @@ __device__ __forceinline__ double w5_m(double a, double b, double c, double d, d
   const double q2 = (11. * o6) * c + (-7. * o6) * d + (2. * o6) * e;
   const double q1 = (2. * o6) * b + (5. * o6) * c + (-o6) * d;
   const double q0 = (-o6) * a + (5. * o6) * b + (2. * o6) * c;
-  return (p0 * q0 + p1 * q1 + p2 * q2) / (p0 + p1 + p2);
+  return (p0 * q0 + p1 * q1 + p2 * q2) * rcp_nr(p0 + p1 + p2);
 }
 
 // WENOT (rec1d_p/m_WENOT with uniform weights, rec1d.c:1412-1578): the
@@ -102,7 +105,7 @@ __device__ __forceinline__ double wt_p(double a, double b, double c, double d, d
   const double q0 = (2. * o6) * a + (-7. * o6) * b + (11. * o6) * c;
   const double q1 = (-o6) * b + (5. * o6) * c + (2. * o6) * d;
   const double q2 = (2. * o6) * c + (5. * o6) * d + (-o6) * e;
-  return (p0 * q0 + p1 * q1 + p2 * q2) / (p0 + p1 + p2);
+  return (p0 * q0 + p1 * q1 + p2 * q2) * rcp_nr(p0 + p1 + p2);
 }
 __device__ __forceinline__ double wt_m(double a, double b, double c, double d, double e) {
   const double s21 = c - 2. * d + e, s22 = 3. * c - 4. * d + e;
@@ -116,7 +119,7 @@ __device__ __forceinline__ double wt_m(double a, double b, double c, double d, d
   const double q2 = (11. * o6) * c + (-7. * o6) * d + (2. * o6) * e;
   const double q1 = (2. * o6) * b + (5. * o6) * c + (-o6) * d;
   const double q0 = (-o6) * a + (5. * o6) * b + (2. * o6) * c;
-  return (p0 * q0 + p1 * q1 + p2 * q2) / (p0 + p1 + p2);
+  return (p0 * q0 + p1 * q1 + p2 * q2) * rcp_nr(p0 + p1 + p2);
 }
 
 // WENO-Z (rec1d_p/m_WENOZ with uniform weights, rec1d.c:1627-1781): weights
@@ -136,7 +139,7 @@ __device__ __forceinline__ double wz_p(double a, double b, double c, double d, d
   const double q0 = (2. * o6) * a + (-7. * o6) * b + (11. * o6) * c;
   const double q1 = (-o6) * b + (5. * o6) * c + (2. * o6) * d;
   const double q2 = (2. * o6) * c + (5. * o6) * d + (-o6) * e;
-  return (p0 * q0 + p1 * q1 + p2 * q2) / (p0 + p1 + p2);
+  return (p0 * q0 + p1 * q1 + p2 * q2) * rcp_nr(p0 + p1 + p2);
 }
 __device__ __forceinline__ double wz_m(double a, double b, double c, double d, double e) {
   const double s21 = c - 2. * d + e, s22 = 3. * c - 4. * d + e;
@@ -152,7 +155,7 @@ __device__ __forceinline__ double wz_m(double a, double b, double c, double d, d
   const double q2 = (11. * o6) * c + (-7. * o6) * d + (2. * o6) * e;
   const double q1 = (2. * o6) * b + (5. * o6) * c + (-o6) * d;
   const double q0 = (-o6) * a + (5. * o6) * b + (2. * o6) * c;
-  return (p0 * q0 + p1 * q1 + p2 * q2) / (p0 + p1 + p2);
+  return (p0 * q0 + p1 * q1 + p2 * q2) * rcp_nr(p0 + p1 + p2);
 }
 
 // both sides of midpoint im of a line q[0..NP-1]: WENOm5_2 (REC 5),
@@ -305,9 +308,10 @@ __device__ __forceinline__ void shell_metric(const FvGeo &g, const double X[3], 
   double s = sqrt(M[DIR][0] * M[DIR][0] + M[DIR][1] * M[DIR][1] + M[DIR][2] * M[DIR][2]);
   soj = g.dXbdX[DIR] * s / ooJ;
   if (s == 0.0) s = 1.0;
-  nrm[0] = M[DIR][0] / s;
-  nrm[1] = M[DIR][1] / s;
-  nrm[2] = M[DIR][2] / s;
+  const double is = 1.0 / s;
+  nrm[0] = M[DIR][0] * is;
+  nrm[1] = M[DIR][1] * is;
+  nrm[2] = M[DIR][2] * is;
 }
 
 // patch coordinate d of element point Xb (XYZ_of_XbYbZb, get_coords.c:384-394)
@@ -391,7 +395,8 @@ __device__ __forceinline__ void fv_line(const double *eu, const Face &face, int 
       shell_metric<DIR>(G, X, sojR, ojm, nR);
     }
     if (i0 == 0) sojL = sojn;
-    const double JR = sojR / wc[i0], JL = sojL / wc[i0];
+    const double iwc = 1.0 / wc[i0];
+    const double JR = sojR * iwc, JL = sojL * iwc;
     sojL = sojR;
     if (i0 > 0) {
 #pragma unroll
