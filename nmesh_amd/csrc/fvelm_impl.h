@@ -549,7 +549,10 @@ __device__ __forceinline__ void fv_element(const FvOut &o, long gbase, const Fac
 
 // element-range kernel of meshes that are FV throughout (one order)
 template <int NP, int REC, int NT, int GEOM, bool EX>
-__global__ __launch_bounds__(NT) void fvelm_stage_kernel(StageArgs a, const double *wc) {
+// two waves per SIMD: the shell instance otherwise takes 256 VGPRs plus 12
+// AGPRs and runs one wave per SIMD
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void fvelm_stage_kernel(StageArgs a,
+                                                                                                  const double *wc) {
   constexpr int NP3 = NP * NP * NP;
   __shared__ double acc[4][NP3];   // divf, summed over x, then y, then z
   __shared__ double s_wc[NP];
