@@ -46,6 +46,61 @@ struct JobView {
   __device__ int nx2() const { return (sw ? a1 - a0 : b1 - b0) + 1; }
 };
 
+// One job with the neighbour's order M known at compile time: the loops
+// over its points are unrolled without branches, so the LDS reads of a row
+// are issued together instead of one round trip per term.
+template <int M>
+__device__ __forceinline__ void face_job(const FaceArgs &a, const JobView &J, int tid, double *s_T, double *s_R1,
+                                         double *s_R2) {
+  const int n = J.n, nx1 = J.nx1(), nx2 = J.nx2();
+  const int n1 = nx1 * M, n2 = nx2 * M;
+  // every load of this face: rows (at most 100 doubles each, two per lane)
+  // and my (v, q2) line of the plane
+  const double *R1 = a.Btab + (J.sw ? J.offB : J.offA) + J.x1lo() * M;
+  const double *R2 = a.Btab + (J.sw ? J.offA : J.offB) + J.x2lo() * M;
+  const double r1a = tid < n1 ? R1[tid] : 0.0, r1b = tid + 64 < n1 ? R1[tid + 64] : 0.0;
+  const double r2a = tid < n2 ? R2[tid] : 0.0, r2b = tid + 64 < n2 ? R2[tid + 64] : 0.0;
+  const bool act = tid < 4 * M;
+  const int v = act ? tid / M : 0, q2 = act ? tid % M : 0;
+  const double *P = a.win + J.soff + J.base + (long)v * M * M * M + q2 * J.t2;
+  double p[M];
+#pragma unroll
+  for (int q1 = 0; q1 < M; ++q1) p[q1] = act ? P[q1 * J.t1] : 0.0;
+  __syncthreads();   // the previous face is done with LDS
+  if (tid < n1) s_R1[tid] = r1a;
+  if (tid + 64 < n1) s_R1[tid + 64] = r1b;
+  if (tid < n2) s_R2[tid] = r2a;
+  if (tid + 64 < n2) s_R2[tid + 64] = r2b;
+  __syncthreads();
+  if (act) {
+    double *T = s_T + v * nx1 * M + q2;
+    for (int x = 0; x < nx1; ++x) {
+      const double *row = s_R1 + x * M;
+      double t = 0.0;
+#pragma unroll
+      for (int q1 = 0; q1 < M; ++q1) t += p[q1] * row[q1];
+      T[x * M] = t;
+    }
+  }
+  __syncthreads();
+  const int nn = n * n;
+  for (int idx = tid; idx < 4 * nn; idx += kFaceThreads) {
+    const int vv = idx / nn, q = idx % nn;
+    const int fa = q % n, fb = q / n;
+    double sum = 0.0;
+    if (fa >= J.a0 && fa <= J.a1 && fb >= J.b0 && fb <= J.b1) {
+      const int x1 = (J.sw ? fb : fa) - J.x1lo(), x2 = (J.sw ? fa : fb) - J.x2lo();
+      const double *T = s_T + (vv * nx1 + x1) * M;
+      const double *row = s_R2 + x2 * M;
+      double sv = 0.0;
+#pragma unroll
+      for (int k = 0; k < M; ++k) sv += T[k] * row[k];
+      sum += sv;
+    }
+    a.aj[J.ajoff + idx] = sum;
+  }
+}
+
 __global__ __launch_bounds__(kFaceThreads) void face_interp_kernel(FaceArgs a) {
   __shared__ double s_T[kFaceT1];    // [v][x1][m]
   __shared__ double s_R1[kFaceR1];   // [x1][m]
@@ -56,54 +111,23 @@ __global__ __launch_bounds__(kFaceThreads) void face_interp_kernel(FaceArgs a) {
   const int f_hi = min((xcd + 1) * chunk, a.nfaces);
   for (int fi = xcd * chunk + (blockIdx.x >> 3); fi < f_hi; fi += g8) {
     const JobView J(as_const(a.jobs) + kJobInts * fi);
-    const int n = J.n, m = J.m, nx1 = J.nx1(), nx2 = J.nx2();
-    const int n1 = nx1 * m, n2 = nx2 * m;
-    // every load of this face: rows (at most 100 doubles each, two per lane)
-    // and my (v, q2) line of the plane
-    const double *R1 = a.Btab + (J.sw ? J.offB : J.offA) + J.x1lo() * m;
-    const double *R2 = a.Btab + (J.sw ? J.offA : J.offB) + J.x2lo() * m;
-    const double r1a = tid < n1 ? R1[tid] : 0.0, r1b = tid + 64 < n1 ? R1[tid + 64] : 0.0;
-    const double r2a = tid < n2 ? R2[tid] : 0.0, r2b = tid + 64 < n2 ? R2[tid + 64] : 0.0;
-    const bool act = tid < 4 * m;
-    const int v = act ? tid / m : 0, q2 = act ? tid % m : 0;
-    const double *P = a.win + J.soff + J.base + (long)v * m * m * m + q2 * J.t2;
-    double p[kMaxFaceN];
-#pragma unroll
-    for (int q1 = 0; q1 < kMaxFaceN; ++q1) p[q1] = (act && q1 < m) ? P[q1 * J.t1] : 0.0;
-    __syncthreads();   // the previous face is done with LDS
-    if (tid < n1) s_R1[tid] = r1a;
-    if (tid + 64 < n1) s_R1[tid + 64] = r1b;
-    if (tid < n2) s_R2[tid] = r2a;
-    if (tid + 64 < n2) s_R2[tid + 64] = r2b;
-    __syncthreads();
-    if (act) {
-      double *T = s_T + v * nx1 * m + q2;
-#pragma unroll
-      for (int x = 0; x < kMaxFaceN; ++x) {
-        if (x < nx1) {
-          double t = 0.0;
-#pragma unroll
-          for (int q1 = 0; q1 < kMaxFaceN; ++q1)
-            if (q1 < m) t += p[q1] * s_R1[x * m + q1];
-          T[x * m] = t;
-        }
-      }
-    }
-    __syncthreads();
-    const int nn = n * n;
-    for (int idx = tid; idx < 4 * nn; idx += kFaceThreads) {
-      const int vv = idx / nn, q = idx % nn;
-      const int fa = q % n, fb = q / n;
-      double sum = 0.0;
-      if (fa >= J.a0 && fa <= J.a1 && fb >= J.b0 && fb <= J.b1) {
-        const int x1 = (J.sw ? fb : fa) - J.x1lo(), x2 = (J.sw ? fa : fb) - J.x2lo();
-        const double *T = s_T + (vv * nx1 + x1) * m;
-        const double *row = s_R2 + x2 * m;
-        double sv = 0.0;
-        for (int k = 0; k < m; ++k) sv += T[k] * row[k];
-        sum += sv;
-      }
-      a.aj[J.ajoff + idx] = sum;
+    switch (J.m) {
+#define NMX_CASE(M_)                                   \
+  case M_:                                             \
+    face_job<M_>(a, J, tid, s_T, s_R1, s_R2);          \
+    break;
+      NMX_CASE(2)
+      NMX_CASE(3)
+      NMX_CASE(4)
+      NMX_CASE(5)
+      NMX_CASE(6)
+      NMX_CASE(7)
+      NMX_CASE(8)
+      NMX_CASE(9)
+      NMX_CASE(10)
+#undef NMX_CASE
+      default:
+        break;   // orders outside 2..10 are refused on the host
     }
   }
 }
