@@ -99,27 +99,30 @@ __global__ __launch_bounds__(kAdjGThreads) void adj_gflux_kernel(AdjAmrArgs a) {
 // ---------------------------------------------------------------------------
 constexpr int kAdjJThreads = 64;
 
-__global__ __launch_bounds__(kAdjJThreads) void adj_jobs_kernel(AdjAmrArgs a) {
-  __shared__ double s_W[4 * kMaxFaceN * kMaxFaceN];
-  __shared__ double s_S[4 * kMaxFaceN * kMaxFaceN];
-  __shared__ double s_R1[kMaxFaceN * kMaxFaceN];
-  __shared__ double s_R2[kMaxFaceN * kMaxFaceN];
-  const int tid = threadIdx.x;
-  for (int ji = blockIdx.x; ji < a.njobs; ji += gridDim.x) {
-    const AdjJob J = a.jobs[ji];
-    const int n = J.n, m = J.m, n2 = n * n;
-    const int x1lo = J.sw ? J.b0 : J.a0, x2lo = J.sw ? J.a0 : J.b0;
-    const int nx1 = (J.sw ? J.b1 - J.b0 : J.a1 - J.a0) + 1;
-    const int nx2 = (J.sw ? J.a1 - J.a0 : J.b1 - J.b0) + 1;
-    const double *R1 = a.Btab + (J.sw ? J.offB : J.offA) + x1lo * m;
-    const double *R2 = a.Btab + (J.sw ? J.offA : J.offB) + x2lo * m;
-    const double *Gp = a.G + a.goff[J.e] + 4 * J.g * n2;
-    __syncthreads();   // the previous job is done with LDS
-    for (int idx = tid; idx < nx1 * m; idx += kAdjJThreads) s_R1[idx] = R1[idx];
-    for (int idx = tid; idx < nx2 * m; idx += kAdjJThreads) s_R2[idx] = R2[idx];
-    for (int idx = tid; idx < 4 * nx1 * nx2; idx += kAdjJThreads) {
-      const int v = idx / (nx1 * nx2), r = idx % (nx1 * nx2);
-      const int x1 = r / nx2, x2 = r % nx2;
+// One job with the receiver's order N known at compile time.  The covered
+// ranges x1 < nx1, x2 < nx2 are padded to N with zero rows of W, R1, R2 and
+// S, so the sums over x1 and x2 unroll without branches (their LDS reads are
+// issued together) and add only exact zeros beyond the ranges.
+template <int N>
+__device__ __forceinline__ void adj_job(const AdjAmrArgs &a, const AdjJob &J, int tid, double *s_W, double *s_S,
+                                        double *s_R1, double *s_R2) {
+  const int m = J.m, n2 = N * N;
+  const int x1lo = J.sw ? J.b0 : J.a0, x2lo = J.sw ? J.a0 : J.b0;
+  const int nx1 = (J.sw ? J.b1 - J.b0 : J.a1 - J.a0) + 1;
+  const int nx2 = (J.sw ? J.a1 - J.a0 : J.b1 - J.b0) + 1;
+  const double *R1 = a.Btab + (J.sw ? J.offB : J.offA) + x1lo * m;
+  const double *R2 = a.Btab + (J.sw ? J.offA : J.offB) + x2lo * m;
+  const double *Gp = a.G + a.goff[J.e] + 4 * J.g * n2;
+  __syncthreads();   // the previous job is done with LDS
+  for (int idx = tid; idx < N * m; idx += kAdjJThreads) {
+    s_R1[idx] = idx < nx1 * m ? R1[idx] : 0.0;
+    s_R2[idx] = idx < nx2 * m ? R2[idx] : 0.0;
+  }
+  for (int idx = tid; idx < 4 * N * N; idx += kAdjJThreads) {
+    const int v = idx / n2, r = idx % n2;
+    const int x1 = r / N, x2 = r % N;
+    double wg = 0.0;
+    if (x1 < nx1 && x2 < nx2) {
       const int fa = J.sw ? x2 + x2lo : x1 + x1lo;
       const int fb = J.sw ? x1 + x1lo : x2 + x2lo;
       double w = 1.0;
@@ -132,25 +135,57 @@ __global__ __launch_bounds__(kAdjJThreads) void adj_jobs_kernel(AdjAmrArgs a) {
         }
         if (cnt > 1) w = 1.0 / cnt;
       }
-      s_W[idx] = w * Gp[v * n2 + fa + n * fb];
+      wg = w * Gp[v * n2 + fa + N * fb];
     }
-    __syncthreads();
-    for (int idx = tid; idx < 4 * nx1 * m; idx += kAdjJThreads) {
-      const int v = idx / (nx1 * m), r = idx % (nx1 * m);
-      const int x1 = r / m, q2 = r % m;
-      const double *W = s_W + (v * nx1 + x1) * nx2;
-      double t = 0.0;
-      for (int x2 = 0; x2 < nx2; ++x2) t += W[x2] * s_R2[x2 * m + q2];
-      s_S[idx] = t;
-    }
-    __syncthreads();
-    double *T = a.T + J.toff;
-    for (int idx = tid; idx < 4 * m * m; idx += kAdjJThreads) {
-      const int v = idx / (m * m), r = idx % (m * m);
-      const int q1 = r % m, q2 = r / m;
-      double t = 0.0;
-      for (int x1 = 0; x1 < nx1; ++x1) t += s_R1[x1 * m + q1] * s_S[(v * nx1 + x1) * m + q2];
-      T[idx] = t;
+    s_W[idx] = wg;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < 4 * N * m; idx += kAdjJThreads) {
+    const int v = idx / (N * m), r = idx % (N * m);
+    const int x1 = r / m, q2 = r % m;
+    const double *W = s_W + (v * N + x1) * N;
+    double t = 0.0;
+#pragma unroll
+    for (int x2 = 0; x2 < N; ++x2) t += W[x2] * s_R2[x2 * m + q2];
+    s_S[idx] = t;
+  }
+  __syncthreads();
+  double *T = a.T + J.toff;
+  for (int idx = tid; idx < 4 * m * m; idx += kAdjJThreads) {
+    const int v = idx / (m * m), r = idx % (m * m);
+    const int q1 = r % m, q2 = r / m;
+    double t = 0.0;
+#pragma unroll
+    for (int x1 = 0; x1 < N; ++x1) t += s_R1[x1 * m + q1] * s_S[(v * N + x1) * m + q2];
+    T[idx] = t;
+  }
+}
+
+__global__ __launch_bounds__(kAdjJThreads) void adj_jobs_kernel(AdjAmrArgs a) {
+  __shared__ double s_W[4 * kMaxFaceN * kMaxFaceN];
+  __shared__ double s_S[4 * kMaxFaceN * kMaxFaceN];
+  __shared__ double s_R1[kMaxFaceN * kMaxFaceN];
+  __shared__ double s_R2[kMaxFaceN * kMaxFaceN];
+  const int tid = threadIdx.x;
+  for (int ji = blockIdx.x; ji < a.njobs; ji += gridDim.x) {
+    const AdjJob J = a.jobs[ji];
+    switch (J.n) {
+#define NMX_CASE(N_)                                    \
+  case N_:                                              \
+    adj_job<N_>(a, J, tid, s_W, s_S, s_R1, s_R2);       \
+    break;
+      NMX_CASE(2)
+      NMX_CASE(3)
+      NMX_CASE(4)
+      NMX_CASE(5)
+      NMX_CASE(6)
+      NMX_CASE(7)
+      NMX_CASE(8)
+      NMX_CASE(9)
+      NMX_CASE(10)
+#undef NMX_CASE
+      default:
+        break;   // orders outside 2..10 are refused on the host
     }
   }
 }
