@@ -480,9 +480,11 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
     return fail(NMX_EUNSUP, "fv_rec: 1, WENOm3, WENOm5, WENOmZ (and _1, _2) and WENOmT_2 are built");
   if ((k == "fv_flux" && v != "fnum") || (k == "fv_divf_extrap" && v != "dnfn_extrap1") ||
       (k == "fv_divf_use_only_right_flux" && v != "yes") || (k == "fv_WENOm3_opt_weightratio" && v != "2") ||
-      (k == "fv_surface_interp" && !surface_interp_mode(v, nullptr, nullptr)) ||
       (k == "fv2dg_interp_use_extrap1" && v != "no"))
     return fail(NMX_EUNSUP, k + ": only nmesh's default is built");
+  if (k == "fv_surface_interp" && !surface_interp_mode(v, nullptr, nullptr))
+    return fail(NMX_EUNSUP,
+                "fv_surface_interp: linear or parabolic, each optionally with Lagrange_if_dg, is built (not WENO)");
   if ((k == "amr_dir_active0" || k == "amr_dir_active1" || k == "amr_dir_active2") && v != "yes")
     return fail(NMX_EUNSUP, k + ": only 3d meshes are built");
   if (k == "coordinates_surface_metric" && v != "sqrtgdiag")
