@@ -79,6 +79,32 @@ def test_adjoint_rhs_dot_product_refined_mixed_shell(numflux, geometry):
         assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(Jv) * np.linalg.norm(w), (lhs, rhs)
 
 
+def test_adjoint_rhs_dot_product_every_order_pair():
+    """orders 2..10 on an h-refined shell, so the transposed interpolation
+    runs for every (receiver, neighbour) order pair (its jobs are compiled
+    per receiver order): dot products against the oracle's RHS"""
+    print("adjoint amr order pairs", flush=True)
+    pars = {"amr_mesh_type": "Shell", "amr_luni": 1, "amr_n0": 4, "amr_n1": 4, "amr_n2": 4,
+            "amr_Shell_rin": 2.0, "amr_Shell_rout": 20.0, "amr_hrefine_sphere_radius": 7,
+            "amr_hrefine_sphere_levels": 1, "scalarwave1_k": "0 0 0", "nmx_path": "batched"}
+    m = nm.Mesh(pars, device=0)
+    nm.amr_setup_mesh(m)
+    om = OMesh("shell", 1, 4, rin=2.0, rout=20.0, radius=7, levels=1)
+    nv = np.random.default_rng(0).integers(2, 11, om.nelem).astype(np.int32)
+    m.set_elm_n(nv)
+    om.set_n(nv)
+    pairs = {(int(nv[e]), int(nv[int(j)])) for e in range(om.nelem) for f in range(6) for j in om.elm_fnb(e, f)[0]}
+    assert len(pairs) == 81
+    rng = np.random.default_rng(13)
+    for _ in range(2):
+        v = rng.standard_normal(5 * om.nnodes)
+        w = rng.standard_normal(5 * om.nnodes)
+        Jv = om.rhs(v, 0.0, K0)
+        JTw = host(adj.adjoint_setrhs(m, dev(w)))
+        lhs, rhs = float(Jv @ w), float(v @ JTw)
+        assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(Jv) * np.linalg.norm(w), (lhs, rhs)
+
+
 @pytest.mark.parametrize("numflux", ["upwind", "LLF"])
 def test_adjoint_rhs_is_the_transposed_matrix_refined_box(numflux):
     print("adjoint amr matrix", numflux, flush=True)

@@ -149,3 +149,26 @@ def test_graph_steps_match_single_steps_and_rhs_after_evolve(shape):
     assert np.array_equal(u1, m2.download())
     t = m1.time
     assert rel(nm.evolve_setrhs_mesh(m1, t=t), om.rhs(u1, t, K)) <= TOL
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_every_order_pair_on_a_refined_shell(shape):
+    """orders 2..10 on an h-refined shell: the face interpolation of every
+    (receiver, neighbour) order pair that the mesh holds (the face pass is
+    compiled per neighbour order, its transpose per receiver order), RHS
+    against the oracle"""
+    print("order pairs", shape, flush=True)
+    base = OMesh("shell", 1, 4, rin=2.0, rout=20.0, radius=7, levels=1)
+    nvec = np.random.default_rng(0).integers(2, 11, base.nelem).astype(np.int32)
+    m, om = make("shell", 1, 4, nvec=nvec, radius=7, levels=1, shape=shape)
+    info = np.array([m.elm_info(e) for e in range(m.nelem)])
+    pairs = set()
+    for e in range(m.nelem):
+        for f in range(6):
+            nb, _ = om.elm_fnb(e, f)
+            for j in nb:
+                pairs.add((int(info[e, 2]), int(info[int(j), 2])))
+    assert len(pairs) == 81, sorted(pairs)   # every (receiver, neighbour) pair of 2..10
+    u = state(om, 21)
+    m.upload(u)
+    assert rel(nm.evolve_setrhs_mesh(m, t=0.1), om.rhs(u, 0.1, K)) <= TOL
