@@ -2686,6 +2686,38 @@ extern "C" int nmx_autotune(nmx_ctx *c) {
       c->par["nmx_batch"] = best_shape;
       lanes_for(best_shape);
     }
+    // with the face kernel's shape, the order buckets' tiles refined on whole
+    // steps, where the buckets run beside one another as they will: each DG
+    // bucket, longest first, keeps the tile that makes the steps fastest with
+    // the others fixed (the tile that is fastest alone need not be)
+    if (P(c, "nmx_batch") == "legacy" && c->anydg && c->buckets.size() > 1) {
+      auto steps_ms = [&](float &ms) -> int {
+        if (int rr = issue_step(c, method, c->dt, 0, false)) return rr;   // warm-up
+        HIPCHK(hipEventRecord(t0, c->stream));
+        for (int s = 0; s < 6; ++s)
+          if (int rr = issue_step(c, method, c->dt, 0, (s & 1) != 0)) return rr;
+        HIPCHK(hipEventRecord(t1, c->stream));
+        HIPCHK(hipEventSynchronize(t1));
+        HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+        return NMX_OK;
+      };
+      float cur = 0.f;
+      if ((r = steps_ms(cur))) return r;
+      for (size_t k = 0; k < c->buckets.size(); ++k) {
+        auto &b = c->buckets[c->border[k]];
+        if (b.fv) continue;
+        const int keep = b.div;
+        for (int div : {1, 2, 4}) {
+          if (div == keep) continue;
+          const int prev = b.div;
+          b.div = div;
+          float ms = 0.f;
+          if ((r = steps_ms(ms))) return r;
+          if (ms < cur) cur = ms;
+          else b.div = prev;
+        }
+      }
+    }
   }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
