@@ -12,6 +12,7 @@
 #include <map>
 #include <sstream>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/nmx.h"
@@ -1391,9 +1392,10 @@ extern "C" int nmx_mesh_hadapt(nmx_ctx *c, const int *flag, int nelem) {
   if (nelem != (int)c->mesh.e.size()) return fail(NMX_EINVAL, "need one flag per element");
   for (const Elm &E : c->mesh.e)
     if (E.fv) return fail(NMX_EUNSUP, "adapting meshes with FV elements is not built");
-  Mesh old = c->mesh;
-  std::string err = hadapt(c->mesh, std::vector<int>(flag, flag + nelem));
-  if (!err.empty()) { c->mesh = old; return fail(NMX_EINVAL, err); }
+  Mesh nw = mesh_skeleton(c->mesh);
+  std::string err = hadapt(nw, std::vector<int>(flag, flag + nelem));
+  if (!err.empty()) return fail(NMX_EINVAL, err);
+  const Mesh old = std::exchange(c->mesh, std::move(nw));
   return remirror_with_state(c, old);
 }
 
@@ -1405,9 +1407,10 @@ extern "C" int nmx_mesh_padapt(nmx_ctx *c, const int *n, int nelem) {
     if (E.fv) return fail(NMX_EUNSUP, "adapting meshes with FV elements is not built");
   for (int i = 0; i < nelem; ++i)
     if (n[i] < 2 || n[i] > 10) return fail(NMX_EUNSUP, "the device kernels take 2 to 10 points per direction");
-  Mesh old = c->mesh;
-  std::string err = set_elm_n(c->mesh, std::vector<int>(n, n + nelem));
-  if (!err.empty()) { c->mesh = old; return fail(NMX_EINVAL, err); }
+  Mesh nw = mesh_skeleton(c->mesh);
+  std::string err = set_elm_n(nw, std::vector<int>(n, n + nelem));
+  if (!err.empty()) return fail(NMX_EINVAL, err);
+  const Mesh old = std::exchange(c->mesh, std::move(nw));
   return remirror_with_state(c, old);
 }
 
@@ -1417,9 +1420,10 @@ extern "C" int nmx_mesh_switch_fv(nmx_ctx *c, const int *to_fv, int nelem) {
   if (nelem != (int)c->mesh.e.size()) return fail(NMX_EINVAL, "need one flag per element");
   for (int i = 0; i < nelem; ++i)
     if (to_fv[i] && c->mesh.e[i].n < 3) return fail(NMX_EUNSUP, "FV elements need n >= 3 points per direction");
-  Mesh old = c->mesh;
-  std::string err = set_fv_elems(c->mesh, std::vector<int>(to_fv, to_fv + nelem));
-  if (!err.empty()) { c->mesh = old; return fail(NMX_EINVAL, err); }
+  Mesh nw = mesh_skeleton(c->mesh);
+  std::string err = set_fv_elems(nw, std::vector<int>(to_fv, to_fv + nelem));
+  if (!err.empty()) return fail(NMX_EINVAL, err);
+  const Mesh old = std::exchange(c->mesh, std::move(nw));
   return remirror_with_state(c, old);
 }
 
@@ -2245,10 +2249,8 @@ enum { CARRY_A = 1, CARRY_W0 = 2, CARRY_W1 = 4 };
 // at the same n (PARENT_n_P_UNIFORM / PARENT_n_P_LGL; trouble.c:186-330,
 // RungeKutta.c:239-312): the tables are rebuilt, U and the carried registers
 // interpolated, the scores kept
-static int trouble_switch(nmx_ctx *c, const std::vector<int> &to_fv, int carry) {
-  const Mesh old = c->mesh;
-  std::string err = set_fv_elems(c->mesh, to_fv);
-  if (!err.empty()) { c->mesh = old; return fail(NMX_EINVAL, err); }
+// (nw: the switched mesh, already built by the caller from the current one)
+static int trouble_switch_to(nmx_ctx *c, Mesh nw, int carry) {
   struct Held { int flag; double *tmp; };
   std::vector<Held> held;
   for (int flag : {CARRY_A, CARRY_W0, CARRY_W1}) {
@@ -2261,12 +2263,12 @@ static int trouble_switch(nmx_ctx *c, const std::vector<int> &to_fv, int carry) 
     if (r) {
       (void)hipFree(tmp);
       for (const Held &h : held) (void)hipFree(h.tmp);
-      c->mesh = old;
       return r;
     }
     held.push_back({flag, tmp});
   }
   const std::vector<int> keep = c->hscore;
+  const Mesh old = std::exchange(c->mesh, std::move(nw));
   int r = remirror_with_state(c, old);
   for (const Held &h : held) {
     double *dst = h.flag == CARRY_A ? c->dA : h.flag == CARRY_W0 ? c->dW0 : c->dW1;
@@ -2280,6 +2282,13 @@ static int trouble_switch(nmx_ctx *c, const std::vector<int> &to_fv, int carry) 
   c->tuned = true;                // mid-run: keep the launch shapes chosen so far
   c->trouble_switches++;
   return trouble_prepare(c);
+}
+
+static int trouble_switch(nmx_ctx *c, const std::vector<int> &to_fv, int carry) {
+  Mesh nw = mesh_skeleton(c->mesh);
+  std::string err = set_fv_elems(nw, to_fv);
+  if (!err.empty()) return fail(NMX_EINVAL, err);
+  return trouble_switch_to(c, std::move(nw), carry);
 }
 
 static int trouble_fix(nmx_ctx *c, double *y, const double *x, const double *x2, const double *r, double sfac,
@@ -2358,7 +2367,7 @@ static int mirror_again(nmx_ctx *c) {
 // the RHS on the switched mesh; with more, each class is switched on its
 // own in turn and its R kept.
 static int redo_rhs(nmx_ctx *c, const TStage &S, int s) {
-  const Mesh M0 = c->mesh;
+  const Mesh M0 = mesh_skeleton(c->mesh);    // the base of each class's rebuild
   const size_t ne = M0.e.size();
   const std::vector<int> score = c->hscore;
   std::vector<int> fv(ne);
@@ -2376,7 +2385,9 @@ static int redo_rhs(nmx_ctx *c, const TStage &S, int s) {
     HIPCHK(hipMemcpyAsync(win0, treg(c, S.win), ns * 8, hipMemcpyDeviceToDevice, c->stream));
   }
   const int carry = CARRY_A | (S.win == 1 || S.wout == 1 ? CARRY_W0 : 0) | (S.win == 2 || S.wout == 2 ? CARRY_W1 : 0);
-  if (int r = trouble_switch(c, fv, carry)) { (void)hipFree(win0); return r; }
+  // the switched mesh is Mall: copied in when the class mirrors follow, else moved
+  const int rs = ncls > 1 ? trouble_switch_to(c, Mall, carry) : trouble_switch_to(c, std::move(Mall), carry);
+  if (rs) { (void)hipFree(win0); return rs; }
   if (ncls <= 1) {
     StageArgs b = base_args(c);
     b.win = treg(c, S.win); b.R = c->dR; b.tidx = s;
@@ -2414,7 +2425,7 @@ static int redo_rhs(nmx_ctx *c, const TStage &S, int s) {
   // back to the switched mesh with its registers, also after a failure
   // (the first error is the one returned)
   const bool held_ok = held[0] && held[1] && held[2] && held[3] && held[4];
-  c->mesh = Mall;
+  c->mesh = std::move(Mall);
   int r2 = mirror_again(c);
   if (!r2 && held_ok) {
     double *dst[5] = {c->dU, c->dA, c->dW0, c->dW1, c->dR};

@@ -10,9 +10,11 @@
 #include "mesh.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
 
 namespace nmx {
@@ -105,24 +107,33 @@ static void make_uniform(int n, Basis &b) {
   }
 }
 
+// The bases are built once per n under a lock; afterwards a lookup only
+// reads the flag (geometry() calls these from several threads).
 const Basis &uniform_basis(int n) {
   static Basis cache[kMaxN + 1];
-  static bool done[kMaxN + 1] = {false};
+  static std::atomic<bool> done[kMaxN + 1];
   static std::mutex mu;
-  std::lock_guard<std::mutex> lk(mu);
-  if (!done[n]) {
-    make_uniform(n, cache[n]);
-    done[n] = true;
+  if (!done[n].load(std::memory_order_acquire)) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (!done[n].load(std::memory_order_relaxed)) {
+      make_uniform(n, cache[n]);
+      done[n].store(true, std::memory_order_release);
+    }
   }
   return cache[n];
 }
 
 const Basis &lgl_basis(int n) {
   static Basis cache[kMaxN + 1];
-  static bool done[kMaxN + 1] = {false};
+  static std::atomic<bool> done[kMaxN + 1];
   static std::mutex mu;
-  std::lock_guard<std::mutex> g(mu);
-  if (!done[n]) { make_basis(n, cache[n]); done[n] = true; }
+  if (!done[n].load(std::memory_order_acquire)) {
+    std::lock_guard<std::mutex> g(mu);
+    if (!done[n].load(std::memory_order_relaxed)) {
+      make_basis(n, cache[n]);
+      done[n].store(true, std::memory_order_release);
+    }
+  }
   return cache[n];
 }
 
@@ -567,16 +578,13 @@ static double find_hmin(const Mesh &M, const Elm &E, bool *pair) {
   return h;
 }
 
-static void geometry(Mesh &M) {
-  long off = 0;
-  for (auto &E : M.e) { E.off = off; off += (long)E.n * E.n * E.n; }
-  M.nnodes = off;
-  long nn = off;
-  M.x.assign(3 * nn, 0.0);
-  M.dXdx.assign(9 * nn, 0.0);
-  M.sqrtgdiag.assign(3 * nn, 0.0);
-  for (auto &E : M.e) {
+// coordinates, dX/dx, sqrtgdiag and hmin of elements [e0, e1)
+static void geometry_range(Mesh &M, size_t e0, size_t e1) {
+  const long nn = M.nnodes;
+  for (size_t ei = e0; ei < e1; ++ei) {
+    Elm &E = M.e[ei];
     const Patch &P = M.pat[E.p];
+    const Basis &bas = E.fv ? uniform_basis(E.n) : lgl_basis(E.n);
     int n = E.n;
     double dXbdX[3];
     for (int d = 0; d < 3; ++d) dXbdX[d] = 2. / (E.bbox[2 * d + 1] - E.bbox[2 * d]);
@@ -585,7 +593,9 @@ static void geometry(Mesh &M) {
         for (int i = 0; i < n; ++i) {
           long g = E.off + i + n * (j + n * k);
           double X[3], x[3], D[3][3];
-          node_X(E, i, j, k, X);
+          const int I[3] = {i, j, k};
+          for (int d = 0; d < 3; ++d)   // node_X with the element's basis looked up once
+            X[d] = 0.5 * ((E.bbox[2 * d + 1] - E.bbox[2 * d]) * bas.x[I[d]] + (E.bbox[2 * d + 1] + E.bbox[2 * d]));
           patch_dXdx(P, X, x, D);
           for (int d = 0; d < 3; ++d) M.x[d * nn + g] = x[d];
           for (int a = 0; a < 3; ++a) {
@@ -599,6 +609,28 @@ static void geometry(Mesh &M) {
         }
     E.hmin = find_hmin(M, E, &E.hmin_pair);
   }
+}
+
+// Every element's geometry; elements are independent, so ranges of them run
+// on up to 16 host threads.
+static void geometry(Mesh &M) {
+  long off = 0;
+  for (auto &E : M.e) { E.off = off; off += (long)E.n * E.n * E.n; }
+  M.nnodes = off;
+  long nn = off;
+  M.x.assign(3 * nn, 0.0);
+  M.dXdx.assign(9 * nn, 0.0);
+  M.sqrtgdiag.assign(3 * nn, 0.0);
+  const size_t ne = M.e.size();
+  for (auto &E : M.e) (void)(E.fv ? uniform_basis(E.n) : lgl_basis(E.n));   // fill the caches first
+  unsigned T = std::thread::hardware_concurrency();
+  T = T < 1 ? 1 : (T > 16 ? 16 : T);
+  if (ne < 64 * (size_t)T) T = 1;
+  if (T == 1) { geometry_range(M, 0, ne); return; }
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < T; ++t)
+    th.emplace_back(geometry_range, std::ref(M), ne * t / T, ne * (t + 1) / T);
+  for (auto &x : th) x.join();
 }
 
 // ------------------------------------------------------------------------
@@ -633,11 +665,10 @@ static double closest_corner(const Mesh &M, const Elm &N, int nf, const double x
   return best;
 }
 
-static std::string face_tables(Mesh &M) {
-  size_t ne = M.e.size();
-  M.fcopy.assign(6 * ne, FaceCopy());
-  M.fint.assign(6 * ne, std::vector<InterpSlot>());
-  for (size_t ei = 0; ei < ne; ++ei) {
+// the face tables of elements [e0, e1): each element writes only its own
+// entries and reads its neighbours' boxes, orders and point types
+static std::string face_tables_range(Mesh &M, size_t e0, size_t e1) {
+  for (size_t ei = e0; ei < e1; ++ei) {
     Elm &E = M.e[ei];
     int n = E.n;
     for (int f = 0; f < 6; ++f) {
@@ -800,6 +831,39 @@ static std::string face_tables(Mesh &M) {
     }
   }
   return "";
+}
+
+// Every element's face tables, ranges of elements on up to 16 host threads;
+// an error is the first element's in element order, as one thread would
+// report it.
+static std::string face_tables(Mesh &M) {
+  size_t ne = M.e.size();
+  M.fcopy.assign(6 * ne, FaceCopy());
+  M.fint.assign(6 * ne, std::vector<InterpSlot>());
+  for (int n = 2; n <= kMaxN; ++n) { (void)uniform_basis(n); (void)lgl_basis(n); }
+  unsigned T = std::thread::hardware_concurrency();
+  T = T < 1 ? 1 : (T > 16 ? 16 : T);
+  if (ne < 64 * (size_t)T) T = 1;
+  if (T == 1) return face_tables_range(M, 0, ne);
+  std::vector<std::string> err(T);
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < T; ++t)
+    th.emplace_back([&M, &err, t, T, ne] { err[t] = face_tables_range(M, ne * t / T, ne * (t + 1) / T); });
+  for (auto &x : th) x.join();
+  for (const auto &e : err)
+    if (!e.empty()) return e;
+  return "";
+}
+
+Mesh mesh_skeleton(const Mesh &M) {
+  Mesh S;
+  S.pat = M.pat;
+  S.e = M.e;
+  S.nnodes = M.nnodes;
+  S.maxlevel = M.maxlevel;
+  S.fv_surface_interp = M.fv_surface_interp;
+  S.fv_lagrange_if_dg = M.fv_lagrange_if_dg;
+  return S;
 }
 
 static std::string finish(Mesh &M) {

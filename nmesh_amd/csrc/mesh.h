@@ -89,6 +89,12 @@ struct Mesh {
   bool fv_lagrange_if_dg = false;   // ... but Lagrange rows into DG elements
 };
 
+// A copy of M's patches, elements and settings without the node arrays
+// (x, dXdx, sqrtgdiag) and face tables (fcopy, fint), which every rebuilder
+// below (set_elm_n, hadapt, set_fv_elems, set_fv_patches) recomputes: the
+// base of a rebuild, without copying the large per-node arrays.
+Mesh mesh_skeleton(const Mesh &M);
+
 // builders (return "" on success, else an error message)
 std::string build_box(Mesh &M, int luni, int n, double dout, const double xc[3]);
 std::string build_shell(Mesh &M, int luni, int n, double rin, double rout,
