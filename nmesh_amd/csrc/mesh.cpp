@@ -13,6 +13,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -611,6 +612,30 @@ static void geometry_range(Mesh &M, size_t e0, size_t e1) {
   }
 }
 
+// fn(t, e0, e1) for the T consecutive ranges t of [0, ne), on up to 16 host
+// threads (one range when ne is small); a range whose thread cannot be
+// started runs on the calling thread.  Returns T.
+template <class F>
+static unsigned for_element_ranges(size_t ne, F fn) {
+  unsigned T = std::thread::hardware_concurrency();
+  T = T < 1 ? 1 : (T > 16 ? 16 : T);
+  if (ne < 64 * (size_t)T) T = 1;
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < T; ++t) {
+    const size_t e0 = ne * t / T, e1 = ne * (t + 1) / T;
+    bool started = false;
+    try {
+      th.emplace_back(fn, t, e0, e1);
+      started = true;
+    } catch (const std::exception &) {
+    }
+    if (!started) fn(t, e0, e1);
+  }
+  fn(0u, (size_t)0, ne / T);
+  for (auto &x : th) x.join();
+  return T;
+}
+
 // Every element's geometry; elements are independent, so ranges of them run
 // on up to 16 host threads.
 static void geometry(Mesh &M) {
@@ -623,14 +648,7 @@ static void geometry(Mesh &M) {
   M.sqrtgdiag.assign(3 * nn, 0.0);
   const size_t ne = M.e.size();
   for (auto &E : M.e) (void)(E.fv ? uniform_basis(E.n) : lgl_basis(E.n));   // fill the caches first
-  unsigned T = std::thread::hardware_concurrency();
-  T = T < 1 ? 1 : (T > 16 ? 16 : T);
-  if (ne < 64 * (size_t)T) T = 1;
-  if (T == 1) { geometry_range(M, 0, ne); return; }
-  std::vector<std::thread> th;
-  for (unsigned t = 0; t < T; ++t)
-    th.emplace_back(geometry_range, std::ref(M), ne * t / T, ne * (t + 1) / T);
-  for (auto &x : th) x.join();
+  for_element_ranges(ne, [&M](unsigned, size_t e0, size_t e1) { geometry_range(M, e0, e1); });
 }
 
 // ------------------------------------------------------------------------
@@ -841,17 +859,11 @@ static std::string face_tables(Mesh &M) {
   M.fcopy.assign(6 * ne, FaceCopy());
   M.fint.assign(6 * ne, std::vector<InterpSlot>());
   for (int n = 2; n <= kMaxN; ++n) { (void)uniform_basis(n); (void)lgl_basis(n); }
-  unsigned T = std::thread::hardware_concurrency();
-  T = T < 1 ? 1 : (T > 16 ? 16 : T);
-  if (ne < 64 * (size_t)T) T = 1;
-  if (T == 1) return face_tables_range(M, 0, ne);
-  std::vector<std::string> err(T);
-  std::vector<std::thread> th;
+  std::vector<std::string> err(16);
+  const unsigned T = for_element_ranges(
+      ne, [&M, &err](unsigned t, size_t e0, size_t e1) { err[t] = face_tables_range(M, e0, e1); });
   for (unsigned t = 0; t < T; ++t)
-    th.emplace_back([&M, &err, t, T, ne] { err[t] = face_tables_range(M, ne * t / T, ne * (t + 1) / T); });
-  for (auto &x : th) x.join();
-  for (const auto &e : err)
-    if (!e.empty()) return e;
+    if (!err[t].empty()) return err[t];
   return "";
 }
 
