@@ -127,7 +127,7 @@ C4_EXTRA = {}
 FV_EXTRA = {}
 
 
-def c4_mesh(nm, path):
+def c4_mesh(nm, path, extra=None):
     """Shell r in [2,20] refined to level 3, then in nested spheres of radius
     6 and 3 (levels 4 and 5): 19,200 elements; per-element N drawn uniformly
     from {3..9} (PCG64, seed 2026); scalar wave, upwind, outgoing, RK4."""
@@ -135,6 +135,7 @@ def c4_mesh(nm, path):
     p.update({"amr_n0": 5, "amr_n1": 5, "amr_n2": 5, "amr_hrefine_sphere_radius": 3,
               "amr_hrefine_sphere_levels": 2, "nmx_path": path})
     p.update(C4_EXTRA)
+    p.update(extra or {})
     m = nm.Mesh(p, device=0)
     nm.amr_setup_mesh(m)
     rng = np.random.default_rng(2026)
@@ -383,6 +384,78 @@ def fvelm_shell_leg(nm, steps=20):
     return out
 
 
+def trouble_leg(nm, steps=20, redo_steps=3):
+    """Troubled elements on the C4 mesh, batched path, RK4 with
+    evolve_redo_troubled = substep (trouble.c:27-400, RungeKutta.c:221-344).
+    1. Scoring only: the RDMP indicator after every substep with
+       nmx_trouble_delta0 = 1e30, so no element is troubled; ms per step
+       against the headline's graph-captured steps.
+    2. Redo: a TROUBLE hook marks 4 elements far apart at the second scoring
+       call (step 0, substep 2) and none after. Those elements take their
+       substep back, become FV and redo it; the step is timed, switch
+       included, then steps on the mixed DG/FV mesh."""
+    import ctypes as C
+    m = c4_mesh(nm, "batched", {"nmx_trouble": "RDMP", "evolve_redo_troubled": "substep",
+                                "nmx_trouble_delta0": "1e30"})
+    u0 = shell_initial_data(m.coords())
+    m.upload(u0)
+    dt = m.dt
+    nm.evolve_myln(m, 1)
+    m.synchronize()
+    t0 = time.perf_counter()
+    nm.evolve_myln(m, steps)
+    m.synchronize()
+    score_ms = 1e3 * (time.perf_counter() - t0) / steps
+    _, redos0, switches0 = m.trouble_scores()
+    print("trouble leg: scoring only %.3f ms per step" % score_ms, file=sys.stderr, flush=True)
+
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    hip.hipMemcpyAsync.restype = C.c_int
+    hip.hipStreamSynchronize.argtypes = [C.c_void_p]
+    hip.hipStreamSynchronize.restype = C.c_int
+    ne = m.nelem
+    marked = [ne // 8, 3 * ne // 8, 5 * ne // 8, 7 * ne // 8]
+    zero = np.zeros(ne, np.int32)
+    mark = zero.copy()
+    mark[marked] = 1
+    calls = [0]
+
+    def hook(u, up, tr, n, stream):
+        t = mark if calls[0] == 1 else zero
+        calls[0] += 1
+        if n != ne or hip.hipMemcpyAsync(tr, t.ctypes.data, 4 * n, 1, stream) != 0:
+            return 1
+        return int(hip.hipStreamSynchronize(stream) != 0)
+
+    m.set_par("nmx_trouble", "hook")
+    m.set_trouble_hook(hook)
+    try:
+        m.time = 0.0
+        m.dt = dt
+        m.upload(u0)
+        m.synchronize()
+        per_step = []
+        for k in range(redo_steps):
+            t0 = time.perf_counter()
+            nm.evolve_myln(m, 1)
+            m.synchronize()
+            per_step.append(1e3 * (time.perf_counter() - t0))
+            print("trouble leg: redo step %d %.1f ms" % (k, per_step[-1]), file=sys.stderr, flush=True)
+        _, redos, switches = m.trouble_scores()
+        fv = int(m.fv_flags().sum())
+    finally:
+        m.set_trouble_hook(None)
+    del m
+    return {"workload": "C4 mesh (19,200 elements, n in {4..10}), batched path, RK4, "
+                        "evolve_redo_troubled = substep",
+            "scoring_only": {"indicator": "RDMP, delta0 1e30 (no element troubled)",
+                             "ms_per_step": score_ms, "steps": steps, "redos": redos0,
+                             "switches": switches0},
+            "redo": {"troubled_elements": marked, "at_call": 1, "ms_per_step": per_step,
+                     "redos": redos, "switches": switches, "fv_elements_after": fv}}
+
+
 def filter_leg(nm, reps=100):
     """The FILTER bin's exponential filter (alp 36, s 32, dn 0, Jacobian
     power 1) of all 5 evolved fields on the 3072-element N=7 shell, in place,
@@ -564,6 +637,8 @@ def main():
     ap.add_argument("--only-filter", action="store_true", help="run only the exponential-filter leg")
     ap.add_argument("--only-fvelm", action="store_true", help="run only the FV-element leg")
     ap.add_argument("--only-crc", action="store_true", help="run only the checkpoint CRC leg")
+    ap.add_argument("--only-trouble", action="store_true", help="run only the troubled-element leg")
+    ap.add_argument("--no-trouble", action="store_true", help="skip the troubled-element leg")
     ap.add_argument("--no-crc", action="store_true", help="skip the checkpoint CRC leg")
     ap.add_argument("--no-fvelm", action="store_true", help="skip the FV-element leg")
     ap.add_argument("--no-filter", action="store_true", help="skip the exponential-filter leg")
@@ -628,6 +703,9 @@ def main():
         return
     if args.only_crc:
         print(json.dumps(crc_leg(nm)), flush=True)
+        return
+    if args.only_trouble:
+        print(json.dumps(trouble_leg(nm)), flush=True)
         return
     if args.only_fv:
         print(json.dumps(fv_leg(args.fv_steps, cpu=not args.no_cpu)), flush=True)
@@ -713,6 +791,11 @@ def main():
         out["fv_elements"] = fvelm_leg(nm)
     if not args.no_crc:
         out["checkpoint_crc"] = crc_leg(nm)
+    if not args.no_trouble:
+        try:
+            out["troubled_elements"] = trouble_leg(nm)
+        except nm.NmxError as exc:    # recorded in the line; the headline stands
+            out["troubled_elements"] = {"error": str(exc)}
     if not args.no_cpu:
         out["cpu_baseline"] = c4_cpu_baseline()
     print(json.dumps(out), flush=True)

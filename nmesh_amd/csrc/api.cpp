@@ -119,6 +119,14 @@ struct nmx_ctx {
   int graph_steps = 0;
   double graph_dt = 0;
   int graph_method = -1;
+  // with troubled elements: one graph per RK stage (the stage's launches
+  // only; the scores are read on the host between stages), and the stage
+  // arguments each was captured with
+  hipGraphExec_t tgexec[4] = {nullptr, nullptr, nullptr, nullptr};
+  StageArgs tgargs[4];
+  // the u_p whose element bounds dldat holds in this step (RDMP scores
+  // every substep against the same u_p); nullptr: not computed
+  const double *ldat_up = nullptr;
   double *hT = nullptr;     // pinned stage-time staging
   double *hstage = nullptr; // pinned state staging (element blocks) for host transfers
   long hstage_n = 0;
@@ -352,7 +360,13 @@ extern "C" int nmx_ctx_create(nmx_ctx **out, int device) {
   return NMX_OK;
 }
 
+static void drop_stage_graphs(nmx_ctx *c) {
+  for (auto &g : c->tgexec)
+    if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+
 static void free_mesh_dev(nmx_ctx *c) {
+  c->ldat_up = nullptr;
   double **bufs[] = {&c->dU, &c->dA, &c->dW0, &c->dW1, &c->dR, &c->dP,
                      &c->dXdx, &c->dXbdX, &c->dxyz, &c->dD, &c->dT, &c->degeo, &c->dxb};
   for (auto b : bufs)
@@ -374,6 +388,7 @@ static void free_mesh_dev(nmx_ctx *c) {
     c->filt_key.clear();
   }
   if (c->dface) { (void)hipFree(c->dface); c->dface = nullptr; }
+  drop_stage_graphs(c);
   if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
   c->graph_steps = 0;
@@ -517,6 +532,7 @@ extern "C" int nmx_set_param(nmx_ctx *c, const char *name, const char *value) {
   if (k == "nmx_batch") c->batch_fixed = true;
   if (k == "nmx_block")
     for (auto &b : c->buckets) b.div = 0;
+  drop_stage_graphs(c);
   if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
   c->graph_steps = 0;
@@ -2220,11 +2236,15 @@ static int trouble_score(nmx_ctx *c, const double *u, const double *up, bool upd
         return fail(NMX_EINVAL, "the TROUBLE hook returned " + std::to_string(r));
     } else {
       if (P(c, "nmx_trouble") != "RDMP") return fail(NMX_ESTATE, "nmx_trouble = hook needs nmx_set_trouble_hook");
-      LimArgs la;
-      la.u = const_cast<double *>(up); la.eoff = c->deoff; la.en = c->den; la.wq = c->dwqall;
-      la.ldat = c->dldat; la.nelem = ne;
-      hipError_t err = launch_lim_data(la, c->stream);
-      if (err != hipSuccess) return fail(NMX_EHIP, std::string("trouble data: ") + hipGetErrorString(err));
+      hipError_t err = hipSuccess;
+      if (c->ldat_up != up) {
+        LimArgs la;
+        la.u = const_cast<double *>(up); la.eoff = c->deoff; la.en = c->den; la.wq = c->dwqall;
+        la.ldat = c->dldat; la.nelem = ne;
+        err = launch_lim_data(la, c->stream);
+        if (err != hipSuccess) return fail(NMX_EHIP, std::string("trouble data: ") + hipGetErrorString(err));
+        c->ldat_up = up;
+      }
       TroubleArgs ta;
       ta.u = u; ta.ldat = c->dldat; ta.eoff = c->deoff; ta.en = c->den; ta.efv = c->defv; ta.nbr = c->dnbr;
       ta.tr = c->dtr; ta.nelem = ne;
@@ -2446,6 +2466,38 @@ static int redo_rhs(nmx_ctx *c, const TStage &S, int s) {
 // element takes its substep back, becomes FV (u, u_p and w interpolated),
 // and redoes it with the RHS of w (evolve_trouble_redo_u_step_mesh,
 // RungeKutta.c:223-330).  The step's times are ts.
+// stage s's launches from a graph captured on the first call with these
+// arguments (the mirror's pointers and parameters drop the graphs when they
+// change); eager launches where steps are not captured
+static int launch_stage_graph(nmx_ctx *c, const StageArgs &a, int s) {
+  if (P(c, "nmx_graph") != "yes" || P(c, "nmx_path") == "perelement" || s < 0 || s > 3) return launch(c, a);
+  const StageArgs &k = c->tgargs[s];
+  const bool same = c->tgexec[s] && k.win == a.win && k.U == a.U && k.Ain == a.Ain && k.Aout == a.Aout &&
+                    k.Wout == a.Wout && k.R == a.R && k.tidx == a.tidx && k.cA == a.cA && k.cW == a.cW &&
+                    k.wmode == a.wmode;
+  if (!same) {
+    if (c->tgexec[s]) { (void)hipGraphExecDestroy(c->tgexec[s]); c->tgexec[s] = nullptr; }
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int r = launch(c, a);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+    if (r || ec != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (r) return r;
+      return fail(NMX_EHIP, std::string("stage capture: ") + hipGetErrorString(ec));
+    }
+    const hipError_t ei = hipGraphInstantiate(&c->tgexec[s], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      c->tgexec[s] = nullptr;
+      return fail(NMX_EHIP, std::string("stage graph: ") + hipGetErrorString(ei));
+    }
+    c->tgargs[s] = a;
+  }
+  HIPCHK(hipGraphLaunch(c->tgexec[s], c->stream));
+  return NMX_OK;
+}
+
 static int trouble_evolve_mesh(nmx_ctx *c, int method, double dt, bool redo, int *nredo) {
   std::vector<TStage> st;
   trouble_stages(method, dt, st);
@@ -2453,14 +2505,21 @@ static int trouble_evolve_mesh(nmx_ctx *c, int method, double dt, bool redo, int
     const TStage &S = st[s];
     StageArgs a = base_args(c);
     a.win = treg(c, S.win); a.U = c->dU; a.Ain = s ? c->dA : nullptr; a.Aout = c->dA;
-    a.Wout = treg(c, S.wout); a.wmode = S.wmode; a.cA = S.cA; a.cW = S.cW; a.R = c->dR; a.tidx = (int)s;
-    if (int r = launch(c, a)) return r;
+    a.Wout = treg(c, S.wout); a.wmode = S.wmode; a.cA = S.cA; a.cW = S.cW; a.tidx = (int)s;
+    if (int r = launch_stage_graph(c, a, (int)s)) return r;
     if (!redo) continue;
     int score = 0;
     if (int r = trouble_score(c, c->dA, c->dU, true, &score)) return r;
     if (score <= 0) continue;
     ++*nredo;
     c->trouble_redos++;
+    // the substep's r again from its w (no stage writes its w input), on
+    // the mesh it was taken on: the stages keep no r, which only a redo reads
+    {
+      StageArgs b = base_args(c);
+      b.win = a.win; b.R = c->dR; b.tidx = (int)s;
+      if (int r = launch(c, b)) return r;
+    }
     // u -= r rfac on the troubled elements, they become FV and get their r
     // again, then u += r rfac and the next w
     if (int r = trouble_fix(c, c->dA, c->dA, nullptr, c->dR, -S.cA, 0)) return r;
@@ -2483,6 +2542,7 @@ static int trouble_step(nmx_ctx *c, int method, double dt) {
   const int nsub = method == 0 ? 4 : method == 1 ? 3 : 1;
   const int notroubles = 11 * (substep ? nsub : 1);   // NOTROUBLES (evolve.h:60), evolve_notroubles
   int nredo = 0, score = 0;
+  c->ldat_up = nullptr;           // u_p is new
   if (int r = trouble_evolve_mesh(c, method, dt, substep, &nredo)) return r;
   if (!substep) {
     if (int r = trouble_score(c, c->dA, c->dU, true, &score)) return r;
@@ -2735,6 +2795,7 @@ extern "C" int nmx_autotune(nmx_ctx *c) {
   HIPCHK(hipMemcpyAsync(c->dU, backup, c->nstate * 8, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   (void)hipFree(backup);
+  drop_stage_graphs(c);
   if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
   c->graph_steps = 0;
@@ -2775,7 +2836,7 @@ extern "C" int nmx_evolve(nmx_ctx *c, int nsteps) {
   const double dt = c->dt;
   if (trouble_on(c)) {
     // troubled elements: scores read on the host after each substep, so one
-    // step at a time, no graphs
+    // step at a time, each stage's launches from a graph of their own
     if (c->lim_on || c->filt_on) return fail(NMX_EUNSUP, "troubled elements with a limiter or filter are not built");
     if (P(c, "nmx_trouble") == "hook" && !c->trouble_hook) return fail(NMX_ESTATE, "nmx_trouble = hook needs nmx_set_trouble_hook");
     HIPCHK(hipEventRecord(c->ev0, c->stream));

@@ -101,10 +101,11 @@ def table_indicator(table, good_from=None):
     return ind
 
 
-def box(method, redo, trouble="hook"):
+def box(method, redo, trouble="hook", extra=None):
     pars = {"amr_mesh_type": "BoxMesh", "amr_luni": 2, "amr_n0": 4, "amr_n1": 4, "amr_n2": 4,
             "scalarwave1_k": "%.17g %.17g %.17g" % tuple(K), "evolve_method": method,
             "nmx_trouble": trouble, "evolve_redo_troubled": redo}
+    pars.update(extra or {})
     m = nm.Mesh(pars, device=0)
     nm.amr_setup_mesh(m)
     return m, OMesh("box", 2, 4)
@@ -165,8 +166,11 @@ def run_both(m, om, method, redo, indicator, nsteps, u0, dt):
     return ot, hook
 
 
-def test_rk4_substep_redo_and_switch_back_on_box():
-    m, om = box("RK4", "substep")
+@pytest.mark.parametrize("graph,path", [("yes", "plain"), ("no", "plain"), ("yes", "batched")])
+def test_rk4_substep_redo_and_switch_back_on_box(graph, path):
+    """the stages from captured graphs (one per RK stage, replayed every
+    step and recaptured after each switch) and eager"""
+    m, om = box("RK4", "substep", extra={"nmx_graph": graph, "nmx_path": path})
     u0, _ = om.profile(K, 0.0)
     dt = om.dt()
     table = {1: {21: 1, 22: 1}, 6: {42: 1}, 7: {21: 1}}
