@@ -456,6 +456,54 @@ def trouble_leg(nm, steps=20, redo_steps=3):
                      "redos": redos, "switches": switches, "fv_elements_after": fv}}
 
 
+def adapt_leg(nm):
+    """Mesh adaptation between steps on the C4 mesh (refine.c:174-506,
+    storage.c:501-760): every 50th element of the coarsest level (3)
+    h-refined into 8 children of its n, those children coarsened back into their parent, then every
+    element's n raised by one (at most 10). Each call rebuilds the host
+    tables and the device mirror and interpolates the state on the device;
+    wall time per call. Refining and coarsening back is exact for the
+    parent's polynomial, so the state must come back to rounding."""
+    m = c4_mesh(nm, "batched")
+    u0 = shell_initial_data(m.coords())
+    m.upload(u0)
+    m.synchronize()
+    ne0 = m.nelem
+    lev = np.array([m.elm_info(e)[1] for e in range(ne0)])
+    flag = np.zeros(ne0, np.int32)
+    flag[np.where(lev == lev.min())[0][::50]] = 1     # children stay 2:1 with their neighbours
+    t0 = time.perf_counter()
+    m.hadapt(flag)
+    m.synchronize()
+    t_ref = time.perf_counter() - t0
+    ne1 = m.nelem
+    back = np.zeros(ne1, np.int32)
+    pos = 0
+    for i in range(ne0):
+        if flag[i]:
+            back[pos:pos + 8] = -1
+            pos += 8
+        else:
+            pos += 1
+    t0 = time.perf_counter()
+    m.hadapt(back)
+    m.synchronize()
+    t_coarse = time.perf_counter() - t0
+    err = float(np.abs(m.download() - u0).max() / np.abs(u0).max())
+    n = np.array([m.elm_info(e)[2] for e in range(m.nelem)], np.int32)
+    t0 = time.perf_counter()
+    m.padapt(np.minimum(n + 1, 10).astype(np.int32))
+    m.synchronize()
+    t_p = time.perf_counter() - t0
+    nodes_p = m.nnodes
+    del m
+    return {"workload": "C4 mesh (19,200 elements, n in {4..10}): every 50th level-3 element h-refined, "
+                        "coarsened back, then n + 1 (at most 10) everywhere",
+            "refine_ms": 1e3 * t_ref, "elements_after_refine": ne1,
+            "coarsen_ms": 1e3 * t_coarse, "roundtrip_rel_err": err,
+            "p_change_ms": 1e3 * t_p, "nodes_after_p_change": nodes_p}
+
+
 def filter_leg(nm, reps=100):
     """The FILTER bin's exponential filter (alp 36, s 32, dn 0, Jacobian
     power 1) of all 5 evolved fields on the 3072-element N=7 shell, in place,
@@ -639,6 +687,8 @@ def main():
     ap.add_argument("--only-crc", action="store_true", help="run only the checkpoint CRC leg")
     ap.add_argument("--only-trouble", action="store_true", help="run only the troubled-element leg")
     ap.add_argument("--no-trouble", action="store_true", help="skip the troubled-element leg")
+    ap.add_argument("--only-adapt", action="store_true", help="run only the mesh-adaptation leg")
+    ap.add_argument("--no-adapt", action="store_true", help="skip the mesh-adaptation leg")
     ap.add_argument("--no-crc", action="store_true", help="skip the checkpoint CRC leg")
     ap.add_argument("--no-fvelm", action="store_true", help="skip the FV-element leg")
     ap.add_argument("--no-filter", action="store_true", help="skip the exponential-filter leg")
@@ -706,6 +756,9 @@ def main():
         return
     if args.only_trouble:
         print(json.dumps(trouble_leg(nm)), flush=True)
+        return
+    if args.only_adapt:
+        print(json.dumps(adapt_leg(nm)), flush=True)
         return
     if args.only_fv:
         print(json.dumps(fv_leg(args.fv_steps, cpu=not args.no_cpu)), flush=True)
@@ -791,6 +844,8 @@ def main():
         out["fv_elements"] = fvelm_leg(nm)
     if not args.no_crc:
         out["checkpoint_crc"] = crc_leg(nm)
+    if not args.no_adapt:
+        out["adaptation"] = adapt_leg(nm)
     if not args.no_trouble:
         try:
             out["troubled_elements"] = trouble_leg(nm)

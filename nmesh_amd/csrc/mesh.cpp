@@ -396,6 +396,30 @@ static uint64_t okey(int p, int l, uint32_t i, uint32_t j, uint32_t k) {
          ((uint64_t)j << 18) | (uint64_t)k;
 }
 
+// fn(t, e0, e1) for the T consecutive ranges t of [0, ne), on up to 16 host
+// threads (one range when ne is small); a range whose thread cannot be
+// started runs on the calling thread.  Returns T.
+template <class F>
+static unsigned for_element_ranges(size_t ne, F fn) {
+  unsigned T = std::thread::hardware_concurrency();
+  T = T < 1 ? 1 : (T > 16 ? 16 : T);
+  if (ne < 64 * (size_t)T) T = 1;
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < T; ++t) {
+    const size_t e0 = ne * t / T, e1 = ne * (t + 1) / T;
+    bool started = false;
+    try {
+      th.emplace_back(fn, t, e0, e1);
+      started = true;
+    } catch (const std::exception &) {
+    }
+    if (!started) fn(t, e0, e1);
+  }
+  fn(0u, (size_t)0, ne / T);
+  for (auto &x : th) x.join();
+  return T;
+}
+
 struct Locator {
   const Mesh &M;
   std::unordered_map<uint64_t, int> leaf;
@@ -474,10 +498,11 @@ static bool other_patch(const Mesh &M, int p, const double x[3], int *q, int *fq
   return false;
 }
 
-static void find_neighbours(Mesh &M) {
-  Locator loc(M);
+// face neighbours of elements [e0, e1): each element writes only its own
+// lists; the locator and the other elements' boxes are read only
+static void find_neighbours_range(Mesh &M, const Locator &loc, size_t e0, size_t e1) {
   const int K = 8;   // samples per face edge: resolves level jumps up to 3
-  for (size_t ei = 0; ei < M.e.size(); ++ei) {
+  for (size_t ei = e0; ei < e1; ++ei) {
     Elm &E = M.e[ei];
     for (int f = 0; f < 6; ++f) {
       std::vector<std::pair<int, int>> found;
@@ -520,6 +545,11 @@ static void find_neighbours(Mesh &M) {
       }
     }
   }
+}
+
+static void find_neighbours(Mesh &M) {
+  const Locator loc(M);
+  for_element_ranges(M.e.size(), [&M, &loc](unsigned, size_t e0, size_t e1) { find_neighbours_range(M, loc, e0, e1); });
   for (auto &P : M.pat)
     for (int f = 0; f < 6; ++f) P.bound[f] = 1;
   for (auto &E : M.e)
@@ -610,30 +640,6 @@ static void geometry_range(Mesh &M, size_t e0, size_t e1) {
         }
     E.hmin = find_hmin(M, E, &E.hmin_pair);
   }
-}
-
-// fn(t, e0, e1) for the T consecutive ranges t of [0, ne), on up to 16 host
-// threads (one range when ne is small); a range whose thread cannot be
-// started runs on the calling thread.  Returns T.
-template <class F>
-static unsigned for_element_ranges(size_t ne, F fn) {
-  unsigned T = std::thread::hardware_concurrency();
-  T = T < 1 ? 1 : (T > 16 ? 16 : T);
-  if (ne < 64 * (size_t)T) T = 1;
-  std::vector<std::thread> th;
-  for (unsigned t = 1; t < T; ++t) {
-    const size_t e0 = ne * t / T, e1 = ne * (t + 1) / T;
-    bool started = false;
-    try {
-      th.emplace_back(fn, t, e0, e1);
-      started = true;
-    } catch (const std::exception &) {
-    }
-    if (!started) fn(t, e0, e1);
-  }
-  fn(0u, (size_t)0, ne / T);
-  for (auto &x : th) x.join();
-  return T;
 }
 
 // Every element's geometry; elements are independent, so ranges of them run
