@@ -33,50 +33,68 @@ namespace nmx {
 
 constexpr int kLimThreads = 128;
 
-__global__ __launch_bounds__(kLimThreads) void lim_data_kernel(LimArgs a) {
+// One wave per element: lane l sums the x lines l, l + 64, ... of the five
+// variables side by side straight from the element block (a line is
+// contiguous, so the block is read once through the caches) and takes their
+// minima and maxima on the way; the y sums of the 5 n rows and the z sums follow from
+// LDS.  Every sum runs in the order of the one-variable quadrature.
+constexpr int kLdThreads = 64;
+
+__global__ __launch_bounds__(kLdThreads) void lim_data_kernel(LimArgs a) {
 #pragma clang fp contract(off)
-  __shared__ double s1[100], s2[10];
-  __shared__ double smn[kLimThreads / 64], smx[kLimThreads / 64];
-  const int tid = threadIdx.x;
+  __shared__ double s1[5 * 100], s2[5 * 10];
+  const int lane = threadIdx.x;
   for (int e = blockIdx.x; e < a.nelem; e += gridDim.x) {
     const int n = a.en[e], n2 = n * n, np = n2 * n;
     const double *w = a.wq + (n - 2) * 10;
     const double *eu = a.u + 5 * a.eoff[e];
-    for (int v = 0; v < 5; ++v) {
-      const double *q = eu + (long)v * np;
-      __syncthreads();
-      if (tid < n2) {
-        double s = 0.0;
-        for (int i = 0; i < n; ++i) s += w[i] * q[i + n * tid];
-        s1[tid] = s;
+    double mn[5], mx[5], s[5];
+    __syncthreads();   // the previous element is done with s1, s2
+#pragma unroll
+    for (int v = 0; v < 5; ++v) mn[v] = mx[v] = eu[(long)v * np];
+    for (int l = lane; l < n2; l += kLdThreads) {
+      // line l of the five variables, point by point: five loads in flight
+#pragma unroll
+      for (int v = 0; v < 5; ++v) s[v] = 0.0;
+#pragma unroll 4
+      for (int i = 0; i < n; ++i) {
+        double x[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) x[v] = eu[(long)v * np + n * l + i];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          s[v] += w[i] * x[v];
+          mn[v] = fmin(mn[v], x[v]);
+          mx[v] = fmax(mx[v], x[v]);
+        }
       }
-      double mn = q[0], mx = q[0];
-      for (int t = tid; t < np; t += kLimThreads) {
-        mn = fmin(mn, q[t]);
-        mx = fmax(mx, q[t]);
-      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) s1[v * n2 + l] = s[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v)
       for (int m = 32; m >= 1; m >>= 1) {
-        mn = fmin(mn, __shfl_xor(mn, m, 64));
-        mx = fmax(mx, __shfl_xor(mx, m, 64));
+        mn[v] = fmin(mn[v], __shfl_xor(mn[v], m, 64));
+        mx[v] = fmax(mx[v], __shfl_xor(mx[v], m, 64));
       }
-      if ((tid & 63) == 0) { smn[tid >> 6] = mn; smx[tid >> 6] = mx; }
-      __syncthreads();
-      if (tid < n) {
-        double s = 0.0;
-        for (int j = 0; j < n; ++j) s += w[j] * s1[j + n * tid];
-        s2[tid] = s;
-      }
-      __syncthreads();
-      if (tid == 0) {
-        double s = 0.0;
-        for (int k = 0; k < n; ++k) s += w[k] * s2[k];
-        double mn2 = smn[0], mx2 = smx[0];
-        for (int i = 1; i < kLimThreads / 64; ++i) { mn2 = fmin(mn2, smn[i]); mx2 = fmax(mx2, smx[i]); }
-        double *o = a.ldat + 20L * e + 4 * v;
-        o[0] = s * 0.125;
-        o[1] = mn2;
-        o[2] = mx2;
-      }
+    __syncthreads();
+    if (lane < 5 * n) {
+      const int v = lane / n, k = lane - v * n;
+      const double *q = s1 + v * n2 + n * k;
+      double y = 0.0;
+      for (int j = 0; j < n; ++j) y += w[j] * q[j];
+      s2[v * n + k] = y;
+    }
+    __syncthreads();
+    if (lane < 5) {
+      const int v = lane;
+      double z = 0.0;
+      for (int k = 0; k < n; ++k) z += w[k] * s2[v * n + k];
+      double *o = a.ldat + 20L * e + 4 * v;
+      o[0] = z * 0.125;
+      // every lane holds the reduced extremes; lane v writes variable v's
+      o[1] = v == 0 ? mn[0] : v == 1 ? mn[1] : v == 2 ? mn[2] : v == 3 ? mn[3] : mn[4];
+      o[2] = v == 0 ? mx[0] : v == 1 ? mx[1] : v == 2 ? mx[2] : v == 3 ? mx[3] : mx[4];
     }
   }
 }
@@ -232,7 +250,7 @@ __global__ __launch_bounds__(kLimThreads) void mm_apply_kernel(LimArgs a) {
 hipError_t launch_lim_data(const LimArgs &a, hipStream_t s) {
   if (a.nelem <= 0) return hipSuccess;
   const int grid = a.nelem < 65536 ? a.nelem : 65536;
-  hipLaunchKernelGGL(lim_data_kernel, dim3(grid), dim3(kLimThreads), 0, s, a);
+  hipLaunchKernelGGL(lim_data_kernel, dim3(grid), dim3(kLdThreads), 0, s, a);
   return hipGetLastError();
 }
 
@@ -242,7 +260,7 @@ hipError_t launch_limiter(const LimArgs &a, hipStream_t s) {
   if (a.kind == 1)
     hipLaunchKernelGGL(mm_data_kernel, dim3(grid), dim3(kLimThreads), 0, s, a);
   else
-    hipLaunchKernelGGL(lim_data_kernel, dim3(grid), dim3(kLimThreads), 0, s, a);
+    hipLaunchKernelGGL(lim_data_kernel, dim3(grid), dim3(kLdThreads), 0, s, a);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
   if (a.kind == 1)

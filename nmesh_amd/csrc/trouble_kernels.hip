@@ -23,38 +23,76 @@
 
 namespace nmx {
 
-constexpr int kTrbThreads = 128;
+constexpr int kTrbThreads = 64;   // rdmp_kernel: one wave per element
 constexpr int kMaxTrouble = 1073741824;
 
+// One wave per element, no barriers.  The lanes fetch the 5 x 24
+// neighbour extremes of u_p (two pairs each) while they stream u for the
+// per-variable minimum, maximum and a NaN flag; wave reductions follow.
+// The element is out of bounds when a variable's minimum is below m - d or
+// its maximum above M + d (or a value is NaN), which is when some value
+// leaves [m - d, M + d].
+__device__ __forceinline__ double wave_min(double x) {
+  for (int m = 32; m >= 1; m >>= 1) x = fmin(x, __shfl_xor(x, m, 64));
+  return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+  for (int m = 32; m >= 1; m >>= 1) x = fmax(x, __shfl_xor(x, m, 64));
+  return x;
+}
+
 __global__ __launch_bounds__(kTrbThreads) void rdmp_kernel(TroubleArgs a) {
-  __shared__ double lo[5], hi[5];
-  const int tid = threadIdx.x;
+  const int lane = threadIdx.x;
   for (int e = blockIdx.x; e < a.nelem; e += gridDim.x) {
     const int n = a.en[e], np = n * n * n;
-    __syncthreads();   // the previous element is done with lo, hi
-    if (tid < 5) {
-      const int v = tid;
-      double m = a.ldat[20L * e + 4 * v + 1], M = a.ldat[20L * e + 4 * v + 2];
-      for (int k = 0; k < 24; ++k) {
-        const int nb = a.nbr[24 * e + k];
-        if (nb < 0) continue;
-        m = fmin(m, a.ldat[20L * nb + 4 * v + 1]);
-        M = fmax(M, a.ldat[20L * nb + 4 * v + 2]);
-      }
-      const double d = fmax(a.delta0, a.eps * (M - m));
-      lo[v] = m - d;
-      hi[v] = M + d;
+    const double *own = a.ldat + 20L * e;
+    // neighbour pairs p = lane and lane + 64 (of 120: p = 24 v + k)
+    double pm[2], pM[2];
+    int pv[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int p = lane + 64 * h;
+      pv[h] = p < 120 ? p / 24 : -1;
+      const int nb = p < 120 ? a.nbr[24 * e + (p - 24 * (p / 24))] : -1;
+      const double *src = nb < 0 ? own : a.ldat + 20L * nb;   // a missing neighbour: the element's own
+      const int v = p < 120 ? pv[h] : 0;
+      pm[h] = src[4 * v + 1];
+      pM[h] = src[4 * v + 2];
     }
-    __syncthreads();
     const double *eu = a.u + 5 * a.eoff[e];
+    int nan = 0;
     int out = 0;
-    for (int t = tid; t < 5 * np; t += kTrbThreads) {
-      const int v = t / np;
-      const double x = eu[t];
-      if (!(x >= lo[v] && x <= hi[v])) out = 1;   // NaN is out of bounds too
+    double umn[5], umx[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) umn[v] = umx[v] = eu[(long)v * np];
+    // the five variables' values at t requested together
+#pragma unroll 2
+    for (int t = lane; t < np; t += kTrbThreads) {
+      double x[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) x[v] = eu[(long)v * np + t];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        nan |= x[v] != x[v];
+        umn[v] = fmin(umn[v], x[v]);
+        umx[v] = fmax(umx[v], x[v]);
+      }
     }
-    out = __syncthreads_or(out);
-    if (tid == 0) a.tr[e] = a.efv[e] ? (out ? 0 : -1) : (out ? 1 : 0);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const double mn = wave_min(umn[v]);
+      const double mx = wave_max(umx[v]);
+      double m = own[4 * v + 1], M = own[4 * v + 2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (pv[h] == v) { m = fmin(m, pm[h]); M = fmax(M, pM[h]); }
+      m = wave_min(m);
+      M = wave_max(M);
+      const double d = fmax(a.delta0, a.eps * (M - m));
+      if (!(mn >= m - d && mx <= M + d)) out = 1;
+    }
+    for (int m = 32; m >= 1; m >>= 1) nan |= __shfl_xor(nan, m, 64);
+    if (lane == 0) a.tr[e] = a.efv[e] ? (out || nan ? 0 : -1) : (out || nan ? 1 : 0);
   }
 }
 
@@ -110,7 +148,7 @@ __global__ __launch_bounds__(256) void trouble_fix_kernel(TroubleFix f) {
 
 hipError_t launch_rdmp(const TroubleArgs &a, hipStream_t s) {
   if (a.nelem <= 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(rdmp_kernel, dim3(a.nelem < 8192 ? a.nelem : 8192), dim3(kTrbThreads), 0, s, a);
+  hipLaunchKernelGGL(rdmp_kernel, dim3(a.nelem < 65536 ? a.nelem : 65536), dim3(kTrbThreads), 0, s, a);
   return hipGetLastError();
 }
 
