@@ -649,9 +649,13 @@ static void geometry(Mesh &M) {
   for (auto &E : M.e) { E.off = off; off += (long)E.n * E.n * E.n; }
   M.nnodes = off;
   long nn = off;
-  M.x.assign(3 * nn, 0.0);
-  M.dXdx.assign(9 * nn, 0.0);
-  M.sqrtgdiag.assign(3 * nn, 0.0);
+  // every entry is written below, so no fill
+  M.x.clear();
+  M.dXdx.clear();
+  M.sqrtgdiag.clear();
+  M.x.resize(3 * nn);
+  M.dXdx.resize(9 * nn);
+  M.sqrtgdiag.resize(3 * nn);
   const size_t ne = M.e.size();
   for (auto &E : M.e) (void)(E.fv ? uniform_basis(E.n) : lgl_basis(E.n));   // fill the caches first
   for_element_ranges(ne, [&M](unsigned, size_t e0, size_t e1) { geometry_range(M, e0, e1); });

@@ -7,6 +7,9 @@
 // (src/main/amr/connections.c:1175-1206).
 #pragma once
 #include <cstdint>
+#include <memory>
+#include <new>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -75,13 +78,30 @@ struct InterpSlot {
   std::vector<double> Ba, Bb;       // Ba[a*m+q]: l_q at my a-th point (covered rows)
 };
 
+// An allocator whose resize leaves new doubles uninitialised: the node
+// arrays are written in full by the threads that build them, so a serial
+// zero fill (and its page faults) before them is wasted.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind { using other = NoInitAlloc<U>; };
+  NoInitAlloc() = default;
+  template <class U>
+  NoInitAlloc(const NoInitAlloc<U> &) {}
+  template <class U>
+  void construct(U *p) { ::new (static_cast<void *>(p)) U; }
+  template <class U, class... A>
+  void construct(U *p, A &&...a) { ::new (static_cast<void *>(p)) U(std::forward<A>(a)...); }
+};
+using NodeArray = std::vector<double, NoInitAlloc<double>>;
+
 struct Mesh {
   std::vector<Patch> pat;
   std::vector<Elm> e;
   long nnodes = 0;
-  std::vector<double> x;            // x[d*nnodes + node]
-  std::vector<double> dXdx;         // dXdx[(3a+b)*nnodes + node]
-  std::vector<double> sqrtgdiag;    // sqrtgdiag[a*nnodes + node]
+  NodeArray x;                      // x[d*nnodes + node]
+  NodeArray dXdx;                   // dXdx[(3a+b)*nnodes + node]
+  NodeArray sqrtgdiag;              // sqrtgdiag[a*nnodes + node]
   std::vector<FaceCopy> fcopy;      // 6 per element
   std::vector<std::vector<InterpSlot>> fint;  // 6 per element
   int maxlevel = 0;
