@@ -1046,23 +1046,30 @@ static int upload_mesh(nmx_ctx *c) {
       if (Pt.type == PAT_CUBEDSHELL && (Pt.xc[0] != 0 || Pt.xc[1] != 0 || Pt.xc[2] != 0))
         c->geom = 1;   // recomputation assumes the shell centre at the origin
   }
-  std::vector<double> h9(c->geom == 1 ? 9 * nn : 1), h3(3 * ne), hx(3 * nn), hg(8 * ne);
+  // element-block copies of the node arrays (every entry written, so no
+  // fill), built over element ranges on several threads
+  NodeArray h9(c->geom == 1 ? 9 * nn : 1), hx(3 * nn);
+  std::vector<double> h3(3 * ne), hg(8 * ne);
   std::vector<int> hdom(ne);
-  for (size_t i = 0; i < ne; ++i) {
-    const Elm &E = M.e[i];
-    long np = (long)E.n * E.n * E.n;
-    for (int d = 0; d < 3; ++d) h3[3 * i + d] = 2. / (E.bbox[2 * d + 1] - E.bbox[2 * d]);
-    for (int q = 0; q < 6; ++q) hg[8 * i + q] = E.bbox[q];
-    hg[8 * i + 6] = M.pat[E.p].s0;
-    hg[8 * i + 7] = M.pat[E.p].s1;
-    hdom[i] = M.pat[E.p].dom;
-    for (long q = 0; q < np; ++q) {
-      long g = E.off + q;
-      if (c->geom == 1)
-        for (int m = 0; m < 9; ++m) h9[9 * E.off + m * np + q] = M.dXdx[m * nn + g];
-      for (int d = 0; d < 3; ++d) hx[3 * E.off + d * np + q] = M.x[d * nn + g];
+  const int geom = c->geom;
+  for_element_ranges(ne, [&](unsigned, size_t e0, size_t e1) {
+    for (size_t i = e0; i < e1; ++i) {
+      const Elm &E = M.e[i];
+      long np = (long)E.n * E.n * E.n;
+      for (int d = 0; d < 3; ++d) h3[3 * i + d] = 2. / (E.bbox[2 * d + 1] - E.bbox[2 * d]);
+      for (int q = 0; q < 6; ++q) hg[8 * i + q] = E.bbox[q];
+      hg[8 * i + 6] = M.pat[E.p].s0;
+      hg[8 * i + 7] = M.pat[E.p].s1;
+      hdom[i] = M.pat[E.p].dom;
+      for (long q = 0; q < np; ++q) {
+        long g = E.off + q;
+        if (geom == 1)
+          for (int m = 0; m < 9; ++m) h9[9 * E.off + m * np + q] = M.dXdx[m * nn + g];
+        for (int d = 0; d < 3; ++d) hx[3 * E.off + d * np + q] = M.x[d * nn + g];
+      }
     }
-  }
+  });
+  if (geom != 1) h9[0] = 0.0;
   if ((r = dev_alloc(&c->dXdx, h9.size())) || (r = dev_alloc(&c->dXbdX, h3.size())) ||
       (r = dev_alloc(&c->dxyz, hx.size())))
     return r;

@@ -396,30 +396,6 @@ static uint64_t okey(int p, int l, uint32_t i, uint32_t j, uint32_t k) {
          ((uint64_t)j << 18) | (uint64_t)k;
 }
 
-// fn(t, e0, e1) for the T consecutive ranges t of [0, ne), on up to 16 host
-// threads (one range when ne is small); a range whose thread cannot be
-// started runs on the calling thread.  Returns T.
-template <class F>
-static unsigned for_element_ranges(size_t ne, F fn) {
-  unsigned T = std::thread::hardware_concurrency();
-  T = T < 1 ? 1 : (T > 16 ? 16 : T);
-  if (ne < 64 * (size_t)T) T = 1;
-  std::vector<std::thread> th;
-  for (unsigned t = 1; t < T; ++t) {
-    const size_t e0 = ne * t / T, e1 = ne * (t + 1) / T;
-    bool started = false;
-    try {
-      th.emplace_back(fn, t, e0, e1);
-      started = true;
-    } catch (const std::exception &) {
-    }
-    if (!started) fn(t, e0, e1);
-  }
-  fn(0u, (size_t)0, ne / T);
-  for (auto &x : th) x.join();
-  return T;
-}
-
 struct Locator {
   const Mesh &M;
   std::unordered_map<uint64_t, int> leaf;

@@ -7,6 +7,8 @@
 // (src/main/amr/connections.c:1175-1206).
 #pragma once
 #include <cstdint>
+#include <exception>
+#include <thread>
 #include <memory>
 #include <new>
 #include <utility>
@@ -108,6 +110,30 @@ struct Mesh {
   int fv_surface_interp = 0;        // rows from FV neighbours: 0 piecewise linear, 1 parabolic
   bool fv_lagrange_if_dg = false;   // ... but Lagrange rows into DG elements
 };
+
+// fn(t, e0, e1) for the T consecutive ranges t of [0, ne), on up to 16 host
+// threads (one range when ne is small); a range whose thread cannot be
+// started runs on the calling thread.  Returns T.
+template <class F>
+inline unsigned for_element_ranges(size_t ne, F fn) {
+  unsigned T = std::thread::hardware_concurrency();
+  T = T < 1 ? 1 : (T > 16 ? 16 : T);
+  if (ne < 64 * (size_t)T) T = 1;
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < T; ++t) {
+    const size_t e0 = ne * t / T, e1 = ne * (t + 1) / T;
+    bool started = false;
+    try {
+      th.emplace_back(fn, t, e0, e1);
+      started = true;
+    } catch (const std::exception &) {
+    }
+    if (!started) fn(t, e0, e1);
+  }
+  fn(0u, (size_t)0, ne / T);
+  for (auto &x : th) x.join();
+  return T;
+}
 
 // A copy of M's patches, elements and settings without the node arrays
 // (x, dXdx, sqrtgdiag) and face tables (fcopy, fint), which every rebuilder
