@@ -25,6 +25,9 @@ def product(kind, luni, n, **kw):
     return m
 
 
+# the 1056-element refined shell is built over element ranges on several
+# host threads (64 elements per thread at least, up to 16 threads); the
+# smaller meshes on one
 CASES = [("box", 2, 5, {}), ("box", 1, 3, {"dout": 1.0}), ("shell", 1, 6, {}),
          ("shell", 2, 4, {}), ("shell", 2, 4, {"radius": 3, "levels": 2}),
          ("shell", 1, 5, {"radius": 9, "levels": 1})]
