@@ -12,6 +12,7 @@
 #include <map>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -1386,6 +1387,16 @@ static int adapt_state(nmx_ctx *c, const Mesh &Mold, const double *uold, double 
 
 // the mesh changed on the host: new device mirror, the state interpolated.
 // If the new mirror cannot be made, the old mesh and state are restored.
+// A replaced mesh is freed on a detached host thread: its node arrays and
+// face tables (over a GB on large meshes) take tens of milliseconds to
+// release, and nothing waits for them.  Without a thread it is freed here.
+static void release_mesh(Mesh &&m) {
+  try {
+    std::thread([](Mesh) {}, std::move(m)).detach();
+  } catch (const std::exception &) {
+  }
+}
+
 static int remirror_with_state(nmx_ctx *c, const Mesh &old) {
   const double dt_old = c->dt;
   if (P(c, "dt") == "auto") c->dt = mesh_dt(c->mesh, Pd(c, "dtfac"), Pd(c, "uniform_dtfac"));
@@ -1418,8 +1429,10 @@ extern "C" int nmx_mesh_hadapt(nmx_ctx *c, const int *flag, int nelem) {
   Mesh nw = mesh_skeleton(c->mesh);
   std::string err = hadapt(nw, std::vector<int>(flag, flag + nelem));
   if (!err.empty()) return fail(NMX_EINVAL, err);
-  const Mesh old = std::exchange(c->mesh, std::move(nw));
-  return remirror_with_state(c, old);
+  Mesh old = std::exchange(c->mesh, std::move(nw));
+  const int r = remirror_with_state(c, old);
+  release_mesh(std::move(old));
+  return r;
 }
 
 extern "C" int nmx_mesh_padapt(nmx_ctx *c, const int *n, int nelem) {
@@ -1433,8 +1446,10 @@ extern "C" int nmx_mesh_padapt(nmx_ctx *c, const int *n, int nelem) {
   Mesh nw = mesh_skeleton(c->mesh);
   std::string err = set_elm_n(nw, std::vector<int>(n, n + nelem));
   if (!err.empty()) return fail(NMX_EINVAL, err);
-  const Mesh old = std::exchange(c->mesh, std::move(nw));
-  return remirror_with_state(c, old);
+  Mesh old = std::exchange(c->mesh, std::move(nw));
+  const int r = remirror_with_state(c, old);
+  release_mesh(std::move(old));
+  return r;
 }
 
 extern "C" int nmx_mesh_switch_fv(nmx_ctx *c, const int *to_fv, int nelem) {
@@ -1446,8 +1461,10 @@ extern "C" int nmx_mesh_switch_fv(nmx_ctx *c, const int *to_fv, int nelem) {
   Mesh nw = mesh_skeleton(c->mesh);
   std::string err = set_fv_elems(nw, std::vector<int>(to_fv, to_fv + nelem));
   if (!err.empty()) return fail(NMX_EINVAL, err);
-  const Mesh old = std::exchange(c->mesh, std::move(nw));
-  return remirror_with_state(c, old);
+  Mesh old = std::exchange(c->mesh, std::move(nw));
+  const int r = remirror_with_state(c, old);
+  release_mesh(std::move(old));
+  return r;
 }
 
 extern "C" int nmx_mesh_set_elm_n(nmx_ctx *c, const int *n, int nelem) {
@@ -2295,13 +2312,14 @@ static int trouble_switch_to(nmx_ctx *c, Mesh nw, int carry) {
     held.push_back({flag, tmp});
   }
   const std::vector<int> keep = c->hscore;
-  const Mesh old = std::exchange(c->mesh, std::move(nw));
+  Mesh old = std::exchange(c->mesh, std::move(nw));
   int r = remirror_with_state(c, old);
   for (const Held &h : held) {
     double *dst = h.flag == CARRY_A ? c->dA : h.flag == CARRY_W0 ? c->dW0 : c->dW1;
     if (!r) r = adapt_state(c, old, h.tmp, dst);
     (void)hipFree(h.tmp);
   }
+  release_mesh(std::move(old));
   if (r) return r;
   // the step's stage times (the mirror's copy is new)
   HIPCHK(hipMemcpyAsync(c->dT, c->hT, 4 * sizeof(double), hipMemcpyHostToDevice, c->stream));
