@@ -2453,7 +2453,7 @@ static int redo_rhs(nmx_ctx *c, const TStage &S, int s) {
   for (int k = 0; k < ncls && !r; ++k) {
     std::vector<int> fk(ne), mk(ne);
     for (size_t e = 0; e < ne; ++e) { mk[e] = cls[e] == k; fk[e] = M0.e[e].fv || mk[e]; }
-    c->mesh = M0;
+    release_mesh(std::exchange(c->mesh, Mesh(M0)));
     err = set_fv_elems(c->mesh, fk);
     if (!err.empty()) { r = fail(NMX_EINVAL, err); break; }
     if ((r = mirror_again(c))) break;
@@ -2470,7 +2470,7 @@ static int redo_rhs(nmx_ctx *c, const TStage &S, int s) {
   // back to the switched mesh with its registers, also after a failure
   // (the first error is the one returned)
   const bool held_ok = held[0] && held[1] && held[2] && held[3] && held[4];
-  c->mesh = std::move(Mall);
+  release_mesh(std::exchange(c->mesh, std::move(Mall)));
   int r2 = mirror_again(c);
   if (!r2 && held_ok) {
     double *dst[5] = {c->dU, c->dA, c->dW0, c->dW1, c->dR};
